@@ -1,0 +1,203 @@
+/*
+ * pftcc.h - C ABI of the MI355X (gfx950) polar-Fourier correlation search.
+ *
+ * Drop-in boundary for SIMPLE's type polarft_corrcalc
+ * (src/main/simple_polarft_corrcalc.f90:68-198).  Every entry point below names the reference
+ * procedure it replaces.  The Fortran module simple3.0_amd/fortran/simple_polarft_corrcalc.f90
+ * binds these with ISO_C_BINDING and keeps the reference's type-bound procedure names.
+ *
+ * Conventions
+ *  - Plain C types only: int32_t/int64_t/float/double and pointers.  Complex single precision
+ *    values are interleaved (re, im) float pairs; a polar FT block is float[nk][pftsz][2],
+ *    which is the memory order of the reference's complex(sp) pft(pftsz, kfromto(1):kfromto(2)).
+ *  - Indices keep the reference's meaning and base: iref in [1, nrefs]; iptcl is a GLOBAL
+ *    particle index in [pfrom, pto] mapped through the pinds table (:276-288, :509-554);
+ *    irot in [1, nrots], irot = 1 <-> 0 degrees (:261-274).
+ *  - Every function returns 0 on success and a non-zero PFTCC_E* code on failure; the message
+ *    is available from pftcc_last_error(h).  The reference stops the program instead
+ *    (THROW_HARD, src/fileio/simple_error.f90:8-31); the Fortran wrapper does the same.
+ *  - Host pointers are borrowed for the duration of the call.  Entry points with the suffix
+ *    _dev take device pointers (memory on the GPU the handle was created on).
+ *  - Setters must not run concurrently with searches on the same handle.  Per-pair calls
+ *    (gencorrs, gencorr_*_for_rot_8) may be issued from several host threads; they are
+ *    serialised on the handle's stream.  Batched calls are the fast path.
+ */
+#ifndef PFTCC_H
+#define PFTCC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFTCC_OK 0
+#define PFTCC_E_ARG 1      /* invalid argument (index out of range, bad size, NULL pointer) */
+#define PFTCC_E_STATE 2    /* call out of order (e.g. particle FFTs not memoized)           */
+#define PFTCC_E_DEVICE 3   /* HIP runtime error or device out of memory                      */
+#define PFTCC_E_UNSUPPORTED 4
+
+#define PFTCC_OBJFUN_CC 0      /* OBJFUN_CC     (simple_defs)  */
+#define PFTCC_OBJFUN_EUCLID 1  /* OBJFUN_EUCLID (simple_defs)  */
+
+/* Explicit replacement of the params_glob fields the reference reads inside the module
+ * (simple_parameters.f90:1182-1193 ring2, 1226-1229 boxmatch, 1262-1264 kfromto/kstop,
+ * 1336-1371 objfun/match_filt/pssnr).  No hidden globals. */
+typedef struct pftcc_cfg {
+  int32_t boxmatch;    /* ldim(1:2) of the Cartesian image; must be even (new :236-243)      */
+  int32_t ring2;       /* pftsz = magic_pftsz(ring2) when pftsz == 0 (new :251)              */
+  int32_t pftsz;       /* explicit half-ring length (even) or 0                               */
+  int32_t kfromto[2];  /* shell range held in memory                                         */
+  int32_t kstop;       /* last shell used by the correlations (<= kfromto[1])                */
+  int32_t objfun;      /* PFTCC_OBJFUN_CC or PFTCC_OBJFUN_EUCLID (cc_objfun)                  */
+  int32_t match_filt;  /* l_match_filt                                                        */
+  int32_t pssnr;       /* l_pssnr                                                             */
+  int32_t ctf;         /* 0: ctf=no; 1: with_ctf (ctf /= 'no', :355-356)                      */
+  int32_t clsfrcs;     /* l_clsfrcs (:415)                                                    */
+  int32_t nthr;        /* host threads that may issue per-pair calls (informational)          */
+} pftcc_cfg;
+
+/* Per-particle CTF parameters (ctfparams from sp_project%get_ctfparams, used by
+ * create_polar_absctfmats :924-953; units as in simple_ctf.f90:54-81). */
+typedef struct pftcc_ctfparams {
+  float smpd;      /* A/pixel                          */
+  float kv;        /* acceleration voltage, kV         */
+  float cs;        /* spherical aberration, mm         */
+  float fraca;     /* amplitude contrast fraction      */
+  float dfx, dfy;  /* defocus, microns                 */
+  float angast;    /* astigmatism angle, degrees       */
+  float phshift;   /* phase-plate shift, radians       */
+  int32_t l_phaseplate;
+} pftcc_ctfparams;
+
+typedef struct pftcc pftcc_t;
+
+/* --------------------------------------------------------------- construction / teardown -- */
+/* new (:209-422).  ptcl_mask (int8[pto-pfrom+1], nonzero = keep) and eoarr (int32, 0 = even)
+ * may be NULL.  The handle is created on the current HIP device. */
+int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto,
+              const int8_t* ptcl_mask, const int32_t* eoarr, pftcc_t** out);
+/* kill (:2234-2279) */
+int pftcc_kill(pftcc_t* h);
+const char* pftcc_last_error(const pftcc_t* h);
+/* Asynchronous work is issued on this stream (hipStream_t; NULL = the null stream). */
+int pftcc_set_stream(pftcc_t* h, void* hip_stream);
+int pftcc_sync(pftcc_t* h);
+
+/* --------------------------------------------------------------------------------- setters -- */
+int pftcc_set_ref_pft(pftcc_t* h, int32_t iref, const float* pft, int32_t iseven);   /* :426 */
+int pftcc_set_ptcl_pft(pftcc_t* h, int32_t iptcl, const float* pft);                /* :438 */
+/* Bulk forms of the two setters (same semantics; count consecutive blocks). */
+int pftcc_set_ref_pfts(pftcc_t* h, int32_t iref_first, int32_t count, const float* pfts,
+                       int32_t iseven);
+int pftcc_set_ptcl_pfts(pftcc_t* h, int32_t iptcl_first, int32_t count, const float* pfts);
+int pftcc_set_ref_pfts_dev(pftcc_t* h, int32_t iref_first, int32_t count, const void* dpfts,
+                           int32_t iseven);
+int pftcc_set_ptcl_pfts_dev(pftcc_t* h, int32_t iptcl_first, int32_t count, const void* dpfts);
+int pftcc_set_ref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, float re, float im,
+                        int32_t iseven);                                              /* :447 */
+int pftcc_set_ptcl_fcomp(pftcc_t* h, int32_t iptcl, int32_t irot, int32_t k, float re,
+                         float im);                                                   /* :471 */
+int pftcc_cp_even2odd_ref(pftcc_t* h, int32_t iref);                                 /* :485 */
+int pftcc_cp_even_ref2ptcl(pftcc_t* h, int32_t iref, int32_t iptcl);                 /* :497 */
+int pftcc_swap_ptclsevenodd(pftcc_t* h);                                             /* :504 */
+int pftcc_reallocate_ptcls(pftcc_t* h, int32_t nptcls, const int32_t* pinds);        /* :509 */
+int pftcc_set_eo(pftcc_t* h, int32_t iptcl, int32_t is_even);                        /* :557 */
+int pftcc_set_eos(pftcc_t* h, const int32_t* eoarr /* [nptcls] */);                  /* :564 */
+int pftcc_set_ref_optlp(pftcc_t* h, int32_t iref, const float* optlp /* [kfrom..kstop] */); /* :587 */
+/* assign_sigma2_noise (:581-585): sigma2[(iptcl - pfrom) * nk + (k - kfrom)] for
+ * iptcl in [pfrom, pto], k in [kfrom, kto].  The data is copied. */
+int pftcc_assign_sigma2_noise(pftcc_t* h, const float* sigma2);
+int pftcc_zero_refs_beyond_kstop(pftcc_t* h);                                        /* :747 */
+
+/* ------------------------------------------------------------------------------ memoizers -- */
+/* create_polar_absctfmats (:906-959).  params[i] belongs to global particle pfrom + i
+ * (i in [0, pto - pfrom]); particles masked out are skipped. */
+int pftcc_create_polar_absctfmats(pftcc_t* h, const pftcc_ctfparams* params);
+int pftcc_memoize_ffts(pftcc_t* h);                                                  /* :848 */
+
+/* --------------------------------------------------------------------------------- getters -- */
+int pftcc_get_nrots(const pftcc_t* h, int32_t* nrots);                               /* :597 */
+int pftcc_get_pdim(const pftcc_t* h, int32_t pdim[3]);                               /* :604 */
+int pftcc_get_pftsz(const pftcc_t* h, int32_t* pftsz);                               /* :611 */
+int pftcc_get_box(const pftcc_t* h, int32_t* box);                                   /* :616 */
+int pftcc_get_rot(const pftcc_t* h, int32_t roind, float* rot);                      /* :623 */
+int pftcc_get_roind(const pftcc_t* h, float rot, int32_t* roind);                    /* :635 */
+int pftcc_get_coord(const pftcc_t* h, int32_t irot, int32_t k, float xy[2]);         /* :648 */
+int pftcc_get_nrefs(const pftcc_t* h, int32_t* nrefs);                               /* :685 */
+int pftcc_get_nptcls(const pftcc_t* h, int32_t* nptcls);                             /* :701 */
+int pftcc_ptcl_iseven(const pftcc_t* h, int32_t iptcl, int32_t* iseven);            /* :695 */
+int pftcc_get_ref_pft(pftcc_t* h, int32_t iref, int32_t iseven, float* pft);        /* :670 */
+int pftcc_get_ptcl_pft(pftcc_t* h, int32_t iptcl, float* pft);                      /* :660 */
+/* Device-resident memo and CTF matrices, for verification: ctfmat float[nk][pftsz];
+ * memo float[nk][pftsz+1][2] = real FFT of length 2*pftsz of the particle's ring (the
+ * reference's PA/PB memo re-packed: W[2t] = 2 PA[t], W[2t+1] = 2 PB[t]). */
+int pftcc_get_ctfmat(pftcc_t* h, int32_t iptcl, float* ctfmat);
+int pftcc_get_memo(pftcc_t* h, int32_t iptcl, float* memo);
+int pftcc_get_sqsum_ptcl(pftcc_t* h, int32_t iptcl, float* sqsum);
+
+/* ------------------------------------------------------------------------- calculators -- */
+/* gencorrs (:173, 1594-1617): cc[nrots].  shvec == NULL selects gencorrs_1, else gencorrs_2
+ * (shift applied to the reference).  objfun CC or EUCLID as configured. */
+int pftcc_gencorrs(pftcc_t* h, int32_t iref, int32_t iptcl, const float* shvec, float* cc);
+/* gencorr_for_rot (:1619) and gencorr_for_rot_8 (:1632) */
+int pftcc_gencorr_for_rot(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2],
+                          int32_t irot, float* f);
+int pftcc_gencorr_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                            int32_t irot, double* f);
+/* gencorr_grad_for_rot_8 (:1849) and gencorr_grad_only_for_rot_8 (:1927) */
+int pftcc_gencorr_grad_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                 int32_t irot, double* f, double grad[2]);
+int pftcc_gencorr_grad_only_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl,
+                                      const double shvec[2], int32_t irot, double grad[2]);
+/* gencorr_sigma_contrib (:2150): sigma_contrib[nk] (+ optional ptcl/ref sums of squares). */
+int pftcc_gencorr_sigma_contrib(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2],
+                                int32_t irot, float* sigma_contrib, float* ptcl_sumsq,
+                                float* ref_sumsq);
+/* specscore (:2190-2205); shvec may be NULL (specscore_1) */
+int pftcc_specscore(pftcc_t* h, int32_t iref, int32_t iptcl, int32_t irot, const float* shvec,
+                    float* score);
+/* calc_roinv_corrmat (:2207-2230): corrmat[nptcls][nptcls] (requires nptcls == nrefs). */
+int pftcc_calc_roinv_corrmat(pftcc_t* h, float* corrmat);
+
+/* ----------------------------------------------------------------- batched fast paths -- */
+/* Exhaustive correlation of particles x references x shifts, all rotations.
+ * iptcls[np] (global ids), irefs[nr] (1-based; NULL = all refs 1..nr), shifts[ns][2]
+ * (NULL with ns == 0 selects the unshifted gencorrs_1 semantics).
+ * cc_out layout: [np][nr][max(ns,1)][nrots] (host memory).  This is what a loop of
+ * gencorrs(iref, iptcl[, shvec], cc) over the same lists returns, bit-compatible with it. */
+int pftcc_gencorrs_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
+                         const int32_t* irefs, int32_t ns, const float* shifts, float* cc_out);
+/* The same search reduced on the device: for each particle the best (ref, shift, irot) over
+ * all nr x max(ns,1) x nrots evaluations, ties to the lowest (ref, shift, irot) in that order.
+ * Outputs (host memory, length np): best_cc, best_iref (1-based), best_ishift (0-based into
+ * shifts, 0 if ns == 0), best_irot (1-based).  The loop it replaces is the greedy ref loop of
+ * strategy3D_greedy%srch (simple_strategy3D_greedy.f90:60-77: gencorrs + maxloc). */
+int pftcc_search(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
+                 int32_t ns, const float* shifts, float* best_cc, int32_t* best_iref,
+                 int32_t* best_ishift, int32_t* best_irot);
+/* Device-pointer form of pftcc_search: all array arguments are device pointers; runs
+ * asynchronously on the handle's stream. */
+int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr,
+                     const int32_t* d_irefs, int32_t ns, const float* d_shifts, float* d_best_cc,
+                     int32_t* d_best_iref, int32_t* d_best_ishift, int32_t* d_best_irot);
+/* Batched gencorr_grad_for_rot_8 over npairs (iref, iptcl, shvec, irot) tuples (the callers are
+ * pftcc_shsrch_grad's cost function, simple_pftcc_shsrch_grad.f90:114-131, per particle and
+ * local reference).  f[np], grad[np][3]: d/dshift_x, d/dshift_y and d/dpsi, psi the continuous
+ * in-plane angle in radians at psi = (irot-1)*pi/pftsz (the band-limited rotational
+ * interpolant of the objective).  grad may be NULL for objective-only evaluation. */
+int pftcc_grad_batch(pftcc_t* h, int32_t npairs, const int32_t* irefs, const int32_t* iptcls,
+                     const double* shvecs, const int32_t* irots, double* f, double* grad);
+int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs,
+                         const int32_t* d_iptcls, const double* d_shvecs, const int32_t* d_irots,
+                         double* d_f, double* d_grad);
+
+/* ------------------------------------------------------------------------- utilities -- */
+/* magic_pftsz (simple_magic_boxes.f90:25-37). */
+int32_t pftcc_magic_pftsz(int32_t ring2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFTCC_H */

@@ -1,0 +1,1570 @@
+// pftcc.hip - C ABI (include/pftcc.h) of the gfx950 polar-FT correlation search.
+//
+// Host-side state mirrors type polarft_corrcalc (simple_polarft_corrcalc.f90:68-198): the polar
+// grid, argument-transfer constants and CTF tables are built on the host in the reference's
+// precision (FP32 grid, FP64 argtransf) and uploaded once; rings, CTF matrices and the particle
+// spectra live in HBM and every calculator runs on the device.  There is no CPU fallback: a
+// missing or failing device is reported as PFTCC_E_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pftcc.h"
+#include "pftcc_kernels.hpp"
+
+using namespace pfk;
+
+namespace {
+
+// ------------------------------------------------------------------ magic_pftsz (host) ---
+// simple_magic_boxes.f90:14-19 (table) and :25-62; simple_math.f90:391-405 (round2even),
+// :2051-2084 (locate_2/find_2).  Evaluated in real(sp) like the reference.
+const int kBoxSizes[] = {
+    32, 36, 40, 48, 52, 56, 64, 66, 70, 72, 80, 84, 88, 100, 104, 108, 112, 120, 128, 130, 132,
+    140, 144, 150, 160, 162, 168, 176, 180, 182, 192, 200, 208, 216, 220, 224, 240, 256, 264, 288,
+    300, 308, 320, 324, 336, 338, 352, 364, 384, 400, 420, 432, 448, 450, 462, 480, 486, 500, 504,
+    512, 520, 528, 546, 560, 576, 588, 600, 640, 648, 650, 660, 672, 686, 700, 702, 704, 720, 726,
+    728, 750, 768, 770, 784, 800, 810, 840, 882, 896, 910, 924, 936, 972, 980, 1008, 1014, 1020,
+    1024, 1296, 1536, 1728, 1944, 2048, 2304, 2592, 3072, 3200, 3456, 3888, 4096, 4608, 5000, 5184,
+    6144, 6250, 6400, 6912, 7776, 8192, 9216, 10240, 12288, 12500};
+const int kNBox = sizeof(kBoxSizes) / sizeof(int);
+const float kPi32 = 3.14159274101257324f;   // acos(-1.) in real(sp)
+const double kDPi = 3.14159265358979323846;
+
+int fnint(float x) { return (int)(x >= 0.f ? std::floor(x + 0.5f) : -std::floor(-x + 0.5f)); }
+
+int round2even(float val) {
+  const int r = fnint(val);
+  if (r % 2 == 0) return r;
+  if (std::fabs((float)(r - 1) - val) <= std::fabs((float)(r + 1) - val)) return r - 1;
+  return r + 1;
+}
+
+int locate_box(int x) {
+  const int n = kNBox;
+  int jl = 0, ju = n + 1;
+  while (ju - jl > 1) {
+    const int jm = (ju + jl) / 2;
+    if ((kBoxSizes[n - 1] >= kBoxSizes[0]) == (x >= kBoxSizes[jm - 1])) jl = jm;
+    else ju = jm;
+  }
+  if (x == kBoxSizes[0]) return 1;
+  if (x == kBoxSizes[n - 1]) return n - 1;
+  return jl;
+}
+
+int find_magic_box(int trial) {
+  int j = std::max(1, locate_box(trial));
+  const int dist1 = kBoxSizes[j - 1] - trial;
+  if (j < kNBox) {
+    const int dist2 = kBoxSizes[j] - trial;
+    if (std::abs(dist1) >= std::abs(dist2)) j = j + 1;
+  }
+  int best = kBoxSizes[j - 1];
+  if (best == kBoxSizes[kNBox - 1] && trial - best > 16) best = round2even((float)trial);
+  return best;
+}
+
+int magic_pftsz_host(int ring2) {
+  const float a = kPi32 * (float)ring2;
+  const int old = round2even(a);
+  const int p = find_magic_box(fnint(a));
+  if ((float)std::abs(p - old) / (float)old > 0.1f) return old;
+  return p;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------------- handle ---
+struct pftcc {
+  pftcc_cfg cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, nrots = 0, kfrom = 0, kto = 0, kstop = 0, nk = 0, kcorr = 0;
+  int nrefs = 0, nptcls = 0, pfrom = 0, pto = 0, ldim = 0;
+  bool with_ctf = false, match_filt = false, pssnr = false, clsfrcs = false, euclid = false;
+  std::vector<int> pinds;           // pinds[iptcl - pfrom] = local 1-based index, 0 = absent
+  std::vector<int8_t> iseven;       // host mirror of the eo flags
+  std::vector<float> angtab_rad, angtab_deg, polar_x, polar_y, spafreq, angmat, pxls;
+  std::vector<double> argx, argy, onex, oney;
+  std::vector<float> optlp;         // [nrefs][kcorr]
+  std::vector<float> sigma2;        // [nptcls][nk] (local order)
+  FftPlan plan{};
+  // device state
+  float2 *d_refs_e = nullptr, *d_refs_o = nullptr, *d_ptcls = nullptr, *d_memo = nullptr;
+  float2 *d_tw2 = nullptr, *d_dptcls = nullptr;
+  float *d_ctf = nullptr, *d_sqsums = nullptr, *d_sqsums_k = nullptr;
+  int8_t* d_iseven = nullptr;
+  double *d_argx = nullptr, *d_argy = nullptr, *d_onex = nullptr, *d_oney = nullptr;
+  float *d_spafreq = nullptr, *d_angmat = nullptr;
+  float *d_refinv_f = nullptr, *d_ofac_f = nullptr, *d_shellw = nullptr;
+  double *d_refinv_d = nullptr, *d_ofac_d = nullptr, *d_sigma2w = nullptr, *d_refpw = nullptr;
+  int* d_optrow = nullptr;
+  bool memo_valid = false, dptcls_valid = false, refnorm_valid = false, have_sigma2 = false;
+  bool ctf_ready = false;
+  // scratch arena (stream ordered; calls on a handle are serialised by mtx)
+  char* d_scratch = nullptr;
+  size_t scratch_cap = 0, scratch_off = 0;
+  std::mutex mtx;
+  std::string err;
+};
+
+namespace {
+
+int fail(pftcc* h, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail((h), PFTCC_E_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                  __FILE__, __LINE__);                                                         \
+  } while (0)
+
+template <typename T>
+int dalloc(pftcc* h, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return PFTCC_OK;
+  HIPCHK(h, hipMalloc((void**)p, count * sizeof(T)));
+  return PFTCC_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// Scratch arena: reserve() before a call, take() pieces, all stream ordered.
+int scratch_reserve(pftcc* h, size_t bytes) {
+  h->scratch_off = 0;
+  if (bytes <= h->scratch_cap) return PFTCC_OK;
+  if (h->d_scratch) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    (void)hipFree(h->d_scratch);
+    h->d_scratch = nullptr;
+  }
+  size_t cap = std::max(bytes, (size_t)1 << 20);
+  HIPCHK(h, hipMalloc((void**)&h->d_scratch, cap));
+  h->scratch_cap = cap;
+  return PFTCC_OK;
+}
+
+template <typename T>
+T* take(pftcc* h, size_t count) {
+  size_t off = (h->scratch_off + 255) & ~(size_t)255;
+  h->scratch_off = off + count * sizeof(T);
+  return reinterpret_cast<T*>(h->d_scratch + off);
+}
+
+size_t aligned(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+// Dynamic LDS per work-group kept within 64 KiB so several groups share a CU.
+constexpr size_t kMaxLds = 64 * 1024;
+
+int local_index(pftcc* h, int iptcl, int* ip) {
+  if (iptcl < h->pfrom || iptcl > h->pto)
+    return fail(h, PFTCC_E_ARG, "iptcl %d outside [%d, %d]", iptcl, h->pfrom, h->pto);
+  const int l = h->pinds[iptcl - h->pfrom];
+  if (l <= 0) return fail(h, PFTCC_E_ARG, "particle %d is not held (pinds = 0)", iptcl);
+  *ip = l - 1;
+  return PFTCC_OK;
+}
+
+int check_ref(pftcc* h, int iref) {
+  if (iref < 1 || iref > h->nrefs) return fail(h, PFTCC_E_ARG, "iref %d outside [1, %d]", iref, h->nrefs);
+  return PFTCC_OK;
+}
+
+int check_rot(pftcc* h, int irot) {
+  if (irot < 1 || irot > h->nrots) return fail(h, PFTCC_E_ARG, "irot %d outside [1, %d]", irot, h->nrots);
+  return PFTCC_OK;
+}
+
+// Polar grid, argtransf and CTF tables exactly as new() / create_polar_absctfmats build them.
+void build_host_grid(pftcc* h) {
+  const int n = h->n, R = h->nrots, nk = h->nk;
+  const float twopi32 = 2.0f * kPi32;
+  const float ang = twopi32 / (float)R;
+  h->angtab_rad.resize(R);
+  h->angtab_deg.resize(R);
+  h->polar_x.resize((size_t)nk * R);
+  h->polar_y.resize((size_t)nk * R);
+  std::vector<float> sn(R), cn(R);
+  for (int i = 0; i < R; ++i) {
+    const float a = (float)i * ang;                 // angtab(irot) = real(irot-1)*ang   (:265)
+    h->angtab_rad[i] = a;
+    sn[i] = std::sin(a);
+    cn[i] = std::cos(a);
+    for (int kk = 0; kk < nk; ++kk) {
+      const float k = (float)(h->kfrom + kk);
+      h->polar_x[(size_t)kk * R + i] = sn[i] * k;   // (:268)
+      h->polar_y[(size_t)kk * R + i] = -cn[i] * k;  // (:269)
+    }
+    h->angtab_deg[i] = (a / kPi32) * 180.0f;        // rad2deg (:274)
+  }
+  const double A = kDPi / (double)(h->ldim / 2);    // (:311)
+  h->argx.resize((size_t)nk * n);
+  h->argy.resize((size_t)nk * n);
+  h->onex.resize(n);
+  h->oney.resize(n);
+  for (int j = 0; j < n; ++j) {
+    h->onex[j] = (double)sn[j] * A;                 // argtransf_shellone (:313-314)
+    h->oney[j] = (double)(-cn[j]) * A;
+    for (int kk = 0; kk < nk; ++kk) {
+      h->argx[(size_t)kk * n + j] = (double)h->polar_x[(size_t)kk * R + j] * A;   // (:316)
+      h->argy[(size_t)kk * n + j] = (double)h->polar_y[(size_t)kk * R + j] * A;   // (:317)
+    }
+  }
+  // spaFreqSq_mat / ang_mat (:927-936)
+  const float inv = 1.0f / (float)h->ldim;
+  h->spafreq.resize((size_t)nk * n);
+  h->angmat.resize((size_t)nk * n);
+  for (int kk = 0; kk < nk; ++kk) {
+    for (int j = 0; j < n; ++j) {
+      const float px = h->polar_x[(size_t)kk * R + j], py = h->polar_y[(size_t)kk * R + j];
+      volatile float hinv = px * inv;
+      volatile float kinv = py * inv;
+      volatile float a2 = hinv * hinv;
+      volatile float b2 = kinv * kinv;
+      h->spafreq[(size_t)kk * n + j] = a2 + b2;
+      h->angmat[(size_t)kk * n + j] = std::atan2(py, px);
+    }
+  }
+  // setup_pxls_p_shell (:879-893)
+  h->pxls.assign(nk, 0.0f);
+  for (int a = 1; a <= h->kto; ++a)
+    for (int b = 1; b <= h->kto; ++b) {
+      const int sh = fnint(std::sqrt((float)(a * a + b * b)));
+      if (sh >= h->kfrom && sh <= h->kto) h->pxls[sh - h->kfrom] += 1.0f;
+    }
+}
+
+int alloc_ptcl_arrays(pftcc* h) {
+  const size_t ring = (size_t)h->nk * h->n;
+  int rc;
+  if ((rc = dalloc(h, &h->d_ptcls, (size_t)h->nptcls * ring))) return rc;
+  if ((rc = dalloc(h, &h->d_memo, (size_t)h->nptcls * h->nk * (h->n + 1)))) return rc;
+  if ((rc = dalloc(h, &h->d_sqsums, (size_t)h->nptcls))) return rc;
+  if ((rc = dalloc(h, &h->d_sqsums_k, (size_t)h->nptcls * h->nk))) return rc;
+  if ((rc = dalloc(h, &h->d_iseven, (size_t)h->nptcls))) return rc;
+  HIPCHK(h, hipMemset(h->d_ptcls, 0, (size_t)h->nptcls * ring * sizeof(float2)));
+  HIPCHK(h, hipMemset(h->d_sqsums, 0, (size_t)h->nptcls * sizeof(float)));
+  HIPCHK(h, hipMemset(h->d_sqsums_k, 0, (size_t)h->nptcls * h->nk * sizeof(float)));
+  h->iseven.assign(h->nptcls, 1);
+  HIPCHK(h, hipMemcpy(h->d_iseven, h->iseven.data(), h->nptcls, hipMemcpyHostToDevice));
+  return PFTCC_OK;
+}
+
+void free_ptcl_arrays(pftcc* h) {
+  dfree(h->d_ptcls);
+  dfree(h->d_memo);
+  dfree(h->d_sqsums);
+  dfree(h->d_sqsums_k);
+  dfree(h->d_iseven);
+  dfree(h->d_ctf);
+  dfree(h->d_dptcls);
+  dfree(h->d_shellw);
+  dfree(h->d_sigma2w);
+  dfree(h->d_optrow);
+  h->memo_valid = h->dptcls_valid = h->ctf_ready = h->have_sigma2 = false;
+  h->refnorm_valid = false;
+}
+
+int upload_iseven(pftcc* h) {
+  HIPCHK(h, hipMemcpyAsync(h->d_iseven, h->iseven.data(), h->nptcls, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+int update_sqsums(pftcc* h, int first, int count) {
+  if (count <= 0) return PFTCC_OK;
+  hipLaunchKernelGGL(k_sqsums, dim3(count), dim3(256), 0, h->stream, (const float2*)h->d_ptcls, h->nk,
+                     h->n, h->kcorr, h->d_sqsums, h->d_sqsums_k, first, count);
+  HIPCHK(h, hipGetLastError());
+  return PFTCC_OK;
+}
+
+// Shell normalisation factors (match_filt / pssnr): refinv = 1/sqrt(pw) (1 if pw <= 1e-12) and
+// ofac = optlp or sqrt(optlp*k), in float and double (shellnorm_and_filter_ref(_8), :756-805).
+int ensure_refnorm(pftcc* h) {
+  if (!(h->match_filt || h->pssnr) || h->refnorm_valid) return PFTCC_OK;
+  const int kc = h->kcorr, nr = h->nrefs;
+  const size_t cnt = (size_t)2 * nr * kc;
+  if (!h->d_refpw) {
+    int rc;
+    if ((rc = dalloc(h, &h->d_refpw, cnt))) return rc;
+    if ((rc = dalloc(h, &h->d_refinv_d, cnt))) return rc;
+    if ((rc = dalloc(h, &h->d_refinv_f, cnt))) return rc;
+  }
+  hipLaunchKernelGGL(k_ref_power, dim3((unsigned)cnt), dim3(256), 0, h->stream, (const float2*)h->d_refs_e,
+                     (const float2*)h->d_refs_o, nr, h->nk, h->n, kc, h->d_refpw);
+  HIPCHK(h, hipGetLastError());
+  std::vector<double> pw(cnt), inv(cnt);
+  std::vector<float> invf(cnt);
+  HIPCHK(h, hipMemcpyAsync(pw.data(), h->d_refpw, cnt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < cnt; ++i) {
+    inv[i] = pw[i] > 1e-12 ? 1.0 / std::sqrt(pw[i]) : 1.0;
+    const float pwf = (float)pw[i];
+    invf[i] = pwf > 1e-12f ? 1.0f / std::sqrt(pwf) : 1.0f;
+  }
+  // ofac rows: references (or particles when clsfrcs, rows indexed by the global particle id)
+  const int rows = nr;
+  std::vector<double> od((size_t)rows * kc);
+  std::vector<float> of((size_t)rows * kc);
+  for (int r = 0; r < rows; ++r)
+    for (int kk = 0; kk < kc; ++kk) {
+      const float o = h->optlp.empty() ? 1.0f : h->optlp[(size_t)r * kc + kk];
+      const int k = h->kfrom + kk;
+      if (h->pssnr) {
+        od[(size_t)r * kc + kk] = std::sqrt((double)o * (double)k);
+        of[(size_t)r * kc + kk] = std::sqrt(o * (float)k);
+      } else {
+        od[(size_t)r * kc + kk] = (double)o;
+        of[(size_t)r * kc + kk] = o;
+      }
+    }
+  if (!h->d_ofac_d) {
+    int rc;
+    if ((rc = dalloc(h, &h->d_ofac_d, (size_t)rows * kc))) return rc;
+    if ((rc = dalloc(h, &h->d_ofac_f, (size_t)rows * kc))) return rc;
+  }
+  HIPCHK(h, hipMemcpy(h->d_refinv_d, inv.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_refinv_f, invf.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_ofac_d, od.data(), od.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_ofac_f, of.data(), of.size() * sizeof(float), hipMemcpyHostToDevice));
+  if (h->clsfrcs) {
+    // ref_optlp(k, iptcl) with the global particle id (:975-979)
+    std::vector<int> row(h->nptcls, 0);
+    for (int g = h->pfrom; g <= h->pto; ++g) {
+      const int l = h->pinds[g - h->pfrom];
+      if (l > 0) {
+        if (g < 1 || g > nr) return fail(h, PFTCC_E_ARG, "clsfrcs needs particle id %d <= nrefs", g);
+        row[l - 1] = g - 1;
+      }
+    }
+    if (!h->d_optrow) {
+      int rc;
+      if ((rc = dalloc(h, &h->d_optrow, (size_t)h->nptcls))) return rc;
+    }
+    HIPCHK(h, hipMemcpy(h->d_optrow, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  h->refnorm_valid = true;
+  return PFTCC_OK;
+}
+
+size_t corr_lds_bytes(int n, int F) {
+  return sizeof(float2) * ((size_t)2 * F * n + (size_t)F * (n + 1) + (n + 1) + 2 * n) +
+         sizeof(float) * ((size_t)n + (size_t)F * n);
+}
+
+int pick_F(int n, int npairs) {
+  int F = 8;
+  while (F > 1 && corr_lds_bytes(n, F) > kMaxLds) F /= 2;
+  while (F > 1 && F / 2 >= npairs) F /= 2;
+  return F;
+}
+
+// Launches the plain FP32 correlation for np particles x nr refs x ns shifts.
+int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist, int ns,
+                const float2* d_shtab, bool shifted, float* d_cc, unsigned long long* d_best) {
+  CorrArgs a{};
+  a.n = h->n;
+  a.nk = h->nk;
+  a.kcorr = h->kcorr;
+  a.mode = h->euclid ? 1 : 0;
+  // gencorrs_cc_2 normalises over all shells (:1531); gencorrs_cc_1 over kfrom..kstop (:984)
+  a.knorm = (shifted && !h->euclid) ? h->nk : h->kcorr;
+  a.nrefs = h->nrefs;
+  a.plan = h->plan;
+  a.refs_even = h->d_refs_e;
+  a.refs_odd = h->d_refs_o;
+  a.memo = h->d_memo;
+  a.ctf = h->with_ctf ? h->d_ctf : nullptr;
+  a.sqsums = h->d_sqsums;
+  a.sqsums_k = h->d_sqsums_k;
+  a.iseven = h->d_iseven;
+  a.shtab = shifted ? d_shtab : nullptr;
+  a.ns = shifted ? ns : 1;
+  // gencorrs_euclid_2 does not shell-normalise (:1573-1577)
+  const bool norm = (h->match_filt || h->pssnr) && !(h->euclid && shifted);
+  a.refinv = norm ? h->d_refinv_f : nullptr;
+  a.ofac = norm ? h->d_ofac_f : nullptr;
+  a.optrow = (norm && h->clsfrcs) ? h->d_optrow : nullptr;
+  a.shellw = h->euclid ? h->d_shellw : nullptr;
+  a.plist = d_plist;
+  a.np = np;
+  a.rlist = d_rlist;
+  a.nr = nr;
+  a.cc_out = d_cc;
+  a.best = d_best;
+  a.tw2g = h->d_tw2;
+  const int npairs = nr * a.ns;
+  a.F = pick_F(h->n, npairs);
+  a.ntiles = (npairs + a.F - 1) / a.F;
+  const size_t lds = corr_lds_bytes(h->n, a.F);
+  const size_t blocks = (size_t)np * a.ntiles;
+  if (blocks == 0) return PFTCC_OK;
+  if (blocks > 0x7fffffffu) return fail(h, PFTCC_E_ARG, "search too large for one launch (%zu blocks)", blocks);
+  hipLaunchKernelGGL(k_corr_generic, dim3((unsigned)blocks), dim3(256), lds, h->stream, a);
+  HIPCHK(h, hipGetLastError());
+  return PFTCC_OK;
+}
+
+int shift_table(pftcc* h, int ns, const double* d_shifts, float2* d_shtab) {
+  const int tot = ns * h->n;
+  hipLaunchKernelGGL(k_shift_table, dim3((tot + 255) / 256), dim3(256), 0, h->stream, d_shifts, ns,
+                     (const double*)h->d_argx, (const double*)h->d_argy, (const double*)h->d_onex,
+                     (const double*)h->d_oney, h->nk, h->n, d_shtab);
+  HIPCHK(h, hipGetLastError());
+  return PFTCC_OK;
+}
+
+int need_memo(pftcc* h) {
+  if (!h->memo_valid)
+    return fail(h, PFTCC_E_STATE, "particle FFTs are not memoized: call memoize_ffts after setting particles");
+  if (h->with_ctf && !h->ctf_ready)
+    return fail(h, PFTCC_E_STATE, "CTF matrices missing: call create_polar_absctfmats");
+  if (h->euclid && !h->have_sigma2)
+    return fail(h, PFTCC_E_STATE, "objfun euclid needs assign_sigma2_noise");
+  return ensure_refnorm(h);
+}
+
+int need_direct(pftcc* h) {
+  if (h->with_ctf && !h->ctf_ready)
+    return fail(h, PFTCC_E_STATE, "CTF matrices missing: call create_polar_absctfmats");
+  if (h->euclid && !h->have_sigma2)
+    return fail(h, PFTCC_E_STATE, "objfun euclid needs assign_sigma2_noise");
+  return ensure_refnorm(h);
+}
+
+}  // namespace
+
+// Angular derivative rings dP/dpsi of every particle (spectral derivative of the full ring).
+__global__ void k_deriv(const float2* __restrict__ memo, int nitems, int nk, int n, FftPlan plan,
+                        const float2* __restrict__ tw2g, float2* __restrict__ dptcls, int F) {
+  extern __shared__ float2 smem[];
+  // 2 transforms per item: even (Ge) and odd (Go) frequency parts
+  const int T = 2 * F;
+  float2* bufA = smem;
+  float2* bufB = bufA + (size_t)T * n;
+  float2* tw2 = bufB + (size_t)T * n;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < 2 * n; i += nt) tw2[i] = tw2g[i];
+  const int item0 = blockIdx.x * F;
+  const int half = n / 2;
+  for (int idx = tid; idx < T * n; idx += nt) {
+    const int tr = idx / n, u = idx - tr * n;
+    const int f = tr >> 1, odd = tr & 1;
+    const int item = item0 + f;
+    float2 g = make_float2(0.f, 0.f);
+    if (item < nitems) {
+      const float2* W = memo + (size_t)item * (n + 1);
+      const int mp = u < half ? u : u - n;          // m' in [-n/2, n/2)
+      const int m = 2 * mp + odd;                   // frequency on the 2n ring
+      if (!(odd == 0 && mp == -half)) {             // m = -n: Nyquist, zero derivative
+        const int am = m < 0 ? -m : m;
+        float2 Fp = W[am];
+        if (m < 0) Fp = cconj(Fp);
+        if (odd) Fp = make_float2(-Fp.y, Fp.x);      // odd bins carry a factor i
+        // G = i m F_P
+        g = make_float2(-(float)m * Fp.y, (float)m * Fp.x);
+      }
+    }
+    bufA[(size_t)tr * n + u] = cconj(g);             // inverse via conj(FFT(conj))
+  }
+  __syncthreads();
+  float2* res = fft_forward_oop(bufA, bufB, T, n, n, plan, tw2, tid, nt);
+  const float inv2n = 1.0f / (2.0f * (float)n);
+  for (int idx = tid; idx < F * n; idx += nt) {
+    const int f = idx / n, t = idx - f * n;
+    const int item = item0 + f;
+    if (item >= nitems) continue;
+    const float2 e = cconj(res[(size_t)(2 * f) * n + t]);
+    const float2 o = cconj(res[(size_t)(2 * f + 1) * n + t]);
+    const float2 ph = cconj(tw2[t]);                 // exp(+i pi t / n)
+    const float2 po = cmul(ph, o);
+    dptcls[(size_t)item * n + t] = make_float2((e.x + po.x) * inv2n, (e.y + po.y) * inv2n);
+  }
+}
+
+namespace {
+
+int ensure_dptcls(pftcc* h) {
+  if (h->dptcls_valid) return PFTCC_OK;
+  int rc;
+  if ((rc = need_memo(h))) return rc;
+  if (!h->d_dptcls && (rc = dalloc(h, &h->d_dptcls, (size_t)h->nptcls * h->nk * h->n))) return rc;
+  int F = 4;
+  while (F > 1 && sizeof(float2) * ((size_t)2 * (2 * F) * h->n + 2 * h->n) > kMaxLds) F /= 2;
+  const int nitems = h->nptcls * h->nk;
+  const size_t lds = sizeof(float2) * ((size_t)2 * (2 * F) * h->n + 2 * h->n);
+  hipLaunchKernelGGL(k_deriv, dim3((nitems + F - 1) / F), dim3(256), lds, h->stream,
+                     (const float2*)h->d_memo, nitems, h->nk, h->n, h->plan, (const float2*)h->d_tw2,
+                     h->d_dptcls, F);
+  HIPCHK(h, hipGetLastError());
+  h->dptcls_valid = true;
+  return PFTCC_OK;
+}
+
+// Launches k_rot8 on device lists.
+int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, const double* d_shifts,
+                const int* d_rhos, double* d_f, double* d_grad, bool angle) {
+  Rot8Args a{};
+  a.n = h->n;
+  a.nk = h->nk;
+  a.kcorr = h->kcorr;
+  a.kfrom = h->kfrom;
+  a.nrefs = h->nrefs;
+  a.mode = h->euclid ? 2 : (h->match_filt ? 1 : 0);
+  a.refs_even = h->d_refs_e;
+  a.refs_odd = h->d_refs_o;
+  a.ptcls = h->d_ptcls;
+  a.dptcls = angle ? h->d_dptcls : nullptr;
+  a.ctf = h->with_ctf ? h->d_ctf : nullptr;
+  a.sqsums = h->d_sqsums;
+  a.iseven = h->d_iseven;
+  const bool norm = (h->match_filt || h->pssnr) && !h->euclid;
+  a.refinv = norm ? h->d_refinv_d : nullptr;
+  a.ofac = norm ? h->d_ofac_d : nullptr;
+  a.optrow = (norm && h->clsfrcs) ? h->d_optrow : nullptr;
+  a.argx = h->d_argx;
+  a.argy = h->d_argy;
+  a.one_x = h->d_onex;
+  a.one_y = h->d_oney;
+  a.sigma2w = h->d_sigma2w;
+  a.irefs = d_irefs;
+  a.ips = d_ips;
+  a.shifts = d_shifts;
+  a.rhos = d_rhos;
+  a.npairs = npairs;
+  a.f = d_f;
+  a.grad = d_grad;
+  if (npairs <= 0) return PFTCC_OK;
+  hipLaunchKernelGGL(k_rot8, dim3(npairs), dim3(256), 0, h->stream, a);
+  HIPCHK(h, hipGetLastError());
+  return PFTCC_OK;
+}
+
+// One (iref, iptcl, shvec, irot) evaluation through k_rot8; returns f and grad[3].
+int rot8_single(pftcc* h, int iref, int iptcl, const double shvec[2], int irot, double* f, double* grad,
+                bool angle) {
+  int rc, ip;
+  if ((rc = check_ref(h, iref)) || (rc = local_index(h, iptcl, &ip)) || (rc = check_rot(h, irot))) return rc;
+  if ((rc = need_direct(h))) return rc;
+  if (angle && (rc = ensure_dptcls(h))) return rc;
+  struct {
+    int iref, ip, rho, pad;
+    double sh[2];
+  } in{iref - 1, ip, irot - 1, 0, {shvec ? shvec[0] : 0.0, shvec ? shvec[1] : 0.0}};
+  if ((rc = scratch_reserve(h, 4096))) return rc;
+  char* d_in = take<char>(h, sizeof in);
+  double* d_out = take<double>(h, 4);
+  HIPCHK(h, hipMemcpyAsync(d_in, &in, sizeof in, hipMemcpyHostToDevice, h->stream));
+  const int* di = reinterpret_cast<const int*>(d_in);
+  if ((rc = launch_rot8(h, 1, di, di + 1, reinterpret_cast<const double*>(d_in + 16), di + 2, d_out, d_out + 1,
+                        angle)))
+    return rc;
+  double out[4];
+  HIPCHK(h, hipMemcpyAsync(out, d_out, sizeof out, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (f) *f = out[0];
+  if (grad) {
+    grad[0] = out[1];
+    grad[1] = out[2];
+    grad[2] = out[3];
+  }
+  return PFTCC_OK;
+}
+
+}  // namespace
+
+// =============================================================================== C ABI ===
+extern "C" {
+
+int32_t pftcc_magic_pftsz(int32_t ring2) { return magic_pftsz_host(ring2); }
+
+const char* pftcc_last_error(const pftcc_t* h) { return h ? h->err.c_str() : "null handle"; }
+
+int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, const int8_t* ptcl_mask,
+              const int32_t* eoarr, pftcc_t** out) {
+  if (!cfg || !out) return PFTCC_E_ARG;
+  *out = nullptr;
+  pftcc* h = new pftcc();
+  h->cfg = *cfg;
+  auto bad = [&](int code) {
+    *out = h;    // hand the handle back so the caller can read the message, then kill it
+    return code;
+  };
+  if (pto - pfrom + 1 < 1) return bad(fail(h, PFTCC_E_ARG, "nptcls (# of particles) must be > 0; new"));
+  if (nrefs < 1) return bad(fail(h, PFTCC_E_ARG, "nrefs (# of reference sections) must be > 0; new"));
+  if (cfg->boxmatch <= 0 || cfg->boxmatch % 2 != 0)
+    return bad(fail(h, PFTCC_E_ARG, "only even logical dims supported; new (boxmatch %d)", cfg->boxmatch));
+  h->ldim = cfg->boxmatch;
+  h->n = cfg->pftsz > 0 ? cfg->pftsz : magic_pftsz_host(cfg->ring2);
+  if (h->n <= 0 || h->n % 2 != 0) return bad(fail(h, PFTCC_E_ARG, "pftsz %d must be even and > 0", h->n));
+  h->plan = make_plan(h->n);
+  if (h->plan.np < 0)
+    return bad(fail(h, PFTCC_E_UNSUPPORTED, "pftsz %d has a prime factor > 13", h->n));
+  h->nrots = 2 * h->n;
+  h->kfrom = cfg->kfromto[0];
+  h->kto = cfg->kfromto[1];
+  h->kstop = cfg->kstop > 0 ? cfg->kstop : h->kto;
+  if (h->kfrom < 1 || h->kto < h->kfrom || h->kstop < h->kfrom || h->kstop > h->kto)
+    return bad(fail(h, PFTCC_E_ARG, "bad shell range kfromto=[%d,%d] kstop=%d", h->kfrom, h->kto, h->kstop));
+  h->nk = h->kto - h->kfrom + 1;
+  h->kcorr = h->kstop - h->kfrom + 1;
+  h->nrefs = nrefs;
+  h->pfrom = pfrom;
+  h->pto = pto;
+  h->with_ctf = cfg->ctf != 0;
+  h->match_filt = cfg->match_filt != 0;
+  h->pssnr = cfg->pssnr != 0;
+  h->clsfrcs = cfg->clsfrcs != 0;
+  h->euclid = cfg->objfun == PFTCC_OBJFUN_EUCLID;
+  // index translation table (:276-290)
+  h->pinds.assign(pto - pfrom + 1, 0);
+  int cnt = 0;
+  for (int i = 0; i < pto - pfrom + 1; ++i) {
+    if (!ptcl_mask || ptcl_mask[i]) h->pinds[i] = ++cnt;
+  }
+  h->nptcls = cnt;
+  if (cnt < 1) return bad(fail(h, PFTCC_E_ARG, "ptcl_mask selects no particle; new"));
+  if (hipGetDevice(&h->device) != hipSuccess) return bad(fail(h, PFTCC_E_DEVICE, "no HIP device"));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bad(fail(h, PFTCC_E_DEVICE, "no HIP device"));
+  build_host_grid(h);
+  int rc;
+  const size_t ring = (size_t)h->nk * h->n;
+  if ((rc = dalloc(h, &h->d_refs_e, (size_t)nrefs * ring)) || (rc = dalloc(h, &h->d_refs_o, (size_t)nrefs * ring)))
+    return bad(rc);
+  if (hipMemset(h->d_refs_e, 0, (size_t)nrefs * ring * sizeof(float2)) != hipSuccess ||
+      hipMemset(h->d_refs_o, 0, (size_t)nrefs * ring * sizeof(float2)) != hipSuccess)
+    return bad(fail(h, PFTCC_E_DEVICE, "hipMemset failed"));
+  if ((rc = alloc_ptcl_arrays(h))) return bad(rc);
+  // eo assignment (:291-307)
+  if (eoarr) {
+    bool allm1 = true;
+    for (int i = 0; i < pto - pfrom + 1; ++i) allm1 = allm1 && eoarr[i] == -1;
+    if (!allm1)
+      for (int i = 0; i < pto - pfrom + 1; ++i)
+        if (h->pinds[i] > 0) h->iseven[h->pinds[i] - 1] = eoarr[i] == 0 ? 1 : 0;
+    if (hipMemcpy(h->d_iseven, h->iseven.data(), h->nptcls, hipMemcpyHostToDevice) != hipSuccess)
+      return bad(fail(h, PFTCC_E_DEVICE, "copy eo"));
+  }
+  // constant tables
+  std::vector<float2> tw(2 * h->n);
+  for (int a = 0; a < 2 * h->n; ++a) {
+    const double ang = -kDPi * a / h->n;
+    tw[a] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  if ((rc = dalloc(h, &h->d_tw2, tw.size())) || (rc = dalloc(h, &h->d_argx, h->argx.size())) ||
+      (rc = dalloc(h, &h->d_argy, h->argy.size())) || (rc = dalloc(h, &h->d_onex, h->onex.size())) ||
+      (rc = dalloc(h, &h->d_oney, h->oney.size())) || (rc = dalloc(h, &h->d_spafreq, h->spafreq.size())) ||
+      (rc = dalloc(h, &h->d_angmat, h->angmat.size())))
+    return bad(rc);
+  if (hipMemcpy(h->d_tw2, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_argx, h->argx.data(), h->argx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_argy, h->argy.data(), h->argy.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_onex, h->onex.data(), h->onex.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_oney, h->oney.data(), h->oney.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_spafreq, h->spafreq.data(), h->spafreq.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_angmat, h->angmat.data(), h->angmat.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return bad(fail(h, PFTCC_E_DEVICE, "upload of constant tables failed"));
+  if (h->match_filt || h->pssnr) h->optlp.assign((size_t)nrefs * h->kcorr, 1.0f);
+  *out = h;
+  return PFTCC_OK;
+}
+
+int pftcc_kill(pftcc_t* h) {
+  if (!h) return PFTCC_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_ptcl_arrays(h);
+  dfree(h->d_refs_e);
+  dfree(h->d_refs_o);
+  dfree(h->d_tw2);
+  dfree(h->d_argx);
+  dfree(h->d_argy);
+  dfree(h->d_onex);
+  dfree(h->d_oney);
+  dfree(h->d_spafreq);
+  dfree(h->d_angmat);
+  dfree(h->d_refinv_f);
+  dfree(h->d_ofac_f);
+  dfree(h->d_refinv_d);
+  dfree(h->d_ofac_d);
+  dfree(h->d_refpw);
+  if (h->d_scratch) (void)hipFree(h->d_scratch);
+  delete h;
+  return PFTCC_OK;
+}
+
+int pftcc_set_stream(pftcc_t* h, void* s) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  h->stream = (hipStream_t)s;
+  return PFTCC_OK;
+}
+
+int pftcc_sync(pftcc_t* h) {
+  if (!h) return PFTCC_E_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+// ------------------------------------------------------------------------------ setters ---
+static int set_refs(pftcc_t* h, int iref_first, int count, const void* src, int iseven, hipMemcpyKind kind) {
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!src || count < 1) return fail(h, PFTCC_E_ARG, "no reference data");
+  if (iref_first < 1 || iref_first + count - 1 > h->nrefs)
+    return fail(h, PFTCC_E_ARG, "refs [%d, %d] outside [1, %d]", iref_first, iref_first + count - 1, h->nrefs);
+  const size_t ring = (size_t)h->nk * h->n;
+  float2* dst = (iseven ? h->d_refs_e : h->d_refs_o) + (size_t)(iref_first - 1) * ring;
+  HIPCHK(h, hipMemcpyAsync(dst, src, (size_t)count * ring * sizeof(float2), kind, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->refnorm_valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_set_ref_pft(pftcc_t* h, int32_t iref, const float* pft, int32_t iseven) {
+  if (!h) return PFTCC_E_ARG;
+  return set_refs(h, iref, 1, pft, iseven, hipMemcpyHostToDevice);
+}
+int pftcc_set_ref_pfts(pftcc_t* h, int32_t first, int32_t count, const float* pfts, int32_t iseven) {
+  if (!h) return PFTCC_E_ARG;
+  return set_refs(h, first, count, pfts, iseven, hipMemcpyHostToDevice);
+}
+int pftcc_set_ref_pfts_dev(pftcc_t* h, int32_t first, int32_t count, const void* d, int32_t iseven) {
+  if (!h) return PFTCC_E_ARG;
+  return set_refs(h, first, count, d, iseven, hipMemcpyDeviceToDevice);
+}
+
+static int set_ptcls(pftcc_t* h, int first, int count, const void* src, hipMemcpyKind kind) {
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!src || count < 1) return fail(h, PFTCC_E_ARG, "no particle data");
+  const size_t ring = (size_t)h->nk * h->n;
+  int ip0, rc;
+  if ((rc = local_index(h, first, &ip0))) return rc;
+  // consecutive global ids usually map to consecutive local slots; copy runs of them at once
+  int i = 0;
+  while (i < count) {
+    int ip;
+    if ((rc = local_index(h, first + i, &ip))) return rc;
+    int len = 1;
+    while (i + len < count) {
+      if (first + i + len > h->pto || h->pinds[first + i + len - h->pfrom] != ip + len + 1) break;
+      ++len;
+    }
+    const char* s = static_cast<const char*>(src) + (size_t)i * ring * sizeof(float2);
+    HIPCHK(h, hipMemcpyAsync(h->d_ptcls + (size_t)ip * ring, s, (size_t)len * ring * sizeof(float2), kind,
+                             h->stream));
+    if ((rc = update_sqsums(h, ip, len))) return rc;
+    i += len;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->memo_valid = false;
+  h->dptcls_valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_set_ptcl_pft(pftcc_t* h, int32_t iptcl, const float* pft) {
+  if (!h) return PFTCC_E_ARG;
+  return set_ptcls(h, iptcl, 1, pft, hipMemcpyHostToDevice);
+}
+int pftcc_set_ptcl_pfts(pftcc_t* h, int32_t first, int32_t count, const float* pfts) {
+  if (!h) return PFTCC_E_ARG;
+  return set_ptcls(h, first, count, pfts, hipMemcpyHostToDevice);
+}
+int pftcc_set_ptcl_pfts_dev(pftcc_t* h, int32_t first, int32_t count, const void* d) {
+  if (!h) return PFTCC_E_ARG;
+  return set_ptcls(h, first, count, d, hipMemcpyDeviceToDevice);
+}
+
+int pftcc_set_ref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, float re, float im, int32_t iseven) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = check_ref(h, iref))) return rc;
+  if (irot < 1 || irot > h->n || k < h->kfrom || k > h->kto)
+    return fail(h, PFTCC_E_ARG, "component (%d, %d) outside the ring block", irot, k);
+  const float2 v = make_float2(re, im);
+  float2* dst = (iseven ? h->d_refs_e : h->d_refs_o) + ((size_t)(iref - 1) * h->nk + (k - h->kfrom)) * h->n + (irot - 1);
+  HIPCHK(h, hipMemcpy(dst, &v, sizeof v, hipMemcpyHostToDevice));
+  h->refnorm_valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_set_ptcl_fcomp(pftcc_t* h, int32_t iptcl, int32_t irot, int32_t k, float re, float im) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = local_index(h, iptcl, &ip))) return rc;
+  if (irot < 1 || irot > h->n || k < h->kfrom || k > h->kto)
+    return fail(h, PFTCC_E_ARG, "component (%d, %d) outside the ring block", irot, k);
+  const float2 v = make_float2(re, im);
+  HIPCHK(h, hipMemcpy(h->d_ptcls + ((size_t)ip * h->nk + (k - h->kfrom)) * h->n + (irot - 1), &v, sizeof v,
+                      hipMemcpyHostToDevice));
+  // set_ptcl_fcomp does not refresh sqsums_ptcls in the reference (:471-476); neither do we.
+  h->memo_valid = false;
+  h->dptcls_valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_cp_even2odd_ref(pftcc_t* h, int32_t iref) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = check_ref(h, iref))) return rc;
+  const size_t ring = (size_t)h->nk * h->n;
+  HIPCHK(h, hipMemcpy(h->d_refs_o + (size_t)(iref - 1) * ring, h->d_refs_e + (size_t)(iref - 1) * ring,
+                      ring * sizeof(float2), hipMemcpyDeviceToDevice));
+  h->refnorm_valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_cp_even_ref2ptcl(pftcc_t* h, int32_t iref, int32_t iptcl) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = check_ref(h, iref)) || (rc = local_index(h, iptcl, &ip))) return rc;
+  const size_t ring = (size_t)h->nk * h->n;
+  HIPCHK(h, hipMemcpyAsync(h->d_ptcls + (size_t)ip * ring, h->d_refs_e + (size_t)(iref - 1) * ring,
+                           ring * sizeof(float2), hipMemcpyDeviceToDevice, h->stream));
+  if ((rc = update_sqsums(h, ip, 1))) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->memo_valid = false;
+  h->dptcls_valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_swap_ptclsevenodd(pftcc_t* h) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  for (auto& e : h->iseven) e = e ? 0 : 1;
+  return upload_iseven(h);
+}
+
+int pftcc_reallocate_ptcls(pftcc_t* h, int32_t nptcls, const int32_t* pinds) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (nptcls < 1 || !pinds) return fail(h, PFTCC_E_ARG, "reallocate_ptcls needs nptcls >= 1 and pinds");
+  int lo = pinds[0], hi = pinds[0];
+  for (int i = 1; i < nptcls; ++i) {
+    lo = std::min(lo, (int)pinds[i]);
+    hi = std::max(hi, (int)pinds[i]);
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (nptcls != h->nptcls) {
+    free_ptcl_arrays(h);
+    h->nptcls = nptcls;
+    int rc;
+    if ((rc = alloc_ptcl_arrays(h))) return rc;
+  } else {
+    // (:543-546) particles, square sums and eo flags are reset in both branches
+    const size_t ring = (size_t)h->nk * h->n;
+    HIPCHK(h, hipMemset(h->d_ptcls, 0, (size_t)h->nptcls * ring * sizeof(float2)));
+    HIPCHK(h, hipMemset(h->d_sqsums, 0, (size_t)h->nptcls * sizeof(float)));
+    HIPCHK(h, hipMemset(h->d_sqsums_k, 0, (size_t)h->nptcls * h->nk * sizeof(float)));
+    h->iseven.assign(h->nptcls, 1);
+    int rc;
+    if ((rc = upload_iseven(h))) return rc;
+  }
+  h->pfrom = lo;
+  h->pto = hi;
+  h->pinds.assign(hi - lo + 1, 0);
+  for (int i = 0; i < nptcls; ++i) h->pinds[pinds[i] - lo] = i + 1;
+  h->memo_valid = h->dptcls_valid = false;
+  h->refnorm_valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_set_eo(pftcc_t* h, int32_t iptcl, int32_t is_even) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = local_index(h, iptcl, &ip))) return rc;
+  h->iseven[ip] = is_even ? 1 : 0;
+  return upload_iseven(h);
+}
+
+int pftcc_set_eos(pftcc_t* h, const int32_t* eoarr) {
+  if (!h || !eoarr) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  bool allm1 = true;
+  for (int i = 0; i < h->nptcls; ++i) allm1 = allm1 && eoarr[i] == -1;
+  for (int i = 0; i < h->nptcls; ++i) h->iseven[i] = allm1 ? 1 : (eoarr[i] == 0 ? 1 : 0);
+  return upload_iseven(h);
+}
+
+int pftcc_set_ref_optlp(pftcc_t* h, int32_t iref, const float* optlp) {
+  if (!h || !optlp) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = check_ref(h, iref))) return rc;
+  if (h->optlp.empty()) h->optlp.assign((size_t)h->nrefs * h->kcorr, 1.0f);
+  std::memcpy(&h->optlp[(size_t)(iref - 1) * h->kcorr], optlp, sizeof(float) * h->kcorr);
+  h->refnorm_valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_assign_sigma2_noise(pftcc_t* h, const float* sigma2) {
+  if (!h || !sigma2) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  const int nk = h->nk;
+  std::vector<float> shellw((size_t)h->nptcls * nk, 0.0f);
+  std::vector<double> s2w((size_t)h->nptcls * nk, 0.0);
+  for (int gidx = h->pfrom; gidx <= h->pto; ++gidx) {
+    const int l = h->pinds[gidx - h->pfrom];
+    if (l <= 0) continue;
+    for (int kk = 0; kk < nk; ++kk) {
+      const float s = sigma2[(size_t)(gidx - h->pfrom) * nk + kk];
+      // gencorrs_euclid: pxls_p_shell(k)/pftsz /(2 sigma2) (:1554-1555); for_rot_8: pxls/pftsz/sigma2
+      shellw[(size_t)(l - 1) * nk + kk] = h->pxls[kk] / (float)h->n / (2.0f * s);
+      s2w[(size_t)(l - 1) * nk + kk] = (double)h->pxls[kk] / (double)h->n / (double)s;
+    }
+  }
+  int rc;
+  if (!h->d_shellw && ((rc = dalloc(h, &h->d_shellw, shellw.size())) || (rc = dalloc(h, &h->d_sigma2w, s2w.size()))))
+    return rc;
+  HIPCHK(h, hipMemcpy(h->d_shellw, shellw.data(), shellw.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->d_sigma2w, s2w.data(), s2w.size() * sizeof(double), hipMemcpyHostToDevice));
+  h->have_sigma2 = true;
+  return PFTCC_OK;
+}
+
+int pftcc_zero_refs_beyond_kstop(pftcc_t* h) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (h->kstop == h->kto) return PFTCC_OK;
+  const size_t ring = (size_t)h->nk * h->n;
+  const size_t off = (size_t)h->kcorr * h->n;
+  const size_t len = (size_t)(h->nk - h->kcorr) * h->n;
+  for (int r = 0; r < h->nrefs; ++r) {
+    HIPCHK(h, hipMemsetAsync(h->d_refs_e + r * ring + off, 0, len * sizeof(float2), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_refs_o + r * ring + off, 0, len * sizeof(float2), h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->refnorm_valid = false;
+  return PFTCC_OK;
+}
+
+// ---------------------------------------------------------------------------- memoizers ---
+int pftcc_create_polar_absctfmats(pftcc_t* h, const pftcc_ctfparams* params) {
+  if (!h || !params) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  std::vector<KCtfConst> cst(h->nptcls);
+  for (auto& c : cst) c.active = 0;
+  for (int gidx = h->pfrom; gidx <= h->pto; ++gidx) {
+    const int l = h->pinds[gidx - h->pfrom];
+    if (l <= 0) continue;
+    const pftcc_ctfparams& p = params[gidx - h->pfrom];
+    KCtfConst c{};
+    // constructor (simple_ctf.f90:54-70) and kV2wl (:522-525), real(sp) arithmetic
+    const float kv = p.kv;
+    const float a = 1000.0f * kv;
+    const float wl = 12.26f / std::sqrt(a + (0.9784f * (a * a)) / 1.0e6f);
+    c.wl = wl / p.smpd;
+    c.cs = (p.cs * 1.0e7f) / p.smpd;
+    const float phaseq = std::sqrt(1.0f - p.fraca * p.fraca);
+    c.ampc = std::atan(p.fraca / phaseq);
+    // init (:73-81)
+    c.dfx = (p.dfx * 1.0e4f) / p.smpd;
+    c.dfy = (p.dfy * 1.0e4f) / p.smpd;
+    c.angast = (p.angast / 180.0f) * kPi32;
+    c.phshift = p.l_phaseplate ? p.phshift : 0.0f;
+    c.active = 1;
+    cst[l - 1] = c;
+  }
+  int rc;
+  const size_t ring = (size_t)h->nk * h->n;
+  if (!h->d_ctf && (rc = dalloc(h, &h->d_ctf, (size_t)h->nptcls * ring))) return rc;
+  HIPCHK(h, hipMemsetAsync(h->d_ctf, 0, (size_t)h->nptcls * ring * sizeof(float), h->stream));
+  if ((rc = scratch_reserve(h, aligned(cst.size() * sizeof(KCtfConst))))) return rc;
+  KCtfConst* d_c = take<KCtfConst>(h, cst.size());
+  HIPCHK(h, hipMemcpyAsync(d_c, cst.data(), cst.size() * sizeof(KCtfConst), hipMemcpyHostToDevice, h->stream));
+  const size_t tot = (size_t)h->nptcls * ring;
+  hipLaunchKernelGGL(k_ctfmats, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, (const KCtfConst*)d_c,
+                     (const float*)h->d_spafreq, (const float*)h->d_angmat, h->nk, h->n, h->nptcls, h->d_ctf);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->ctf_ready = true;
+  return PFTCC_OK;
+}
+
+int pftcc_memoize_ffts(pftcc_t* h) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int F = 8;
+  while (F > 1 && sizeof(float2) * ((size_t)2 * F * h->n + 2 * h->n) > kMaxLds) F /= 2;
+  const int nitems = h->nptcls * h->nk;
+  const size_t lds = sizeof(float2) * ((size_t)2 * F * h->n + 2 * h->n);
+  hipLaunchKernelGGL(k_memoize, dim3((nitems + F - 1) / F), dim3(256), lds, h->stream, (const float2*)h->d_ptcls, 0,
+                     h->nptcls, h->nk, h->n, h->plan, (const float2*)h->d_tw2, h->d_memo, F);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->memo_valid = true;
+  h->dptcls_valid = false;
+  return PFTCC_OK;
+}
+
+// ------------------------------------------------------------------------------ getters ---
+int pftcc_get_nrots(const pftcc_t* h, int32_t* v) { if (!h || !v) return PFTCC_E_ARG; *v = h->nrots; return 0; }
+int pftcc_get_pftsz(const pftcc_t* h, int32_t* v) { if (!h || !v) return PFTCC_E_ARG; *v = h->n; return 0; }
+int pftcc_get_box(const pftcc_t* h, int32_t* v) { if (!h || !v) return PFTCC_E_ARG; *v = h->ldim; return 0; }
+int pftcc_get_nrefs(const pftcc_t* h, int32_t* v) { if (!h || !v) return PFTCC_E_ARG; *v = h->nrefs; return 0; }
+int pftcc_get_nptcls(const pftcc_t* h, int32_t* v) { if (!h || !v) return PFTCC_E_ARG; *v = h->nptcls; return 0; }
+int pftcc_get_pdim(const pftcc_t* h, int32_t pdim[3]) {
+  if (!h || !pdim) return PFTCC_E_ARG;
+  pdim[0] = h->n;
+  pdim[1] = h->kfrom;
+  pdim[2] = h->kto;
+  return 0;
+}
+int pftcc_get_rot(const pftcc_t* h, int32_t roind, float* rot) {
+  if (!h || !rot) return PFTCC_E_ARG;
+  if (roind < 1 || roind > h->nrots)
+    return fail(const_cast<pftcc*>(h), PFTCC_E_ARG, "roind is out of range; get_rot (roind %d, nrots %d)", roind,
+                h->nrots);
+  *rot = h->angtab_deg[roind - 1];
+  return 0;
+}
+int pftcc_get_roind(const pftcc_t* h, float rot, int32_t* ind) {
+  if (!h || !ind) return PFTCC_E_ARG;
+  int best = 0;
+  float bd = 0.f;
+  for (int i = 0; i < h->nrots; ++i) {
+    float d = std::fabs(h->angtab_deg[i] - rot);
+    if (d > 180.f) d = 360.f - d;
+    if (i == 0 || d < bd) {
+      bd = d;
+      best = i;
+    }
+  }
+  *ind = best + 1;
+  return 0;
+}
+int pftcc_get_coord(const pftcc_t* h, int32_t irot, int32_t k, float xy[2]) {
+  if (!h || !xy) return PFTCC_E_ARG;
+  if (irot < 1 || irot > h->nrots || k < h->kfrom || k > h->kto)
+    return fail(const_cast<pftcc*>(h), PFTCC_E_ARG, "get_coord (%d, %d) out of range", irot, k);
+  xy[0] = h->polar_x[(size_t)(k - h->kfrom) * h->nrots + irot - 1];
+  xy[1] = h->polar_y[(size_t)(k - h->kfrom) * h->nrots + irot - 1];
+  return 0;
+}
+int pftcc_ptcl_iseven(const pftcc_t* h, int32_t iptcl, int32_t* v) {
+  if (!h || !v) return PFTCC_E_ARG;
+  int ip, rc;
+  if ((rc = local_index(const_cast<pftcc*>(h), iptcl, &ip))) return rc;
+  *v = h->iseven[ip];
+  return 0;
+}
+int pftcc_get_ref_pft(pftcc_t* h, int32_t iref, int32_t iseven, float* pft) {
+  if (!h || !pft) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = check_ref(h, iref))) return rc;
+  const size_t ring = (size_t)h->nk * h->n;
+  HIPCHK(h, hipMemcpy(pft, (iseven ? h->d_refs_e : h->d_refs_o) + (size_t)(iref - 1) * ring, ring * sizeof(float2),
+                      hipMemcpyDeviceToHost));
+  return 0;
+}
+int pftcc_get_ptcl_pft(pftcc_t* h, int32_t iptcl, float* pft) {
+  if (!h || !pft) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = local_index(h, iptcl, &ip))) return rc;
+  const size_t ring = (size_t)h->nk * h->n;
+  HIPCHK(h, hipMemcpy(pft, h->d_ptcls + (size_t)ip * ring, ring * sizeof(float2), hipMemcpyDeviceToHost));
+  return 0;
+}
+int pftcc_get_ctfmat(pftcc_t* h, int32_t iptcl, float* out) {
+  if (!h || !out) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = local_index(h, iptcl, &ip))) return rc;
+  if (!h->ctf_ready) return fail(h, PFTCC_E_STATE, "CTF matrices not created");
+  const size_t ring = (size_t)h->nk * h->n;
+  HIPCHK(h, hipMemcpy(out, h->d_ctf + (size_t)ip * ring, ring * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+int pftcc_get_memo(pftcc_t* h, int32_t iptcl, float* out) {
+  if (!h || !out) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = local_index(h, iptcl, &ip))) return rc;
+  if (!h->memo_valid) return fail(h, PFTCC_E_STATE, "memo not computed");
+  const size_t per = (size_t)h->nk * (h->n + 1);
+  HIPCHK(h, hipMemcpy(out, h->d_memo + (size_t)ip * per, per * sizeof(float2), hipMemcpyDeviceToHost));
+  return 0;
+}
+int pftcc_get_sqsum_ptcl(pftcc_t* h, int32_t iptcl, float* out) {
+  if (!h || !out) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = local_index(h, iptcl, &ip))) return rc;
+  HIPCHK(h, hipMemcpy(out, h->d_sqsums + ip, sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ------------------------------------------------------------------------- calculators ---
+int pftcc_gencorrs(pftcc_t* h, int32_t iref, int32_t iptcl, const float* shvec, float* cc) {
+  if (!h || !cc) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = check_ref(h, iref)) || (rc = local_index(h, iptcl, &ip))) return rc;
+  if ((rc = need_memo(h))) return rc;
+  const size_t ring = (size_t)h->nk * h->n;
+  if ((rc = scratch_reserve(h, aligned(64) + aligned(16) + aligned(ring * 8) + aligned(h->nrots * 4)))) return rc;
+  int* d_idx = take<int>(h, 2);
+  double* d_sh = take<double>(h, 2);
+  float2* d_tab = take<float2>(h, ring);
+  float* d_cc = take<float>(h, h->nrots);
+  const int idx[2] = {ip, iref - 1};
+  HIPCHK(h, hipMemcpyAsync(d_idx, idx, sizeof idx, hipMemcpyHostToDevice, h->stream));
+  if (shvec) {
+    const double sh[2] = {(double)shvec[0], (double)shvec[1]};
+    HIPCHK(h, hipMemcpyAsync(d_sh, sh, sizeof sh, hipMemcpyHostToDevice, h->stream));
+    if ((rc = shift_table(h, 1, d_sh, d_tab))) return rc;
+  }
+  if ((rc = launch_corr(h, 1, d_idx, 1, d_idx + 1, 1, d_tab, shvec != nullptr, d_cc, nullptr))) return rc;
+  HIPCHK(h, hipMemcpyAsync(cc, d_cc, h->nrots * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+int pftcc_gencorr_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2], int32_t irot, double* f) {
+  if (!h || !f) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  return rot8_single(h, iref, iptcl, shvec, irot, f, nullptr, false);
+}
+
+int pftcc_gencorr_for_rot(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2], int32_t irot, float* f) {
+  if (!h || !f) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  const double sh[2] = {shvec ? (double)shvec[0] : 0.0, shvec ? (double)shvec[1] : 0.0};
+  double v = 0.0;
+  const int rc = rot8_single(h, iref, iptcl, sh, irot, &v, nullptr, false);
+  *f = (float)v;
+  return rc;
+}
+
+int pftcc_gencorr_grad_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2], int32_t irot,
+                                 double* f, double grad[2]) {
+  if (!h || !f || !grad) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  double gg[3];
+  const int rc = rot8_single(h, iref, iptcl, shvec, irot, f, gg, false);
+  grad[0] = gg[0];
+  grad[1] = gg[1];
+  return rc;
+}
+
+int pftcc_gencorr_grad_only_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2], int32_t irot,
+                                      double grad[2]) {
+  if (!h || !grad) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  double gg[3], f;
+  const int rc = rot8_single(h, iref, iptcl, shvec, irot, &f, gg, false);
+  grad[0] = gg[0];
+  grad[1] = gg[1];
+  return rc;
+}
+
+int pftcc_gencorrs_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs, int32_t ns,
+                         const float* shifts, float* cc_out) {
+  if (!h || !iptcls || !cc_out || np < 1 || nr < 1 || ns < 0) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  std::vector<int> pl(np), rl(nr);
+  for (int i = 0; i < np; ++i)
+    if ((rc = local_index(h, iptcls[i], &pl[i]))) return rc;
+  for (int i = 0; i < nr; ++i) {
+    const int r = irefs ? irefs[i] : i + 1;
+    if ((rc = check_ref(h, r))) return rc;
+    rl[i] = r - 1;
+  }
+  if ((rc = need_memo(h))) return rc;
+  const int nse = std::max(ns, 1);
+  const size_t ring = (size_t)h->nk * h->n;
+  const size_t ncc = (size_t)np * nr * nse * h->nrots;
+  if ((rc = scratch_reserve(h, aligned(np * 4) + aligned(nr * 4) + aligned(nse * 16) + aligned(nse * ring * 8) +
+                                   aligned(ncc * 4))))
+    return rc;
+  int* d_pl = take<int>(h, np);
+  int* d_rl = take<int>(h, nr);
+  double* d_sh = take<double>(h, 2 * nse);
+  float2* d_tab = take<float2>(h, nse * ring);
+  float* d_cc = take<float>(h, ncc);
+  HIPCHK(h, hipMemcpyAsync(d_pl, pl.data(), np * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_rl, rl.data(), nr * 4, hipMemcpyHostToDevice, h->stream));
+  if (ns > 0) {
+    std::vector<double> sh(2 * ns);
+    for (int i = 0; i < 2 * ns; ++i) sh[i] = (double)shifts[i];
+    HIPCHK(h, hipMemcpyAsync(d_sh, sh.data(), sh.size() * 8, hipMemcpyHostToDevice, h->stream));
+    if ((rc = shift_table(h, ns, d_sh, d_tab))) return rc;
+  }
+  if ((rc = launch_corr(h, np, d_pl, nr, d_rl, nse, d_tab, ns > 0, d_cc, nullptr))) return rc;
+  HIPCHK(h, hipMemcpyAsync(cc_out, d_cc, ncc * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+static int search_impl(pftcc_t* h, int np, const int* d_pl, int nr, const int* d_rl, int ns, const float2* d_tab,
+                       float* d_cc, int* d_ir, int* d_is, int* d_rot) {
+  int rc;
+  unsigned long long* d_best = take<unsigned long long>(h, np);
+  hipLaunchKernelGGL(k_fill_u64, dim3((np + 255) / 256), dim3(256), 0, h->stream, d_best, np, 0ull);
+  HIPCHK(h, hipGetLastError());
+  if ((rc = launch_corr(h, np, d_pl, nr, d_rl, std::max(ns, 1), d_tab, ns > 0, nullptr, d_best))) return rc;
+  hipLaunchKernelGGL(k_best_decode, dim3((np + 255) / 256), dim3(256), 0, h->stream, (const unsigned long long*)d_best,
+                     np, nr, std::max(ns, 1), h->nrots, d_rl, d_cc, d_ir, d_is, d_rot);
+  HIPCHK(h, hipGetLastError());
+  return PFTCC_OK;
+}
+
+int pftcc_search(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs, int32_t ns,
+                 const float* shifts, float* best_cc, int32_t* best_iref, int32_t* best_ishift, int32_t* best_irot) {
+  if (!h || !iptcls || np < 1 || nr < 1 || ns < 0 || !best_cc || !best_iref || !best_ishift || !best_irot)
+    return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  std::vector<int> pl(np), rl(nr);
+  for (int i = 0; i < np; ++i)
+    if ((rc = local_index(h, iptcls[i], &pl[i]))) return rc;
+  for (int i = 0; i < nr; ++i) {
+    const int r = irefs ? irefs[i] : i + 1;
+    if ((rc = check_ref(h, r))) return rc;
+    rl[i] = r - 1;
+  }
+  if ((rc = need_memo(h))) return rc;
+  const int nse = std::max(ns, 1);
+  const size_t ring = (size_t)h->nk * h->n;
+  if ((rc = scratch_reserve(h, aligned(np * 4) + aligned(nr * 4) + aligned(nse * 16) + aligned(nse * ring * 8) +
+                                   5 * aligned((size_t)np * 8))))
+    return rc;
+  int* d_pl = take<int>(h, np);
+  int* d_rl = take<int>(h, nr);
+  double* d_sh = take<double>(h, 2 * nse);
+  float2* d_tab = take<float2>(h, nse * ring);
+  float* d_cc = take<float>(h, np);
+  int* d_ir = take<int>(h, np);
+  int* d_is = take<int>(h, np);
+  int* d_rot = take<int>(h, np);
+  HIPCHK(h, hipMemcpyAsync(d_pl, pl.data(), np * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_rl, rl.data(), nr * 4, hipMemcpyHostToDevice, h->stream));
+  if (ns > 0) {
+    std::vector<double> sh(2 * ns);
+    for (int i = 0; i < 2 * ns; ++i) sh[i] = (double)shifts[i];
+    HIPCHK(h, hipMemcpyAsync(d_sh, sh.data(), sh.size() * 8, hipMemcpyHostToDevice, h->stream));
+    if ((rc = shift_table(h, ns, d_sh, d_tab))) return rc;
+  }
+  if ((rc = search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, d_cc, d_ir, d_is, d_rot))) return rc;
+  HIPCHK(h, hipMemcpyAsync(best_cc, d_cc, np * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(best_iref, d_ir, np * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(best_ishift, d_is, np * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(best_irot, d_rot, np * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+// Device-list form: d_iptcls are LOCAL 0-based particle slots here would be ambiguous, so the
+// global ids are translated on the device side by the caller-visible contract: d_iptcls hold
+// global ids; the translation table is uploaded once per call.
+__global__ void k_translate(const int* __restrict__ gids, int np, const int* __restrict__ pinds, int pfrom,
+                                   int span, int* __restrict__ out, int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int g = gids[i] - pfrom;
+  int l = (g >= 0 && g < span) ? pinds[g] : 0;
+  if (l <= 0) {
+    *bad = 1;
+    l = 1;
+  }
+  out[i] = l - 1;
+}
+
+__global__ void k_refs0(const int* __restrict__ irefs, int nr, int nrefs, int* __restrict__ out,
+                               int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  int r = irefs ? irefs[i] : i + 1;
+  if (r < 1 || r > nrefs) {
+    *bad = 1;
+    r = 1;
+  }
+  out[i] = r - 1;
+}
+
+__global__ void k_f2d(const float* __restrict__ s, int n, double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (double)s[i];
+}
+
+int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr, const int32_t* d_irefs, int32_t ns,
+                     const float* d_shifts, float* d_best_cc, int32_t* d_best_iref, int32_t* d_best_ishift,
+                     int32_t* d_best_irot) {
+  if (!h || !d_iptcls || np < 1 || nr < 1 || ns < 0) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if (!d_irefs && nr != h->nrefs) return fail(h, PFTCC_E_ARG, "nr must equal nrefs when irefs is NULL");
+  if ((rc = need_memo(h))) return rc;
+  const int nse = std::max(ns, 1);
+  const size_t ring = (size_t)h->nk * h->n;
+  const int span = h->pto - h->pfrom + 1;
+  if ((rc = scratch_reserve(h, aligned(np * 4) + aligned(nr * 4) + aligned(nse * 16) + aligned(nse * ring * 8) +
+                                   aligned((size_t)span * 4) + aligned(64) + aligned((size_t)np * 8))))
+    return rc;
+  int* d_pl = take<int>(h, np);
+  int* d_rl = take<int>(h, nr);
+  double* d_sh = take<double>(h, 2 * nse);
+  float2* d_tab = take<float2>(h, nse * ring);
+  int* d_pinds = take<int>(h, span);
+  int* d_bad = take<int>(h, 16);
+  HIPCHK(h, hipMemcpyAsync(d_pinds, h->pinds.data(), (size_t)span * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(d_bad, 0, 4, h->stream));
+  hipLaunchKernelGGL(k_translate, dim3((np + 255) / 256), dim3(256), 0, h->stream, d_iptcls, np, (const int*)d_pinds,
+                     h->pfrom, span, d_pl, d_bad);
+  hipLaunchKernelGGL(k_refs0, dim3((nr + 255) / 256), dim3(256), 0, h->stream, d_irefs, nr, h->nrefs, d_rl, d_bad);
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_f2d, dim3((2 * ns + 255) / 256), dim3(256), 0, h->stream, d_shifts, 2 * ns, d_sh);
+    if ((rc = shift_table(h, ns, d_sh, d_tab))) return rc;
+  }
+  int bad = 0;
+  HIPCHK(h, hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (bad) return fail(h, PFTCC_E_ARG, "search_dev: particle or reference index out of range");
+  return search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, d_best_cc, d_best_iref, d_best_ishift, d_best_irot);
+}
+
+int pftcc_grad_batch(pftcc_t* h, int32_t npairs, const int32_t* irefs, const int32_t* iptcls, const double* shvecs,
+                     const int32_t* irots, double* f, double* grad) {
+  if (!h || npairs < 0 || (npairs > 0 && (!irefs || !iptcls || !shvecs || !irots || !f))) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (npairs == 0) return PFTCC_OK;
+  int rc;
+  std::vector<int> ir(npairs), ip(npairs), rho(npairs);
+  for (int i = 0; i < npairs; ++i) {
+    if ((rc = check_ref(h, irefs[i])) || (rc = local_index(h, iptcls[i], &ip[i])) || (rc = check_rot(h, irots[i])))
+      return rc;
+    ir[i] = irefs[i] - 1;
+    rho[i] = irots[i] - 1;
+  }
+  if ((rc = need_direct(h))) return rc;
+  const bool angle = grad != nullptr && !h->euclid;
+  if (angle && (rc = ensure_dptcls(h))) return rc;
+  if ((rc = scratch_reserve(h, 3 * aligned((size_t)npairs * 4) + aligned((size_t)npairs * 16) +
+                                   aligned((size_t)npairs * 8) + aligned((size_t)npairs * 24))))
+    return rc;
+  int* d_ir = take<int>(h, npairs);
+  int* d_ip = take<int>(h, npairs);
+  int* d_rho = take<int>(h, npairs);
+  double* d_sh = take<double>(h, 2 * (size_t)npairs);
+  double* d_f = take<double>(h, npairs);
+  double* d_g = take<double>(h, 3 * (size_t)npairs);
+  HIPCHK(h, hipMemcpyAsync(d_ir, ir.data(), npairs * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_ip, ip.data(), npairs * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_rho, rho.data(), npairs * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_sh, shvecs, (size_t)npairs * 16, hipMemcpyHostToDevice, h->stream));
+  if ((rc = launch_rot8(h, npairs, d_ir, d_ip, d_sh, d_rho, d_f, grad ? d_g : nullptr, angle))) return rc;
+  HIPCHK(h, hipMemcpyAsync(f, d_f, (size_t)npairs * 8, hipMemcpyDeviceToHost, h->stream));
+  if (grad) HIPCHK(h, hipMemcpyAsync(grad, d_g, (size_t)npairs * 24, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs, const int32_t* d_iptcls,
+                         const double* d_shvecs, const int32_t* d_irots, double* d_f, double* d_grad) {
+  if (!h || npairs < 0) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (npairs == 0) return PFTCC_OK;
+  int rc;
+  if ((rc = need_direct(h))) return rc;
+  const bool angle = d_grad != nullptr && !h->euclid;
+  if (angle && (rc = ensure_dptcls(h))) return rc;
+  const int span = h->pto - h->pfrom + 1;
+  if ((rc = scratch_reserve(h, 3 * aligned((size_t)npairs * 4) + aligned((size_t)span * 4) + aligned(64)))) return rc;
+  int* d_ir = take<int>(h, npairs);
+  int* d_ip = take<int>(h, npairs);
+  int* d_rho = take<int>(h, npairs);
+  int* d_pinds = take<int>(h, span);
+  int* d_bad = take<int>(h, 16);
+  HIPCHK(h, hipMemcpyAsync(d_pinds, h->pinds.data(), (size_t)span * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(d_bad, 0, 4, h->stream));
+  hipLaunchKernelGGL(k_translate, dim3((npairs + 255) / 256), dim3(256), 0, h->stream, d_iptcls, npairs,
+                     (const int*)d_pinds, h->pfrom, span, d_ip, d_bad);
+  hipLaunchKernelGGL(k_refs0, dim3((npairs + 255) / 256), dim3(256), 0, h->stream, d_irefs, npairs, h->nrefs, d_ir,
+                     d_bad);
+  hipLaunchKernelGGL(k_refs0, dim3((npairs + 255) / 256), dim3(256), 0, h->stream, d_irots, npairs, h->nrots, d_rho,
+                     d_bad);
+  int bad = 0;
+  HIPCHK(h, hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (bad) return fail(h, PFTCC_E_ARG, "grad_batch_dev: index out of range");
+  return launch_rot8(h, npairs, d_ir, d_ip, d_shvecs, d_rho, d_f, d_grad, angle);
+}
+
+// Per-shell terms at one rotation (genfrc / calc_frc / gencorr_sigma_contrib), FP64 direct sums:
+// out[kk] = {Re sum q conj(P_rot), sum |q|^2, sum |p|^2, sum |q - P_rot|^2} for every shell.
+__global__ void k_shell_terms(const float2* __restrict__ ref, const float2* __restrict__ ptc,
+                                     const float* __restrict__ ctf, const float2* __restrict__ shtab, int nk, int n,
+                                     int rho, const float* __restrict__ scale /*[nk] or null*/,
+                                     double* __restrict__ out) {
+  const int kk = blockIdx.x;
+  __shared__ double red[4][256];
+  double acc[4] = {0, 0, 0, 0};
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    float2 r = ref[(size_t)kk * n + j];
+    float c = ctf ? ctf[(size_t)kk * n + j] : 1.0f;
+    // FP32 preparation as prep_ref4corr / gencorr_sigma_contrib do
+    float2 q = make_float2(r.x * c, r.y * c);
+    if (scale) q = make_float2(q.x * scale[kk], q.y * scale[kk]);
+    if (shtab) q = cmul(q, shtab[(size_t)kk * n + j]);
+    int t = j + rho;
+    bool cj = false;
+    if (t >= 2 * n) t -= 2 * n;
+    if (t >= n) { t -= n; cj = true; }
+    const float2 pv = ptc[(size_t)kk * n + t];
+    const double pr = pv.x, pim = cj ? -(double)pv.y : (double)pv.y;
+    acc[0] += (double)q.x * pr + (double)q.y * pim;
+    acc[1] += (double)q.x * q.x + (double)q.y * q.y;
+    const float2 pj = ptc[(size_t)kk * n + j];
+    acc[2] += (double)pj.x * pj.x + (double)pj.y * pj.y;
+    const double dr = q.x - pr, di = q.y - pim;
+    acc[3] += dr * dr + di * di;
+  }
+  for (int v = 0; v < 4; ++v) red[v][threadIdx.x] = acc[v];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int v = 0; v < 4; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int v = 0; v < 4; ++v) out[4 * kk + v] = red[v][0];
+}
+
+static int shell_terms(pftcc_t* h, int iref, int iptcl, const float* shvec, int irot, bool shellnorm,
+                       std::vector<double>& terms) {
+  int rc, ip;
+  if ((rc = check_ref(h, iref)) || (rc = local_index(h, iptcl, &ip)) || (rc = check_rot(h, irot))) return rc;
+  if ((rc = need_direct(h))) return rc;
+  const size_t ring = (size_t)h->nk * h->n;
+  if ((rc = scratch_reserve(h, aligned(16) + aligned(ring * 8) + aligned(h->nk * 4) + aligned(h->nk * 32)))) return rc;
+  double* d_sh = take<double>(h, 2);
+  float2* d_tab = take<float2>(h, ring);
+  float* d_scale = take<float>(h, h->nk);
+  double* d_out = take<double>(h, 4 * (size_t)h->nk);
+  if (shvec) {
+    const double sh[2] = {(double)shvec[0], (double)shvec[1]};
+    HIPCHK(h, hipMemcpyAsync(d_sh, sh, sizeof sh, hipMemcpyHostToDevice, h->stream));
+    if ((rc = shift_table(h, 1, d_sh, d_tab))) return rc;
+  }
+  const bool even = h->iseven[ip] != 0;
+  bool use_scale = shellnorm && (h->match_filt || h->pssnr);
+  if (use_scale) {
+    // per-shell scale of this (eo, ref): ofac[row] * refinv, computed on the host from the tables
+    std::vector<float> inv((size_t)2 * h->nrefs * h->kcorr), of;
+    HIPCHK(h, hipMemcpy(inv.data(), h->d_refinv_f, inv.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<float> sc(h->nk, 1.0f);
+    const int row = h->clsfrcs ? (iptcl - 1) : (iref - 1);
+    for (int kk = 0; kk < h->kcorr; ++kk) {
+      const float o = h->optlp.empty() ? 1.0f : h->optlp[(size_t)row * h->kcorr + kk];
+      const float ofv = h->pssnr ? std::sqrt(o * (float)(h->kfrom + kk)) : o;
+      sc[kk] = ofv * inv[((size_t)(even ? 0 : 1) * h->nrefs + (iref - 1)) * h->kcorr + kk];
+    }
+    HIPCHK(h, hipMemcpyAsync(d_scale, sc.data(), h->nk * 4, hipMemcpyHostToDevice, h->stream));
+  }
+  const float2* ref = (even ? h->d_refs_e : h->d_refs_o) + (size_t)(iref - 1) * ring;
+  hipLaunchKernelGGL(k_shell_terms, dim3(h->nk), dim3(256), 0, h->stream, ref,
+                     (const float2*)(h->d_ptcls + (size_t)ip * ring), h->with_ctf ? (const float*)(h->d_ctf + ip * ring) : nullptr,
+                     shvec ? (const float2*)d_tab : nullptr, h->nk, h->n, irot - 1,
+                     use_scale ? (const float*)d_scale : nullptr, d_out);
+  HIPCHK(h, hipGetLastError());
+  terms.resize(4 * (size_t)h->nk);
+  HIPCHK(h, hipMemcpyAsync(terms.data(), d_out, terms.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+int pftcc_gencorr_sigma_contrib(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2], int32_t irot,
+                                float* sigma_contrib, float* ptcl_sumsq, float* ref_sumsq) {
+  if (!h || !sigma_contrib) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  std::vector<double> t;
+  // gencorr_sigma_contrib applies CTF and shift, no shell normalisation (:2166-2175)
+  int rc = shell_terms(h, iref, iptcl, shvec, irot, false, t);
+  if (rc) return rc;
+  for (int kk = 0; kk < h->nk; ++kk) {
+    sigma_contrib[kk] = (float)(0.5 * t[4 * kk + 3] / (double)h->n);
+    if (ptcl_sumsq) ptcl_sumsq[kk] = (float)t[4 * kk + 2];
+    if (ref_sumsq) ref_sumsq[kk] = (float)t[4 * kk + 1];
+  }
+  return PFTCC_OK;
+}
+
+int pftcc_specscore(pftcc_t* h, int32_t iref, int32_t iptcl, int32_t irot, const float* shvec, float* score) {
+  if (!h || !score) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  std::vector<double> t;
+  int rc = shell_terms(h, iref, iptcl, shvec, irot, true, t);
+  if (rc) return rc;
+  std::vector<float> frc(h->kcorr);
+  for (int kk = 0; kk < h->kcorr; ++kk) {
+    const float den = (float)std::sqrt(t[4 * kk + 2] * t[4 * kk + 1]);
+    frc[kk] = den < 1e-12f ? 0.0f : (float)(t[4 * kk] / std::sqrt(t[4 * kk + 2] * t[4 * kk + 1]));
+  }
+  // median_nocopy (simple_math.f90:2111-2130)
+  std::vector<float> s = frc;
+  std::sort(s.begin(), s.end());
+  const int nf = (int)s.size();
+  float med = (nf % 2 == 0) ? (s[nf / 2 - 1] + s[nf / 2]) / 2.0f : s[nf / 2];
+  *score = std::max(0.0f, med);
+  return PFTCC_OK;
+}
+
+int pftcc_calc_roinv_corrmat(pftcc_t* h, float* corrmat) {
+  if (!h || !corrmat) return PFTCC_E_ARG;
+  int np, nr;
+  pftcc_get_nptcls(h, &np);
+  pftcc_get_nrefs(h, &nr);
+  if (np != nr) return fail(h, PFTCC_E_ARG, "nptcls == nrefs in pftcc required; calc_roinv_corrmat");
+  std::vector<int> ids(np);
+  {
+    std::lock_guard<std::mutex> g(h->mtx);
+    int c = 0;
+    for (int gidx = h->pfrom; gidx <= h->pto; ++gidx)
+      if (h->pinds[gidx - h->pfrom] > 0) ids[c++] = gidx;
+  }
+  std::vector<float> cc((size_t)np * nr * h->nrots);
+  int rc = pftcc_gencorrs_batch(h, np, ids.data(), nr, nullptr, 0, nullptr, cc.data());
+  if (rc) return rc;
+  // corrmat(iref, iptcl) for iref < iptcl, symmetric, unit diagonal (:2219-2228); Fortran order
+  for (int i = 0; i < np * np; ++i) corrmat[i] = 1.0f;
+  for (int p = 0; p < np; ++p)
+    for (int r = 0; r < p; ++r) {
+      const float* v = &cc[((size_t)p * nr + r) * h->nrots];
+      float m = v[0];
+      for (int t = 1; t < h->nrots; ++t) m = std::max(m, v[t]);
+      corrmat[(size_t)p * np + r] = m;
+      corrmat[(size_t)r * np + p] = m;
+    }
+  return PFTCC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,290 @@
+"""Pins the FP64 oracle (oracle/pftcc_oracle.py) without the reference binary.
+
+The reference ships no golden vectors for this path, so each oracle function is checked
+against something a plausible mistake would break: an independent formulation of the same
+quantity (FFT route vs brute-force direct sums, Euclid via correlations vs via differences),
+a closed form (planted rotations and shifts give cc = 1, amplitude-contrast-only CTF equals
+the amplitude fraction, relativistic electron wavelength), hand-derived integer tables, or
+central finite differences.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pftcc_oracle as O
+from cases import make_case, random_pfts
+
+
+# --------------------------------------------------------------------------------- a3 / a2 ---
+@pytest.mark.parametrize("box,pftsz", [(128, 192), (200, 308), (256, 400), (64, 100)])
+def test_magic_pftsz_known_boxes(box, pftsz):
+    """Hand-derived: ring2 = round2even(box/2-2); pi*ring2 -> nearest FFT-friendly size.
+
+    box 128: ring2 62, pi*62 = 194.8 -> 192 (|192-195|=3 < |200-195|=5).
+    box 200: ring2 98, pi*98 = 307.9 -> 308.  box 256: ring2 126, pi*126 = 395.8 -> 400.
+    box 64: ring2 30, pi*30 = 94.2 -> tie between 88 and 100 resolves to the larger, 100.
+    """
+    assert O.magic_pftsz(O.default_ring2(box)) == pftsz
+
+
+def test_round2even_hand_values():
+    assert O.round2even(194.78) == 194      # nint 195 is odd; 194 is closer
+    assert O.round2even(3.0) == 2           # tie 2 vs 4 goes down (<=)
+    assert O.round2even(5.5) == 6           # nint 6 even
+    assert O.round2even(62.0) == 62
+
+
+def test_boxmatch_hand_values():
+    # box 256 with msk 62: 2*(62+10) = 144 < 256 -> nearest magic box to 144 is 144
+    assert O.find_boxmatch(256, 62.0) == 144
+    assert O.find_boxmatch(128, 62.0) == 128   # 128 <= 144 keeps the box
+
+
+def test_grid_closed_form():
+    g = O.make_grid(192, 2, 40, 128)
+    rho = np.arange(g.nrots)
+    ang = 2 * np.pi * rho / g.nrots
+    ks = g.ks[:, None]
+    assert np.allclose(g.polar_x, ks * np.sin(ang)[None, :], atol=4e-5 * 40)
+    assert np.allclose(g.polar_y, -ks * np.cos(ang)[None, :], atol=4e-5 * 40)
+    assert np.allclose(g.angtab_deg, rho * 360.0 / g.nrots, atol=2e-4)
+    # argtransf = polar * pi/(ldim/2)
+    assert np.allclose(g.argtransf_x, g.polar_x[:, :192].astype(np.float64) * np.pi / 64, rtol=1e-12)
+    for irot in (1, 2, 97, 384):
+        assert O.get_roind(g, float(g.angtab_deg[irot - 1])) == irot
+    assert O.get_roind(g, 359.99) == 1
+
+
+def test_pxls_p_shell_hand_and_area():
+    # h,k in {1,2}: (1,1)->1, (1,2),(2,1)->2, (2,2)->3 (outside [1,2])
+    assert list(O.pxls_p_shell(1, 2)) == [1.0, 2.0]
+    p = O.pxls_p_shell(2, 60)
+    k = np.arange(2, 61)
+    # a quarter ring of radius k holds about pi*k/2 lattice points
+    sel = k >= 20
+    assert np.all(np.abs(p[sel] / (np.pi * k[sel] / 2) - 1) < 0.15)
+
+
+# -------------------------------------------------------------------------------------- a6 ---
+def test_ctf_wavelength_300kv():
+    """Relativistic electron wavelength at 300 kV is 0.019687 A (textbook, 12.2643/sqrt(V(1+
+    0.97848e-6 V))); the reference's constant 12.26 makes it 3.1e-4 smaller.  Dropping the
+    relativistic term would be 14 % off, so rtol 5e-4 pins the formula."""
+    g = O.make_grid(16, 2, 7, 32)
+    # Recover wl from a pure-defocus CTF: chi = pi*wl*s2*df when Cs = fraca = 0.
+    p = O.CtfParams(smpd=1.0, kv=300.0, cs=0.0, fraca=0.0, dfx=1e-4, dfy=1e-4, angast=0.0)
+    v = O.ctf_abs_mat(g, p)
+    s2 = (g.polar_x[:, :16].astype(float) / 32) ** 2 + (g.polar_y[:, :16].astype(float) / 32) ** 2
+    df = 1e-4 * 1e4
+    wl = np.arcsin(v) / (np.pi * s2 * df)
+    assert np.allclose(wl, 0.019687, rtol=5e-4)
+
+
+def test_ctf_amplitude_contrast_only():
+    """df = Cs = 0: |sin(atan(fa/sqrt(1-fa^2)))| = fa everywhere."""
+    g = O.make_grid(24, 2, 9, 40)
+    p = O.CtfParams(smpd=1.2, kv=200.0, cs=0.0, fraca=0.07, dfx=0.0, dfy=0.0, angast=33.0)
+    assert np.allclose(O.ctf_abs_mat(g, p), 0.07, atol=1e-12)
+
+
+def test_ctf_phase_plate_quarter_wave():
+    g = O.make_grid(24, 2, 9, 40)
+    p = O.CtfParams(smpd=1.2, kv=200.0, cs=0.0, fraca=0.0, dfx=0.0, dfy=0.0, angast=0.0,
+                    phshift=np.pi / 2, phaseplate=True)
+    assert np.allclose(O.ctf_abs_mat(g, p), 1.0, atol=1e-12)
+
+
+def test_ctf_isotropic_is_ring_constant_and_astig_axes():
+    g = O.make_grid(32, 2, 12, 48)
+    iso = O.CtfParams(smpd=1.3, kv=300, cs=2.7, fraca=0.1, dfx=2.0, dfy=2.0, angast=17.0)
+    v = O.ctf_abs_mat(g, iso)
+    # the FP32 grid radius varies by ~1 ulp around a ring; |dCTF| <= |d chi| ~ 2 eps32 chi
+    tol = 4 * 6e-8 * O.ctf_chi_abs(g, iso).max()
+    assert np.allclose(v, v[:, :1], atol=tol)
+    # astigmatic, angast = 0: along theta = pi/2 (x axis) df = dfx; along theta = 0 (y axis) dfy
+    ast = O.CtfParams(smpd=1.3, kv=300, cs=2.7, fraca=0.1, dfx=2.5, dfy=1.5, angast=0.0)
+    va = O.ctf_abs_mat(g, ast)
+    vx = O.ctf_abs_mat(g, O.CtfParams(1.3, 300, 2.7, 0.1, 2.5, 2.5, 0.0))
+    vy = O.ctf_abs_mat(g, O.CtfParams(1.3, 300, 2.7, 0.1, 1.5, 1.5, 0.0))
+    assert np.allclose(va[:, 16], vx[:, 16], atol=1e-5)   # j = n/2 -> theta = 90 deg
+    assert np.allclose(va[:, 0], vy[:, 0], atol=1e-5)     # j = 0   -> theta = 0
+
+
+# ------------------------------------------------------------------------------------- a11 ---
+def test_shmat_recurrence_matches_closed_form():
+    """De Moivre: shmat(:,k) = exp(i(arg_kfrom + (k-kfrom) arg_1)) exactly (to FP64 rounding);
+    against exp(i*argtransf) it differs only by the FP32 rounding of polar = sin*k."""
+    g = O.make_grid(64, 2, 30, 100)
+    for sh in ([1.5, -2.25], [0.0, 3.0], [-4.0, 0.5]):
+        arg0 = g.argtransf_x[0] * sh[0] + g.argtransf_y[0] * sh[1]
+        arg1 = g.shellone_x * sh[0] + g.shellone_y * sh[1]
+        steps = np.arange(g.nk)[:, None]
+        exact = np.exp(1j * (arg0[None, :] + steps * arg1[None, :]))
+        assert np.max(np.abs(O.shmat_8(g, sh) - exact)) < 1e-12
+        argmax = np.max(np.abs(g.argtransf_x) * abs(sh[0]) + np.abs(g.argtransf_y) * abs(sh[1]))
+        assert np.max(np.abs(O.shmat_8(g, sh) - O.shmat_closed(g, sh))) < 4 * 6e-8 * argmax
+
+
+# -------------------------------------------------------------------------- a9 / a10 / a12 ---
+@pytest.mark.parametrize("n", [8, 12, 16, 24, 40])
+def test_fourcase_equals_modular_index(n):
+    rng = np.random.default_rng(n)
+    q = random_pfts(rng, 1, 3, n)[0].astype(np.complex128)
+    p = random_pfts(rng, 1, 3, n)[0]
+    for irot in range(1, 2 * n + 1):
+        assert np.allclose(O.corrk_for_rot(q, p, irot), O.corrk_for_rot_fourcase(q, p, irot),
+                           rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,ctf,shift", [(8, False, None), (16, True, None), (24, True, (1.0, -0.5)),
+                                         (40, True, (-2.0, 1.25)), (192, True, None)])
+def test_fft_route_equals_direct_sum(n, ctf, shift):
+    """calc_corrs_over_k's FFT identity (4 conj(A)PA / 4 conj(B)PB, c2r_2n, /(2 nrots)) equals
+    the brute-force rotational sum; fixes the scaling and the conjugation convention."""
+    kto = 7 if n < 100 else 20
+    pf, _ = make_case(seed=n, n=n, kto=kto, ctf=ctf, nrefs=2, nptcls=2, ldim=4 * kto)
+    for iref in range(2):
+        for i in range(2):
+            a = pf.gencorrs(iref, i, shift, method="direct")
+            b = pf.gencorrs(iref, i, shift, method="fft")
+            assert np.allclose(a, b, rtol=1e-11, atol=1e-13)
+
+
+def _plant_rotation(ref_half, rho0):
+    """Particle = reference rotated so that P[j + rho0] = ref_full[j]."""
+    n = ref_half.shape[-1]
+    full = O.full_ring(ref_half.astype(np.complex128))
+    rolled = np.roll(full, rho0, axis=-1)
+    return rolled[..., :n].astype(np.complex64)
+
+
+@pytest.mark.parametrize("rho0", [0, 1, 5, 16, 17, 31])
+def test_planted_rotation_gives_unit_correlation(rho0):
+    pf, _ = make_case(seed=3, n=16, kto=9, ctf=False, nrefs=1, nptcls=1)
+    pf.ptcls[0] = _plant_rotation(pf.refs_even[0], rho0)
+    cc = pf.gencorrs(0, 0)
+    assert abs(cc[rho0] - 1.0) < 1e-6
+    assert int(np.argmax(cc)) == rho0
+    assert np.all(np.abs(cc) <= 1 + 1e-12)                      # Cauchy-Schwarz
+
+
+def test_rotation_equivariance():
+    pf, _ = make_case(seed=5, n=24, kto=8, ctf=False, nrefs=1, nptcls=1)
+    cc0 = pf.gencorrs(0, 0)
+    base = pf.ptcls[0].copy()
+    full = O.full_ring(base.astype(np.complex128))
+    for d in (1, 7, 30):
+        pf.ptcls[0] = np.roll(full, d, axis=-1)[..., :24].astype(np.complex64)
+        assert np.allclose(pf.gencorrs(0, 0), np.roll(cc0, d), atol=1e-6)
+    pf.ptcls[0] = base
+
+
+def test_shift_recovery_invariant():
+    """simple_test_shiftsrch's invariant: a particle carrying the phase of shift s0 is matched
+    best by gencorrs with shvec = s0 (the shift is applied to the reference, :1529)."""
+    pf, _ = make_case(seed=7, n=32, kto=12, ctf=False, nrefs=1, nptcls=1, ldim=40)
+    s0 = (1.0, -1.0)
+    pf.ptcls[0] = (pf.refs_even[0].astype(np.complex128) *
+                   O.shmat_closed(pf.grid, s0)).astype(np.complex64)
+    best = max(((pf.gencorrs(0, 0, (x, y)).max(), (x, y))
+                for x in (-2.0, -1.0, 0.0, 1.0, 2.0) for y in (-2.0, -1.0, 0.0, 1.0, 2.0)))
+    assert best[1] == s0 and abs(best[0] - 1.0) < 1e-5
+
+
+# ------------------------------------------------------------------------------- a13 / a14 ---
+@pytest.mark.parametrize("match_filt", [False, True])
+def test_for_rot_8_consistent_with_gencorrs(match_filt):
+    """With match_filt (unit weights, memoised S_p) and optlp = 1, gencorr_for_rot_8 equals the
+    shifted gencorrs at irot when kstop == kto (both normalise over the same shells)."""
+    pf, _ = make_case(seed=11, n=16, kto=8, ctf=True, nrefs=2, nptcls=2, match_filt=match_filt)
+    if match_filt:
+        pf.ref_optlp[:] = 1.0
+    sh = (0.7, -1.3)
+    cc = pf.gencorrs(1, 1, sh)
+    for irot in (1, 5, 17, 30):
+        v = pf.gencorr_for_rot_8(1, 1, sh, irot)
+        if match_filt:
+            # shellnorm with optlp=1 rescales each shell; recompute gencorrs with it applied
+            q = pf.prep_ref(1, 1, sh)
+            c = O.corrk_for_rot(q, pf.ptcls[1], irot).sum()
+            ref = c / math.sqrt(np.sum(np.abs(q) ** 2) * pf.sqsum_ptcl(1))
+            assert abs(v - ref) < 1e-12
+        else:
+            assert np.isfinite(v) and abs(v) <= 1.0
+    # the planted case: weights k give exactly 1 as well
+    pf2, _ = make_case(seed=12, n=16, kto=8, ctf=False, nrefs=1, nptcls=1)
+    pf2.ptcls[0] = _plant_rotation(pf2.refs_even[0], 9)
+    assert abs(pf2.gencorr_for_rot_8(0, 0, (0.0, 0.0), 10) - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("match_filt", [False, True])
+def test_shift_gradient_matches_central_fd(match_filt):
+    pf, _ = make_case(seed=21, n=24, kto=10, ctf=True, nrefs=2, nptcls=2, match_filt=match_filt)
+    h = 1e-3
+    for (iref, i, sh, irot) in [(0, 0, (0.3, -0.8), 4), (1, 1, (-2.1, 1.7), 37)]:
+        f, g = pf.gencorr_grad_for_rot_8(iref, i, sh, irot)
+        assert abs(f - pf.gencorr_for_rot_8(iref, i, sh, irot)) < 1e-13
+        for d in range(2):
+            sp = list(sh); sm = list(sh)
+            sp[d] += h; sm[d] -= h
+            fd = (pf.gencorr_for_rot_8(iref, i, sp, irot) -
+                  pf.gencorr_for_rot_8(iref, i, sm, irot)) / (2 * h)
+            assert abs(fd - g[d]) <= 1e-6 * max(1.0, abs(g[d]))
+
+
+# ------------------------------------------------------------------------------------- a15 ---
+def test_angle_interpolant_and_derivative():
+    pf, _ = make_case(seed=31, n=16, kto=8, ctf=True, nrefs=1, nptcls=1)
+    sh = (0.4, 0.2)
+    for irot in (1, 6, 20):
+        psi = (irot - 1) * np.pi / 16
+        assert abs(pf.corr_at_angle(0, 0, sh, psi) - pf.gencorr_for_rot_8(0, 0, sh, irot)) < 1e-12
+    h = 1e-5
+    for psi in (0.0, 0.37, 2.0, 5.5):
+        fd = (pf.corr_at_angle(0, 0, sh, psi + h) - pf.corr_at_angle(0, 0, sh, psi - h)) / (2 * h)
+        assert abs(fd - pf.grad_angle(0, 0, sh, psi)) < 1e-7
+
+
+# ------------------------------------------------------------------------------------- a16 ---
+def test_euclid_two_formulations_agree():
+    """gencorrs_euclid (via correlations and square sums) equals calc_euclid_for_rot_8's direct
+    differences once the latter's extra factor k is divided out through sigma2."""
+    pf, _ = make_case(seed=41, n=16, kto=8, ctf=True, nrefs=2, nptcls=2, euclid=True)
+    e_fft = pf.gencorrs_euclid(1, 0, None, method="fft")
+    e_dir = pf.gencorrs_euclid(1, 0, None, method="direct")
+    assert np.allclose(e_fft, e_dir, rtol=1e-11, atol=1e-12)
+    s2 = pf.sigma2.copy()
+    pf.sigma2 = (s2 * np.arange(pf.kfrom, pf.grid.kto + 1)[None, :]).astype(np.float64)
+    for irot in (1, 9, 17, 32):
+        assert abs(pf.euclid_for_rot_8(1, 0, (0.0, 0.0), irot) -
+                   e_dir[irot - 1] * 1.0) < 1e-9 * max(1, abs(e_dir[irot - 1]))
+    pf.sigma2 = s2
+
+
+def test_euclid_gradient_fd():
+    pf, _ = make_case(seed=43, n=16, kto=8, ctf=True, nrefs=1, nptcls=1, euclid=True)
+    sh, irot, h = (0.5, -0.25), 7, 1e-4
+    f, g = pf.euclid_grad_for_rot_8(0, 0, sh, irot)
+    for d in range(2):
+        sp = list(sh); sm = list(sh)
+        sp[d] += h; sm[d] -= h
+        fd = (pf.euclid_for_rot_8(0, 0, sp, irot) - pf.euclid_for_rot_8(0, 0, sm, irot)) / (2 * h)
+        assert abs(fd - g[d]) <= 1e-6 * max(1.0, abs(g[d]))
+
+
+def test_sigma_contrib_zero_for_identical():
+    pf, _ = make_case(seed=44, n=16, kto=8, ctf=False, nrefs=1, nptcls=1, euclid=True)
+    pf.ptcls[0] = pf.refs_even[0]
+    assert np.max(np.abs(pf.sigma_contrib(0, 0, (0.0, 0.0), 1))) < 1e-10
+
+
+# ------------------------------------------------------------------------------------- a18 ---
+def test_specscore_planted_and_bounds():
+    pf, _ = make_case(seed=51, n=16, kto=9, ctf=False, nrefs=1, nptcls=1)
+    pf.ptcls[0] = _plant_rotation(pf.refs_even[0], 3)
+    assert abs(pf.specscore(0, 0, 4) - 1.0) < 1e-6
+    pf2, _ = make_case(seed=52, n=16, kto=9, ctf=True, nrefs=1, nptcls=1)
+    frc = pf2.genfrc(0, 0, 5)
+    assert np.all(np.abs(frc) <= 1 + 1e-12)
+    assert pf2.specscore(0, 0, 5) >= 0.0
