@@ -193,6 +193,14 @@ int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs,
                          const int32_t* d_iptcls, const double* d_shvecs, const int32_t* d_irots,
                          double* d_f, double* d_grad);
 
+/* ------------------------------------------------------------------- instrumentation -- */
+/* No reference counterpart.  When enabled, each launch of the correlation kernels (the plain
+ * FP32 search/gencorrs kernel) is bracketed by HIP events on the handle's stream;
+ * pftcc_get_timing synchronises and returns the summed device time and launch count since the
+ * last reset. */
+int pftcc_set_timing(pftcc_t* h, int32_t enable);
+int pftcc_get_timing(pftcc_t* h, double* total_ms, int64_t* launches);
+
 /* ------------------------------------------------------------------------- utilities -- */
 /* magic_pftsz (simple_magic_boxes.f90:25-37). */
 int32_t pftcc_magic_pftsz(int32_t ring2);

@@ -37,6 +37,14 @@ def build_hip():
     return out
 
 
+def build_synth():
+    out = os.path.join(LIB, "libpftcc_synth.so")
+    tmp = out + ".tmp"
+    run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "synth.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_fortran():
     """Fortran module simple_polarft_corrcalc (ISO_C_BINDING over the C ABI) + test driver."""
     src_mod = os.path.join(FSRC, "simple_polarft_corrcalc.f90")
@@ -57,6 +65,7 @@ def build_fortran():
 
 def main():
     build_hip()
+    build_synth()
     build_fortran()
 
 

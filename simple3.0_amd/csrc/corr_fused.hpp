@@ -1,0 +1,432 @@
+// corr_fused.hpp - compile-time-size correlation kernel (the hot path of the plain FP32 search).
+//
+// Same arithmetic as k_corr_generic (pftcc_kernels.hpp), organised for gfx950:
+//  * the ring length N and the pair tile F are template constants, so every index, radix and
+//    loop bound folds at compile time;
+//  * pass 1 (radix 4) builds its inputs straight from the reference, CTF and shift rings:
+//    z[j], z[j+N/4], z[j+N/2], z[j+3N/4] all come from q[2j..2j+1] and q[2j+N/2..2j+N/2+1];
+//  * the last pass runs the mirrored butterflies j and Ns-j in one thread, so the real-FFT split
+//    V[m] = f(Zc[m], Zc[N-m]) and the product conj(V) W_k accumulate in registers;
+//  * the spectra Z of the F pairs stay in VGPRs across all shells; the pre-processing of the
+//    final inverse transform (X[m], X[N-m] -> Y[m]) is also done from those registers.
+// Deterministic: a pair's arithmetic does not depend on the tile it belongs to.
+#pragma once
+#include "pftcc_kernels.hpp"
+
+namespace pfk {
+
+// radix order for the fused kernel: one 4 first, the remaining factors in descending order
+__host__ __device__ constexpr FftPlan make_fused_plan(int n) {
+  FftPlan p = make_plan(n);
+  if (p.np < 2 || n % 4 != 0) return p;
+  FftPlan q{0, {0}};
+  q.r[q.np++] = 4;
+  int rest[16] = {0};
+  int nr = 0;
+  bool dropped = false;
+  for (int i = 0; i < p.np; ++i) {
+    if (!dropped && p.r[i] == 4) {
+      dropped = true;
+      continue;
+    }
+    rest[nr++] = p.r[i];
+  }
+  for (int i = 0; i < nr; ++i)
+    for (int k = i + 1; k < nr; ++k)
+      if (rest[k] > rest[i]) {
+        const int t = rest[i];
+        rest[i] = rest[k];
+        rest[k] = t;
+      }
+  for (int i = 0; i < nr; ++i) q.r[q.np++] = rest[i];
+  return q;
+}
+
+template <int N>
+struct Fp {
+  static constexpr FftPlan plan = make_fused_plan(N);
+  static constexpr int np = plan.np;
+  static constexpr int R(int i) { return plan.r[i]; }
+  static constexpr int Ns(int i) {
+    int s = 1;
+    for (int k = 0; k < i; ++k) s *= plan.r[k];
+    return s;
+  }
+  static constexpr bool fusable = np >= 2 && plan.r[0] == 4 && N % 4 == 0;
+};
+
+// In-place Stockham pass P of F transforms held in LDS (stride N); each thread stages its
+// butterflies' inputs in registers between two barriers.
+template <int N, int F, int NT, int P>
+__device__ __forceinline__ void lds_pass(float2* __restrict__ buf, const float2* __restrict__ tw2, int tid) {
+  constexpr int R = Fp<N>::R(P);
+  constexpr int Ns = Fp<N>::Ns(P);
+  constexpr int nb = N / R;
+  constexpr int TASKS = F * nb;
+  constexpr int TPT = (TASKS + NT - 1) / NT;
+  constexpr int tstep = N / (Ns * R);
+  float2 v[TPT][R];
+#pragma unroll
+  for (int c = 0; c < TPT; ++c) {
+    const int task = tid + c * NT;
+    if (TASKS % NT == 0 || task < TASKS) {
+      const int f = task / nb, j = task % nb;
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[c][t] = buf[f * N + j + t * nb];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < TPT; ++c) {
+    const int task = tid + c * NT;
+    if (TASKS % NT == 0 || task < TASKS) {
+      const int f = task / nb, j = task % nb;
+      const int jm = j % Ns;
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int t = 1; t < R; ++t) v[c][t] = cmul(v[c][t], tw2[2 * jm * t * tstep]);
+      }
+      Dft<R>::run(v[c]);
+      const int base = (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int t = 0; t < R; ++t) buf[f * N + base + t * Ns] = v[c][t];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int F, int NT, int P, int PEND>
+__device__ __forceinline__ void lds_passes(float2* buf, const float2* tw2, int tid) {
+  if constexpr (P < PEND) {
+    lds_pass<N, F, NT, P>(buf, tw2, tid);
+    lds_passes<N, F, NT, P + 1, PEND>(buf, tw2, tid);
+  }
+}
+
+template <int N, int NT>
+constexpr int fused_F() {
+  constexpr int RL = Fp<N>::R(Fp<N>::np - 1);
+  constexpr int G = (N / RL) / 2 + 1;
+  int F = NT / G;
+  if (F < 1) F = 1;
+  if (F > 32) F = 32;
+  return F;
+}
+
+template <int N, int F>
+constexpr size_t fused_lds_bytes() {
+  // buf[F N] + Wk[N+1] + tw2[2N] float2;  ctfk[N] + sq/off slots 2 F N/4 floats
+  return sizeof(float2) * ((size_t)F * N + (N + 1) + 2 * N) + sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
+}
+
+// Value select (a conditional on array lvalues would select an address and force the array
+// out of registers).
+__device__ __forceinline__ float2 vsel(bool c, float2 a, float2 b) {
+  return make_float2(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+// V[m] of the real FFT of length 2N from the packed FFT values A = Zc[m], B = Zc[N-m].
+__device__ __forceinline__ float2 split_bin(float2 A, float2 B, float2 w) {
+  const float2 sum = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y - B.y));
+  const float2 dif = make_float2(0.5f * (A.x - B.x), 0.5f * (A.y + B.y));
+  const float2 wd = cmul(w, dif);
+  return make_float2(sum.x + wd.y, sum.y - wd.x);
+}
+
+// Y[m] of the inverse real FFT from X[m] and X[N-m]; e = exp(+i pi m / N).  Returns conj(Y).
+__device__ __forceinline__ float2 unsplit_bin_conj(float2 X, float2 Xn, float2 e) {
+  const float2 s = make_float2(X.x + Xn.x, X.y - Xn.y);
+  const float2 d = make_float2(X.x - Xn.x, X.y + Xn.y);
+  const float2 ed = cmul(e, d);
+  return make_float2(s.x - ed.y, -(s.y + ed.x));
+}
+
+template <int N, int F, int NT>
+__global__ void __launch_bounds__(NT) k_corr_fused(CorrArgs a) {
+  static_assert(Fp<N>::fusable, "first radix must be 4");
+  constexpr int NP = Fp<N>::np;
+  constexpr int RL = Fp<N>::R(NP - 1);      // last radix
+  constexpr int NsL = N / RL;               // product of the other radices
+  constexpr int G = NsL / 2 + 1;            // mirrored butterfly groups of the last pass
+  constexpr int TL = F * G;
+  constexpr int TPTL = (TL + NT - 1) / NT;
+  constexpr int NB = 2 * RL + 1;            // bins per group: pair 2 RL; group 0: RL + Nyquist
+  constexpr int Q = N / 4;                  // pass-1 butterflies per transform
+  constexpr int T1 = F * Q;
+  constexpr int TPT1 = (T1 + NT - 1) / NT;
+
+  extern __shared__ float2 smem[];
+  float2* buf = smem;                        // [F][N]
+  float2* Wk = buf + F * N;                  // [N+1]
+  float2* tw2 = Wk + (N + 1);                // [2N]   exp(-i pi a / N)
+  float* ctfk = reinterpret_cast<float*>(tw2 + 2 * N);   // [N]
+  float* sqs = ctfk + N;                     // [F][Q]
+  float* offs = sqs + F * Q;                 // [F][Q]
+  __shared__ float sq[F], off[F];
+  __shared__ float pscal;
+  __shared__ unsigned long long red[NT];
+
+  const int tid = threadIdx.x;
+  const int pi = blockIdx.x / a.ntiles;
+  const int tile = blockIdx.x - pi * a.ntiles;
+  const int ip = a.plist[pi];
+  const bool even = a.iseven[ip] != 0;
+  const float2* refs = even ? a.refs_even : a.refs_odd;
+  const int npairs = a.nr * a.ns;
+  const int nk = a.nk;
+
+  for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
+
+  // pass-1 slots: pair, reference and shift rows are fixed for all shells
+  float sqp[TPT1], offp[TPT1];
+  const float2* rrow[TPT1];
+  const float2* srow[TPT1];
+  int slotref[TPT1];
+  bool slotok[TPT1];
+#pragma unroll
+  for (int c = 0; c < TPT1; ++c) {
+    const int task = tid + c * NT;
+    const int f = task / Q;
+    const int P = tile * F + f;
+    slotok[c] = task < T1 && P < npairs;
+    sqp[c] = 0.f;
+    offp[c] = 0.f;
+    rrow[c] = refs;
+    srow[c] = nullptr;
+    slotref[c] = 0;
+    if (slotok[c]) {
+      const int ir = P / a.ns, is = P % a.ns;
+      const int iref = a.rlist ? a.rlist[ir] : ir;
+      slotref[c] = iref;
+      rrow[c] = refs + (size_t)iref * nk * N;
+      srow[c] = a.shtab ? a.shtab + (size_t)is * nk * N : nullptr;
+    }
+  }
+
+  float2 acc[TPTL][NB];
+#pragma unroll
+  for (int c = 0; c < TPTL; ++c)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[c][b] = make_float2(0.f, 0.f);
+
+  const int kloop = a.kcorr > a.knorm ? a.kcorr : a.knorm;
+  for (int kk = 0; kk < kloop; ++kk) {
+    const bool docorr = kk < a.kcorr;
+    const bool donorm = kk < a.knorm;
+    __syncthreads();   // the previous shell's last pass has finished with buf and Wk
+    const size_t prow = (size_t)ip * nk + kk;
+    for (int j = tid; j < N; j += NT) ctfk[j] = a.ctf ? a.ctf[prow * N + j] : 1.0f;
+    if (docorr)
+      for (int m = tid; m <= N; m += NT) Wk[m] = a.memo[prow * (N + 1) + m];
+    const float wk = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
+    __syncthreads();
+    // ---- pass 1: z from q = ref * |CTF| [* shift] [* shell scale]; radix-4 DFT (Ns = 1) ----
+#pragma unroll
+    for (int c = 0; c < TPT1; ++c) {
+      const int task = tid + c * NT;
+      if (!(T1 % NT == 0 || task < T1)) continue;
+      const int f = task / Q, j = task % Q;
+      float2 y[4];
+      if (slotok[c]) {
+        const float2* r = rrow[c] + (size_t)kk * N;
+        const float4 ra = *reinterpret_cast<const float4*>(r + 2 * j);
+        const float4 rb = *reinterpret_cast<const float4*>(r + 2 * j + N / 2);
+        const float2 c01 = *reinterpret_cast<const float2*>(ctfk + 2 * j);
+        const float2 c23 = *reinterpret_cast<const float2*>(ctfk + 2 * j + N / 2);
+        float2 q0 = make_float2(ra.x * c01.x, ra.y * c01.x), q1 = make_float2(ra.z * c01.y, ra.w * c01.y);
+        float2 q2 = make_float2(rb.x * c23.x, rb.y * c23.x), q3 = make_float2(rb.z * c23.y, rb.w * c23.y);
+        if (srow[c]) {
+          const float2* s = srow[c] + (size_t)kk * N;
+          const float4 sa = *reinterpret_cast<const float4*>(s + 2 * j);
+          const float4 sb = *reinterpret_cast<const float4*>(s + 2 * j + N / 2);
+          q0 = cmul(q0, make_float2(sa.x, sa.y));
+          q1 = cmul(q1, make_float2(sa.z, sa.w));
+          q2 = cmul(q2, make_float2(sb.x, sb.y));
+          q3 = cmul(q3, make_float2(sb.z, sb.w));
+        }
+        if (a.refinv && docorr) {
+          const int iref = slotref[c];
+          const int row = a.optrow ? a.optrow[ip] : iref;
+          const float sc = a.ofac[(size_t)row * a.kcorr + kk] *
+                           a.refinv[((size_t)(even ? 0 : 1) * a.nrefs + iref) * a.kcorr + kk];
+          q0 = cscale(q0, sc);
+          q1 = cscale(q1, sc);
+          q2 = cscale(q2, sc);
+          q3 = cscale(q3, sc);
+        }
+        const float e2 = q0.x * q0.x + q0.y * q0.y + q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y +
+                         q3.x * q3.x + q3.y * q3.y;
+        if (donorm) sqp[c] += e2;
+        if (a.mode == 1 && docorr) offp[c] = fmaf(wk, e2, offp[c]);
+        // z[j], z[j+Q], z[j+2Q], z[j+3Q]
+        y[0] = make_float2(q0.x + q0.y, q1.x + q1.y);
+        y[1] = make_float2(q2.x + q2.y, q3.x + q3.y);
+        y[2] = make_float2(q0.x - q0.y, q1.x - q1.y);
+        y[3] = make_float2(q2.x - q2.y, q3.x - q3.y);
+      } else {
+        y[0] = y[1] = y[2] = y[3] = make_float2(0.f, 0.f);
+      }
+      if (docorr) {
+        Dft<4>::run(y);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) buf[f * N + 4 * j + t] = y[t];
+      }
+    }
+    if (!docorr) continue;
+    __syncthreads();
+    // ---- middle passes in LDS ----
+    lds_passes<N, F, NT, 1, NP - 1>(buf, tw2, tid);
+    // ---- last pass (radix RL, Ns = NsL) on mirrored butterflies + split + conj(V) W_k ----
+#pragma unroll
+    for (int c = 0; c < TPTL; ++c) {
+      const int task = tid + c * NT;
+      if (!(TL % NT == 0 || task < TL)) continue;
+      const int f = task / G, g = task % G;
+      const bool pair = g != 0 && 2 * g != NsL;
+      const int j2 = NsL - g;
+      float2 y1[RL], y2[RL];
+#pragma unroll
+      for (int t = 0; t < RL; ++t) y1[t] = buf[f * N + g + t * NsL];
+#pragma unroll
+      for (int t = 1; t < RL; ++t) y1[t] = cmul(y1[t], tw2[2 * g * t]);
+      Dft<RL>::run(y1);
+      if (pair) {
+#pragma unroll
+        for (int t = 0; t < RL; ++t) y2[t] = buf[f * N + j2 + t * NsL];
+#pragma unroll
+        for (int t = 1; t < RL; ++t) y2[t] = cmul(y2[t], tw2[2 * j2 * t]);
+        Dft<RL>::run(y2);
+      } else {
+#pragma unroll
+        for (int t = 0; t < RL; ++t) y2[t] = make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int m = g + t * NsL;
+        const float2 b0 = y1[(RL - t) % RL], b1 = y2[RL - 1 - t], b2 = y1[RL - 1 - t];
+        const float2 B = vsel(g == 0, b0, vsel(pair, b1, b2));
+        const float2 V = split_bin(y1[t], B, tw2[m]);
+        const float2 p = cmulcj(V, Wk[m]);
+        acc[c][t].x = fmaf(wk, p.x, acc[c][t].x);
+        acc[c][t].y = fmaf(wk, p.y, acc[c][t].y);
+      }
+      if (pair) {
+#pragma unroll
+        for (int t = 0; t < RL; ++t) {
+          const int m = j2 + t * NsL;
+          const float2 V = split_bin(y2[t], y1[RL - 1 - t], tw2[m]);
+          const float2 p = cmulcj(V, Wk[m]);
+          acc[c][RL + t].x = fmaf(wk, p.x, acc[c][RL + t].x);
+          acc[c][RL + t].y = fmaf(wk, p.y, acc[c][RL + t].y);
+        }
+      }
+      if (g == 0) {
+        const float2 V = split_bin(y1[0], y1[0], tw2[N]);
+        const float2 p = cmulcj(V, Wk[N]);
+        acc[c][2 * RL].x = fmaf(wk, p.x, acc[c][2 * RL].x);
+        acc[c][2 * RL].y = fmaf(wk, p.y, acc[c][2 * RL].y);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- normalisation sums (fixed order) ----
+#pragma unroll
+  for (int c = 0; c < TPT1; ++c) {
+    const int task = tid + c * NT;
+    if (task < T1) {
+      sqs[task] = sqp[c];
+      offs[task] = offp[c];
+    }
+  }
+  // ---- Y = pre-processed inverse input, conj'd, from the register spectra ----
+#pragma unroll
+  for (int c = 0; c < TPTL; ++c) {
+    const int task = tid + c * NT;
+    if (!(TL % NT == 0 || task < TL)) continue;
+    const int f = task / G, g = task % G;
+    const bool pair = g != 0 && 2 * g != NsL;
+    const int j2 = NsL - g;
+#pragma unroll
+    for (int t = 0; t < RL; ++t) {
+      const int m = g + t * NsL;
+      float2 X = acc[c][t];
+      const float2 x0 = (t == 0) ? acc[c][2 * RL] : acc[c][(RL - t) % RL];
+      const float2 x1 = acc[c][2 * RL - 1 - t], x2 = acc[c][RL - 1 - t];
+      float2 Xn = vsel(g == 0, x0, vsel(pair, x1, x2));
+      if (m == 0) {
+        X.y = 0.f;
+        Xn.y = 0.f;
+      }
+      buf[f * N + m] = unsplit_bin_conj(X, Xn, cconj(tw2[m]));
+    }
+    if (pair) {
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int m = j2 + t * NsL;
+        buf[f * N + m] = unsplit_bin_conj(acc[c][RL + t], acc[c][RL - 1 - t], cconj(tw2[m]));
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < F) {
+    float s0 = 0.f, s1 = 0.f;
+    for (int u = 0; u < Q; ++u) {
+      s0 += sqs[tid * Q + u];
+      s1 += offs[tid * Q + u];
+    }
+    sq[tid] = s0;
+    off[tid] = s1;
+  }
+  if (tid == 0) {
+    if (a.mode == 1) {
+      float s = 0.f;
+      for (int kk = 0; kk < a.kcorr; ++kk)
+        s += a.shellw[(size_t)ip * nk + kk] * a.sqsums_k[(size_t)ip * nk + kk];
+      pscal = s;
+    } else {
+      pscal = a.sqsums[ip];
+    }
+  }
+  // ---- inverse FFT of every pair (all passes in LDS) ----
+  lds_passes<N, F, NT, 0, NP>(buf, tw2, tid);
+  const float inv4n = 1.0f / (4.0f * (float)N);
+  unsigned long long bestkey = 0ull;
+  for (int idx = tid; idx < F * N; idx += NT) {
+    const int f = idx / N, j = idx % N;
+    const int P = tile * F + f;
+    if (P >= npairs) continue;
+    const float2 y = buf[idx];
+    float x0 = y.x, x1 = -y.y;
+    if (a.mode == 1) {
+      const float o = off[f] + pscal;
+      x0 = 2.f * x0 * inv4n - o;
+      x1 = 2.f * x1 * inv4n - o;
+    } else {
+      const float sc = inv4n / sqrtf(sq[f] * pscal);
+      x0 *= sc;
+      x1 *= sc;
+    }
+    if (a.cc_out) {
+      float2* o = reinterpret_cast<float2*>(a.cc_out + ((size_t)pi * npairs + P) * (2 * N));
+      o[j] = make_float2(x0, x1);
+    }
+    if (a.best) {
+      const unsigned int lin0 = (unsigned int)P * (unsigned int)(2 * N) + 2u * j;
+      const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
+      const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
+      bestkey = k0 > bestkey ? k0 : bestkey;
+      bestkey = k1 > bestkey ? k1 : bestkey;
+    }
+  }
+  if (a.best) {
+    red[tid] = bestkey;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+      if (tid < w && red[tid + w] > red[tid]) red[tid] = red[tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) atomicMax(&a.best[pi], red[0]);
+  }
+}
+
+}  // namespace pfk

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -18,6 +19,7 @@
 
 #include "pftcc.h"
 #include "pftcc_kernels.hpp"
+#include "corr_fused.hpp"
 
 using namespace pfk;
 
@@ -109,6 +111,13 @@ struct pftcc {
   int* d_optrow = nullptr;
   bool memo_valid = false, dptcls_valid = false, refnorm_valid = false, have_sigma2 = false;
   bool ctf_ready = false;
+  bool use_fused = true;            // PFTCC_GENERIC=1 forces the runtime-size kernel
+  // kernel timing (pftcc_set_timing)
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t nev = 0;
+  double timed_ms = 0.0;
+  int64_t timed_launches = 0;
   // scratch arena (stream ordered; calls on a handle are serialised by mtx)
   char* d_scratch = nullptr;
   size_t scratch_cap = 0, scratch_off = 0;
@@ -381,6 +390,34 @@ int pick_F(int n, int npairs) {
   return F;
 }
 
+// Compile-time ring lengths of the fused kernel: the benchmark shapes (192, 308, 400), common
+// box sizes (64 -> 100, 160 -> 240, 320 -> 500, 384 -> 600) and small sizes used in tests.
+template <int N>
+struct FusedLaunch {
+  static constexpr int NT = 256;
+  static constexpr int F = fused_F<N, NT>();
+  static constexpr size_t lds = fused_lds_bytes<N, F>();
+  static void go(unsigned blocks, hipStream_t s, const CorrArgs& a) {
+    hipLaunchKernelGGL((k_corr_fused<N, F, NT>), dim3(blocks), dim3(NT), lds, s, a);
+  }
+};
+
+#define PFTCC_FUSED_SIZES(X) X(16) X(24) X(32) X(40) X(48) X(64) X(100) X(192) X(240) X(308) X(400) X(500)
+
+// F of the fused kernel for ring length n, 0 if n has no fused instantiation
+int fused_tile(int n) {
+#define X(N) if (n == N) return FusedLaunch<N>::F;
+  PFTCC_FUSED_SIZES(X)
+#undef X
+  return 0;
+}
+
+void fused_go(int n, unsigned blocks, hipStream_t s, const CorrArgs& a) {
+#define X(N) if (n == N) { FusedLaunch<N>::go(blocks, s, a); return; }
+  PFTCC_FUSED_SIZES(X)
+#undef X
+}
+
 // Launches the plain FP32 correlation for np particles x nr refs x ns shifts.
 int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist, int ns,
                 const float2* d_shtab, bool shifted, float* d_cc, unsigned long long* d_best) {
@@ -416,14 +453,42 @@ int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist
   a.best = d_best;
   a.tw2g = h->d_tw2;
   const int npairs = nr * a.ns;
-  a.F = pick_F(h->n, npairs);
+  const int Ff = h->use_fused ? fused_tile(h->n) : 0;
+  a.F = Ff > 0 ? Ff : pick_F(h->n, npairs);
   a.ntiles = (npairs + a.F - 1) / a.F;
   const size_t lds = corr_lds_bytes(h->n, a.F);
   const size_t blocks = (size_t)np * a.ntiles;
   if (blocks == 0) return PFTCC_OK;
   if (blocks > 0x7fffffffu) return fail(h, PFTCC_E_ARG, "search too large for one launch (%zu blocks)", blocks);
-  hipLaunchKernelGGL(k_corr_generic, dim3((unsigned)blocks), dim3(256), lds, h->stream, a);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->timing) {
+    if (h->nev == h->events.size()) {
+      hipEvent_t x, y;
+      HIPCHK(h, hipEventCreate(&x));
+      HIPCHK(h, hipEventCreate(&y));
+      h->events.emplace_back(x, y);
+    }
+    e0 = h->events[h->nev].first;
+    e1 = h->events[h->nev].second;
+    ++h->nev;
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+  }
+  if (Ff > 0) fused_go(h->n, (unsigned)blocks, h->stream, a);
+  else hipLaunchKernelGGL(k_corr_generic, dim3((unsigned)blocks), dim3(256), lds, h->stream, a);
   HIPCHK(h, hipGetLastError());
+  if (h->timing) HIPCHK(h, hipEventRecord(e1, h->stream));
+  return PFTCC_OK;
+}
+
+int collect_timing(pftcc* h) {
+  for (size_t i = 0; i < h->nev; ++i) {
+    HIPCHK(h, hipEventSynchronize(h->events[i].second));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->events[i].first, h->events[i].second));
+    h->timed_ms += ms;
+    ++h->timed_launches;
+  }
+  h->nev = 0;
   return PFTCC_OK;
 }
 
@@ -638,6 +703,10 @@ int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, c
   h->pssnr = cfg->pssnr != 0;
   h->clsfrcs = cfg->clsfrcs != 0;
   h->euclid = cfg->objfun == PFTCC_OBJFUN_EUCLID;
+  {
+    const char* gen = std::getenv("PFTCC_GENERIC");
+    h->use_fused = !(gen && gen[0] == '1');
+  }
   // index translation table (:276-290)
   h->pinds.assign(pto - pfrom + 1, 0);
   int cnt = 0;
@@ -692,9 +761,34 @@ int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, c
   return PFTCC_OK;
 }
 
+int pftcc_set_timing(pftcc_t* h, int32_t enable) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = collect_timing(h))) return rc;
+  h->timing = enable != 0;
+  h->timed_ms = 0.0;
+  h->timed_launches = 0;
+  return PFTCC_OK;
+}
+
+int pftcc_get_timing(pftcc_t* h, double* total_ms, int64_t* launches) {
+  if (!h || !total_ms || !launches) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = collect_timing(h))) return rc;
+  *total_ms = h->timed_ms;
+  *launches = h->timed_launches;
+  return PFTCC_OK;
+}
+
 int pftcc_kill(pftcc_t* h) {
   if (!h) return PFTCC_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto& ev : h->events) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
   free_ptcl_arrays(h);
   dfree(h->d_refs_e);
   dfree(h->d_refs_o);

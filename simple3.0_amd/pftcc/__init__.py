@@ -104,6 +104,8 @@ SIGNATURES = [
     ("pftcc_search_dev", _I, [_P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P]),
     ("pftcc_grad_batch", _I, [_P, _I, _PI, _PI, _PD, _PI, _PD, _PD]),
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    ("pftcc_set_timing", _I, [_P, _I]),
+    ("pftcc_get_timing", _I, [_P, _PD, C.POINTER(C.c_int64)]),
     ("pftcc_magic_pftsz", _I, [_I]),
 ]
 
@@ -503,6 +505,16 @@ class PolarFTCorrCalc:
                                                  C.c_void_p(d_irots), C.c_void_p(d_f),
                                                  C.c_void_p(d_grad) if d_grad else None),
                   "grad_batch_dev")
+
+    def set_timing(self, enable=True):
+        self._chk(self._lib.pftcc_set_timing(self._h, int(enable)), "set_timing")
+
+    def get_timing(self):
+        """(summed device ms of the correlation kernel launches, number of launches)."""
+        ms = C.c_double()
+        n = C.c_int64()
+        self._chk(self._lib.pftcc_get_timing(self._h, C.byref(ms), C.byref(n)), "get_timing")
+        return ms.value, n.value
 
     def set_stream(self, stream_ptr):
         self._chk(self._lib.pftcc_set_stream(self._h, C.c_void_p(stream_ptr)), "set_stream")

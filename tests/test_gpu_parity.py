@@ -288,3 +288,25 @@ def test_grid_matches_oracle():
     for irot in (1, 2, 155, 616):
         assert lib.get_rot(irot) == g.angtab_deg[irot - 1]
     lib.kill()
+
+
+@pytest.mark.parametrize("n,kto,ldim", [(16, 8, 32), (48, 14, 48), (192, 30, 128), (308, 30, 200)])
+def test_fused_and_generic_kernels_agree(n, kto, ldim, monkeypatch):
+    """The compile-time-size kernel and the runtime-size kernel compute the same correlations
+    (different radix orders, so agreement is to FP32 rounding, not bitwise)."""
+    say(f"fused vs generic kernel n={n}")
+    pf, cps = make_case(seed=200 + n, n=n, kto=kto, ctf=True, nrefs=3, nptcls=3, ldim=ldim)
+    shifts = np.array([[0.0, 0.0], [0.5, -1.0]], np.float32)
+    lib = to_lib(pf, cps)
+    fused = lib.gencorrs_batch(np.arange(1, 4), None, shifts)
+    lib.kill()
+    monkeypatch.setenv("PFTCC_GENERIC", "1")
+    lib = to_lib(pf, cps)
+    gen = lib.gencorrs_batch(np.arange(1, 4), None, shifts)
+    lib.kill()
+    assert np.max(np.abs(fused - gen)) < 2e-6
+    for p in range(3):
+        for r in range(3):
+            for s in range(2):
+                ref = pf.gencorrs(r, p, tuple(shifts[s]) if s else None, method="fft")
+                assert peak_rel_err(fused[p, r, s], ref) < FP32_TOL
