@@ -51,44 +51,37 @@ def cpu_baseline(pftcc, w, shifts, nthreads, target_s=12.0):
 
     n, nk = w.pftsz, w.nk
     grid = O.make_grid(n, w.kfromto[0], w.kfromto[1], w.box)
-    nr = 4
-    ns = 4
-    sel_sh = shifts[np.linspace(0, len(shifts) - 1, ns).astype(int)]
-    shm = np.stack([O.shmat_8(grid, s) for s in sel_sh]).astype(np.complex64)
-    refs = np.stack([pftcc.get_ref_pft(r + 1, True) for r in range(nr)])
-
-    def sample(npt):
-        pt = np.stack([pftcc.get_ptcl_pft(i + 1) for i in range(npt)])
-        ctf = np.stack([pftcc.get_ctfmat(i + 1) for i in range(npt)])
-        sq = np.array([pftcc.get_sqsum_ptcl(i + 1) for i in range(npt)], np.float32)
-        return pt, ctf, sq
-
-    # calibrate on one particle per thread
-    pt, ctf, sq = sample(nthreads)
+    npt = 2 * nthreads                      # two particles per thread (static schedule)
+    pt = np.stack([pftcc.get_ptcl_pft(i + 1) for i in range(npt)])
+    ctf = np.stack([pftcc.get_ctfmat(i + 1) for i in range(npt)])
+    sq = np.array([pftcc.get_sqsum_ptcl(i + 1) for i in range(npt)], np.float32)
     pa, pb = cpu_port.memoize(pt, nthreads)
-    t0 = time.perf_counter()
-    cpu_port.search_fp32(refs, ctf, sq, pa, pb, shm, nk, nthreads)
-    dt = time.perf_counter() - t0
-    reps = max(1, int(target_s / max(dt, 1e-3)))
-    npt = nthreads * reps
-    pt, ctf, sq = sample(npt)
-    pa, pb = cpu_port.memoize(pt, nthreads)
-    t0 = time.perf_counter()
-    ccc, cir, cish, crot = cpu_port.search_fp32(refs, ctf, sq, pa, pb, shm, nk, nthreads)
-    dt = time.perf_counter() - t0
+
+    def run(nr, ns):
+        sel = shifts[np.linspace(0, len(shifts) - 1, ns).astype(int)]
+        shm = np.stack([O.shmat_8(grid, s) for s in sel]).astype(np.complex64)
+        refs = np.stack([pftcc.get_ref_pft(r + 1, True) for r in range(nr)])
+        t0 = time.perf_counter()
+        res = cpu_port.search_fp32(refs, ctf, sq, pa, pb, shm, nk, nthreads)
+        return time.perf_counter() - t0, sel, res
+
+    nr, ns = 2, 2
+    dt, _, _ = run(nr, ns)                   # calibration
+    scale = target_s / max(dt, 1e-3)
+    ns = int(min(len(shifts), max(2, ns * scale)))
+    scale = target_s / max(dt * ns / 2, 1e-3)
+    nr = int(min(w.nrefs, max(2, nr * scale)))
+    dt, sel, (ccc, cir, cish, crot) = run(nr, ns)
     evals = npt * nr * ns * 2 * n
-    # GPU on the same sample (same particles, references and shifts)
-    gcc, gir, gish, grot = pftcc.search(np.arange(1, npt + 1), np.arange(1, nr + 1), sel_sh)
+    gcc, gir, gish, grot = pftcc.search(np.arange(1, npt + 1), np.arange(1, nr + 1), sel)
     same = (gir == cir) & (gish == cish) & (grot == crot)
-    gap_ok = np.abs(gcc - ccc) < 1e-5
     return {
         "value": evals / dt, "unit": "evals/s", "cores": nthreads, "kind": "port",
-        "sample": f"{npt} ptcls x {nr} refs x {ns} shifts x {2 * n} rots of the same workload, "
-                  f"FP32 FFT route (r2c_n + c2c_n + c2r_2n per shell, own mixed-radix FFT; "
-                  f"FFTW absent), OpenMP over particles, {dt:.1f} s",
+        "sample": f"{npt} ptcls x {nr} refs x {ns} shifts x {2 * n} rots of the same workload "
+                  f"({dt:.1f} s): the reference's FP32 FFT route (r2c_n + c2c_n + c2r_2n per shell, "
+                  f"own mixed-radix FFT since FFTW is absent), OpenMP over particles",
         "gpu_assignment_agreement": float(np.mean(same)),
         "max_abs_best_cc_diff": float(np.max(np.abs(gcc - ccc))),
-        "agreement_where_cc_within_1e-5": float(np.mean(same[gap_ok])) if gap_ok.any() else None,
     }
 
 
@@ -181,7 +174,7 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "VALU fp32 (no MFMA; packed-fp32 disabled, issue peak 78.6)",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_corr_generic", "avg_launch_ms": avg_ms, "launches": launches,
+                     "kernel": "k_corr_fused<308,11,256>", "avg_launch_ms": avg_ms, "launches": launches,
                      "flops_per_pair": plain_flops_per_pair(pftsz, w.nk), "pairs_per_launch": pairs_launch,
                      "hbm_compulsory_GBs": bytes_launch / (avg_ms * 1e-3) / 1e9,
                      "hbm_frac": bytes_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},

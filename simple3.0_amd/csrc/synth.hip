@@ -132,14 +132,15 @@ int synth_refs(int nb, const float* blobs, int nrefs, const float* uvs, int nk, 
   float* d_uv = nullptr;
   if (hipMalloc(&d_b, sizeof(Blob) * nb) != hipSuccess) return 3;
   if (hipMalloc(&d_uv, sizeof(float) * 6 * nrefs) != hipSuccess) return 3;
-  hipMemcpy(d_b, blobs, sizeof(Blob) * nb, hipMemcpyHostToDevice);
-  hipMemcpy(d_uv, uvs, sizeof(float) * 6 * nrefs, hipMemcpyHostToDevice);
+  if (hipMemcpy(d_b, blobs, sizeof(Blob) * nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_uv, uvs, sizeof(float) * 6 * nrefs, hipMemcpyHostToDevice) != hipSuccess)
+    return 3;
   const size_t tot = (size_t)nrefs * nk * n;
   hipLaunchKernelGGL(k_refs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, d_b, nb, d_uv, nrefs, nk, n, kfrom,
                      box, (float2*)d_out);
   const hipError_t e = hipDeviceSynchronize();
-  hipFree(d_b);
-  hipFree(d_uv);
+  (void)hipFree(d_b);
+  (void)hipFree(d_uv);
   return e == hipSuccess ? 0 : 3;
 }
 
@@ -152,16 +153,17 @@ int synth_ptcls(int nb, const float* blobs, int nrefs, const float* uvs, int np,
   if (hipMalloc(&d_b, sizeof(Blob) * nb) != hipSuccess) return 3;
   if (hipMalloc(&d_uv, sizeof(float) * 6 * nrefs) != hipSuccess) return 3;
   if (hipMalloc(&d_p, sizeof(PtclParam) * np) != hipSuccess) return 3;
-  hipMemcpy(d_b, blobs, sizeof(Blob) * nb, hipMemcpyHostToDevice);
-  hipMemcpy(d_uv, uvs, sizeof(float) * 6 * nrefs, hipMemcpyHostToDevice);
-  hipMemcpy(d_p, prm, sizeof(PtclParam) * np, hipMemcpyHostToDevice);
+  if (hipMemcpy(d_b, blobs, sizeof(Blob) * nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_uv, uvs, sizeof(float) * 6 * nrefs, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_p, prm, sizeof(PtclParam) * np, hipMemcpyHostToDevice) != hipSuccess)
+    return 3;
   const size_t tot = (size_t)np * nk * n;
   hipLaunchKernelGGL(k_ptcls, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, d_b, nb, d_uv, d_p, np, nk, n,
                      kfrom, box, with_ctf, (float2*)d_out);
   const hipError_t e = hipDeviceSynchronize();
-  hipFree(d_b);
-  hipFree(d_uv);
-  hipFree(d_p);
+  (void)hipFree(d_b);
+  (void)hipFree(d_uv);
+  (void)hipFree(d_p);
   return e == hipSuccess ? 0 : 3;
 }
 
@@ -189,9 +191,9 @@ void* synth_event_create() {
 }
 int synth_event_record(void* e, void* stream) { return hipEventRecord((hipEvent_t)e, (hipStream_t)stream) == hipSuccess ? 0 : 3; }
 float synth_event_elapsed_ms(void* a, void* b) {
-  hipEventSynchronize((hipEvent_t)b);
   float ms = -1.f;
-  hipEventElapsedTime(&ms, (hipEvent_t)a, (hipEvent_t)b);
+  if (hipEventSynchronize((hipEvent_t)b) != hipSuccess) return ms;
+  if (hipEventElapsedTime(&ms, (hipEvent_t)a, (hipEvent_t)b) != hipSuccess) return -1.f;
   return ms;
 }
 int synth_device_sync() { return hipDeviceSynchronize() == hipSuccess ? 0 : 3; }

@@ -1,0 +1,186 @@
+! pftcc_fortran_driver - drives module simple_polarft_corrcalc the way SIMPLE's search code does.
+!
+!   pftcc_fortran_driver parity <in.bin> <out.bin>
+!       reads references, particles, CTF parameters, shifts and single-rotation queries, runs
+!       new / set_ref_pft / set_ptcl_pft / create_polar_absctfmats / memoize_ffts / gencorrs /
+!       search / gencorr_for_rot_8 / gencorr_grad_for_rot_8 / specscore and writes the results
+!       (raw little-endian; layout in tests/test_gpu_fortran.py).
+!   pftcc_fortran_driver shiftsrch
+!       the invariant of production/tests/test_shiftsrch (simple_test_shiftsrch.f90:40-118):
+!       8 (ref, ptcl) pairs, the particle carrying the reference's phase for a +-1 px shift in
+!       8 directions; a grid search over shifts in [-2,2]^2 with gencorrs must recover it.
+!       Exits non-zero if any shift is not recovered.
+program pftcc_fortran_driver
+use, intrinsic :: iso_c_binding
+use simple_polarft_corrcalc
+implicit none
+character(len=512) :: mode, fin, fout
+call get_command_argument(1, mode)
+select case(trim(mode))
+    case('parity')
+        call get_command_argument(2, fin)
+        call get_command_argument(3, fout)
+        call run_parity(trim(fin), trim(fout))
+    case('shiftsrch')
+        call run_shiftsrch
+    case default
+        write(*,*) 'usage: pftcc_fortran_driver parity <in> <out> | shiftsrch'
+        error stop 2
+end select
+
+contains
+
+    subroutine run_parity( fin, fout )
+        character(len=*), intent(in) :: fin, fout
+        type(polarft_corrcalc) :: pftcc
+        type(pftcc_params)     :: p
+        integer(c_int32_t)     :: hdr(16)
+        complex(c_float_complex), allocatable :: refse(:,:,:), refso(:,:,:), ptcls(:,:,:)
+        type(ctfparams), allocatable :: cps(:)
+        real(c_float),  allocatable :: shifts(:,:), cc(:), ccall(:,:,:,:), best_cc(:), scores(:)
+        integer,        allocatable :: best_iref(:), best_irot(:), best_ishift(:), ids(:)
+        integer(c_int32_t), allocatable :: qref(:), qptcl(:), qrot(:)
+        real(c_double), allocatable :: qsh(:,:), f8(:), g8(:,:)
+        integer :: nrefs, nptcls, n, kfrom, kto, kstop, box, withctf, nsh, nq, pfrom, nk, nrots
+        integer :: u, i, r, s, q
+        open(newunit=u, file=fin, access='stream', form='unformatted', status='old')
+        read(u) hdr
+        nrefs = hdr(1); nptcls = hdr(2); n = hdr(3); kfrom = hdr(4); kto = hdr(5); kstop = hdr(6)
+        box = hdr(7); withctf = hdr(8); nsh = hdr(9); nq = hdr(10); pfrom = hdr(11)
+        nk = kto - kfrom + 1
+        nrots = 2 * n
+        allocate(refse(n, kfrom:kto, nrefs), refso(n, kfrom:kto, nrefs), ptcls(n, kfrom:kto, nptcls))
+        read(u) refse
+        read(u) refso
+        read(u) ptcls
+        allocate(cps(nptcls))
+        if( withctf /= 0 ) read(u) cps
+        allocate(shifts(2, max(nsh,1)))
+        if( nsh > 0 ) read(u) shifts(:, 1:nsh)
+        allocate(qref(max(nq,1)), qptcl(max(nq,1)), qrot(max(nq,1)), qsh(2, max(nq,1)))
+        if( nq > 0 )then
+            read(u) qref(1:nq)
+            read(u) qptcl(1:nq)
+            read(u) qrot(1:nq)
+            read(u) qsh(:, 1:nq)
+        endif
+        close(u)
+        ! the calls SIMPLE makes (preppftcc4align / build_pftcc_batch_particles / srch)
+        p%boxmatch = box
+        p%pftsz    = n
+        p%kfromto  = [kfrom, kto]
+        p%kstop    = kstop
+        p%with_ctf = withctf /= 0
+        call pftcc%new(nrefs, [pfrom, pfrom + nptcls - 1], p)
+        do r = 1, nrefs
+            call pftcc%set_ref_pft(r, refse(:,:,r), .true.)
+            call pftcc%set_ref_pft(r, refso(:,:,r), .false.)
+        end do
+        do i = 1, nptcls
+            call pftcc%set_ptcl_pft(pfrom + i - 1, ptcls(:,:,i))
+        end do
+        if( withctf /= 0 ) call pftcc%create_polar_absctfmats(cps)
+        call pftcc%memoize_ffts
+        allocate(cc(nrots), ccall(nrots, 0:nsh, nrefs, nptcls))
+        do i = 1, nptcls
+            do r = 1, nrefs
+                call pftcc_glob%gencorrs(r, pfrom + i - 1, cc)
+                ccall(:, 0, r, i) = cc
+                do s = 1, nsh
+                    call pftcc_glob%gencorrs(r, pfrom + i - 1, shifts(:, s), cc)
+                    ccall(:, s, r, i) = cc
+                end do
+            end do
+        end do
+        allocate(best_cc(nptcls), best_iref(nptcls), best_irot(nptcls), best_ishift(nptcls), ids(nptcls))
+        ids = [(pfrom + i - 1, i = 1, nptcls)]
+        if( nsh > 0 )then
+            call pftcc%search(ids, best_cc, best_iref, best_irot, shifts=shifts(:, 1:nsh), best_ishift=best_ishift)
+        else
+            call pftcc%search(ids, best_cc, best_iref, best_irot)
+            best_ishift = 0
+        endif
+        allocate(f8(max(nq,1)), g8(2, max(nq,1)), scores(max(nq,1)))
+        do q = 1, nq
+            f8(q) = pftcc%gencorr_for_rot_8(qref(q), qptcl(q), qsh(:,q), qrot(q))
+            call pftcc%gencorr_grad_for_rot_8(qref(q), qptcl(q), qsh(:,q), qrot(q), f8(q), g8(:,q))
+            scores(q) = pftcc%specscore(qref(q), qptcl(q), qrot(q))
+        end do
+        call pftcc%kill
+        open(newunit=u, file=fout, access='stream', form='unformatted', status='replace')
+        write(u) ccall
+        write(u) best_cc
+        write(u) int(best_iref, c_int32_t)
+        write(u) int(best_ishift, c_int32_t)
+        write(u) int(best_irot, c_int32_t)
+        if( nq > 0 )then
+            write(u) f8(1:nq)
+            write(u) g8(:, 1:nq)
+            write(u) scores(1:nq)
+        endif
+        close(u)
+        write(*,'(a,i0,a,i0,a,i0,a)') 'parity run done: ', nptcls, ' ptcls x ', nrefs, ' refs x ', nsh, ' shifts'
+    end subroutine run_parity
+
+    subroutine run_shiftsrch
+        integer, parameter :: NPAIR = 8, BOX = 64, KFROM = 2, KTO = 20
+        real,    parameter :: SHMAG = 1.0, PI = acos(-1.0)
+        type(polarft_corrcalc) :: pftcc
+        type(pftcc_params)     :: p
+        complex(c_float_complex), allocatable :: ref(:,:), ptcl(:,:)
+        real(c_float), allocatable :: corrs(:)
+        real    :: dirs(2, NPAIR), corr, corrmax, xy(2), ph, rnd(2)
+        integer :: i, j, k, xsh, ysh, xbest, ybest, nbad, n, seed(64)
+        dirs = reshape([SHMAG,0., 0.,SHMAG, -SHMAG,0., 0.,-SHMAG, SHMAG,SHMAG, -SHMAG,-SHMAG, &
+            &-SHMAG,SHMAG, SHMAG,-SHMAG], [2, NPAIR])
+        p%boxmatch = BOX
+        p%ring2    = 30
+        p%kfromto  = [KFROM, KTO]
+        p%with_ctf = .false.
+        call pftcc%new(NPAIR, [1, NPAIR], p)
+        n = pftcc%get_pftsz()
+        allocate(ref(n, KFROM:KTO), ptcl(n, KFROM:KTO), corrs(pftcc%get_nrots()))
+        seed = 42
+        call random_seed(put=seed)
+        do i = 1, NPAIR
+            do k = KFROM, KTO
+                do j = 1, n
+                    call random_number(rnd)
+                    ref(j,k) = cmplx(rnd(1) - 0.5, rnd(2) - 0.5) / real(k)
+                    ! the particle carries the phase exp(i pi (x sx + y sy) / (box/2)) of the shift
+                    xy = pftcc%get_coord(j, k)
+                    ph = PI * (xy(1) * dirs(1,i) + xy(2) * dirs(2,i)) / real(BOX / 2)
+                    ptcl(j,k) = ref(j,k) * cmplx(cos(ph), sin(ph))
+                end do
+            end do
+            call pftcc%set_ref_pft(i, ref, .true.)
+            call pftcc%set_ptcl_pft(i, ptcl)
+        end do
+        call pftcc%memoize_ffts
+        nbad = 0
+        do i = 1, NPAIR
+            corrmax = -1.
+            do xsh = -2, 2
+                do ysh = -2, 2
+                    call pftcc%gencorrs(i, i, real([xsh, ysh]), corrs)
+                    corr = maxval(corrs)
+                    if( corr > corrmax )then
+                        corrmax = corr
+                        xbest   = xsh
+                        ybest   = ysh
+                    endif
+                end do
+            end do
+            write(*,'(a,i0,a,2i3,a,2f5.1,a,f9.6)') 'pair ', i, ' best shift', xbest, ybest, ' applied', &
+                &dirs(:,i), ' corr', corrmax
+            if( xbest /= nint(dirs(1,i)) .or. ybest /= nint(dirs(2,i)) .or. corrmax < 0.9999 ) nbad = nbad + 1
+        end do
+        call pftcc%kill
+        if( nbad > 0 )then
+            write(*,'(a,i0,a)') 'shiftsrch: ', nbad, ' shifts not recovered'
+            error stop 1
+        endif
+        write(*,'(a)') 'shiftsrch: all shifts recovered'
+    end subroutine run_shiftsrch
+
+end program pftcc_fortran_driver
