@@ -193,6 +193,24 @@ int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs,
                          const int32_t* d_iptcls, const double* d_shvecs, const int32_t* d_irots,
                          double* d_f, double* d_grad);
 
+/* -------------------------------------------------------- precision (MFMA) form -- */
+/* Exhaustive search without shifts (the 3D-refinement use of gencorrs_1) computed as one dense
+ * complex contraction over shells per angular frequency m on MFMA, followed by the inverse FFT
+ * over m.  Exact for ring-constant |CTF| (ctf = no or no astigmatism); with astigmatism the
+ * particle's |CTF| is replaced by its ring mean per shell (the reference applies the particle's
+ * angle-dependent |CTF| in the reference frame, :982, which no per-m contraction can express) and
+ * candidates should be rescored with pftcc_grad_batch / pftcc_gencorr_for_rot_8.
+ * prec: PFTCC_PREC_SPLIT_BF16 (hi+lo split, FP32 accumulation) or PFTCC_PREC_BF16.
+ * Denominators sqrt(sum_k cbar_k^2 S_ref,k * S_ptcl) are formed in FP64.  No counterpart
+ * entry point in the reference; it replaces the same greedy loop as pftcc_search. */
+#define PFTCC_PREC_BF16 1
+#define PFTCC_PREC_SPLIT_BF16 3
+int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
+                      int32_t prec, float* best_cc, int32_t* best_iref, int32_t* best_irot);
+/* cc_out[np][nr][nrots] of the same computation (for verification and small batches). */
+int pftcc_gencorrs_mfma_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
+                              int32_t prec, float* cc_out);
+
 /* ------------------------------------------------------------------- instrumentation -- */
 /* No reference counterpart.  When enabled, each launch of the correlation kernels (the plain
  * FP32 search/gencorrs kernel) is bracketed by HIP events on the handle's stream;
@@ -200,6 +218,9 @@ int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs,
  * last reset. */
 int pftcc_set_timing(pftcc_t* h, int32_t enable);
 int pftcc_get_timing(pftcc_t* h, double* total_ms, int64_t* launches);
+/* Per kernel family: 0 plain correlation, 1 MFMA contraction, 2 inverse FFT of the MFMA form,
+ * 3 operand packing of the MFMA form. */
+int pftcc_get_timing_channel(pftcc_t* h, int32_t channel, double* total_ms, int64_t* launches);
 
 /* ------------------------------------------------------------------------- utilities -- */
 /* magic_pftsz (simple_magic_boxes.f90:25-37). */

@@ -1,0 +1,419 @@
+// mfma_search.hpp - the precision form: per-angular-frequency contraction on MFMA.
+//
+// With a ring-constant |CTF| (ctf = no, no astigmatism, or its ring mean c_p,k), the rotational
+// correlation of every (reference r, particle p) pair is
+//   Z[r,p,m] = sum_k conj(V_r[k,m]) * c_p,k * W_p[k,m],   m = 0..n,
+//   cc[r,p,rho] = irfft_2n(Z[r,p,:])[rho] / (4n) / sqrt(sum_k c_p,k^2 S_r[k] * S_p),
+// with V_r, W_p the length-2n real FFTs of the rings (the particle memo, and the same transform of
+// the references).  For each m this is one dense complex GEMM over the shells, which runs on
+// MFMA in split BF16: x = hi + lo (hi = bf16(x), lo = bf16(x - hi)), products hi*hi + hi*lo +
+// lo*hi accumulated in FP32 (the lo*lo term, 2^-16 relative, is dropped).  Real formulation with
+// k' = 2k + {re, im}: Zr = A1 . B, Zi = A2 . B, A1 = (Re V, Im V), A2 = (-Im V, Re V), B = (Re W', Im W').
+// The spectra Z (FP32) go through HBM to the inverse-FFT kernel, which normalises with FP64
+// denominators and reduces each particle to its best (reference, rotation).  Z is stored
+// particle-group major, [r][p / 16][m][p % 16], so that one inverse-FFT work-group (one reference,
+// 16 particles) reads one contiguous block of (n + 1) x 16 complex values.
+#pragma once
+#include "corr_fused.hpp"
+
+namespace pfk {
+
+constexpr int ZG = 16;   // particles per Z group (the MFMA tile's column count)
+
+__host__ __device__ __forceinline__ size_t zidx(int r, int p, int m, int n, int cols_pad) {
+  return ((size_t)(r * (cols_pad / ZG) + p / ZG) * (n + 1) + m) * ZG + (p % ZG);
+}
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+// Ring mean of |CTF| per (particle, shell) and its square in FP64.
+__global__ void k_ring_mean_ctf(const float* __restrict__ ctf, int np, int nk, int n, float* __restrict__ cbar,
+                                double* __restrict__ cbar2) {
+  const int item = blockIdx.x * blockDim.y + threadIdx.y;   // (p, kk)
+  if (item >= np * nk) return;
+  const float* c = ctf + (size_t)item * n;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) s += c[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (threadIdx.x == 0) {
+    const double m = s / n;
+    cbar[item] = (float)m;
+    cbar2[item] = m * m;
+  }
+}
+
+// FP64 per-shell sums of squares of the reference rings: sr[r][kk], kk < kcount.
+__global__ void k_ring_sq64(const float2* __restrict__ rings, int count, int nk, int n, int kcount,
+                            double* __restrict__ out) {
+  const int item = blockIdx.x * blockDim.y + threadIdx.y;   // (r, kk)
+  if (item >= count * kcount) return;
+  const int r = item / kcount, kk = item % kcount;
+  const float2* v = rings + ((size_t)r * nk + kk) * n;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) s += (double)v[j].x * v[j].x + (double)v[j].y * v[j].y;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (threadIdx.x == 0) out[item] = s;
+}
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  const __bf16 h = (__bf16)x;   // round to nearest even
+  return *reinterpret_cast<const unsigned short*>(&h);
+}
+__device__ __forceinline__ float bf16_float(unsigned short b) {
+  return __uint_as_float((unsigned int)b << 16);
+}
+
+// Packs spectra spec[row][kk][m] (float2, m in [0, n]) times an optional per-(row, kk) scale into
+// split-BF16 operand tiles hi/lo[m][row0 + row][k'] (k' = 2 kk + {0: re, 1: im}, zero past 2 kcorr).
+// One thread per (m, row, kk-pair); rows beyond count and k' beyond 2 kcorr are written as zero.
+__global__ void k_pack_split(const float2* __restrict__ spec, const float* __restrict__ scale, int count, int rows_pad,
+                             int nk, int kcorr, int n, int KP, unsigned int* __restrict__ hi,
+                             unsigned int* __restrict__ lo) {
+  const size_t KW = KP / 2;                                  // 32-bit words (one complex) per row
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t tot = (size_t)(n + 1) * rows_pad * KW;
+  if (idx >= tot) return;
+  const int kk = (int)(idx % KW);
+  const int row = (int)((idx / KW) % rows_pad);
+  const int m = (int)(idx / (KW * rows_pad));
+  float re = 0.f, im = 0.f;
+  if (row < count && kk < kcorr) {
+    const float2 v = spec[((size_t)row * nk + kk) * (n + 1) + m];
+    const float s = scale ? scale[(size_t)row * nk + kk] : 1.0f;
+    re = v.x * s;
+    im = v.y * s;
+  }
+  const unsigned short hr = bf16_bits(re), hiim = bf16_bits(im);
+  const unsigned short lr = bf16_bits(re - bf16_float(hr)), li = bf16_bits(im - bf16_float(hiim));
+  hi[idx] = (unsigned int)hr | ((unsigned int)hiim << 16);
+  lo[idx] = (unsigned int)lr | ((unsigned int)li << 16);
+}
+
+// A2 fragment from A1: per complex (re, im) word -> (-im, re).
+__device__ __forceinline__ bf16x8 rot_neg(bf16x8 a) {
+  union U {
+    bf16x8 v;
+    unsigned int w[4];
+  } u;
+  u.v = a;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) u.w[i] = ((u.w[i] >> 16) | (u.w[i] << 16)) ^ 0x00008000u;
+  return u.v;
+}
+
+// Z[r][p][m] = sum_k' A[m][r][k'] (x) B[m][p][k'] for one 64 x 64 (ref x particle) tile per wave.
+// split = 3: hi.hi + hi.lo + lo.hi; split = 1: hi.hi only (plain BF16, for error studies).
+template <int SPLIT>
+__global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
+                                              const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
+                                              int rows_pad, int cols_pad, int KP, float2* __restrict__ Z) {
+  const int l = threadIdx.x;
+  const int p0 = blockIdx.x * 64, r0 = blockIdx.y * 64, m = blockIdx.z;
+  const int n = gridDim.z - 1;
+  const int KV = KP / 8;                                  // bf16x8 vectors per row
+  const size_t abase = ((size_t)m * rows_pad + r0) * KV;
+  const size_t bbase = ((size_t)m * cols_pad + p0) * KV;
+  f32x4 zr[4][4], zi[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      zr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      zi[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  const int lr = l % 16, lk = l / 16;
+  for (int ks = 0; ks < KV; ks += 4) {                    // 32 k' per step, lane takes 8 of them
+    bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const size_t ao = abase + (size_t)(i * 16 + lr) * KV + ks + lk;
+      ah[i] = Ahi[ao];
+      if (SPLIT == 3) al[i] = Alo[ao];
+      const size_t bo = bbase + (size_t)(i * 16 + lr) * KV + ks + lk;
+      bh[i] = Bhi[bo];
+      if (SPLIT == 3) bl[i] = Blo[bo];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bf16x8 a2h = rot_neg(ah[i]);
+      bf16x8 a2l;
+      if (SPLIT == 3) a2l = rot_neg(al[i]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (SPLIT == 3) {
+          zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], zr[i][j], 0, 0, 0);
+          zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], zr[i][j], 0, 0, 0);
+          zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l, bh[j], zi[i][j], 0, 0, 0);
+          zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bl[j], zi[i][j], 0, 0, 0);
+        }
+        zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], zr[i][j], 0, 0, 0);
+        zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bh[j], zi[i][j], 0, 0, 0);
+      }
+    }
+  }
+  // D layout: lane l, element e -> row 4 (l/16) + e, column l % 16
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = r0 + i * 16 + 4 * lk + e;
+        const int p = p0 + j * 16 + lr;
+        Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e], zi[i][j][e]);
+      }
+}
+
+struct ZifftArgs {
+  int n, nrefs, np, rows_pad, cols_pad, kcorr;
+  const float2* Z;             // zidx layout
+  const double* sr;            // [nrefs][kcorr] reference shell sums of squares
+  const double* cbar2;         // [np_chunk][nk] ring-mean |CTF|^2 of the chunk's particles (or null)
+  int nk;
+  const double* sp;            // [np_chunk] particle sums of squares (FP64)
+  const int* rlist;            // 0-based reference ids of the rows (null = row index)
+  const int* outpos;           // [np_chunk] output position of each chunk particle
+  float* cc_out;               // [npos][nrefs][2n] or null
+  unsigned long long* best;    // [npos] or null
+  const float2* tw2g;
+  FftPlan plan;
+};
+
+// Inverse real FFT, FP64-normalised correlations and argmax for F pairs (one reference row,
+// F consecutive particles) per work-group.  Dynamic LDS: X[F][n+1], buf A/B [F][n], tw2[2n].
+template <int F>
+__global__ void __launch_bounds__(256) k_zifft(ZifftArgs a) {
+  extern __shared__ float2 smem[];
+  const int n = a.n, tid = threadIdx.x, nt = blockDim.x;
+  float2* X = smem;                        // [F][n+1], reused as the FFT's second buffer
+  float2* bufA = X + F * (n + 1);
+  float2* bufB = X;
+  float2* tw2 = bufA + F * n;
+  __shared__ double den[F];
+  __shared__ unsigned long long red[256];
+  const int r = blockIdx.y;               // reference row
+  const int p0 = blockIdx.x * F;
+  for (int i = tid; i < 2 * n; i += nt) tw2[i] = a.tw2g[i];
+  // gather Z[m][r][p0 .. p0+F-1]
+  for (int idx = tid; idx < (n + 1) * F; idx += nt) {
+    const int m = idx / F, f = idx % F;
+    X[f * (n + 1) + m] = a.Z[zidx(r, p0 + f, m, n, a.cols_pad)];
+  }
+  if (tid < F) {
+    const int p = p0 + tid;
+    double d = 0.0;
+    if (p < a.np) {
+      double sq = 0.0;
+      for (int kk = 0; kk < a.kcorr; ++kk)
+        sq += a.sr[(size_t)r * a.kcorr + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
+      d = sqrt(sq * a.sp[p]);
+    }
+    den[tid] = d;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < F * n; idx += nt) {
+    const int f = idx / n, m = idx % n;
+    float2 x = X[f * (n + 1) + m];
+    float2 xn = X[f * (n + 1) + (n - m)];
+    if (m == 0) {
+      x.y = 0.f;
+      xn.y = 0.f;
+    }
+    bufA[f * n + m] = unsplit_bin_conj(x, xn, cconj(tw2[m]));
+  }
+  __syncthreads();
+  float2* res = fft_forward_oop(bufA, bufB, F, n, n, a.plan, tw2, tid, nt);
+  const double inv4n = 1.0 / (4.0 * n);
+  unsigned long long bestkey[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) bestkey[f] = 0ull;
+  const int iref = a.rlist ? a.rlist[r] : r;
+  for (int idx = tid; idx < F * n; idx += nt) {
+    const int f = idx / n, j = idx % n;
+    const int p = p0 + f;
+    if (p >= a.np || r >= a.nrefs) continue;
+    const float2 y = res[f * n + j];
+    const double sc = inv4n / den[f];
+    const float x0 = (float)(y.x * sc), x1 = (float)(-y.y * sc);
+    if (a.cc_out) {
+      float2* o = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * n));
+      o[j] = make_float2(x0, x1);
+    }
+    if (a.best) {
+      const unsigned int lin0 = (unsigned int)iref * (unsigned int)(2 * n) + 2u * j;
+      const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
+      const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
+#pragma unroll
+      for (int g = 0; g < F; ++g)
+        if (g == f) {
+          bestkey[g] = k0 > bestkey[g] ? k0 : bestkey[g];
+          bestkey[g] = k1 > bestkey[g] ? k1 : bestkey[g];
+        }
+    }
+  }
+  if (a.best) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      red[tid] = bestkey[f];
+      __syncthreads();
+      for (int w = nt / 2; w > 0; w >>= 1) {
+        if (tid < w && red[tid + w] > red[tid]) red[tid] = red[tid + w];
+        __syncthreads();
+      }
+      if (tid == 0 && p0 + f < a.np && r < a.nrefs) atomicMax(&a.best[a.outpos[p0 + f]], red[0]);
+      __syncthreads();
+    }
+  }
+}
+
+// Compile-time-size inverse stage: F particles (of one Z group) x one reference per work-group.
+//  * Z[m] and Z[N-m] of a pair are read together and turned into the conjugated inverse-FFT
+//    inputs Y[m], Y[N-m] on the way into LDS (reads are contiguous over f for each m);
+//  * all passes run register-staged in LDS (lds_passes);
+//  * the FP64 denominator sqrt(sum_k cbar_k^2 S_r,k * S_p) is split over NT/F threads per pair;
+//  * the best key of each pair is reduced by NT/F lanes of one wavefront with shuffles.
+template <int N, int F, int NT>
+__global__ void __launch_bounds__(NT) k_zifft_ct(ZifftArgs a) {
+  static_assert(ZG % F == 0, "F divides the Z group");
+  constexpr int TPF = NT / F;            // threads per pair
+  static_assert(TPF * F == NT && TPF <= 64 && (TPF & (TPF - 1)) == 0, "TPF is a power of two <= 64");
+  constexpr int H = N / 2 + 1;           // mirrored bin pairs (m, N - m), m = 0 .. N/2
+  extern __shared__ float2 smem[];
+  float2* buf = smem;                    // [F][N]
+  float2* tw2 = buf + F * N;             // [2N]
+  __shared__ double den[F];
+  const int tid = threadIdx.x;
+  const int r = blockIdx.y;
+  const int p0 = blockIdx.x * F;
+  for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
+  // FP64 denominators: TPF threads per pair, shuffle-reduced
+  {
+    const int f = tid / TPF, u = tid % TPF;
+    const int p = p0 + f;
+    double s = 0.0;
+    if (p < a.np)
+      for (int kk = u; kk < a.kcorr; kk += TPF)
+        s += a.sr[(size_t)r * a.kcorr + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
+#pragma unroll
+    for (int o = TPF / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, TPF);
+    if (u == 0) den[f] = p < a.np ? sqrt(s * a.sp[p]) : 1.0;
+  }
+  __syncthreads();   // tw2 complete
+  const float2* Zb = a.Z + zidx(r, p0, 0, N, a.cols_pad);   // group base + p0 % ZG
+  for (int t = tid; t < H * F; t += NT) {
+    const int m = t / F, f = t % F;
+    float2 X = Zb[(size_t)m * ZG + f];
+    float2 Xn = Zb[(size_t)(N - m) * ZG + f];
+    if (m == 0) {
+      X.y = 0.f;
+      Xn.y = 0.f;
+    }
+    buf[f * N + m] = unsplit_bin_conj(X, Xn, cconj(tw2[m]));
+    if (m != 0 && 2 * m != N) buf[f * N + (N - m)] = unsplit_bin_conj(Xn, X, cconj(tw2[N - m]));
+  }
+  __syncthreads();
+  lds_passes<N, F, NT, 0, Fp<N>::np>(buf, tw2, tid);
+  const double inv4n = 1.0 / (4.0 * N);
+  if (a.cc_out) {
+    for (int idx = tid; idx < F * N; idx += NT) {
+      const int f = idx / N, j = idx % N;
+      const int p = p0 + f;
+      if (p >= a.np || r >= a.nrefs) continue;
+      const float2 y = buf[idx];
+      const double sc = inv4n / den[f];
+      float2* o = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * N));
+      o[j] = make_float2((float)(y.x * sc), (float)(-y.y * sc));
+    }
+  }
+  if (a.best) {
+    const int f = tid / TPF, u = tid % TPF;
+    const int p = p0 + f;
+    const int iref = a.rlist ? a.rlist[r] : r;
+    const double sc = inv4n / den[f];
+    unsigned long long bk = 0ull;
+    for (int j = u; j < N; j += TPF) {
+      const float2 y = buf[f * N + j];
+      const float x0 = (float)(y.x * sc), x1 = (float)(-y.y * sc);
+      const unsigned int lin0 = (unsigned int)iref * (unsigned int)(2 * N) + 2u * j;
+      const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
+      const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
+      bk = k0 > bk ? k0 : bk;
+      bk = k1 > bk ? k1 : bk;
+    }
+#pragma unroll
+    for (int o = TPF / 2; o > 0; o >>= 1) {
+      const unsigned long long v = __shfl_xor(bk, o, TPF);
+      bk = v > bk ? v : bk;
+    }
+    if (u == 0 && p < a.np && r < a.nrefs) atomicMax(&a.best[a.outpos[p]], bk);
+  }
+}
+
+template <int N>
+struct ZifftCt {
+  static constexpr int NT = 256;
+  static constexpr int F = (sizeof(float2) * (16 * N + 2 * N) <= 65536) ? 16
+                         : (sizeof(float2) * (8 * N + 2 * N) <= 65536) ? 8 : 4;
+  static constexpr size_t lds = sizeof(float2) * ((size_t)F * N + 2 * N);
+};
+
+// Gathers rows of a [N][nk][len] float2 array by a list into [count][nk][len].
+__global__ void k_gather_rows(const float2* __restrict__ src, const int* __restrict__ list, int count, size_t rowlen,
+                              float2* __restrict__ dst) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)count * rowlen) return;
+  const size_t row = idx / rowlen, e = idx % rowlen;
+  dst[idx] = src[(size_t)list[row] * rowlen + e];
+}
+
+// Ring mean of |CTF| for listed particles: cbar [count][nk] (float), cbar2 (double).
+__global__ void k_ring_mean_ctf_list(const float* __restrict__ ctf, const int* __restrict__ list, int count, int nk,
+                                     int n, float* __restrict__ cbar, double* __restrict__ cbar2) {
+  const int item = blockIdx.x * blockDim.y + threadIdx.y;   // (p, kk)
+  if (item >= count * nk) return;
+  const int p = item / nk, kk = item % nk;
+  const float* c = ctf + ((size_t)list[p] * nk + kk) * n;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) s += c[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (threadIdx.x == 0) {
+    const double m = s / n;
+    cbar[item] = (float)m;
+    cbar2[item] = m * m;
+  }
+}
+
+// FP64 sum |p|^2 over shells kk < kcount for listed particles (one wave per particle).
+__global__ void k_ring_sq64_list(const float2* __restrict__ rings, const int* __restrict__ list, int count, int nk,
+                                 int n, int kcount, double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.y + threadIdx.y;
+  if (p >= count) return;
+  const float2* v = rings + (size_t)list[p] * nk * n;
+  double s = 0.0;
+  for (int e = threadIdx.x; e < kcount * n; e += blockDim.x) s += (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (threadIdx.x == 0) out[p] = s;
+}
+
+// Decodes keys whose low word encodes iref * nrots + rho.
+__global__ void k_best_decode_rot(const unsigned long long* __restrict__ best, int np, int nrots,
+                                  float* __restrict__ cc, int* __restrict__ iref, int* __restrict__ irot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const unsigned long long k = best[i];
+  if (k == 0ull) {
+    cc[i] = __uint_as_float(0x7FC00000u);
+    iref[i] = 0;
+    irot[i] = 0;
+    return;
+  }
+  const unsigned int hi = (unsigned int)(k >> 32);
+  const unsigned int bits = (hi & 0x80000000u) ? (hi & 0x7FFFFFFFu) : ~hi;
+  const unsigned int lin = 0xFFFFFFFFu - (unsigned int)(k & 0xFFFFFFFFull);
+  cc[i] = __uint_as_float(bits);
+  iref[i] = (int)(lin / (unsigned)nrots) + 1;
+  irot[i] = (int)(lin % (unsigned)nrots) + 1;
+}
+
+}  // namespace pfk

@@ -20,6 +20,7 @@
 #include "pftcc.h"
 #include "pftcc_kernels.hpp"
 #include "corr_fused.hpp"
+#include "mfma_search.hpp"
 
 using namespace pfk;
 
@@ -112,12 +113,21 @@ struct pftcc {
   bool memo_valid = false, dptcls_valid = false, refnorm_valid = false, have_sigma2 = false;
   bool ctf_ready = false;
   bool use_fused = true;            // PFTCC_GENERIC=1 forces the runtime-size kernel
+  // precision form: cached split-BF16 reference operands per eo class
+  struct APack {
+    bool valid = false;
+    std::vector<int> refs;            // 0-based reference ids of the rows
+    int rows_pad = 0, KP = 0, kcorr = 0;
+    unsigned int *hi = nullptr, *lo = nullptr;
+    double* sr = nullptr;             // [nr][kcorr]
+  } apack[2];
   // kernel timing (pftcc_set_timing)
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  std::vector<int> event_ch;
   size_t nev = 0;
-  double timed_ms = 0.0;
-  int64_t timed_launches = 0;
+  double timed_ms[4] = {0, 0, 0, 0};
+  int64_t timed_launches[4] = {0, 0, 0, 0};
   // scratch arena (stream ordered; calls on a handle are serialised by mtx)
   char* d_scratch = nullptr;
   size_t scratch_cap = 0, scratch_off = 0;
@@ -390,6 +400,9 @@ int pick_F(int n, int npairs) {
   return F;
 }
 
+int timing_begin(pftcc* h, int ch, hipEvent_t* e1);
+int timing_end(pftcc* h, hipEvent_t e1);
+
 // Compile-time ring lengths of the fused kernel: the benchmark shapes (192, 308, 400), common
 // box sizes (64 -> 100, 160 -> 240, 320 -> 500, 384 -> 600) and small sizes used in tests.
 template <int N>
@@ -460,23 +473,34 @@ int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist
   const size_t blocks = (size_t)np * a.ntiles;
   if (blocks == 0) return PFTCC_OK;
   if (blocks > 0x7fffffffu) return fail(h, PFTCC_E_ARG, "search too large for one launch (%zu blocks)", blocks);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (h->timing) {
-    if (h->nev == h->events.size()) {
-      hipEvent_t x, y;
-      HIPCHK(h, hipEventCreate(&x));
-      HIPCHK(h, hipEventCreate(&y));
-      h->events.emplace_back(x, y);
-    }
-    e0 = h->events[h->nev].first;
-    e1 = h->events[h->nev].second;
-    ++h->nev;
-    HIPCHK(h, hipEventRecord(e0, h->stream));
-  }
+  hipEvent_t e1 = nullptr;
+  int rc;
+  if ((rc = timing_begin(h, 0, &e1))) return rc;
   if (Ff > 0) fused_go(h->n, (unsigned)blocks, h->stream, a);
   else hipLaunchKernelGGL(k_corr_generic, dim3((unsigned)blocks), dim3(256), lds, h->stream, a);
   HIPCHK(h, hipGetLastError());
-  if (h->timing) HIPCHK(h, hipEventRecord(e1, h->stream));
+  return timing_end(h, e1);
+}
+
+int timing_begin(pftcc* h, int ch, hipEvent_t* e1) {
+  *e1 = nullptr;
+  if (!h->timing) return PFTCC_OK;
+  if (h->nev == h->events.size()) {
+    hipEvent_t x, y;
+    HIPCHK(h, hipEventCreate(&x));
+    HIPCHK(h, hipEventCreate(&y));
+    h->events.emplace_back(x, y);
+    h->event_ch.push_back(0);
+  }
+  HIPCHK(h, hipEventRecord(h->events[h->nev].first, h->stream));
+  h->event_ch[h->nev] = ch;
+  *e1 = h->events[h->nev].second;
+  ++h->nev;
+  return PFTCC_OK;
+}
+
+int timing_end(pftcc* h, hipEvent_t e1) {
+  if (e1) HIPCHK(h, hipEventRecord(e1, h->stream));
   return PFTCC_OK;
 }
 
@@ -485,8 +509,8 @@ int collect_timing(pftcc* h) {
     HIPCHK(h, hipEventSynchronize(h->events[i].second));
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->events[i].first, h->events[i].second));
-    h->timed_ms += ms;
-    ++h->timed_launches;
+    h->timed_ms[h->event_ch[i]] += ms;
+    ++h->timed_launches[h->event_ch[i]];
   }
   h->nev = 0;
   return PFTCC_OK;
@@ -624,6 +648,195 @@ int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, cons
   if (npairs <= 0) return PFTCC_OK;
   hipLaunchKernelGGL(k_rot8, dim3(npairs), dim3(256), 0, h->stream, a);
   HIPCHK(h, hipGetLastError());
+  return PFTCC_OK;
+}
+
+// ------------------------------------------------------------------ precision form ---
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Compile-time inverse stage for the fused ring lengths; false if n has none.
+bool zifft_ct_go(int n, int cnt, int nr, hipStream_t s, const ZifftArgs& a) {
+#define X(N)                                                                                          \
+  if (n == N) {                                                                                       \
+    hipLaunchKernelGGL((k_zifft_ct<N, ZifftCt<N>::F, ZifftCt<N>::NT>),                               \
+                       dim3((cnt + ZifftCt<N>::F - 1) / ZifftCt<N>::F, nr), dim3(ZifftCt<N>::NT),    \
+                       ZifftCt<N>::lds, s, a);                                                        \
+    return true;                                                                                      \
+  }
+  PFTCC_FUSED_SIZES(X)
+#undef X
+  return false;
+}
+
+size_t zifft_lds(int F, int n) { return sizeof(float2) * ((size_t)F * (n + 1) + (size_t)F * n + 2 * n); }
+
+// Split-BF16 reference operand of eo class eo over the (0-based) reference list rl; cached.
+int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl) {
+  pftcc::APack& ap = h->apack[eo];
+  if (ap.valid && ap.refs == rl && ap.kcorr == h->kcorr) return PFTCC_OK;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  dfree(ap.hi);
+  dfree(ap.lo);
+  dfree(ap.sr);
+  ap.valid = false;
+  const int nr = (int)rl.size();
+  const int n = h->n, nk = h->nk;
+  ap.rows_pad = round_up(nr, 64);
+  ap.KP = round_up(2 * h->kcorr, 32);
+  const size_t words = (size_t)(n + 1) * ap.rows_pad * (ap.KP / 2);
+  int rc;
+  if ((rc = dalloc(h, &ap.hi, words)) || (rc = dalloc(h, &ap.lo, words)) || (rc = dalloc(h, &ap.sr, (size_t)nr * h->kcorr)))
+    return rc;
+  int* d_list = nullptr;
+  float2 *rings = nullptr, *spec = nullptr;
+  if ((rc = dalloc(h, &d_list, (size_t)nr)) || (rc = dalloc(h, &rings, (size_t)nr * nk * n)) ||
+      (rc = dalloc(h, &spec, (size_t)nr * nk * (n + 1))))
+    return rc;
+  HIPCHK(h, hipMemcpyAsync(d_list, rl.data(), nr * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  const size_t rowlen = (size_t)nk * n;
+  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((nr * rowlen + 255) / 256)), dim3(256), 0, h->stream,
+                     (const float2*)(eo ? h->d_refs_o : h->d_refs_e), (const int*)d_list, nr, rowlen, rings);
+  int F = 8;
+  while (F > 1 && sizeof(float2) * ((size_t)2 * F * n + 2 * n) > kMaxLds) F /= 2;
+  hipLaunchKernelGGL(k_memoize, dim3((nr * nk + F - 1) / F), dim3(256), sizeof(float2) * ((size_t)2 * F * n + 2 * n),
+                     h->stream, (const float2*)rings, 0, nr, nk, n, h->plan, (const float2*)h->d_tw2, spec, F);
+  hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream, (const float2*)spec,
+                     (const float*)nullptr, nr, ap.rows_pad, nk, h->kcorr, n, ap.KP, ap.hi, ap.lo);
+  hipLaunchKernelGGL(k_ring_sq64, dim3((nr * h->kcorr + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)rings, nr,
+                     nk, n, h->kcorr, ap.sr);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  dfree(d_list);
+  dfree(rings);
+  dfree(spec);
+  ap.refs = rl;
+  ap.kcorr = h->kcorr;
+  ap.valid = true;
+  return PFTCC_OK;
+}
+
+// Runs the precision form for particles pl (local ids) that share eo class eo, against the cached
+// operand of refs rl.  outpos[i] = position of pl[i] in best/cc_out.
+int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<int>& outpos,
+               const std::vector<int>& rl, int prec, unsigned long long* d_best, float* d_cc) {
+  int rc;
+  if ((rc = ensure_apack(h, eo, rl))) return rc;
+  const pftcc::APack& ap = h->apack[eo];
+  const int n = h->n, nk = h->nk, KP = ap.KP, nr = (int)rl.size();
+  const int total = (int)pl.size();
+  // chunk of particles: keep Z (n+1) x rows_pad x Pc float2 within ~4 GiB
+  int Pc = 256;
+  while (Pc > 64 && (size_t)(n + 1) * ap.rows_pad * Pc * 8 > ((size_t)4 << 30)) Pc -= 64;
+  Pc = std::min(Pc, round_up(total, 64));
+  const size_t words = (size_t)(n + 1) * Pc * (KP / 2);
+  const size_t zsz = (size_t)(n + 1) * ap.rows_pad * Pc;
+  if ((rc = scratch_reserve(h, aligned((size_t)total * 4) * 2 + aligned(nr * 4) + aligned((size_t)Pc * nk * (n + 1) * 8) +
+                                   aligned((size_t)Pc * nk * 4) + aligned((size_t)Pc * nk * 8) + aligned((size_t)Pc * 8) +
+                                   2 * aligned(words * 4) + aligned(zsz * 8))))
+    return rc;
+  int* d_pl = take<int>(h, total);
+  int* d_pos = take<int>(h, total);
+  int* d_rl = take<int>(h, nr);
+  float2* wg = take<float2>(h, (size_t)Pc * nk * (n + 1));
+  float* cbar = take<float>(h, (size_t)Pc * nk);
+  double* cbar2 = take<double>(h, (size_t)Pc * nk);
+  double* sp = take<double>(h, Pc);
+  unsigned int* bhi = take<unsigned int>(h, words);
+  unsigned int* blo = take<unsigned int>(h, words);
+  float2* Z = take<float2>(h, zsz);
+  HIPCHK(h, hipMemcpyAsync(d_pl, pl.data(), total * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_pos, outpos.data(), total * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_rl, rl.data(), nr * 4, hipMemcpyHostToDevice, h->stream));
+  for (int c0 = 0; c0 < total; c0 += Pc) {
+    const int cnt = std::min(Pc, total - c0);
+    hipEvent_t ev;
+    if ((rc = timing_begin(h, 3, &ev))) return rc;
+    const size_t rowlen = (size_t)nk * (n + 1);
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((cnt * rowlen + 255) / 256)), dim3(256), 0, h->stream,
+                       (const float2*)h->d_memo, (const int*)(d_pl + c0), cnt, rowlen, wg);
+    const bool ctf = h->with_ctf;
+    if (ctf)
+      hipLaunchKernelGGL(k_ring_mean_ctf_list, dim3((cnt * nk + 3) / 4), dim3(64, 4), 0, h->stream,
+                         (const float*)h->d_ctf, (const int*)(d_pl + c0), cnt, nk, n, cbar, cbar2);
+    hipLaunchKernelGGL(k_ring_sq64_list, dim3((cnt + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)h->d_ptcls,
+                       (const int*)(d_pl + c0), cnt, nk, n, h->kcorr, sp);
+    hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream, (const float2*)wg,
+                       ctf ? (const float*)cbar : nullptr, cnt, Pc, nk, h->kcorr, n, KP, bhi, blo);
+    HIPCHK(h, hipGetLastError());
+    if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 1, &ev))) return rc;
+    const dim3 gg(Pc / 64, ap.rows_pad / 64, n + 1);
+    if (prec == PFTCC_PREC_BF16)
+      hipLaunchKernelGGL(k_zgemm<1>, gg, dim3(64), 0, h->stream, (const bf16x8*)ap.hi, (const bf16x8*)ap.lo,
+                         (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, Z);
+    else
+      hipLaunchKernelGGL(k_zgemm<3>, gg, dim3(64), 0, h->stream, (const bf16x8*)ap.hi, (const bf16x8*)ap.lo,
+                         (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, Z);
+    HIPCHK(h, hipGetLastError());
+    if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 2, &ev))) return rc;
+    ZifftArgs za{};
+    za.n = n;
+    za.nrefs = nr;
+    za.np = cnt;
+    za.rows_pad = ap.rows_pad;
+    za.cols_pad = Pc;
+    za.kcorr = h->kcorr;
+    za.Z = Z;
+    za.sr = ap.sr;
+    za.cbar2 = ctf ? cbar2 : nullptr;
+    za.nk = nk;
+    za.sp = sp;
+    za.rlist = d_rl;
+    za.outpos = d_pos + c0;
+    za.cc_out = d_cc;
+    za.best = d_best;
+    za.tw2g = h->d_tw2;
+    za.plan = h->plan;
+    if (zifft_ct_go(n, cnt, nr, h->stream, za)) {
+      HIPCHK(h, hipGetLastError());
+      if ((rc = timing_end(h, ev))) return rc;
+      continue;
+    }
+    int Fz = 8;
+    while (Fz > 2 && zifft_lds(Fz, n) > kMaxLds) Fz /= 2;
+    const size_t lds = zifft_lds(Fz, n);
+    if (Fz == 8)
+      hipLaunchKernelGGL(k_zifft<8>, dim3((cnt + 7) / 8, nr), dim3(256), lds, h->stream, za);
+    else if (Fz == 4)
+      hipLaunchKernelGGL(k_zifft<4>, dim3((cnt + 3) / 4, nr), dim3(256), lds, h->stream, za);
+    else
+      hipLaunchKernelGGL(k_zifft<2>, dim3((cnt + 1) / 2, nr), dim3(256), lds, h->stream, za);
+    HIPCHK(h, hipGetLastError());
+    if ((rc = timing_end(h, ev))) return rc;
+  }
+  return PFTCC_OK;
+}
+
+// Common driver: validates, splits particles by eo class and runs both groups.
+int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* irefs, int prec,
+             unsigned long long* d_best, float* d_cc) {
+  int rc;
+  if (prec != PFTCC_PREC_BF16 && prec != PFTCC_PREC_SPLIT_BF16) return fail(h, PFTCC_E_ARG, "unknown precision %d", prec);
+  if (h->euclid || h->match_filt || h->pssnr)
+    return fail(h, PFTCC_E_UNSUPPORTED, "the MFMA form implements objfun cc without match filter");
+  if (zifft_lds(2, h->n) > kMaxLds) return fail(h, PFTCC_E_UNSUPPORTED, "pftsz %d too large for the MFMA form", h->n);
+  if (nr > 65535) return fail(h, PFTCC_E_ARG, "the MFMA form takes at most 65535 references per call");
+  if ((rc = need_memo(h))) return rc;
+  std::vector<int> rl(nr);
+  for (int i = 0; i < nr; ++i) {
+    const int r = irefs ? irefs[i] : i + 1;
+    if ((rc = check_ref(h, r))) return rc;
+    rl[i] = r - 1;
+  }
+  std::vector<int> pl[2], pos[2];
+  for (int i = 0; i < np; ++i) {
+    int ip;
+    if ((rc = local_index(h, iptcls[i], &ip))) return rc;
+    const int eo = h->iseven[ip] ? 0 : 1;
+    pl[eo].push_back(ip);
+    pos[eo].push_back(i);
+  }
+  for (int eo = 0; eo < 2; ++eo)
+    if (!pl[eo].empty() && (rc = mfma_group(h, eo, pl[eo], pos[eo], rl, prec, d_best, d_cc))) return rc;
   return PFTCC_OK;
 }
 
@@ -767,8 +980,10 @@ int pftcc_set_timing(pftcc_t* h, int32_t enable) {
   int rc;
   if ((rc = collect_timing(h))) return rc;
   h->timing = enable != 0;
-  h->timed_ms = 0.0;
-  h->timed_launches = 0;
+  for (int c = 0; c < 4; ++c) {
+    h->timed_ms[c] = 0.0;
+    h->timed_launches[c] = 0;
+  }
   return PFTCC_OK;
 }
 
@@ -777,8 +992,18 @@ int pftcc_get_timing(pftcc_t* h, double* total_ms, int64_t* launches) {
   std::lock_guard<std::mutex> g(h->mtx);
   int rc;
   if ((rc = collect_timing(h))) return rc;
-  *total_ms = h->timed_ms;
-  *launches = h->timed_launches;
+  *total_ms = h->timed_ms[0];
+  *launches = h->timed_launches[0];
+  return PFTCC_OK;
+}
+
+int pftcc_get_timing_channel(pftcc_t* h, int32_t ch, double* total_ms, int64_t* launches) {
+  if (!h || !total_ms || !launches || ch < 0 || ch > 3) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = collect_timing(h))) return rc;
+  *total_ms = h->timed_ms[ch];
+  *launches = h->timed_launches[ch];
   return PFTCC_OK;
 }
 
@@ -804,6 +1029,11 @@ int pftcc_kill(pftcc_t* h) {
   dfree(h->d_refinv_d);
   dfree(h->d_ofac_d);
   dfree(h->d_refpw);
+  for (auto& ap : h->apack) {
+    dfree(ap.hi);
+    dfree(ap.lo);
+    dfree(ap.sr);
+  }
   if (h->d_scratch) (void)hipFree(h->d_scratch);
   delete h;
   return PFTCC_OK;
@@ -833,6 +1063,7 @@ static int set_refs(pftcc_t* h, int iref_first, int count, const void* src, int 
   HIPCHK(h, hipMemcpyAsync(dst, src, (size_t)count * ring * sizeof(float2), kind, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->refnorm_valid = false;
+  h->apack[0].valid = h->apack[1].valid = false;
   return PFTCC_OK;
 }
 
@@ -901,6 +1132,7 @@ int pftcc_set_ref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, float
   float2* dst = (iseven ? h->d_refs_e : h->d_refs_o) + ((size_t)(iref - 1) * h->nk + (k - h->kfrom)) * h->n + (irot - 1);
   HIPCHK(h, hipMemcpy(dst, &v, sizeof v, hipMemcpyHostToDevice));
   h->refnorm_valid = false;
+  h->apack[0].valid = h->apack[1].valid = false;
   return PFTCC_OK;
 }
 
@@ -929,6 +1161,7 @@ int pftcc_cp_even2odd_ref(pftcc_t* h, int32_t iref) {
   HIPCHK(h, hipMemcpy(h->d_refs_o + (size_t)(iref - 1) * ring, h->d_refs_e + (size_t)(iref - 1) * ring,
                       ring * sizeof(float2), hipMemcpyDeviceToDevice));
   h->refnorm_valid = false;
+  h->apack[0].valid = h->apack[1].valid = false;
   return PFTCC_OK;
 }
 
@@ -1055,6 +1288,7 @@ int pftcc_zero_refs_beyond_kstop(pftcc_t* h) {
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->refnorm_valid = false;
+  h->apack[0].valid = h->apack[1].valid = false;
   return PFTCC_OK;
 }
 
@@ -1444,6 +1678,53 @@ int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (bad) return fail(h, PFTCC_E_ARG, "search_dev: particle or reference index out of range");
   return search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, d_best_cc, d_best_iref, d_best_ishift, d_best_irot);
+}
+
+int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs, int32_t prec,
+                      float* best_cc, int32_t* best_iref, int32_t* best_irot) {
+  if (!h || !iptcls || np < 1 || nr < 1 || !best_cc || !best_iref || !best_irot) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  unsigned long long* d_best = nullptr;
+  float* d_cc = nullptr;
+  int *d_ir = nullptr, *d_rot = nullptr;
+  int rc;
+  if ((rc = dalloc(h, &d_best, (size_t)np)) || (rc = dalloc(h, &d_cc, (size_t)np)) || (rc = dalloc(h, &d_ir, (size_t)np)) ||
+      (rc = dalloc(h, &d_rot, (size_t)np)))
+    return rc;
+  hipLaunchKernelGGL(k_fill_u64, dim3((np + 255) / 256), dim3(256), 0, h->stream, d_best, np, 0ull);
+  rc = mfma_run(h, np, iptcls, nr, irefs, prec, d_best, nullptr);
+  if (rc == PFTCC_OK) {
+    hipLaunchKernelGGL(k_best_decode_rot, dim3((np + 255) / 256), dim3(256), 0, h->stream,
+                       (const unsigned long long*)d_best, np, h->nrots, d_cc, d_ir, d_rot);
+    if (hipMemcpyAsync(best_cc, d_cc, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipMemcpyAsync(best_iref, d_ir, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipMemcpyAsync(best_irot, d_rot, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      rc = fail(h, PFTCC_E_DEVICE, "search_mfma: copy of the results failed");
+  }
+  (void)hipStreamSynchronize(h->stream);
+  dfree(d_best);
+  dfree(d_cc);
+  dfree(d_ir);
+  dfree(d_rot);
+  return rc;
+}
+
+int pftcc_gencorrs_mfma_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
+                              int32_t prec, float* cc_out) {
+  if (!h || !iptcls || np < 1 || nr < 1 || !cc_out) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  float* d_cc = nullptr;
+  int rc;
+  const size_t ncc = (size_t)np * nr * h->nrots;
+  if ((rc = dalloc(h, &d_cc, ncc))) return rc;
+  rc = mfma_run(h, np, iptcls, nr, irefs, prec, nullptr, d_cc);
+  if (rc == PFTCC_OK && (hipMemcpyAsync(cc_out, d_cc, ncc * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                         hipStreamSynchronize(h->stream) != hipSuccess))
+    rc = fail(h, PFTCC_E_DEVICE, "gencorrs_mfma_batch: copy failed");
+  (void)hipStreamSynchronize(h->stream);
+  dfree(d_cc);
+  return rc;
 }
 
 int pftcc_grad_batch(pftcc_t* h, int32_t npairs, const int32_t* irefs, const int32_t* iptcls, const double* shvecs,

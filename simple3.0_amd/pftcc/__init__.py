@@ -24,6 +24,8 @@ LIBPATH = os.path.join(LIBDIR, "libpftcc_hip.so")
 
 PFTCC_OBJFUN_CC = 0
 PFTCC_OBJFUN_EUCLID = 1
+PFTCC_PREC_BF16 = 1
+PFTCC_PREC_SPLIT_BF16 = 3
 
 
 class PftccError(RuntimeError):
@@ -104,6 +106,9 @@ SIGNATURES = [
     ("pftcc_search_dev", _I, [_P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P]),
     ("pftcc_grad_batch", _I, [_P, _I, _PI, _PI, _PD, _PI, _PD, _PD]),
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    ("pftcc_search_mfma", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _PI, _PI]),
+    ("pftcc_gencorrs_mfma_batch", _I, [_P, _I, _PI, _I, _PI, _I, _PF]),
+    ("pftcc_get_timing_channel", _I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
     ("pftcc_set_timing", _I, [_P, _I]),
     ("pftcc_get_timing", _I, [_P, _PD, C.POINTER(C.c_int64)]),
     ("pftcc_magic_pftsz", _I, [_I]),
@@ -505,6 +510,34 @@ class PolarFTCorrCalc:
                                                  C.c_void_p(d_irots), C.c_void_p(d_f),
                                                  C.c_void_p(d_grad) if d_grad else None),
                   "grad_batch_dev")
+
+    def search_mfma(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16):
+        """Precision form of search() without shifts: (best_cc, best_iref, best_irot)."""
+        p = np.ascontiguousarray(iptcls, np.int32)
+        r = None if irefs is None else np.ascontiguousarray(irefs, np.int32)
+        nr = self.nrefs if r is None else r.size
+        cc = np.zeros(p.size, np.float32)
+        ir = np.zeros(p.size, np.int32)
+        rot = np.zeros(p.size, np.int32)
+        self._chk(self._lib.pftcc_search_mfma(self._h, p.size, _ip(p), nr, None if r is None else _ip(r), prec,
+                                              _fp(cc), _ip(ir), _ip(rot)), "search_mfma")
+        return cc, ir, rot
+
+    def gencorrs_mfma_batch(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16):
+        """cc[np, nr, nrots] of the precision form."""
+        p = np.ascontiguousarray(iptcls, np.int32)
+        r = None if irefs is None else np.ascontiguousarray(irefs, np.int32)
+        nr = self.nrefs if r is None else r.size
+        out = np.zeros((p.size, nr, self.nrots), np.float32)
+        self._chk(self._lib.pftcc_gencorrs_mfma_batch(self._h, p.size, _ip(p), nr, None if r is None else _ip(r),
+                                                      prec, _fp(out)), "gencorrs_mfma_batch")
+        return out
+
+    def get_timing_channel(self, ch):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._chk(self._lib.pftcc_get_timing_channel(self._h, ch, C.byref(ms), C.byref(n)), "get_timing_channel")
+        return ms.value, n.value
 
     def set_timing(self, enable=True):
         self._chk(self._lib.pftcc_set_timing(self._h, int(enable)), "set_timing")

@@ -1,0 +1,143 @@
+"""Parity of the precision (MFMA, split-BF16) form with the FP64 oracle.
+
+The form is exact for ring-constant |CTF| (no CTF, or no astigmatism); with astigmatism it uses the
+particle's ring-mean |CTF| per shell, and is then checked against the oracle fed that same
+ring-mean CTF.  Tolerance: |cc_mfma - cc_oracle| <= 1e-4 relative to the correlation peak (split
+BF16 keeps ~16 mantissa bits per operand; the dropped lo*lo term is 2^-16 relative), best
+(reference, rotation) identical wherever the oracle's top-2 gap exceeds 10x the measured error.
+"""
+import numpy as np
+import pytest
+
+import pftcc_oracle as O
+from cases import make_case
+from gpu_helpers import peak_rel_err, say, to_lib, top2_gap
+from pftcc import PFTCC_PREC_BF16, PFTCC_PREC_SPLIT_BF16, PftccError
+
+pytestmark = pytest.mark.gpu
+
+MFMA_TOL = 1e-4
+
+
+def oracle_cube(pf, ptcl_idx, ref_idx):
+    """[np, nr, nrots] FP64 correlations of the oracle (FFT route, unshifted)."""
+    return np.stack([np.stack([pf.gencorrs(r, p, None, method="fft") for r in ref_idx]) for p in ptcl_idx])
+
+
+def ring_mean_case(pf):
+    """The oracle case with each particle's |CTF| replaced by its per-shell ring mean."""
+    m = pf.ctfmats.astype(np.float64).mean(axis=-1, keepdims=True)
+    pf.ctfmats = np.broadcast_to(m, pf.ctfmats.shape).astype(np.float32).copy()
+    return pf
+
+
+def check_cube(got, ref):
+    errs = []
+    for p in range(ref.shape[0]):
+        e = peak_rel_err(got[p], ref[p])
+        errs.append(e)
+        assert e < MFMA_TOL, (p, e)
+        if top2_gap(ref[p]) > 10 * e * np.max(np.abs(ref[p])):
+            assert int(np.argmax(got[p])) == int(np.argmax(ref[p]))
+    return max(errs)
+
+
+@pytest.mark.parametrize("n,kfrom,kto,ldim,ctf,astig", [
+    (40, 2, 12, 32, False, 0.0),
+    (64, 2, 20, 64, True, 0.0),
+    (192, 2, 40, 128, True, 0.0),
+    (308, 2, 60, 200, False, 0.0),
+    (400, 2, 80, 256, True, 0.0),
+])
+def test_mfma_exact_for_ring_constant_ctf(n, kfrom, kto, ldim, ctf, astig):
+    say(f"mfma vs oracle n={n} K=[{kfrom},{kto}] ctf={ctf} astig={astig}")
+    pf, cps = make_case(seed=300 + n, n=n, kfrom=kfrom, kto=kto, ctf=ctf, astig=astig, nrefs=5, nptcls=7,
+                        ldim=ldim, eo_mixed=True)
+    lib = to_lib(pf, cps)
+    ids = np.arange(1, 8)
+    got = lib.gencorrs_mfma_batch(ids)
+    ref = oracle_cube(pf, range(7), range(5))
+    e = check_cube(got, ref)
+    say(f"  split-bf16 max peak-relative error {e:.2e}")
+    lo = lib.gencorrs_mfma_batch(ids, prec=PFTCC_PREC_BF16)
+    say(f"  plain bf16 max peak-relative error {max(peak_rel_err(lo[p], ref[p]) for p in range(7)):.2e}")
+    lib.kill()
+
+
+def test_mfma_astigmatic_uses_ring_mean_ctf():
+    say("mfma astigmatic ctf = oracle with ring-mean ctf")
+    pf, cps = make_case(seed=41, n=192, kfrom=2, kto=40, ctf=True, astig=0.3, nrefs=4, nptcls=6, ldim=128)
+    lib = to_lib(pf, cps)
+    got = lib.gencorrs_mfma_batch(np.arange(1, 7))
+    exact = oracle_cube(pf, range(6), range(4))
+    ref = oracle_cube(ring_mean_case(pf), range(6), range(4))
+    e = check_cube(got, ref)
+    say(f"  vs ring-mean oracle {e:.2e}; vs exact astigmatic oracle "
+        f"{max(peak_rel_err(got[p], exact[p]) for p in range(6)):.2e}")
+    lib.kill()
+
+
+def test_mfma_subsets_order_and_search():
+    """Particle / reference subsets in any order, eo classes mixed, and search_mfma = argmax."""
+    say("mfma subsets + search_mfma")
+    pf, cps = make_case(seed=42, n=64, kfrom=2, kto=20, ctf=True, astig=0.0, nrefs=9, nptcls=11, ldim=64,
+                        eo_mixed=True)
+    lib = to_lib(pf, cps)
+    pids = np.array([7, 2, 11, 1, 5], np.int32)
+    rids = np.array([9, 3, 4, 1], np.int32)
+    got = lib.gencorrs_mfma_batch(pids, rids)
+    ref = oracle_cube(pf, pids - 1, rids - 1)
+    check_cube(got, ref)
+    cc, ir, rot = lib.search_mfma(pids, rids)
+    for i in range(len(pids)):
+        flat = int(np.argmax(got[i]))
+        r, t = np.unravel_index(flat, got[i].shape)
+        assert (ir[i], rot[i]) == (rids[r], t + 1)
+        assert cc[i] == got[i][r, t]
+    # a second search with the same reference list reuses the packed operand
+    cc2, ir2, rot2 = lib.search_mfma(pids[::-1], rids)
+    assert np.array_equal(cc2, cc[::-1]) and np.array_equal(ir2, ir[::-1]) and np.array_equal(rot2, rot[::-1])
+    # changing a reference invalidates it
+    lib.set_ref_pft(rids[0], np.zeros_like(pf.refs_even[0]), True)
+    lib.set_ref_pft(rids[0], np.zeros_like(pf.refs_odd[0]), False)
+    pf.refs_even[rids[0] - 1] = 0
+    pf.refs_odd[rids[0] - 1] = 0
+    cc3, ir3, _ = lib.search_mfma(pids, rids[1:])
+    assert not np.any(ir3 == rids[0])
+    lib.kill()
+
+
+def test_mfma_many_particles_chunks():
+    """More particles than one chunk (256) and more references than one 64-row tile."""
+    say("mfma 600 ptcls x 70 refs, n=40")
+    pf, cps = make_case(seed=43, n=40, kfrom=2, kto=12, ctf=True, astig=0.0, nrefs=70, nptcls=600, ldim=32,
+                        eo_mixed=True)
+    lib = to_lib(pf, cps)
+    pids = np.arange(1, 601)
+    cc, ir, rot = lib.search_mfma(pids)
+    sel = np.array([0, 1, 255, 256, 257, 511, 512, 599])
+    got = lib.gencorrs_mfma_batch(pids[sel])
+    ref = oracle_cube(pf, sel, range(70))
+    check_cube(got, ref)
+    for i, p in enumerate(sel):
+        r, t = np.unravel_index(int(np.argmax(got[i])), got[i].shape)
+        assert (ir[p], rot[p]) == (r + 1, t + 1)
+    lib.kill()
+
+
+def test_mfma_rejects_unsupported():
+    say("mfma errors")
+    pf, cps = make_case(seed=44, n=40, kto=12, ctf=False, nrefs=2, nptcls=2, euclid=True)
+    lib = to_lib(pf, cps)
+    with pytest.raises(PftccError):
+        lib.search_mfma(np.arange(1, 3))
+    lib.kill()
+    pf, cps = make_case(seed=44, n=40, kto=12, ctf=False, nrefs=2, nptcls=2)
+    lib = to_lib(pf, cps)
+    with pytest.raises(PftccError):
+        lib.search_mfma(np.arange(1, 3), prec=2)
+    with pytest.raises(PftccError):
+        lib.search_mfma(np.array([5], np.int32))
+    with pytest.raises(PftccError):
+        lib.search_mfma(np.arange(1, 3), np.array([3], np.int32))
+    lib.kill()
