@@ -202,11 +202,16 @@ int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs,
  * candidates should be rescored with pftcc_grad_batch / pftcc_gencorr_for_rot_8.
  * prec: PFTCC_PREC_SPLIT_BF16 (hi+lo split, FP32 accumulation) or PFTCC_PREC_BF16.
  * Denominators sqrt(sum_k cbar_k^2 S_ref,k * S_ptcl) are formed in FP64.  No counterpart
- * entry point in the reference; it replaces the same greedy loop as pftcc_search. */
+ * entry point in the reference; it replaces the same greedy loop as pftcc_search
+ * (simple_strategy3D_greedy.f90:60-77, gencorrs over all references + maxloc).
+ * ntop = 0: the best (reference, rotation) of the contraction itself.  ntop in [1, 16]: the ntop
+ * references with the highest contraction score per particle are rescored exactly in FP64 (the
+ * reference's gencorrs formula with the particle's own |CTF|) at their best rotation and its two
+ * neighbours, and the best rescored candidate is returned with its FP64 cc. */
 #define PFTCC_PREC_BF16 1
 #define PFTCC_PREC_SPLIT_BF16 3
 int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
-                      int32_t prec, float* best_cc, int32_t* best_iref, int32_t* best_irot);
+                      int32_t prec, int32_t ntop, float* best_cc, int32_t* best_iref, int32_t* best_irot);
 /* cc_out[np][nr][nrots] of the same computation (for verification and small batches). */
 int pftcc_gencorrs_mfma_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
                               int32_t prec, float* cc_out);
@@ -219,7 +224,7 @@ int pftcc_gencorrs_mfma_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int
 int pftcc_set_timing(pftcc_t* h, int32_t enable);
 int pftcc_get_timing(pftcc_t* h, double* total_ms, int64_t* launches);
 /* Per kernel family: 0 plain correlation, 1 MFMA contraction, 2 inverse FFT of the MFMA form,
- * 3 operand packing of the MFMA form. */
+ * 3 operand packing, candidate selection and rescoring of the MFMA form. */
 int pftcc_get_timing_channel(pftcc_t* h, int32_t channel, double* total_ms, int64_t* launches);
 
 /* ------------------------------------------------------------------------- utilities -- */

@@ -176,6 +176,7 @@ struct ZifftArgs {
   const int* outpos;           // [np_chunk] output position of each chunk particle
   float* cc_out;               // [npos][nrefs][2n] or null
   unsigned long long* best;    // [npos] or null
+  unsigned long long* prbest;  // [npos][nrefs] best key of each (particle, reference), or null
   const float2* tw2g;
   FftPlan plan;
 };
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(256) k_zifft(ZifftArgs a) {
       float2* o = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * n));
       o[j] = make_float2(x0, x1);
     }
-    if (a.best) {
+    if (a.best || a.prbest) {
       const unsigned int lin0 = (unsigned int)iref * (unsigned int)(2 * n) + 2u * j;
       const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
       const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(256) k_zifft(ZifftArgs a) {
         }
     }
   }
-  if (a.best) {
+  if (a.best || a.prbest) {
 #pragma unroll
     for (int f = 0; f < F; ++f) {
       red[tid] = bestkey[f];
@@ -261,7 +262,10 @@ __global__ void __launch_bounds__(256) k_zifft(ZifftArgs a) {
         if (tid < w && red[tid + w] > red[tid]) red[tid] = red[tid + w];
         __syncthreads();
       }
-      if (tid == 0 && p0 + f < a.np && r < a.nrefs) atomicMax(&a.best[a.outpos[p0 + f]], red[0]);
+      if (tid == 0 && p0 + f < a.np && r < a.nrefs) {
+        if (a.best) atomicMax(&a.best[a.outpos[p0 + f]], red[0]);
+        if (a.prbest) a.prbest[(size_t)a.outpos[p0 + f] * a.nrefs + r] = red[0];
+      }
       __syncthreads();
     }
   }
@@ -326,7 +330,7 @@ __global__ void __launch_bounds__(NT) k_zifft_ct(ZifftArgs a) {
       o[j] = make_float2((float)(y.x * sc), (float)(-y.y * sc));
     }
   }
-  if (a.best) {
+  if (a.best || a.prbest) {
     const int f = tid / TPF, u = tid % TPF;
     const int p = p0 + f;
     const int iref = a.rlist ? a.rlist[r] : r;
@@ -346,7 +350,10 @@ __global__ void __launch_bounds__(NT) k_zifft_ct(ZifftArgs a) {
       const unsigned long long v = __shfl_xor(bk, o, TPF);
       bk = v > bk ? v : bk;
     }
-    if (u == 0 && p < a.np && r < a.nrefs) atomicMax(&a.best[a.outpos[p]], bk);
+    if (u == 0 && p < a.np && r < a.nrefs) {
+      if (a.best) atomicMax(&a.best[a.outpos[p]], bk);
+      if (a.prbest) a.prbest[(size_t)a.outpos[p] * a.nrefs + r] = bk;
+    }
   }
 }
 
@@ -394,6 +401,148 @@ __global__ void k_ring_sq64_list(const float2* __restrict__ rings, const int* __
   for (int e = threadIdx.x; e < kcount * n; e += blockDim.x) s += (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   if (threadIdx.x == 0) out[p] = s;
+}
+
+// The T largest (particle, reference) keys of each particle, in descending order (0 when fewer
+// than T references exist).  One work-group per particle; keys are distinct (their low words
+// encode distinct (reference, rotation) indices), so round t takes the maximum below round t-1's.
+__global__ void __launch_bounds__(256) k_topt(const unsigned long long* __restrict__ prbest, int np, int nrefs, int T,
+                                              unsigned long long* __restrict__ cand) {
+  __shared__ unsigned long long red[256];
+  const int i = blockIdx.x, tid = threadIdx.x;
+  if (i >= np) return;
+  const unsigned long long* row = prbest + (size_t)i * nrefs;
+  unsigned long long thr = ~0ull;
+  for (int t = 0; t < T; ++t) {
+    unsigned long long m = 0ull;
+    for (int r = tid; r < nrefs; r += 256) {
+      const unsigned long long k = row[r];
+      if (k < thr && k > m) m = k;
+    }
+    red[tid] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w && red[tid + w] > red[tid]) red[tid] = red[tid + w];
+      __syncthreads();
+    }
+    thr = red[0];
+    if (tid == 0) cand[(size_t)i * T + t] = thr;
+    __syncthreads();
+    if (thr == 0ull) {
+      for (int u = t + 1 + tid; u < T; u += 256) cand[(size_t)i * T + u] = 0ull;
+      return;
+    }
+  }
+}
+
+struct RescoreArgs {
+  int n, nk, kcorr, T;
+  const float2* refs_even;
+  const float2* refs_odd;
+  const float2* ptcls;         // [nptcls][nk][n]
+  const float* ctf;            // [nptcls][nk][n] or null
+  const int8_t* iseven;
+  const int* plist;            // [np] local particle index of each output position
+  const unsigned long long* cand;   // [np][T] keys (iref * 2n + rho in the low word)
+  int np;
+  float* best_cc;
+  int* best_iref;
+  int* best_irot;
+  double* best_cc64;           // or null
+};
+
+// Exact rescoring of the candidates in FP64: for each candidate (reference, rho) the rotations
+// rho - 1, rho, rho + 1 are evaluated as in gencorrs (unshifted): q = ref * |CTF| with the
+// particle's own angle-dependent |CTF|, cc = sum_k<kcorr sum_j Re q[k,j] conj(P[k,(j+rho) mod 2n])
+// / sqrt(sum |q|^2 * sum |p|^2).  One wavefront per candidate, one work-group (T wavefronts) per
+// particle; the best cc wins, ties to the lowest (reference, rotation).
+__global__ void __launch_bounds__(1024) k_rescore64(RescoreArgs a) {
+  __shared__ double ccs[16 * 3];
+  __shared__ unsigned int lins[16 * 3];
+  const int i = blockIdx.x;
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int n = a.n, n2 = 2 * n;
+  const unsigned long long key = a.cand[(size_t)i * a.T + w];
+  const int ip = a.plist[i];
+  if (key != 0ull) {
+    const unsigned int lin = 0xFFFFFFFFu - (unsigned int)(key & 0xFFFFFFFFull);
+    const int iref = (int)(lin / (unsigned)n2), rho = (int)(lin % (unsigned)n2);
+    const int rr[3] = {(rho + n2 - 1) % n2, rho, (rho + 1) % n2};
+    const float2* ref = (a.iseven[ip] ? a.refs_even : a.refs_odd) + (size_t)iref * a.nk * n;
+    const float2* pt = a.ptcls + (size_t)ip * a.nk * n;
+    const float* ct = a.ctf ? a.ctf + (size_t)ip * a.nk * n : nullptr;
+    double num0 = 0.0, num1 = 0.0, num2 = 0.0, sq = 0.0, sp = 0.0;
+    for (int e = lane; e < a.kcorr * n; e += 64) {
+      const int k = e / n, j = e % n;
+      const float2 rv = ref[e];
+      const double c = ct ? (double)ct[e] : 1.0;
+      const double qx = rv.x * c, qy = rv.y * c;
+      sq += qx * qx + qy * qy;
+      const float2 pv = pt[e];
+      sp += (double)pv.x * pv.x + (double)pv.y * pv.y;
+      const float2* prow = pt + (size_t)k * n;
+      double nm[3];
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        int idx = j + rr[t];
+        if (idx >= n2) idx -= n2;
+        float2 P;
+        if (idx < n) {
+          P = prow[idx];
+        } else {
+          P = prow[idx - n];
+          P.y = -P.y;
+        }
+        nm[t] = qx * P.x + qy * P.y;
+      }
+      num0 += nm[0];
+      num1 += nm[1];
+      num2 += nm[2];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      num0 += __shfl_xor(num0, o);
+      num1 += __shfl_xor(num1, o);
+      num2 += __shfl_xor(num2, o);
+      sq += __shfl_xor(sq, o);
+      sp += __shfl_xor(sp, o);
+    }
+    if (lane == 0) {
+      const double d = sqrt(sq * sp);
+      const double nums[3] = {num0, num1, num2};
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        ccs[w * 3 + t] = nums[t] / d;
+        lins[w * 3 + t] = (unsigned int)iref * (unsigned int)n2 + (unsigned int)rr[t];
+      }
+    }
+  } else if (lane == 0) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      ccs[w * 3 + t] = -INFINITY;
+      lins[w * 3 + t] = 0xFFFFFFFFu;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bc = -INFINITY;
+    unsigned int bl = 0xFFFFFFFFu;
+    for (int c = 0; c < 3 * a.T; ++c)
+      if (lins[c] != 0xFFFFFFFFu && (ccs[c] > bc || (ccs[c] == bc && lins[c] < bl))) {
+        bc = ccs[c];
+        bl = lins[c];
+      }
+    if (bl == 0xFFFFFFFFu) {
+      a.best_cc[i] = __uint_as_float(0x7FC00000u);
+      a.best_iref[i] = 0;
+      a.best_irot[i] = 0;
+    } else {
+      a.best_cc[i] = (float)bc;
+      a.best_iref[i] = (int)(bl / (unsigned)n2) + 1;
+      a.best_irot[i] = (int)(bl % (unsigned)n2) + 1;
+    }
+    if (a.best_cc64) a.best_cc64[i] = bc;
+  }
 }
 
 // Decodes keys whose low word encodes iref * nrots + rho.

@@ -718,7 +718,8 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl) {
 // Runs the precision form for particles pl (local ids) that share eo class eo, against the cached
 // operand of refs rl.  outpos[i] = position of pl[i] in best/cc_out.
 int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<int>& outpos,
-               const std::vector<int>& rl, int prec, unsigned long long* d_best, float* d_cc) {
+               const std::vector<int>& rl, int prec, unsigned long long* d_best, float* d_cc,
+               unsigned long long* d_prbest) {
   int rc;
   if ((rc = ensure_apack(h, eo, rl))) return rc;
   const pftcc::APack& ap = h->apack[eo];
@@ -789,6 +790,7 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     za.outpos = d_pos + c0;
     za.cc_out = d_cc;
     za.best = d_best;
+    za.prbest = d_prbest;
     za.tw2g = h->d_tw2;
     za.plan = h->plan;
     if (zifft_ct_go(n, cnt, nr, h->stream, za)) {
@@ -813,7 +815,8 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
 
 // Common driver: validates, splits particles by eo class and runs both groups.
 int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* irefs, int prec,
-             unsigned long long* d_best, float* d_cc) {
+             unsigned long long* d_best, float* d_cc, unsigned long long* d_prbest = nullptr,
+             std::vector<int>* plocal = nullptr) {
   int rc;
   if (prec != PFTCC_PREC_BF16 && prec != PFTCC_PREC_SPLIT_BF16) return fail(h, PFTCC_E_ARG, "unknown precision %d", prec);
   if (h->euclid || h->match_filt || h->pssnr)
@@ -834,9 +837,10 @@ int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* ire
     const int eo = h->iseven[ip] ? 0 : 1;
     pl[eo].push_back(ip);
     pos[eo].push_back(i);
+    if (plocal) plocal->push_back(ip);
   }
   for (int eo = 0; eo < 2; ++eo)
-    if (!pl[eo].empty() && (rc = mfma_group(h, eo, pl[eo], pos[eo], rl, prec, d_best, d_cc))) return rc;
+    if (!pl[eo].empty() && (rc = mfma_group(h, eo, pl[eo], pos[eo], rl, prec, d_best, d_cc, d_prbest))) return rc;
   return PFTCC_OK;
 }
 
@@ -1681,21 +1685,53 @@ int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr
 }
 
 int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs, int32_t prec,
-                      float* best_cc, int32_t* best_iref, int32_t* best_irot) {
+                      int32_t ntop, float* best_cc, int32_t* best_iref, int32_t* best_irot) {
   if (!h || !iptcls || np < 1 || nr < 1 || !best_cc || !best_iref || !best_irot) return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
-  unsigned long long* d_best = nullptr;
+  if (ntop < 0 || ntop > 16) return fail(h, PFTCC_E_ARG, "ntop %d outside [0, 16]", ntop);
+  unsigned long long *d_best = nullptr, *d_prbest = nullptr, *d_cand = nullptr;
   float* d_cc = nullptr;
-  int *d_ir = nullptr, *d_rot = nullptr;
+  int *d_ir = nullptr, *d_rot = nullptr, *d_pl = nullptr;
   int rc;
   if ((rc = dalloc(h, &d_best, (size_t)np)) || (rc = dalloc(h, &d_cc, (size_t)np)) || (rc = dalloc(h, &d_ir, (size_t)np)) ||
       (rc = dalloc(h, &d_rot, (size_t)np)))
     return rc;
+  if (ntop > 0 && ((rc = dalloc(h, &d_prbest, (size_t)np * nr)) || (rc = dalloc(h, &d_cand, (size_t)np * ntop)) ||
+                   (rc = dalloc(h, &d_pl, (size_t)np))))
+    return rc;
   hipLaunchKernelGGL(k_fill_u64, dim3((np + 255) / 256), dim3(256), 0, h->stream, d_best, np, 0ull);
-  rc = mfma_run(h, np, iptcls, nr, irefs, prec, d_best, nullptr);
-  if (rc == PFTCC_OK) {
+  std::vector<int> plocal;
+  rc = mfma_run(h, np, iptcls, nr, irefs, prec, ntop > 0 ? nullptr : d_best, nullptr, d_prbest, &plocal);
+  if (rc == PFTCC_OK && ntop > 0) {
+    hipEvent_t ev;
+    if ((rc = timing_begin(h, 3, &ev))) return rc;
+    HIPCHK(h, hipMemcpyAsync(d_pl, plocal.data(), np * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_topt, dim3(np), dim3(256), 0, h->stream, (const unsigned long long*)d_prbest, np, nr, ntop,
+                       d_cand);
+    RescoreArgs ra{};
+    ra.n = h->n;
+    ra.nk = h->nk;
+    ra.kcorr = h->kcorr;
+    ra.T = ntop;
+    ra.refs_even = h->d_refs_e;
+    ra.refs_odd = h->d_refs_o;
+    ra.ptcls = h->d_ptcls;
+    ra.ctf = h->with_ctf ? h->d_ctf : nullptr;
+    ra.iseven = h->d_iseven;
+    ra.plist = d_pl;
+    ra.cand = d_cand;
+    ra.np = np;
+    ra.best_cc = d_cc;
+    ra.best_iref = d_ir;
+    ra.best_irot = d_rot;
+    hipLaunchKernelGGL(k_rescore64, dim3(np), dim3(64 * ntop), 0, h->stream, ra);
+    HIPCHK(h, hipGetLastError());
+    if ((rc = timing_end(h, ev))) return rc;
+  } else if (rc == PFTCC_OK) {
     hipLaunchKernelGGL(k_best_decode_rot, dim3((np + 255) / 256), dim3(256), 0, h->stream,
                        (const unsigned long long*)d_best, np, h->nrots, d_cc, d_ir, d_rot);
+  }
+  if (rc == PFTCC_OK) {
     if (hipMemcpyAsync(best_cc, d_cc, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipMemcpyAsync(best_iref, d_ir, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipMemcpyAsync(best_irot, d_rot, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
@@ -1707,6 +1743,9 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
   dfree(d_cc);
   dfree(d_ir);
   dfree(d_rot);
+  dfree(d_prbest);
+  dfree(d_cand);
+  dfree(d_pl);
   return rc;
 }
 

@@ -106,7 +106,7 @@ SIGNATURES = [
     ("pftcc_search_dev", _I, [_P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P]),
     ("pftcc_grad_batch", _I, [_P, _I, _PI, _PI, _PD, _PI, _PD, _PD]),
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
-    ("pftcc_search_mfma", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _PI, _PI]),
+    ("pftcc_search_mfma", _I, [_P, _I, _PI, _I, _PI, _I, _I, _PF, _PI, _PI]),
     ("pftcc_gencorrs_mfma_batch", _I, [_P, _I, _PI, _I, _PI, _I, _PF]),
     ("pftcc_get_timing_channel", _I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
     ("pftcc_set_timing", _I, [_P, _I]),
@@ -511,8 +511,10 @@ class PolarFTCorrCalc:
                                                  C.c_void_p(d_grad) if d_grad else None),
                   "grad_batch_dev")
 
-    def search_mfma(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16):
-        """Precision form of search() without shifts: (best_cc, best_iref, best_irot)."""
+    def search_mfma(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16, ntop=0):
+        """Precision form of search() without shifts: (best_cc, best_iref, best_irot).
+
+        ntop > 0 rescores the ntop best references per particle exactly in FP64."""
         p = np.ascontiguousarray(iptcls, np.int32)
         r = None if irefs is None else np.ascontiguousarray(irefs, np.int32)
         nr = self.nrefs if r is None else r.size
@@ -520,7 +522,7 @@ class PolarFTCorrCalc:
         ir = np.zeros(p.size, np.int32)
         rot = np.zeros(p.size, np.int32)
         self._chk(self._lib.pftcc_search_mfma(self._h, p.size, _ip(p), nr, None if r is None else _ip(r), prec,
-                                              _fp(cc), _ip(ir), _ip(rot)), "search_mfma")
+                                              ntop, _fp(cc), _ip(ir), _ip(rot)), "search_mfma")
         return cc, ir, rot
 
     def gencorrs_mfma_batch(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16):

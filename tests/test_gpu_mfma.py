@@ -141,3 +141,36 @@ def test_mfma_rejects_unsupported():
     with pytest.raises(PftccError):
         lib.search_mfma(np.arange(1, 3), np.array([3], np.int32))
     lib.kill()
+
+
+@pytest.mark.parametrize("astig", [0.0, 0.3])
+def test_mfma_rescored_search_is_exact(astig):
+    """ntop > 0: the candidates are rescored in FP64 with the particle's own |CTF|, so the result is
+    the oracle's exact argmax (astigmatic CTF included) and its cc the oracle's FP64 value."""
+    say(f"mfma rescored search astig={astig}")
+    pf, cps = make_case(seed=45, n=192, kfrom=2, kto=40, ctf=True, astig=astig, nrefs=12, nptcls=9, ldim=128,
+                        eo_mixed=True)
+    # plant rotated, CTF-multiplied copies of references so that each particle has a clear winner
+    rng = np.random.default_rng(46)
+    for i in range(9):
+        r = int(rng.integers(0, 12))
+        rho = int(rng.integers(0, 2 * pf.pftsz))
+        src = pf.refs_even[r] if pf.iseven[i] else pf.refs_odd[r]
+        rolled = np.roll(O.full_ring(src.astype(np.complex128)), rho, axis=-1)[:, :pf.pftsz]
+        noise = 0.3 * (rng.standard_normal(rolled.shape) + 1j * rng.standard_normal(rolled.shape))
+        pf.ptcls[i] = (rolled + noise * np.abs(rolled).mean()).astype(np.complex64)
+    lib = to_lib(pf, cps)
+    ids = np.arange(1, 10)
+    ref = oracle_cube(pf, range(9), range(12))
+    cc, ir, rot = lib.search_mfma(ids, ntop=4)
+    pcc, pir, _, prot = lib.search(ids)
+    for i in range(9):
+        r, t = np.unravel_index(int(np.argmax(ref[i])), ref[i].shape)
+        assert (ir[i], rot[i]) == (r + 1, t + 1)
+        assert abs(cc[i] - ref[i][r, t]) <= 1e-7
+        assert (pir[i], prot[i]) == (ir[i], rot[i])
+    raw_cc, raw_ir, raw_rot = lib.search_mfma(ids, ntop=0)
+    say(f"  raw vs exact max |dcc| {np.max(np.abs(raw_cc - cc)):.2e}")
+    with pytest.raises(PftccError):
+        lib.search_mfma(ids, ntop=17)
+    lib.kill()
