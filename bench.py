@@ -30,6 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "simple3.0_amd"))
 METRIC = ("ptcl×ref×rot corr evals/s: plain FP32 vs split-BF16 MFMA @ rel err ≤1e-4")
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
@@ -40,6 +41,92 @@ def plain_flops_per_pair(n, K):
     """Algorithmic work of the reference FFT route per (ref, ptcl, shift) pair:
     K*(12.5 n log2 n + 25 n) (r2c_n + c2c_n + c2r_2n per shell at 5 N log2 N, plus products)."""
     return K * (12.5 * n * math.log2(n) + 25.0 * n)
+
+
+def mfma_flops_per_pair(n, kcorr):
+    """Algorithmic work of the per-angular-frequency contraction per (ref, ptcl) pair: one complex
+    multiply-add (8 real flops) per shell for each of the n+1 angular frequencies."""
+    return 8.0 * kcorr * (n + 1)
+
+
+def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
+    """Split-BF16 MFMA form at the 3D-refinement shape, next to the plain FP32 form on the same data."""
+    import pftcc
+    from pftcc import Params, PolarFTCorrCalc, PFTCC_PREC_SPLIT_BF16
+    from pftcc import synth
+
+    box, smpd, kfromto = 256, 1.06, (2, 80)
+    pftsz = pftcc.magic_pftsz(126)
+    w = synth.Workload(box=box, smpd=smpd, kfromto=kfromto, pftsz=pftsz, nrefs=nrefs, nptcls=nptcls, snr=0.03,
+                       seed=20251203, dfrange=(0.8, 3.5), astig=0.1, max_shift=0.0, uniform_dirs=True)
+    synth.make_params(w)
+    lib = PolarFTCorrCalc().new(nrefs, (1, nptcls), Params(boxmatch=box, kfromto=kfromto, pftsz=pftsz, ctf=True))
+    rbuf_refs = synth.DeviceBuffer(nrefs * w.nk * pftsz * 8)
+    synth.gen_refs(w, rbuf_refs)
+    t0 = time.perf_counter()
+    synth.load_into(lib, w, chunk=5000)
+    log(f"precision form setup: {nptcls} particles x {nrefs} refs in {time.perf_counter() - t0:.1f} s")
+    S = synth.synth_lib()
+    ids = np.arange(1, nptcls + 1, dtype=np.int32)
+    nrots = 2 * pftsz
+    kcorr = w.nk
+    lib.search_mfma(ids[:256], prec=PFTCC_PREC_SPLIT_BF16, ntop=ntop)      # warm-up (code objects)
+    S.synth_device_sync()
+    lib.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        # new references every step, as every refinement iteration: the reference operand is rebuilt
+        lib.set_ref_pfts_dev(1, nrefs, rbuf_refs.ptr, True)
+        lib.set_ref_pfts_dev(1, nrefs, rbuf_refs.ptr, False)
+        cc, ir, rot = lib.search_mfma(ids, prec=PFTCC_PREC_SPLIT_BF16, ntop=ntop)
+    S.synth_device_sync()
+    wall = (time.perf_counter() - t0) / steps
+    ch = {name: lib.get_timing_channel(c) for c, name in ((1, "zgemm"), (2, "zifft"), (3, "pack_rescore"))}
+    lib.set_timing(False)
+    rbuf_refs.close()
+    value = nptcls * nrefs * nrots / wall
+    raw_cc, raw_ir, raw_rot = lib.search_mfma(ids[:plain_sample], prec=PFTCC_PREC_SPLIT_BF16, ntop=0)
+    # the plain FP32 form on a sample of the same particles
+    lib.search(ids[:64])
+    S.synth_device_sync()
+    t0 = time.perf_counter()
+    pcc, pir, _, prot = lib.search(ids[:plain_sample])
+    S.synth_device_sync()
+    pwall = time.perf_counter() - t0
+    plain_value = plain_sample * nrefs * nrots / pwall
+    same = (pir == ir[:plain_sample]) & (prot == rot[:plain_sample])
+    same_raw = (pir == raw_ir) & (prot == raw_rot)
+    lib.kill()
+    zg_ms, zg_n = ch["zgemm"]
+    zi_ms, zi_n = ch["zifft"]
+    pairs = nptcls * nrefs
+    zg_tflops = mfma_flops_per_pair(pftsz, kcorr) * pairs / (zg_ms * 1e-3) / 1e12
+    zbytes = pairs * (pftsz + 1) * 8.0
+    return {
+        "workload": "split-BF16 MFMA exhaustive search (no shifts), 3D-refinement shape, top-4 references "
+                    "per particle rescored in FP64 with the exact per-pixel |CTF|",
+        "value": value, "unit": "evals/s", "ms_per_step": 1e3 * wall, "steps": steps,
+        "refs": nrefs, "ptcls": nptcls, "box": box, "smpd": smpd, "kfromto": list(kfromto), "pftsz": pftsz,
+        "nrots": nrots, "ctf": "astigmatic <= 0.1 um, defocus U(0.8,3.5) um", "snr": 0.03,
+        "ref_directions": "even spiral on the sphere (~3 deg)",
+        "plain_fp32_same_shape": {"value": plain_value, "unit": "evals/s", "sample_ptcls": plain_sample},
+        "speedup_vs_plain_fp32": value / plain_value,
+        "assignment_agreement_vs_plain_fp32": float(np.mean(same)),
+        "max_abs_best_cc_diff_vs_plain_fp32": float(np.max(np.abs(pcc - cc[:plain_sample]))),
+        "unrescored": {"assignment_agreement_vs_plain_fp32": float(np.mean(same_raw)),
+                       "max_abs_best_cc_diff_vs_plain_fp32": float(np.max(np.abs(pcc - raw_cc)))},
+        "kernels": {
+            "k_zgemm<3>": {"ms_per_step": zg_ms / steps, "launches_per_step": zg_n / steps,
+                           "achieved": zg_tflops, "unit": "TFLOP/s (algorithmic complex contraction)",
+                           "peak": BF16_PEAK_TFLOPS, "frac": zg_tflops / BF16_PEAK_TFLOPS,
+                           "mfma_issued_frac": 3.0 * zg_tflops / BF16_PEAK_TFLOPS},
+            "k_zifft_ct<400>": {"ms_per_step": zi_ms / steps, "launches_per_step": zi_n / steps,
+                                "achieved": zbytes / (zi_ms / steps * 1e-3) / 1e9,
+                                "unit": "GB/s (spectra read)", "peak": HBM_PEAK_GBS,
+                                "frac": zbytes / (zi_ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "pack_topk_rescore": {"ms_per_step": ch["pack_rescore"][0] / steps},
+        },
+    }
 
 
 def cpu_baseline(pftcc, w, shifts, nthreads, target_s=12.0):
@@ -94,6 +181,8 @@ def main():
     ap.add_argument("--nptcls", type=int, default=100000)
     ap.add_argument("--nrefs", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true", help="skip the MFMA precision-form line")
+    ap.add_argument("--precision-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the search shards particles "
@@ -186,6 +275,8 @@ def main():
         except Exception as e:  # the GPU figure stands on its own
             out["cpu_baseline"] = {"error": repr(e)}
     lib.kill()
+    if not args.no_precision:
+        out["precision_form"] = precision_form(args.precision_steps)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -357,6 +357,164 @@ __global__ void __launch_bounds__(NT) k_zifft_ct(ZifftArgs a) {
   }
 }
 
+// The inverse stage with its first and last passes in registers.
+//  * pass 1 (radix 4, from global): a thread takes the mirrored butterflies g and N/4 - g, whose
+//    inputs Y[m] = f(Z[m], Z[N-m]) need exactly the same 8 spectrum values;
+//  * middle passes in LDS;
+//  * the last pass leaves its outputs in registers, where they are normalised, written (cc_out)
+//    and reduced to per-particle best keys (LDS 64-bit max, order independent).
+template <int N, int F, int NT>
+__global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
+  static_assert(ZG % F == 0, "F divides the Z group");
+  static_assert(Fp<N>::fusable, "first radix must be 4");
+  constexpr int TPF = NT / F;
+  static_assert(TPF * F == NT && TPF <= 64 && (TPF & (TPF - 1)) == 0, "TPF is a power of two <= 64");
+  constexpr int NP = Fp<N>::np;
+  constexpr int Q = N / 4;
+  constexpr int G1 = Q / 2 + 1;                  // mirrored pass-1 groups per transform
+  constexpr int T1 = F * G1;
+  constexpr int TPT1 = (T1 + NT - 1) / NT;
+  constexpr int RL = Fp<N>::R(NP - 1);
+  constexpr int NsL = N / RL;
+  constexpr int TL = F * NsL;
+  constexpr int TPTL = (TL + NT - 1) / NT;
+  extern __shared__ float2 smem[];
+  float2* buf = smem;                    // [F][N]
+  float2* tw2 = buf + F * N;             // [2N]
+  __shared__ double den[F];
+  __shared__ unsigned long long bkey[F];
+  const int tid = threadIdx.x;
+  const int r = blockIdx.y;
+  const int p0 = blockIdx.x * F;
+  for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
+  if (tid < F) bkey[tid] = 0ull;
+  {
+    const int f = tid / TPF, u = tid % TPF;
+    const int p = p0 + f;
+    double s = 0.0;
+    if (p < a.np)
+      for (int kk = u; kk < a.kcorr; kk += TPF)
+        s += a.sr[(size_t)r * a.kcorr + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
+#pragma unroll
+    for (int o = TPF / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, TPF);
+    if (u == 0) den[f] = p < a.np ? sqrt(s * a.sp[p]) : 1.0;
+  }
+  // ---- pass 1: loads first (they do not depend on LDS), then the butterflies ----
+  const float2* Zb = a.Z + zidx(r, p0, 0, N, a.cols_pad);
+  float2 za[TPT1][4], zb[TPT1][4], zn[TPT1];
+#pragma unroll
+  for (int c = 0; c < TPT1; ++c) {
+    const int task = tid + c * NT;
+    if (!(T1 % NT == 0 || task < T1)) continue;
+    const int g = task / F, f = task % F;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      za[c][t] = Zb[(size_t)(g + t * Q) * ZG + f];
+      zb[c][t] = Zb[(size_t)((t + 1) * Q - g) * ZG + f];
+    }
+    zn[c] = zb[c][3];   // Z[N - g]; for g = 0 this is Z[N]
+  }
+  __syncthreads();   // tw2
+#pragma unroll
+  for (int c = 0; c < TPT1; ++c) {
+    const int task = tid + c * NT;
+    if (!(T1 % NT == 0 || task < T1)) continue;
+    const int g = task / F, f = task % F;
+    float2* o = buf + f * N;
+    float2 y[4];
+    if (g == 0) {
+      float2 x0 = za[c][0], xn = zn[c];
+      x0.y = 0.f;
+      xn.y = 0.f;
+      y[0] = unsplit_bin_conj(x0, xn, cconj(tw2[0]));
+      y[1] = unsplit_bin_conj(za[c][1], za[c][3], cconj(tw2[Q]));
+      y[2] = unsplit_bin_conj(za[c][2], za[c][2], cconj(tw2[2 * Q]));
+      y[3] = unsplit_bin_conj(za[c][3], za[c][1], cconj(tw2[3 * Q]));
+      Dft<4>::run(y);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) o[t] = y[t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) y[t] = unsplit_bin_conj(za[c][t], zb[c][3 - t], cconj(tw2[g + t * Q]));
+      Dft<4>::run(y);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) o[4 * g + t] = y[t];
+      if (2 * g != Q) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) y[t] = unsplit_bin_conj(zb[c][t], za[c][3 - t], cconj(tw2[(t + 1) * Q - g]));
+        Dft<4>::run(y);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) o[4 * (Q - g) + t] = y[t];
+      }
+    }
+  }
+  __syncthreads();
+  lds_passes<N, F, NT, 1, NP - 1>(buf, tw2, tid);
+  // ---- last pass in registers, then normalisation, output and best keys ----
+  const double inv4n = 1.0 / (4.0 * N);
+  const int iref = a.rlist ? a.rlist[r] : r;
+  const bool want_key = a.best || a.prbest;
+#pragma unroll
+  for (int c = 0; c < TPTL; ++c) {
+    const int task = tid + c * NT;
+    if (!(TL % NT == 0 || task < TL)) continue;
+    const int f = task / NsL, j = task % NsL;
+    const int p = p0 + f;
+    float2 v[RL];
+#pragma unroll
+    for (int t = 0; t < RL; ++t) v[t] = buf[f * N + j + t * NsL];
+#pragma unroll
+    for (int t = 1; t < RL; ++t) v[t] = cmul(v[t], tw2[2 * j * t]);
+    Dft<RL>::run(v);
+    if (p >= a.np) continue;
+    const double sc = inv4n / den[f];
+    unsigned long long bk = 0ull;
+#pragma unroll
+    for (int t = 0; t < RL; ++t) {
+      const int jj = j + t * NsL;                   // output index: rotations 2 jj, 2 jj + 1
+      const float x0 = (float)(v[t].x * sc), x1 = (float)(-v[t].y * sc);
+      if (a.cc_out) {
+        float2* oo = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * N));
+        oo[jj] = make_float2(x0, x1);
+      }
+      if (want_key) {
+        const unsigned int lin0 = (unsigned int)iref * (unsigned int)(2 * N) + 2u * jj;
+        const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
+        const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
+        bk = k0 > bk ? k0 : bk;
+        bk = k1 > bk ? k1 : bk;
+      }
+    }
+    if (want_key) atomicMax(&bkey[f], bk);
+  }
+  if (want_key) {
+    __syncthreads();
+    if (tid < F) {
+      const int p = p0 + tid;
+      if (p < a.np) {
+        if (a.best) atomicMax(&a.best[a.outpos[p]], bkey[tid]);
+        if (a.prbest) a.prbest[(size_t)a.outpos[p] * a.nrefs + r] = bkey[tid];
+      }
+    }
+  }
+}
+
+#ifndef PFTCC_ZIFFT_F
+#define PFTCC_ZIFFT_F 8
+#endif
+#ifndef PFTCC_ZIFFT_NT
+#define PFTCC_ZIFFT_NT 256
+#endif
+
+template <int N>
+struct ZifftV2 {
+  static constexpr int NT = PFTCC_ZIFFT_NT;
+  static constexpr int F0 = PFTCC_ZIFFT_F;
+  static constexpr int F = (sizeof(float2) * (F0 * N + 2 * N) <= 65536) ? F0
+                         : (sizeof(float2) * (F0 / 2 * N + 2 * N) <= 65536) ? F0 / 2 : F0 / 4;
+  static constexpr size_t lds = sizeof(float2) * ((size_t)F * N + 2 * N);
+};
+
 template <int N>
 struct ZifftCt {
   static constexpr int NT = 256;

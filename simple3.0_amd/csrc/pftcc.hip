@@ -658,9 +658,9 @@ int round_up(int x, int m) { return (x + m - 1) / m * m; }
 bool zifft_ct_go(int n, int cnt, int nr, hipStream_t s, const ZifftArgs& a) {
 #define X(N)                                                                                          \
   if (n == N) {                                                                                       \
-    hipLaunchKernelGGL((k_zifft_ct<N, ZifftCt<N>::F, ZifftCt<N>::NT>),                               \
-                       dim3((cnt + ZifftCt<N>::F - 1) / ZifftCt<N>::F, nr), dim3(ZifftCt<N>::NT),    \
-                       ZifftCt<N>::lds, s, a);                                                        \
+    hipLaunchKernelGGL((k_zifft_v2<N, ZifftV2<N>::F, ZifftV2<N>::NT>),                               \
+                       dim3((cnt + ZifftV2<N>::F - 1) / ZifftV2<N>::F, nr), dim3(ZifftV2<N>::NT),    \
+                       ZifftV2<N>::lds, s, a);                                                        \
     return true;                                                                                      \
   }
   PFTCC_FUSED_SIZES(X)
@@ -732,7 +732,7 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
   const size_t words = (size_t)(n + 1) * Pc * (KP / 2);
   const size_t zsz = (size_t)(n + 1) * ap.rows_pad * Pc;
   if ((rc = scratch_reserve(h, aligned((size_t)total * 4) * 2 + aligned(nr * 4) + aligned((size_t)Pc * nk * (n + 1) * 8) +
-                                   aligned((size_t)Pc * nk * 4) + aligned((size_t)Pc * nk * 8) + aligned((size_t)Pc * 8) +
+                                   aligned((size_t)Pc * nk * 4) + aligned((size_t)Pc * nk * 8) + aligned((size_t)total * 8) +
                                    2 * aligned(words * 4) + aligned(zsz * 8))))
     return rc;
   int* d_pl = take<int>(h, total);
@@ -741,13 +741,16 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
   float2* wg = take<float2>(h, (size_t)Pc * nk * (n + 1));
   float* cbar = take<float>(h, (size_t)Pc * nk);
   double* cbar2 = take<double>(h, (size_t)Pc * nk);
-  double* sp = take<double>(h, Pc);
+  double* spall = take<double>(h, total);
   unsigned int* bhi = take<unsigned int>(h, words);
   unsigned int* blo = take<unsigned int>(h, words);
   float2* Z = take<float2>(h, zsz);
   HIPCHK(h, hipMemcpyAsync(d_pl, pl.data(), total * 4, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(d_pos, outpos.data(), total * 4, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(d_rl, rl.data(), nr * 4, hipMemcpyHostToDevice, h->stream));
+  // FP64 particle sums of squares over the correlated shells, all particles of the group at once
+  hipLaunchKernelGGL(k_ring_sq64_list, dim3((total + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)h->d_ptcls,
+                     (const int*)d_pl, total, nk, n, h->kcorr, spall);
   for (int c0 = 0; c0 < total; c0 += Pc) {
     const int cnt = std::min(Pc, total - c0);
     hipEvent_t ev;
@@ -759,8 +762,6 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     if (ctf)
       hipLaunchKernelGGL(k_ring_mean_ctf_list, dim3((cnt * nk + 3) / 4), dim3(64, 4), 0, h->stream,
                          (const float*)h->d_ctf, (const int*)(d_pl + c0), cnt, nk, n, cbar, cbar2);
-    hipLaunchKernelGGL(k_ring_sq64_list, dim3((cnt + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)h->d_ptcls,
-                       (const int*)(d_pl + c0), cnt, nk, n, h->kcorr, sp);
     hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream, (const float2*)wg,
                        ctf ? (const float*)cbar : nullptr, cnt, Pc, nk, h->kcorr, n, KP, bhi, blo);
     HIPCHK(h, hipGetLastError());
@@ -785,7 +786,7 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     za.sr = ap.sr;
     za.cbar2 = ctf ? cbar2 : nullptr;
     za.nk = nk;
-    za.sp = sp;
+    za.sp = spall + c0;
     za.rlist = d_rl;
     za.outpos = d_pos + c0;
     za.cc_out = d_cc;

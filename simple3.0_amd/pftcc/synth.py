@@ -117,6 +117,7 @@ class Workload:
     astig: float = 0.1
     max_shift: float = 5.0
     nblobs: int = 50
+    uniform_dirs: bool = False
     # filled by generate
     blobs: np.ndarray = field(default=None, repr=False)
     uvs: np.ndarray = field(default=None, repr=False)
@@ -142,9 +143,24 @@ def phantom(rng, nblobs=50, radius=40.0):
     return out
 
 
-def projection_axes(rng, count):
-    """Orthonormal (u, v) of planes perpendicular to uniform random directions: [count, 6]."""
-    d = rng.standard_normal((count, 3))
+def spiral_directions(count):
+    """Near-uniform directions on the sphere (golden-angle spiral): [count, 3]."""
+    i = np.arange(count, dtype=np.float64) + 0.5
+    z = 1.0 - 2.0 * i / count
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = i * math.pi * (3.0 - math.sqrt(5.0))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def projection_axes(rng, count, uniform=False):
+    """Orthonormal (u, v) of planes perpendicular to the projection directions: [count, 6].
+
+    Directions are uniform random, or an even spiral sampling of the sphere when uniform is set
+    (4684 directions are ~3 degrees apart)."""
+    if uniform:
+        d = spiral_directions(count)
+    else:
+        d = rng.standard_normal((count, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     helper = np.where(np.abs(d[:, 2:3]) < 0.9, np.array([[0, 0, 1.0]]), np.array([[1.0, 0, 0]]))
     u = np.cross(d, helper)
@@ -164,7 +180,7 @@ def ctf_pixel_constants(p: CtfP):
 def make_params(w: Workload):
     rng = np.random.default_rng(w.seed)
     w.blobs = phantom(rng, w.nblobs)
-    w.uvs = projection_axes(rng, w.nrefs)
+    w.uvs = projection_axes(rng, w.nrefs, w.uniform_dirs)
     prm = np.zeros(w.nptcls, PTCL_DTYPE)
     prm["ref"] = rng.integers(0, w.nrefs, w.nptcls)
     prm["psi"] = rng.uniform(0, 2 * np.pi, w.nptcls)
