@@ -204,12 +204,15 @@ int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs,
  * Denominators sqrt(sum_k cbar_k^2 S_ref,k * S_ptcl) are formed in FP64.  No counterpart
  * entry point in the reference; it replaces the same greedy loop as pftcc_search
  * (simple_strategy3D_greedy.f90:60-77, gencorrs over all references + maxloc).
- * ntop = 0: the best (reference, rotation) of the contraction itself.  ntop in [1, 16]: the ntop
+ * ntop = 0: the best (reference, rotation) of the contraction itself.  ntop in [1, 64]: the ntop
  * references with the highest contraction score per particle are rescored exactly in FP64 (the
  * reference's gencorrs formula with the particle's own |CTF|) at their best rotation and its two
  * neighbours, and the best rescored candidate is returned with its FP64 cc. */
 #define PFTCC_PREC_BF16 1
 #define PFTCC_PREC_SPLIT_BF16 3
+/* FP8 e4m3 coarse pass: per-shell power-of-two scales balance the two operands, a global power of
+ * two per operand maps its largest component to <= 448; meant for ntop > 0 (exact rescoring). */
+#define PFTCC_PREC_FP8 8
 int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
                       int32_t prec, int32_t ntop, float* best_cc, int32_t* best_iref, int32_t* best_irot);
 /* cc_out[np][nr][nrots] of the same computation (for verification and small batches). */

@@ -271,92 +271,6 @@ __global__ void __launch_bounds__(256) k_zifft(ZifftArgs a) {
   }
 }
 
-// Compile-time-size inverse stage: F particles (of one Z group) x one reference per work-group.
-//  * Z[m] and Z[N-m] of a pair are read together and turned into the conjugated inverse-FFT
-//    inputs Y[m], Y[N-m] on the way into LDS (reads are contiguous over f for each m);
-//  * all passes run register-staged in LDS (lds_passes);
-//  * the FP64 denominator sqrt(sum_k cbar_k^2 S_r,k * S_p) is split over NT/F threads per pair;
-//  * the best key of each pair is reduced by NT/F lanes of one wavefront with shuffles.
-template <int N, int F, int NT>
-__global__ void __launch_bounds__(NT) k_zifft_ct(ZifftArgs a) {
-  static_assert(ZG % F == 0, "F divides the Z group");
-  constexpr int TPF = NT / F;            // threads per pair
-  static_assert(TPF * F == NT && TPF <= 64 && (TPF & (TPF - 1)) == 0, "TPF is a power of two <= 64");
-  constexpr int H = N / 2 + 1;           // mirrored bin pairs (m, N - m), m = 0 .. N/2
-  extern __shared__ float2 smem[];
-  float2* buf = smem;                    // [F][N]
-  float2* tw2 = buf + F * N;             // [2N]
-  __shared__ double den[F];
-  const int tid = threadIdx.x;
-  const int r = blockIdx.y;
-  const int p0 = blockIdx.x * F;
-  for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
-  // FP64 denominators: TPF threads per pair, shuffle-reduced
-  {
-    const int f = tid / TPF, u = tid % TPF;
-    const int p = p0 + f;
-    double s = 0.0;
-    if (p < a.np)
-      for (int kk = u; kk < a.kcorr; kk += TPF)
-        s += a.sr[(size_t)r * a.kcorr + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
-#pragma unroll
-    for (int o = TPF / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, TPF);
-    if (u == 0) den[f] = p < a.np ? sqrt(s * a.sp[p]) : 1.0;
-  }
-  __syncthreads();   // tw2 complete
-  const float2* Zb = a.Z + zidx(r, p0, 0, N, a.cols_pad);   // group base + p0 % ZG
-  for (int t = tid; t < H * F; t += NT) {
-    const int m = t / F, f = t % F;
-    float2 X = Zb[(size_t)m * ZG + f];
-    float2 Xn = Zb[(size_t)(N - m) * ZG + f];
-    if (m == 0) {
-      X.y = 0.f;
-      Xn.y = 0.f;
-    }
-    buf[f * N + m] = unsplit_bin_conj(X, Xn, cconj(tw2[m]));
-    if (m != 0 && 2 * m != N) buf[f * N + (N - m)] = unsplit_bin_conj(Xn, X, cconj(tw2[N - m]));
-  }
-  __syncthreads();
-  lds_passes<N, F, NT, 0, Fp<N>::np>(buf, tw2, tid);
-  const double inv4n = 1.0 / (4.0 * N);
-  if (a.cc_out) {
-    for (int idx = tid; idx < F * N; idx += NT) {
-      const int f = idx / N, j = idx % N;
-      const int p = p0 + f;
-      if (p >= a.np || r >= a.nrefs) continue;
-      const float2 y = buf[idx];
-      const double sc = inv4n / den[f];
-      float2* o = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * N));
-      o[j] = make_float2((float)(y.x * sc), (float)(-y.y * sc));
-    }
-  }
-  if (a.best || a.prbest) {
-    const int f = tid / TPF, u = tid % TPF;
-    const int p = p0 + f;
-    const int iref = a.rlist ? a.rlist[r] : r;
-    const double sc = inv4n / den[f];
-    unsigned long long bk = 0ull;
-    for (int j = u; j < N; j += TPF) {
-      const float2 y = buf[f * N + j];
-      const float x0 = (float)(y.x * sc), x1 = (float)(-y.y * sc);
-      const unsigned int lin0 = (unsigned int)iref * (unsigned int)(2 * N) + 2u * j;
-      const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
-      const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
-      bk = k0 > bk ? k0 : bk;
-      bk = k1 > bk ? k1 : bk;
-    }
-#pragma unroll
-    for (int o = TPF / 2; o > 0; o >>= 1) {
-      const unsigned long long v = __shfl_xor(bk, o, TPF);
-      bk = v > bk ? v : bk;
-    }
-    if (u == 0 && p < a.np && r < a.nrefs) {
-      if (a.best) atomicMax(&a.best[a.outpos[p]], bk);
-      if (a.prbest) a.prbest[(size_t)a.outpos[p] * a.nrefs + r] = bk;
-    }
-  }
-}
-
 // The inverse stage with its first and last passes in registers.
 //  * pass 1 (radix 4, from global): a thread takes the mirrored butterflies g and N/4 - g, whose
 //    inputs Y[m] = f(Z[m], Z[N-m]) need exactly the same 8 spectrum values;
@@ -515,13 +429,168 @@ struct ZifftV2 {
   static constexpr size_t lds = sizeof(float2) * ((size_t)F * N + 2 * N);
 };
 
-template <int N>
-struct ZifftCt {
-  static constexpr int NT = 256;
-  static constexpr int F = (sizeof(float2) * (16 * N + 2 * N) <= 65536) ? 16
-                         : (sizeof(float2) * (8 * N + 2 * N) <= 65536) ? 8 : 4;
-  static constexpr size_t lds = sizeof(float2) * ((size_t)F * N + 2 * N);
-};
+// ---------------------------------------------------------------- FP8 e4m3 coarse pass ---
+// Operands are scaled per shell by a power of two, V'_k = a_k V_k and W'_k = W_k / a_k, so the
+// contraction sum_k conj(V_k) W_k is unchanged before rounding (a_k balances the magnitudes of the
+// two operands shell by shell), and then by a global power of two per operand that maps its largest
+// component to at most 448, the e4m3 maximum.  Z is scaled back by the two global factors.
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+// Per-shell sums of |spec|^2 over listed rows (list null: rows 0..count-1) and m; partial sums
+// [kk][gridDim.y] in a fixed order.
+__global__ void __launch_bounds__(256) k_shell_ms(const float2* __restrict__ spec, const int* __restrict__ list,
+                                                  int count, int nk, int n, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int kk = blockIdx.x, chunk = blockIdx.y, nch = gridDim.y;
+  const int per = (count + nch - 1) / nch;
+  const int r0 = chunk * per, r1 = min(count, r0 + per);
+  double s = 0.0;
+  const int len = n + 1;
+  for (int e = threadIdx.x; e < (r1 - r0) * len; e += 256) {
+    const int rl = r0 + e / len, m = e % len;
+    const int row = list ? list[rl] : rl;
+    const float2 v = spec[((size_t)row * nk + kk) * len + m];
+    s += (double)v.x * v.x + (double)v.y * v.y;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)kk * nch + chunk] = red[0];
+}
+
+// Largest |component| of spec[row][kk][m] * rowscale[row][kk] * shellscale[kk] (kk < kcorr), as
+// the bits of a non-negative float (atomicMax on the bit pattern orders non-negative floats).
+// Grid-stride loop, 256 threads, one atomic per work-group.
+__global__ void k_absmax_scaled(const float2* __restrict__ spec, const float* __restrict__ rowscale,
+                                const float* __restrict__ shellscale, int count, int nk, int n, int kcorr,
+                                unsigned int* __restrict__ out) {
+  const size_t tot = (size_t)count * kcorr * (n + 1);
+  float mx = 0.f;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(idx % (n + 1));
+    const int kk = (int)((idx / (n + 1)) % kcorr);
+    const int row = (int)(idx / ((size_t)(n + 1) * kcorr));
+    const float2 v = spec[((size_t)row * nk + kk) * (n + 1) + m];
+    const float sc = (rowscale ? rowscale[(size_t)row * nk + kk] : 1.0f) * shellscale[kk];
+    mx = fmaxf(mx, fmaxf(fabsf(v.x * sc), fabsf(v.y * sc)));
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  __shared__ float wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float b = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    if (b > 0.f) atomicMax(out, __float_as_uint(b));
+  }
+}
+
+// Power of two g with max * g <= 448.
+__device__ __forceinline__ float fp8_gain(const unsigned int* gmax) {
+  const float m = __uint_as_float(*gmax);
+  if (!(m > 0.f)) return 1.0f;
+  int e;
+  (void)frexpf(448.0f / m, &e);   // 448 / m = f 2^e, f in [0.5, 1)
+  return ldexpf(1.0f, e - 1);
+}
+
+// e4m3 operand tiles out[m][row][k'] (bytes; k' = 2 kk + {re, im}, zero past 2 kcorr), one 32-bit
+// word (two complex values) per thread.
+__global__ void k_pack_fp8(const float2* __restrict__ spec, const float* __restrict__ rowscale,
+                           const float* __restrict__ shellscale, const unsigned int* __restrict__ gmax, int count,
+                           int rows_pad, int nk, int kcorr, int n, int KP8, unsigned int* __restrict__ out) {
+  const size_t KW = KP8 / 4;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t tot = (size_t)(n + 1) * rows_pad * KW;
+  if (idx >= tot) return;
+  const int w = (int)(idx % KW);
+  const int row = (int)((idx / KW) % rows_pad);
+  const int m = (int)(idx / (KW * rows_pad));
+  const float g = fp8_gain(gmax);
+  float v[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int kk = 2 * w + h;
+    if (row < count && kk < kcorr) {
+      const float2 x = spec[((size_t)row * nk + kk) * (n + 1) + m];
+      const float sc = (rowscale ? rowscale[(size_t)row * nk + kk] : 1.0f) * shellscale[kk] * g;
+      v[2 * h] = x.x * sc;
+      v[2 * h + 1] = x.y * sc;
+    }
+  }
+  unsigned int word = (unsigned int)__builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+  word = (unsigned int)__builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], (int)word, true);
+  out[idx] = word;
+}
+
+// (re, im) byte pairs -> (-im, re); a zero keeps a clear sign bit.
+__device__ __forceinline__ unsigned int rot_neg8_word(unsigned int w) {
+  const unsigned int sw = ((w & 0x00FF00FFu) << 8) | ((w >> 8) & 0x00FF00FFu);
+  unsigned int flip = 0u;
+  if (sw & 0x0000007Fu) flip |= 0x00000080u;
+  if (sw & 0x007F0000u) flip |= 0x00800000u;
+  return sw ^ flip;
+}
+
+__device__ __forceinline__ i32x8 rot_neg8(i32x8 a) {
+  i32x8 r;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r[i] = (int)rot_neg8_word((unsigned int)a[i]);
+  return r;
+}
+
+// FP8 contraction: Z[r][p][m] = (A1 . B, A2 . B) / (gA gB) for one 64 x 64 tile per wave, 128 k'
+// per MFMA (16x16x128, e4m3 x e4m3, FP32 accumulation).
+__global__ void __launch_bounds__(64) k_zgemm8(const i32x8* __restrict__ A, const i32x8* __restrict__ B, int rows_pad,
+                                               int cols_pad, int KP8, float2* __restrict__ Z,
+                                               const unsigned int* __restrict__ gmaxA,
+                                               const unsigned int* __restrict__ gmaxB) {
+  const int l = threadIdx.x;
+  const int p0 = blockIdx.x * 64, r0 = blockIdx.y * 64, m = blockIdx.z;
+  const int n = gridDim.z - 1;
+  const int KV = KP8 / 32;                                 // i32x8 vectors per row
+  const size_t abase = ((size_t)m * rows_pad + r0) * KV;
+  const size_t bbase = ((size_t)m * cols_pad + p0) * KV;
+  f32x4 zr[4][4], zi[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      zr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      zi[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  const int lr = l % 16, lk = l / 16;
+  for (int ks = 0; ks < KV; ks += 4) {
+    i32x8 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = A[abase + (size_t)(i * 16 + lr) * KV + ks + lk];
+      b[i] = B[bbase + (size_t)(i * 16 + lr) * KV + ks + lk];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const i32x8 a2 = rot_neg8(a[i]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        zr[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], zr[i][j], 0, 0, 0, 0, 0, 0);
+        zi[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a2, b[j], zi[i][j], 0, 0, 0, 0, 0, 0);
+      }
+    }
+  }
+  const float inv = 1.0f / (fp8_gain(gmaxA) * fp8_gain(gmaxB));
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = r0 + i * 16 + 4 * lk + e;
+        const int p = p0 + j * 16 + lr;
+        Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e] * inv, zi[i][j][e] * inv);
+      }
+}
 
 // Gathers rows of a [N][nk][len] float2 array by a list into [count][nk][len].
 __global__ void k_gather_rows(const float2* __restrict__ src, const int* __restrict__ list, int count, size_t rowlen,
@@ -612,29 +681,39 @@ struct RescoreArgs {
 // Exact rescoring of the candidates in FP64: for each candidate (reference, rho) the rotations
 // rho - 1, rho, rho + 1 are evaluated as in gencorrs (unshifted): q = ref * |CTF| with the
 // particle's own angle-dependent |CTF|, cc = sum_k<kcorr sum_j Re q[k,j] conj(P[k,(j+rho) mod 2n])
-// / sqrt(sum |q|^2 * sum |p|^2).  One wavefront per candidate, one work-group (T wavefronts) per
-// particle; the best cc wins, ties to the lowest (reference, rotation).
-__global__ void __launch_bounds__(1024) k_rescore64(RescoreArgs a) {
-  __shared__ double ccs[16 * 3];
-  __shared__ unsigned int lins[16 * 3];
+// / sqrt(sum |q|^2 * sum |p|^2).  One work-group of RW wavefronts per particle, each wavefront
+// taking candidates w, w + RW, ...; the best cc wins, ties to the lowest (reference, rotation).
+constexpr int RESCORE_MAX = 64;
+constexpr int RESCORE_WAVES = 16;
+__global__ void __launch_bounds__(64 * RESCORE_WAVES) k_rescore64(RescoreArgs a) {
+  __shared__ double ccs[RESCORE_MAX * 3];
+  __shared__ unsigned int lins[RESCORE_MAX * 3];
   const int i = blockIdx.x;
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int n = a.n, n2 = 2 * n;
-  const unsigned long long key = a.cand[(size_t)i * a.T + w];
   const int ip = a.plist[i];
-  if (key != 0ull) {
+  const float2* pt = a.ptcls + (size_t)ip * a.nk * n;
+  const float* ct = a.ctf ? a.ctf + (size_t)ip * a.nk * n : nullptr;
+  for (int c = w; c < a.T; c += RESCORE_WAVES) {
+    const unsigned long long key = a.cand[(size_t)i * a.T + c];
+    if (key == 0ull) {
+      if (lane == 0)
+        for (int t = 0; t < 3; ++t) {
+          ccs[c * 3 + t] = -INFINITY;
+          lins[c * 3 + t] = 0xFFFFFFFFu;
+        }
+      continue;
+    }
     const unsigned int lin = 0xFFFFFFFFu - (unsigned int)(key & 0xFFFFFFFFull);
     const int iref = (int)(lin / (unsigned)n2), rho = (int)(lin % (unsigned)n2);
     const int rr[3] = {(rho + n2 - 1) % n2, rho, (rho + 1) % n2};
     const float2* ref = (a.iseven[ip] ? a.refs_even : a.refs_odd) + (size_t)iref * a.nk * n;
-    const float2* pt = a.ptcls + (size_t)ip * a.nk * n;
-    const float* ct = a.ctf ? a.ctf + (size_t)ip * a.nk * n : nullptr;
     double num0 = 0.0, num1 = 0.0, num2 = 0.0, sq = 0.0, sp = 0.0;
     for (int e = lane; e < a.kcorr * n; e += 64) {
       const int k = e / n, j = e % n;
       const float2 rv = ref[e];
-      const double c = ct ? (double)ct[e] : 1.0;
-      const double qx = rv.x * c, qy = rv.y * c;
+      const double cv = ct ? (double)ct[e] : 1.0;
+      const double qx = rv.x * cv, qy = rv.y * cv;
       sq += qx * qx + qy * qy;
       const float2 pv = pt[e];
       sp += (double)pv.x * pv.x + (double)pv.y * pv.y;
@@ -668,17 +747,10 @@ __global__ void __launch_bounds__(1024) k_rescore64(RescoreArgs a) {
     if (lane == 0) {
       const double d = sqrt(sq * sp);
       const double nums[3] = {num0, num1, num2};
-#pragma unroll
       for (int t = 0; t < 3; ++t) {
-        ccs[w * 3 + t] = nums[t] / d;
-        lins[w * 3 + t] = (unsigned int)iref * (unsigned int)n2 + (unsigned int)rr[t];
+        ccs[c * 3 + t] = nums[t] / d;
+        lins[c * 3 + t] = (unsigned int)iref * (unsigned int)n2 + (unsigned int)rr[t];
       }
-    }
-  } else if (lane == 0) {
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      ccs[w * 3 + t] = -INFINITY;
-      lins[w * 3 + t] = 0xFFFFFFFFu;
     }
   }
   __syncthreads();

@@ -118,8 +118,17 @@ struct pftcc {
     bool valid = false;
     std::vector<int> refs;            // 0-based reference ids of the rows
     int rows_pad = 0, KP = 0, kcorr = 0;
+    bool fp8 = false;                 // e4m3 operand (a8) instead of split BF16 (hi, lo)
     unsigned int *hi = nullptr, *lo = nullptr;
     double* sr = nullptr;             // [nr][kcorr]
+    // FP8: per-shell mean squares of the reference spectra, shell scales a_k (device copies of
+    // a_k and 1/a_k), the operand's largest component and the operand itself
+    std::vector<double> msV;
+    std::vector<float> ak;
+    float *d_ak = nullptr, *d_bk = nullptr;
+    unsigned int* gmax = nullptr;
+    int KP8 = 0;
+    unsigned int* a8 = nullptr;
   } apack[2];
   // kernel timing (pftcc_set_timing)
   bool timing = false;
@@ -670,23 +679,71 @@ bool zifft_ct_go(int n, int cnt, int nr, hipStream_t s, const ZifftArgs& a) {
 
 size_t zifft_lds(int F, int n) { return sizeof(float2) * ((size_t)F * (n + 1) + (size_t)F * n + 2 * n); }
 
-// Split-BF16 reference operand of eo class eo over the (0-based) reference list rl; cached.
-int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl) {
-  pftcc::APack& ap = h->apack[eo];
-  if (ap.valid && ap.refs == rl && ap.kcorr == h->kcorr) return PFTCC_OK;
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+void free_apack(pftcc::APack& ap) {
   dfree(ap.hi);
   dfree(ap.lo);
   dfree(ap.sr);
+  dfree(ap.d_ak);
+  dfree(ap.d_bk);
+  dfree(ap.gmax);
+  dfree(ap.a8);
   ap.valid = false;
+}
+
+// FP8 shell scales a_k = 2^round(log2(msW_k / msV_k) / 2): conj(a_k V) (W / a_k) = conj(V) W.
+std::vector<float> fp8_shell_scales(const std::vector<double>& msV, const std::vector<double>& msW) {
+  std::vector<float> ak(msV.size(), 1.0f);
+  for (size_t k = 0; k < msV.size(); ++k)
+    if (msV[k] > 0.0 && msW[k] > 0.0) ak[k] = (float)std::ldexp(1.0, (int)std::lround(0.5 * std::log2(msW[k] / msV[k])));
+  return ak;
+}
+
+// Per-shell mean |spec|^2 over count listed rows (list null: 0..count-1), deterministic order.
+int shell_ms(pftcc* h, const float2* spec, const int* d_list, int count, std::vector<double>& out) {
+  const int kc = h->kcorr, nch = 64;
+  double* part = nullptr;
+  int rc;
+  if ((rc = dalloc(h, &part, (size_t)kc * nch))) return rc;
+  hipLaunchKernelGGL(k_shell_ms, dim3(kc, nch), dim3(256), 0, h->stream, spec, d_list, count, h->nk, h->n, part);
+  std::vector<double> hp((size_t)kc * nch);
+  const bool ok = hipGetLastError() == hipSuccess &&
+                  hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream) == hipSuccess &&
+                  hipStreamSynchronize(h->stream) == hipSuccess;
+  dfree(part);
+  if (!ok) return fail(h, PFTCC_E_DEVICE, "shell statistics failed");
+  out.assign(kc, 0.0);
+  for (int k = 0; k < kc; ++k) {
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += hp[(size_t)k * nch + c];
+    out[k] = s / ((double)count * (h->n + 1));
+  }
+  return PFTCC_OK;
+}
+
+// Reference operand of eo class eo over the (0-based) reference list rl; cached.  Split BF16, or
+// FP8 e4m3 with shell scales balanced against the particles' mean squares msW.
+int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, bool fp8, const std::vector<double>* msW) {
+  pftcc::APack& ap = h->apack[eo];
+  if (ap.valid && ap.refs == rl && ap.kcorr == h->kcorr && ap.fp8 == fp8 &&
+      (!fp8 || fp8_shell_scales(ap.msV, *msW) == ap.ak))
+    return PFTCC_OK;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  free_apack(ap);
   const int nr = (int)rl.size();
   const int n = h->n, nk = h->nk;
   ap.rows_pad = round_up(nr, 64);
   ap.KP = round_up(2 * h->kcorr, 32);
-  const size_t words = (size_t)(n + 1) * ap.rows_pad * (ap.KP / 2);
+  ap.KP8 = round_up(2 * h->kcorr, 128);
+  const size_t words = fp8 ? (size_t)(n + 1) * ap.rows_pad * (ap.KP8 / 4) : (size_t)(n + 1) * ap.rows_pad * (ap.KP / 2);
   int rc;
-  if ((rc = dalloc(h, &ap.hi, words)) || (rc = dalloc(h, &ap.lo, words)) || (rc = dalloc(h, &ap.sr, (size_t)nr * h->kcorr)))
+  if ((rc = dalloc(h, &ap.sr, (size_t)nr * h->kcorr))) return rc;
+  if (fp8) {
+    if ((rc = dalloc(h, &ap.a8, words)) || (rc = dalloc(h, &ap.d_ak, (size_t)h->kcorr)) ||
+        (rc = dalloc(h, &ap.d_bk, (size_t)h->kcorr)) || (rc = dalloc(h, &ap.gmax, (size_t)1)))
+      return rc;
+  } else if ((rc = dalloc(h, &ap.hi, words)) || (rc = dalloc(h, &ap.lo, words))) {
     return rc;
+  }
   int* d_list = nullptr;
   float2 *rings = nullptr, *spec = nullptr;
   if ((rc = dalloc(h, &d_list, (size_t)nr)) || (rc = dalloc(h, &rings, (size_t)nr * nk * n)) ||
@@ -700,8 +757,25 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl) {
   while (F > 1 && sizeof(float2) * ((size_t)2 * F * n + 2 * n) > kMaxLds) F /= 2;
   hipLaunchKernelGGL(k_memoize, dim3((nr * nk + F - 1) / F), dim3(256), sizeof(float2) * ((size_t)2 * F * n + 2 * n),
                      h->stream, (const float2*)rings, 0, nr, nk, n, h->plan, (const float2*)h->d_tw2, spec, F);
-  hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream, (const float2*)spec,
-                     (const float*)nullptr, nr, ap.rows_pad, nk, h->kcorr, n, ap.KP, ap.hi, ap.lo);
+  if (fp8) {
+    if ((rc = shell_ms(h, spec, nullptr, nr, ap.msV))) return rc;
+    ap.ak = fp8_shell_scales(ap.msV, *msW);
+    std::vector<float> bk(ap.ak.size());
+    for (size_t k = 0; k < bk.size(); ++k) bk[k] = 1.0f / ap.ak[k];
+    HIPCHK(h, hipMemcpyAsync(ap.d_ak, ap.ak.data(), ap.ak.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ap.d_bk, bk.data(), bk.size() * 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, h->stream, ap.gmax, 1, 0u);
+    const size_t tot = (size_t)nr * h->kcorr * (n + 1);
+    hipLaunchKernelGGL(k_absmax_scaled, dim3((unsigned)std::min<size_t>(1024, (tot + 255) / 256)), dim3(256), 0, h->stream,
+                       (const float2*)spec, (const float*)nullptr, (const float*)ap.d_ak, nr, nk, n, h->kcorr, ap.gmax);
+    hipLaunchKernelGGL(k_pack_fp8, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream, (const float2*)spec,
+                       (const float*)nullptr, (const float*)ap.d_ak, (const unsigned int*)ap.gmax, nr, ap.rows_pad, nk,
+                       h->kcorr, n, ap.KP8, ap.a8);
+  } else {
+    hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
+                       (const float2*)spec, (const float*)nullptr, nr, ap.rows_pad, nk, h->kcorr, n, ap.KP, ap.hi,
+                       ap.lo);
+  }
   hipLaunchKernelGGL(k_ring_sq64, dim3((nr * h->kcorr + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)rings, nr,
                      nk, n, h->kcorr, ap.sr);
   HIPCHK(h, hipGetLastError());
@@ -711,6 +785,7 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl) {
   dfree(spec);
   ap.refs = rl;
   ap.kcorr = h->kcorr;
+  ap.fp8 = fp8;
   ap.valid = true;
   return PFTCC_OK;
 }
@@ -719,21 +794,22 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl) {
 // operand of refs rl.  outpos[i] = position of pl[i] in best/cc_out.
 int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<int>& outpos,
                const std::vector<int>& rl, int prec, unsigned long long* d_best, float* d_cc,
-               unsigned long long* d_prbest) {
+               unsigned long long* d_prbest, const std::vector<double>* msW) {
   int rc;
-  if ((rc = ensure_apack(h, eo, rl))) return rc;
+  const bool fp8 = prec == PFTCC_PREC_FP8;
+  if ((rc = ensure_apack(h, eo, rl, fp8, msW))) return rc;
   const pftcc::APack& ap = h->apack[eo];
-  const int n = h->n, nk = h->nk, KP = ap.KP, nr = (int)rl.size();
+  const int n = h->n, nk = h->nk, KP = fp8 ? ap.KP8 : ap.KP, nr = (int)rl.size();
   const int total = (int)pl.size();
   // chunk of particles: keep Z (n+1) x rows_pad x Pc float2 within ~4 GiB
   int Pc = 256;
   while (Pc > 64 && (size_t)(n + 1) * ap.rows_pad * Pc * 8 > ((size_t)4 << 30)) Pc -= 64;
   Pc = std::min(Pc, round_up(total, 64));
-  const size_t words = (size_t)(n + 1) * Pc * (KP / 2);
+  const size_t words = (size_t)(n + 1) * Pc * (fp8 ? KP / 4 : KP / 2);
   const size_t zsz = (size_t)(n + 1) * ap.rows_pad * Pc;
   if ((rc = scratch_reserve(h, aligned((size_t)total * 4) * 2 + aligned(nr * 4) + aligned((size_t)Pc * nk * (n + 1) * 8) +
                                    aligned((size_t)Pc * nk * 4) + aligned((size_t)Pc * nk * 8) + aligned((size_t)total * 8) +
-                                   2 * aligned(words * 4) + aligned(zsz * 8))))
+                                   2 * aligned(words * 4) + aligned(zsz * 8) + aligned(4))))
     return rc;
   int* d_pl = take<int>(h, total);
   int* d_pos = take<int>(h, total);
@@ -745,6 +821,7 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
   unsigned int* bhi = take<unsigned int>(h, words);
   unsigned int* blo = take<unsigned int>(h, words);
   float2* Z = take<float2>(h, zsz);
+  unsigned int* gmaxB = take<unsigned int>(h, 1);
   HIPCHK(h, hipMemcpyAsync(d_pl, pl.data(), total * 4, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(d_pos, outpos.data(), total * 4, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(d_rl, rl.data(), nr * 4, hipMemcpyHostToDevice, h->stream));
@@ -762,12 +839,26 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     if (ctf)
       hipLaunchKernelGGL(k_ring_mean_ctf_list, dim3((cnt * nk + 3) / 4), dim3(64, 4), 0, h->stream,
                          (const float*)h->d_ctf, (const int*)(d_pl + c0), cnt, nk, n, cbar, cbar2);
-    hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream, (const float2*)wg,
-                       ctf ? (const float*)cbar : nullptr, cnt, Pc, nk, h->kcorr, n, KP, bhi, blo);
+    if (fp8) {
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, h->stream, gmaxB, 1, 0u);
+      const size_t tot = (size_t)cnt * h->kcorr * (n + 1);
+      hipLaunchKernelGGL(k_absmax_scaled, dim3((unsigned)std::min<size_t>(1024, (tot + 255) / 256)), dim3(256), 0, h->stream,
+                         (const float2*)wg, ctf ? (const float*)cbar : nullptr, (const float*)ap.d_bk, cnt, nk, n,
+                         h->kcorr, gmaxB);
+      hipLaunchKernelGGL(k_pack_fp8, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
+                         (const float2*)wg, ctf ? (const float*)cbar : nullptr, (const float*)ap.d_bk,
+                         (const unsigned int*)gmaxB, cnt, Pc, nk, h->kcorr, n, KP, bhi);
+    } else {
+      hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
+                         (const float2*)wg, ctf ? (const float*)cbar : nullptr, cnt, Pc, nk, h->kcorr, n, KP, bhi, blo);
+    }
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 1, &ev))) return rc;
     const dim3 gg(Pc / 64, ap.rows_pad / 64, n + 1);
-    if (prec == PFTCC_PREC_BF16)
+    if (fp8)
+      hipLaunchKernelGGL(k_zgemm8, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad, Pc,
+                         KP, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB);
+    else if (prec == PFTCC_PREC_BF16)
       hipLaunchKernelGGL(k_zgemm<1>, gg, dim3(64), 0, h->stream, (const bf16x8*)ap.hi, (const bf16x8*)ap.lo,
                          (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, Z);
     else
@@ -819,7 +910,8 @@ int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* ire
              unsigned long long* d_best, float* d_cc, unsigned long long* d_prbest = nullptr,
              std::vector<int>* plocal = nullptr) {
   int rc;
-  if (prec != PFTCC_PREC_BF16 && prec != PFTCC_PREC_SPLIT_BF16) return fail(h, PFTCC_E_ARG, "unknown precision %d", prec);
+  if (prec != PFTCC_PREC_BF16 && prec != PFTCC_PREC_SPLIT_BF16 && prec != PFTCC_PREC_FP8)
+    return fail(h, PFTCC_E_ARG, "unknown precision %d", prec);
   if (h->euclid || h->match_filt || h->pssnr)
     return fail(h, PFTCC_E_UNSUPPORTED, "the MFMA form implements objfun cc without match filter");
   if (zifft_lds(2, h->n) > kMaxLds) return fail(h, PFTCC_E_UNSUPPORTED, "pftsz %d too large for the MFMA form", h->n);
@@ -840,8 +932,21 @@ int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* ire
     pos[eo].push_back(i);
     if (plocal) plocal->push_back(ip);
   }
+  // FP8: per-shell mean squares of the particles' spectra (for the shell scales)
+  std::vector<double> msW;
+  if (prec == PFTCC_PREC_FP8) {
+    std::vector<int> all;
+    for (int eo = 0; eo < 2; ++eo) all.insert(all.end(), pl[eo].begin(), pl[eo].end());
+    int* d_all = nullptr;
+    if ((rc = dalloc(h, &d_all, all.size()))) return rc;
+    HIPCHK(h, hipMemcpyAsync(d_all, all.data(), all.size() * 4, hipMemcpyHostToDevice, h->stream));
+    rc = shell_ms(h, h->d_memo, d_all, (int)all.size(), msW);
+    dfree(d_all);
+    if (rc) return rc;
+  }
   for (int eo = 0; eo < 2; ++eo)
-    if (!pl[eo].empty() && (rc = mfma_group(h, eo, pl[eo], pos[eo], rl, prec, d_best, d_cc, d_prbest))) return rc;
+    if (!pl[eo].empty() && (rc = mfma_group(h, eo, pl[eo], pos[eo], rl, prec, d_best, d_cc, d_prbest, &msW)))
+      return rc;
   return PFTCC_OK;
 }
 
@@ -1034,11 +1139,7 @@ int pftcc_kill(pftcc_t* h) {
   dfree(h->d_refinv_d);
   dfree(h->d_ofac_d);
   dfree(h->d_refpw);
-  for (auto& ap : h->apack) {
-    dfree(ap.hi);
-    dfree(ap.lo);
-    dfree(ap.sr);
-  }
+  for (auto& ap : h->apack) free_apack(ap);
   if (h->d_scratch) (void)hipFree(h->d_scratch);
   delete h;
   return PFTCC_OK;
@@ -1689,7 +1790,7 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
                       int32_t ntop, float* best_cc, int32_t* best_iref, int32_t* best_irot) {
   if (!h || !iptcls || np < 1 || nr < 1 || !best_cc || !best_iref || !best_irot) return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
-  if (ntop < 0 || ntop > 16) return fail(h, PFTCC_E_ARG, "ntop %d outside [0, 16]", ntop);
+  if (ntop < 0 || ntop > RESCORE_MAX) return fail(h, PFTCC_E_ARG, "ntop %d outside [0, %d]", ntop, RESCORE_MAX);
   unsigned long long *d_best = nullptr, *d_prbest = nullptr, *d_cand = nullptr;
   float* d_cc = nullptr;
   int *d_ir = nullptr, *d_rot = nullptr, *d_pl = nullptr;
@@ -1725,7 +1826,7 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
     ra.best_cc = d_cc;
     ra.best_iref = d_ir;
     ra.best_irot = d_rot;
-    hipLaunchKernelGGL(k_rescore64, dim3(np), dim3(64 * ntop), 0, h->stream, ra);
+    hipLaunchKernelGGL(k_rescore64, dim3(np), dim3(64 * std::min(ntop, RESCORE_WAVES)), 0, h->stream, ra);
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev))) return rc;
   } else if (rc == PFTCC_OK) {

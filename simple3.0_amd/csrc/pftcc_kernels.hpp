@@ -42,6 +42,11 @@ __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
   if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // memoize_sqsum_ptcl (:841-845) and per-shell sums; one block per particle.
 __global__ void k_sqsums(const float2* __restrict__ ptcls, int nk, int n, int kcount,
                          float* __restrict__ sqsums, float* __restrict__ sqsums_k, int first, int count) {

@@ -26,6 +26,7 @@ PFTCC_OBJFUN_CC = 0
 PFTCC_OBJFUN_EUCLID = 1
 PFTCC_PREC_BF16 = 1
 PFTCC_PREC_SPLIT_BF16 = 3
+PFTCC_PREC_FP8 = 8
 
 
 class PftccError(RuntimeError):

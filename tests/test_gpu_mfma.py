@@ -6,13 +6,15 @@ ring-mean CTF.  Tolerance: |cc_mfma - cc_oracle| <= 1e-4 relative to the correla
 BF16 keeps ~16 mantissa bits per operand; the dropped lo*lo term is 2^-16 relative), best
 (reference, rotation) identical wherever the oracle's top-2 gap exceeds 10x the measured error.
 """
+import copy
+
 import numpy as np
 import pytest
 
 import pftcc_oracle as O
 from cases import make_case
 from gpu_helpers import peak_rel_err, say, to_lib, top2_gap
-from pftcc import PFTCC_PREC_BF16, PFTCC_PREC_SPLIT_BF16, PftccError
+from pftcc import PFTCC_PREC_BF16, PFTCC_PREC_FP8, PFTCC_PREC_SPLIT_BF16, PftccError
 
 pytestmark = pytest.mark.gpu
 
@@ -172,5 +174,39 @@ def test_mfma_rescored_search_is_exact(astig):
     raw_cc, raw_ir, raw_rot = lib.search_mfma(ids, ntop=0)
     say(f"  raw vs exact max |dcc| {np.max(np.abs(raw_cc - cc)):.2e}")
     with pytest.raises(PftccError):
-        lib.search_mfma(ids, ntop=17)
+        lib.search_mfma(ids, ntop=65)
+    lib.kill()
+
+
+def test_fp8_coarse_pass_error_and_rescoring():
+    """FP8 e4m3 contraction: a coarse score (e4m3 keeps 3 mantissa bits, ~3.6% rms per operand
+    element, so the score error is a few percent of the peak on noise-only data); with ntop
+    candidates rescored in FP64 the search returns the oracle's exact argmax and cc."""
+    say("fp8 coarse pass + rescoring")
+    pf, cps = make_case(seed=47, n=192, kfrom=2, kto=40, ctf=True, astig=0.1, nrefs=24, nptcls=10, ldim=128,
+                        eo_mixed=True)
+    ids = np.arange(1, 11)
+    lib = to_lib(pf, cps)
+    got = lib.gencorrs_mfma_batch(ids, prec=PFTCC_PREC_FP8)
+    ref = oracle_cube(ring_mean_case(copy.deepcopy(pf)), range(10), range(24))
+    errs = [peak_rel_err(got[p], ref[p]) for p in range(10)]
+    say(f"  fp8 peak-relative error vs ring-mean oracle: max {max(errs):.2e}")
+    assert max(errs) < 0.15
+    lib.kill()
+    # planted winners: exact after rescoring
+    rng = np.random.default_rng(48)
+    for i in range(10):
+        r = int(rng.integers(0, 24))
+        rho = int(rng.integers(0, 2 * pf.pftsz))
+        src = pf.refs_even[r] if pf.iseven[i] else pf.refs_odd[r]
+        rolled = np.roll(O.full_ring(src.astype(np.complex128)), rho, axis=-1)[:, :pf.pftsz]
+        noise = 1.0 * (rng.standard_normal(rolled.shape) + 1j * rng.standard_normal(rolled.shape))
+        pf.ptcls[i] = (rolled * pf.ctfmats[i] + noise * np.abs(rolled).mean()).astype(np.complex64)
+    lib = to_lib(pf, cps)
+    exact = oracle_cube(pf, range(10), range(24))
+    cc, ir, rot = lib.search_mfma(ids, prec=PFTCC_PREC_FP8, ntop=16)
+    for i in range(10):
+        r, t = np.unravel_index(int(np.argmax(exact[i])), exact[i].shape)
+        assert (ir[i], rot[i]) == (r + 1, t + 1)
+        assert abs(cc[i] - exact[i][r, t]) <= 1e-7
     lib.kill()
