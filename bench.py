@@ -31,6 +31,7 @@ METRIC = ("ptcl×ref×rot corr evals/s: plain FP32 vs split-BF16 MFMA @ rel err 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def log(msg):
@@ -129,6 +130,75 @@ def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
     }
 
 
+def gradient_form(steps, nptcls=50000, nrefs=500, fd_sample=256, h=1e-3):
+    """Batched adjoint (gencorr_grad_for_rot_8 + in-plane angle derivative) at the continuous-
+    refinement shape: every particle paired with 1..8 references, shifts within +-4 px."""
+    import pftcc
+    from pftcc import Params, PolarFTCorrCalc
+    from pftcc import synth
+
+    box, smpd, kfromto = 200, 1.0, (2, 70)
+    pftsz = pftcc.magic_pftsz(98)
+    w = synth.Workload(box=box, smpd=smpd, kfromto=kfromto, pftsz=pftsz, nrefs=nrefs, nptcls=nptcls, snr=0.03,
+                       seed=20251204, dfrange=(0.8, 3.5), astig=0.1, max_shift=4.0, uniform_dirs=True)
+    synth.make_params(w)
+    lib = PolarFTCorrCalc().new(nrefs, (1, nptcls), Params(boxmatch=box, kfromto=kfromto, pftsz=pftsz, ctf=True))
+    synth.load_into(lib, w, chunk=10000)
+    rng = np.random.default_rng(20251205)
+    per = rng.integers(1, 9, nptcls)
+    iptcls = np.repeat(np.arange(1, nptcls + 1, dtype=np.int32), per)
+    irefs = rng.integers(1, nrefs + 1, iptcls.size).astype(np.int32)
+    shv = rng.uniform(-4.0, 4.0, (iptcls.size, 2))
+    irots = rng.integers(1, 2 * pftsz + 1, iptcls.size).astype(np.int32)
+    npairs = iptcls.size
+    S = synth.synth_lib()
+    d_r = synth.DeviceBuffer.from_array(irefs)
+    d_p = synth.DeviceBuffer.from_array(iptcls)
+    d_s = synth.DeviceBuffer.from_array(shv)
+    d_o = synth.DeviceBuffer.from_array(irots)
+    d_f = synth.DeviceBuffer(8 * npairs)
+    d_g = synth.DeviceBuffer(24 * npairs)
+    lib.grad_batch_dev(npairs, d_r.ptr, d_p.ptr, d_s.ptr, d_o.ptr, d_f.ptr, d_g.ptr)     # warm-up
+    S.synth_device_sync()
+    lib.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lib.grad_batch_dev(npairs, d_r.ptr, d_p.ptr, d_s.ptr, d_o.ptr, d_f.ptr, d_g.ptr)
+    S.synth_device_sync()
+    wall = (time.perf_counter() - t0) / steps
+    k_ms, k_n = lib.get_timing_channel(4)
+    lib.set_timing(False)
+    g = d_g.to_array(np.float64, (npairs, 3))
+    # central finite differences of the FP64 objective on a sample, h = 1e-3 px
+    sel = rng.choice(npairs, fd_sample, replace=False)
+    fd = np.zeros((fd_sample, 2))
+    for d in range(2):
+        e = np.zeros(2)
+        e[d] = h
+        fp = lib.grad_batch(irefs[sel], iptcls[sel], shv[sel] + e, irots[sel], with_grad=False)
+        fm = lib.grad_batch(irefs[sel], iptcls[sel], shv[sel] - e, irots[sel], with_grad=False)
+        fd[:, d] = (fp - fm) / (2 * h)
+    scale = np.maximum(np.abs(fd), 1e-3 * np.max(np.abs(fd)))
+    fd_err = np.abs(g[sel, :2] - fd) / scale
+    for b in (d_r, d_p, d_s, d_o, d_f, d_g):
+        b.close()
+    lib.kill()
+    K = w.nk
+    flops_pair = 32.0 * K * pftsz
+    avg_ms = k_ms / max(k_n, 1)
+    achieved = flops_pair * npairs / (avg_ms * 1e-3) / 1e12
+    return {
+        "workload": "batched FP64 objective + adjoint (d/dx, d/dy, d/dpsi), continuous-refinement shape",
+        "value": npairs / wall, "unit": "(f, grad f) per s", "ms_per_step": 1e3 * wall, "steps": steps,
+        "ptcls": nptcls, "refs": nrefs, "pairs": int(npairs), "refs_per_ptcl": "U{1..8}", "box": box,
+        "kfromto": list(kfromto), "pftsz": pftsz, "shift_range_px": 4.0,
+        "fd_check": {"h_px": h, "sample_pairs": fd_sample, "max_rel_err": float(np.max(fd_err)),
+                     "p99_rel_err": float(np.percentile(fd_err, 99))},
+        "kernel": {"name": "k_rot8", "avg_launch_ms": avg_ms, "achieved": achieved, "unit": "TFLOP/s",
+                   "flops_per_pair": flops_pair, "peak": FP64_PEAK_TFLOPS, "frac": achieved / FP64_PEAK_TFLOPS},
+    }
+
+
 def cpu_baseline(pftcc, w, shifts, nthreads, target_s=12.0):
     """Times the reference CPU algorithm (C++/OpenMP port, oracle/pftcc_cpu.cpp) on a bounded
     sample of the same workload and checks the GPU's assignments on that sample."""
@@ -183,6 +253,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the MFMA precision-form line")
     ap.add_argument("--precision-steps", type=int, default=2)
+    ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-kernel line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the search shards particles "
@@ -277,6 +348,8 @@ def main():
     lib.kill()
     if not args.no_precision:
         out["precision_form"] = precision_form(args.precision_steps)
+    if not args.no_gradient:
+        out["gradient_form"] = gradient_form(3)
     print(json.dumps(out), flush=True)
     return 0
 

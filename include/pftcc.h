@@ -198,9 +198,9 @@ int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs,
  * complex contraction over shells per angular frequency m on MFMA, followed by the inverse FFT
  * over m.  Exact for ring-constant |CTF| (ctf = no or no astigmatism); with astigmatism the
  * particle's |CTF| is replaced by its ring mean per shell (the reference applies the particle's
- * angle-dependent |CTF| in the reference frame, :982, which no per-m contraction can express) and
- * candidates should be rescored with pftcc_grad_batch / pftcc_gencorr_for_rot_8.
- * prec: PFTCC_PREC_SPLIT_BF16 (hi+lo split, FP32 accumulation) or PFTCC_PREC_BF16.
+ * angle-dependent |CTF| in the reference frame, :982, which no per-m contraction can express),
+ * and ntop > 0 rescores the candidates exactly.
+ * prec: PFTCC_PREC_SPLIT_BF16 (hi+lo split, FP32 accumulation), PFTCC_PREC_BF16 or PFTCC_PREC_FP8.
  * Denominators sqrt(sum_k cbar_k^2 S_ref,k * S_ptcl) are formed in FP64.  No counterpart
  * entry point in the reference; it replaces the same greedy loop as pftcc_search
  * (simple_strategy3D_greedy.f90:60-77, gencorrs over all references + maxloc).
@@ -227,7 +227,8 @@ int pftcc_gencorrs_mfma_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int
 int pftcc_set_timing(pftcc_t* h, int32_t enable);
 int pftcc_get_timing(pftcc_t* h, double* total_ms, int64_t* launches);
 /* Per kernel family: 0 plain correlation, 1 MFMA contraction, 2 inverse FFT of the MFMA form,
- * 3 operand packing, candidate selection and rescoring of the MFMA form. */
+ * 3 operand packing, candidate selection and rescoring of the MFMA form, 4 the FP64 objective /
+ * adjoint kernel (gencorr_for_rot_8, grad_batch). */
 int pftcc_get_timing_channel(pftcc_t* h, int32_t channel, double* total_ms, int64_t* launches);
 
 /* ------------------------------------------------------------------------- utilities -- */

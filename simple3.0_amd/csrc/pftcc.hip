@@ -135,8 +135,8 @@ struct pftcc {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
   std::vector<int> event_ch;
   size_t nev = 0;
-  double timed_ms[4] = {0, 0, 0, 0};
-  int64_t timed_launches[4] = {0, 0, 0, 0};
+  double timed_ms[5] = {0, 0, 0, 0, 0};
+  int64_t timed_launches[5] = {0, 0, 0, 0, 0};
   // scratch arena (stream ordered; calls on a handle are serialised by mtx)
   char* d_scratch = nullptr;
   size_t scratch_cap = 0, scratch_off = 0;
@@ -655,9 +655,12 @@ int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, cons
   a.f = d_f;
   a.grad = d_grad;
   if (npairs <= 0) return PFTCC_OK;
+  hipEvent_t ev;
+  int rc;
+  if ((rc = timing_begin(h, 4, &ev))) return rc;
   hipLaunchKernelGGL(k_rot8, dim3(npairs), dim3(256), 0, h->stream, a);
   HIPCHK(h, hipGetLastError());
-  return PFTCC_OK;
+  return timing_end(h, ev);
 }
 
 // ------------------------------------------------------------------ precision form ---
@@ -1090,7 +1093,7 @@ int pftcc_set_timing(pftcc_t* h, int32_t enable) {
   int rc;
   if ((rc = collect_timing(h))) return rc;
   h->timing = enable != 0;
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < 5; ++c) {
     h->timed_ms[c] = 0.0;
     h->timed_launches[c] = 0;
   }
@@ -1108,7 +1111,7 @@ int pftcc_get_timing(pftcc_t* h, double* total_ms, int64_t* launches) {
 }
 
 int pftcc_get_timing_channel(pftcc_t* h, int32_t ch, double* total_ms, int64_t* launches) {
-  if (!h || !total_ms || !launches || ch < 0 || ch > 3) return PFTCC_E_ARG;
+  if (!h || !total_ms || !launches || ch < 0 || ch > 4) return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
   int rc;
   if ((rc = collect_timing(h))) return rc;
