@@ -31,6 +31,11 @@ METRIC = ("ptcl×ref×rot corr evals/s: plain FP32 vs split-BF16 MFMA @ rel err 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+# HBM bytes per launch of k_corr_fused<308,6,256> at 1000 particles x 200 refs x 169 shifts, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script (--legs plain --copy-calibration):
+# FETCH 1318733024 KiB x 2.000 + WRITE 176062 KiB x 1.000, the factors read from k_copy moving
+# 1 GiB in the same runs (profiles/k_corr_fused_308_plain_search.md).
+PLAIN_TRAFFIC_BYTES_B1000 = (1318733024.25 * 2.0 + 176062.5) * 1024.0
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -254,6 +259,10 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the MFMA precision-form line")
     ap.add_argument("--precision-steps", type=int, default=2)
     ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-kernel line")
+    ap.add_argument("--legs", default="plain,precision,gradient",
+                    help="comma list of plain, precision, gradient (profiling runs select one)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 1 GiB device copy (k_copy), the reference for HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the search shards particles "
@@ -263,6 +272,24 @@ def main():
     import pftcc
     from pftcc import Params, PolarFTCorrCalc
     from pftcc import synth
+
+    legs = set(args.legs.split(","))
+    if args.copy_calibration:
+        S0 = synth.synth_lib()
+        a, b = synth.DeviceBuffer(1 << 30), synth.DeviceBuffer(1 << 30)
+        if S0.synth_copy(b.ptr, a.ptr, 1 << 30):
+            raise RuntimeError("calibration copy failed")
+        a.close()
+        b.close()
+        log("copy calibration: 1 GiB read + 1 GiB written by k_copy")
+    if "plain" not in legs:
+        out = {"metric": METRIC, "n_gpus": 1, "note": "plain leg skipped (--legs)"}
+        if "precision" in legs:
+            out["precision_form"] = precision_form(args.precision_steps)
+        if "gradient" in legs:
+            out["gradient_form"] = gradient_form(3)
+        print(json.dumps(out), flush=True)
+        return 0
 
     box, smpd, kfromto = 200, 1.0, (2, 60)
     pftsz = pftcc.magic_pftsz(98)       # ring2 = round2even(box/2 - 2)
@@ -333,8 +360,12 @@ def main():
                    "snr": 0.03, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "pipe": "VALU fp32 (no MFMA; packed-fp32 disabled, issue peak 78.6)",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_corr_fused<308,11,256>", "avg_launch_ms": avg_ms, "launches": launches,
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": PLAIN_TRAFFIC_BYTES_B1000 if (B == 1000 and w.nrefs == 200) else None,
+                     "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated; "
+                                     "compulsory bytes per launch are in hbm_compulsory_bytes",
+                     "hbm_compulsory_bytes": bytes_launch,
+                     "kernel": "k_corr_fused<308,6,256>", "avg_launch_ms": avg_ms, "launches": launches,
                      "flops_per_pair": plain_flops_per_pair(pftsz, w.nk), "pairs_per_launch": pairs_launch,
                      "hbm_compulsory_GBs": bytes_launch / (avg_ms * 1e-3) / 1e9,
                      "hbm_frac": bytes_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
@@ -346,9 +377,9 @@ def main():
         except Exception as e:  # the GPU figure stands on its own
             out["cpu_baseline"] = {"error": repr(e)}
     lib.kill()
-    if not args.no_precision:
+    if not args.no_precision and "precision" in legs:
         out["precision_form"] = precision_form(args.precision_steps)
-    if not args.no_gradient:
+    if not args.no_gradient and "gradient" in legs:
         out["gradient_form"] = gradient_form(3)
     print(json.dumps(out), flush=True)
     return 0

@@ -113,6 +113,14 @@ constexpr int fused_F() {
   return F;
 }
 
+// Pair tile measured fastest on MI355X for the ring lengths of the benchmark shapes (n = 192, 308:
+// a smaller tile keeps the kernel at 3 wavefronts per SIMD); fused_F elsewhere.
+template <int N, int NT>
+constexpr int fused_F_tuned() {
+  if (NT == 256 && (N == 192 || N == 308)) return N == 192 ? 5 : 6;
+  return fused_F<N, NT>();
+}
+
 template <int N, int F>
 constexpr size_t fused_lds_bytes() {
   // buf[F N] + Wk[N+1] + tw2[2N] float2;  ctfk[N] + sq/off slots 2 F N/4 floats
@@ -157,7 +165,7 @@ __global__ void __launch_bounds__(NT) k_corr_fused(CorrArgs a) {
 
   extern __shared__ float2 smem[];
   float2* buf = smem;                        // [F][N]
-  float2* Wk = buf + F * N;                  // [N+1]
+  float2* Wk = buf + F * N;       // [N+1]
   float2* tw2 = Wk + (N + 1);                // [2N]   exp(-i pi a / N)
   float* ctfk = reinterpret_cast<float*>(tw2 + 2 * N);   // [N]
   float* sqs = ctfk + N;                     // [F][Q]
@@ -395,7 +403,7 @@ __global__ void __launch_bounds__(NT) k_corr_fused(CorrArgs a) {
     const int f = idx / N, j = idx % N;
     const int P = tile * F + f;
     if (P >= npairs) continue;
-    const float2 y = buf[idx];
+    const float2 y = buf[f * N + j];
     float x0 = y.x, x1 = -y.y;
     if (a.mode == 1) {
       const float o = off[f] + pscal;

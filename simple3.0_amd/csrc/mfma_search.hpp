@@ -104,13 +104,27 @@ __device__ __forceinline__ bf16x8 rot_neg(bf16x8 a) {
 
 // Z[r][p][m] = sum_k' A[m][r][k'] (x) B[m][p][k'] for one 64 x 64 (ref x particle) tile per wave.
 // split = 3: hi.hi + hi.lo + lo.hi; split = 1: hi.hi only (plain BF16, for error studies).
+// Tile coordinates of a 1-D launch of gx * gy * (n+1) work-groups: the XCD-aware order puts the
+// particle tiles that share one reference tile (and m) on one XCD, next to each other in time.
+struct ZTile {
+  int p0, r0, m;
+};
+__device__ __forceinline__ ZTile ztile(int gx, int gy) {
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  ZTile z;
+  z.p0 = (t % gx) * 64;
+  z.r0 = ((t / gx) % gy) * 64;
+  z.m = t / (gx * gy);
+  return z;
+}
+
 template <int SPLIT>
 __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
                                               const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
-                                              int rows_pad, int cols_pad, int KP, float2* __restrict__ Z) {
+                                              int rows_pad, int cols_pad, int KP, int n, float2* __restrict__ Z) {
   const int l = threadIdx.x;
-  const int p0 = blockIdx.x * 64, r0 = blockIdx.y * 64, m = blockIdx.z;
-  const int n = gridDim.z - 1;
+  const ZTile zt = ztile(cols_pad / 64, rows_pad / 64);
+  const int p0 = zt.p0, r0 = zt.r0, m = zt.m;
   const int KV = KP / 8;                                  // bf16x8 vectors per row
   const size_t abase = ((size_t)m * rows_pad + r0) * KV;
   const size_t bbase = ((size_t)m * cols_pad + p0) * KV;
@@ -298,8 +312,12 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   __shared__ double den[F];
   __shared__ unsigned long long bkey[F];
   const int tid = threadIdx.x;
-  const int r = blockIdx.y;
-  const int p0 = blockIdx.x * F;
+  // 1-D launch of gx * nrefs work-groups; XCD-aware order keeps the work-groups that read one Z
+  // group (consecutive particle blocks of one reference) on one XCD
+  const int gx = (a.np + F - 1) / F;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = t / gx;
+  const int p0 = (t % gx) * F;
   for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
   if (tid < F) bkey[tid] = 0ull;
   {
@@ -544,12 +562,12 @@ __device__ __forceinline__ i32x8 rot_neg8(i32x8 a) {
 // FP8 contraction: Z[r][p][m] = (A1 . B, A2 . B) / (gA gB) for one 64 x 64 tile per wave, 128 k'
 // per MFMA (16x16x128, e4m3 x e4m3, FP32 accumulation).
 __global__ void __launch_bounds__(64) k_zgemm8(const i32x8* __restrict__ A, const i32x8* __restrict__ B, int rows_pad,
-                                               int cols_pad, int KP8, float2* __restrict__ Z,
+                                               int cols_pad, int KP8, int n, float2* __restrict__ Z,
                                                const unsigned int* __restrict__ gmaxA,
                                                const unsigned int* __restrict__ gmaxB) {
   const int l = threadIdx.x;
-  const int p0 = blockIdx.x * 64, r0 = blockIdx.y * 64, m = blockIdx.z;
-  const int n = gridDim.z - 1;
+  const ZTile zt = ztile(cols_pad / 64, rows_pad / 64);
+  const int p0 = zt.p0, r0 = zt.r0, m = zt.m;
   const int KV = KP8 / 32;                                 // i32x8 vectors per row
   const size_t abase = ((size_t)m * rows_pad + r0) * KV;
   const size_t bbase = ((size_t)m * cols_pad + p0) * KV;

@@ -414,11 +414,20 @@ int timing_end(pftcc* h, hipEvent_t e1);
 
 // Compile-time ring lengths of the fused kernel: the benchmark shapes (192, 308, 400), common
 // box sizes (64 -> 100, 160 -> 240, 320 -> 500, 384 -> 600) and small sizes used in tests.
+#ifndef PFTCC_FUSED_F_OVERRIDE
+#define PFTCC_FUSED_F_OVERRIDE 0   // tile size for tuning builds (0: fused_F)
+#endif
+#ifndef PFTCC_FUSED_NT
+#define PFTCC_FUSED_NT 256
+#endif
 template <int N>
 struct FusedLaunch {
-  static constexpr int NT = 256;
-  static constexpr int F = fused_F<N, NT>();
+  static constexpr int NT = PFTCC_FUSED_NT;
+  static constexpr int F = (PFTCC_FUSED_F_OVERRIDE > 0 && fused_lds_bytes<N, PFTCC_FUSED_F_OVERRIDE + 0>() <= 65536)
+                               ? PFTCC_FUSED_F_OVERRIDE
+                               : fused_F_tuned<N, NT>();
   static constexpr size_t lds = fused_lds_bytes<N, F>();
+  static_assert(lds <= 65536, "fused kernel LDS");
   static void go(unsigned blocks, hipStream_t s, const CorrArgs& a) {
     hipLaunchKernelGGL((k_corr_fused<N, F, NT>), dim3(blocks), dim3(NT), lds, s, a);
   }
@@ -658,7 +667,9 @@ int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, cons
   hipEvent_t ev;
   int rc;
   if ((rc = timing_begin(h, 4, &ev))) return rc;
-  hipLaunchKernelGGL(k_rot8, dim3(npairs), dim3(256), 0, h->stream, a);
+  // one thread per ring position j (rounded up to whole wavefronts)
+  const int nt = std::min(1024, (h->n + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_rot8, dim3(npairs), dim3(nt), 0, h->stream, a);
   HIPCHK(h, hipGetLastError());
   return timing_end(h, ev);
 }
@@ -671,8 +682,8 @@ bool zifft_ct_go(int n, int cnt, int nr, hipStream_t s, const ZifftArgs& a) {
 #define X(N)                                                                                          \
   if (n == N) {                                                                                       \
     hipLaunchKernelGGL((k_zifft_v2<N, ZifftV2<N>::F, ZifftV2<N>::NT>),                               \
-                       dim3((cnt + ZifftV2<N>::F - 1) / ZifftV2<N>::F, nr), dim3(ZifftV2<N>::NT),    \
-                       ZifftV2<N>::lds, s, a);                                                        \
+                       dim3((unsigned)((cnt + ZifftV2<N>::F - 1) / ZifftV2<N>::F * nr)),               \
+                       dim3(ZifftV2<N>::NT), ZifftV2<N>::lds, s, a);                                  \
     return true;                                                                                      \
   }
   PFTCC_FUSED_SIZES(X)
@@ -857,16 +868,16 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     }
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 1, &ev))) return rc;
-    const dim3 gg(Pc / 64, ap.rows_pad / 64, n + 1);
+    const dim3 gg((unsigned)((size_t)(Pc / 64) * (ap.rows_pad / 64) * (n + 1)));
     if (fp8)
       hipLaunchKernelGGL(k_zgemm8, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad, Pc,
-                         KP, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB);
+                         KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB);
     else if (prec == PFTCC_PREC_BF16)
       hipLaunchKernelGGL(k_zgemm<1>, gg, dim3(64), 0, h->stream, (const bf16x8*)ap.hi, (const bf16x8*)ap.lo,
-                         (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, Z);
+                         (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
     else
       hipLaunchKernelGGL(k_zgemm<3>, gg, dim3(64), 0, h->stream, (const bf16x8*)ap.hi, (const bf16x8*)ap.lo,
-                         (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, Z);
+                         (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 2, &ev))) return rc;
     ZifftArgs za{};

@@ -458,8 +458,18 @@ struct Rot8Args {
 
 // One block of 256 threads per (ref, particle, shift, rotation) item; threads own angles j and
 // run the shells in order so the shift phase follows gen_shmat_8's recurrence (:1098-1120).
-__global__ void __launch_bounds__(256) k_rot8(Rot8Args a) {
-  const int item = blockIdx.x;
+// Work-groups are dispatched round-robin over the 8 XCDs (each with its own L2): block b runs on
+// XCD b % 8.  This gives each XCD a contiguous range of logical items, so that consecutive items
+// (e.g. the pairs of one particle) share an L2.  A bijection of [0, nblocks).
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+  constexpr int NX = 8;
+  const int per = nblocks / NX, rem = nblocks % NX;
+  const int x = b % NX, q = b / NX;
+  return x * per + (x < rem ? x : rem) + q;
+}
+
+__global__ void __launch_bounds__(1024) k_rot8(Rot8Args a) {
+  const int item = xcd_remap(blockIdx.x, gridDim.x);
   if (item >= a.npairs) return;
   const int n = a.n;
   const int iref = a.irefs[item];
@@ -529,18 +539,20 @@ __global__ void __launch_bounds__(256) k_rot8(Rot8Args a) {
       shi = i2;
     }
   }
-  __shared__ double red[6][256];
+  // wavefront sums, then the waves' partial sums in a fixed order
+  __shared__ double red[6][16];
 #pragma unroll
-  for (int v = 0; v < 6; ++v) red[v][threadIdx.x] = acc[v];
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+  for (int v = 0; v < 6; ++v) {
+    double x = acc[v];
 #pragma unroll
-      for (int v = 0; v < 6; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + w];
-    }
-    __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if ((threadIdx.x & 63) == 0) red[v][threadIdx.x >> 6] = x;
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int v = 0; v < 6; ++v)
+      for (int w = 1; w < nw; ++w) red[v][0] += red[v][w];
     double f, g0, g1, g2;
     if (a.mode == 2) {
       f = red[0][0];

@@ -121,9 +121,22 @@ __global__ void k_ptcls(const Blob* __restrict__ blobs, int nb, const float* __r
   out[idx] = v;
 }
 
+// Plain copy (one float4 per thread); a kernel of known traffic for calibrating HBM counters.
+__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+// dst[0:bytes) = src[0:bytes); bytes a multiple of 16.
+int synth_copy(void* dst, const void* src, size_t bytes) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, (const float4*)src, (float4*)dst, n);
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 3;
+}
 
 // blobs: [nb][5] (cx, cy, cz, sigma, amp); uvs: [nrefs][6] projection axes u, v.
 int synth_refs(int nb, const float* blobs, int nrefs, const float* uvs, int nk, int n, int kfrom, int box,

@@ -46,6 +46,8 @@ def synth_lib():
         lib.synth_event_elapsed_ms.restype = C.c_float
         lib.synth_event_elapsed_ms.argtypes = [C.c_void_p, C.c_void_p]
         lib.synth_device_sync.restype = C.c_int
+        lib.synth_copy.restype = C.c_int
+        lib.synth_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         _SYNTH = lib
     return _SYNTH
 
