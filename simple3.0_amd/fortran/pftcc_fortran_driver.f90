@@ -37,8 +37,8 @@ contains
         integer(c_int32_t)     :: hdr(16)
         complex(c_float_complex), allocatable :: refse(:,:,:), refso(:,:,:), ptcls(:,:,:)
         type(ctfparams), allocatable :: cps(:)
-        real(c_float),  allocatable :: shifts(:,:), cc(:), ccall(:,:,:,:), best_cc(:), scores(:)
-        integer,        allocatable :: best_iref(:), best_irot(:), best_ishift(:), ids(:)
+        real(c_float),  allocatable :: shifts(:,:), cc(:), ccall(:,:,:,:), best_cc(:), scores(:), mcc(:)
+        integer,        allocatable :: best_iref(:), best_irot(:), best_ishift(:), ids(:), miref(:), mirot(:)
         integer(c_int32_t), allocatable :: qref(:), qptcl(:), qrot(:)
         real(c_double), allocatable :: qsh(:,:), f8(:), g8(:,:)
         integer :: nrefs, nptcls, n, kfrom, kto, kstop, box, withctf, nsh, nq, pfrom, nk, nrots
@@ -100,6 +100,9 @@ contains
             call pftcc%search(ids, best_cc, best_iref, best_irot)
             best_ishift = 0
         endif
+        ! precision form with exact rescoring of the 3 best references
+        allocate(mcc(nptcls), miref(nptcls), mirot(nptcls))
+        call pftcc%search_mfma(ids, mcc, miref, mirot, PREC_SPLIT_BF16, 3)
         allocate(f8(max(nq,1)), g8(2, max(nq,1)), scores(max(nq,1)))
         do q = 1, nq
             f8(q) = pftcc%gencorr_for_rot_8(qref(q), qptcl(q), qsh(:,q), qrot(q))
@@ -118,6 +121,9 @@ contains
             write(u) g8(:, 1:nq)
             write(u) scores(1:nq)
         endif
+        write(u) mcc
+        write(u) int(miref, c_int32_t)
+        write(u) int(mirot, c_int32_t)
         close(u)
         write(*,'(a,i0,a,i0,a,i0,a)') 'parity run done: ', nptcls, ' ptcls x ', nrefs, ' refs x ', nsh, ' shifts'
     end subroutine run_parity

@@ -17,10 +17,12 @@ use, intrinsic :: iso_c_binding
 implicit none
 
 public :: polarft_corrcalc, pftcc_glob, pftcc_params, ctfparams, OBJFUN_CC, OBJFUN_EUCLID
+public :: PREC_BF16, PREC_SPLIT_BF16, PREC_FP8
 private
 
 integer, parameter :: sp = c_float, dp = c_double
 integer, parameter :: OBJFUN_CC = 0, OBJFUN_EUCLID = 1
+integer, parameter :: PREC_BF16 = 1, PREC_SPLIT_BF16 = 3, PREC_FP8 = 8   ! precision form
 
 ! params_glob fields read by polarft_corrcalc (simple_parameters.f90:1182-1371)
 type :: pftcc_params
@@ -96,6 +98,7 @@ type :: polarft_corrcalc
     generic            :: specscore => specscore_1, specscore_2
     procedure          :: calc_roinv_corrmat
     procedure          :: search
+    procedure          :: search_mfma
     procedure          :: grad_batch
     procedure          :: kill
 end type polarft_corrcalc
@@ -329,6 +332,16 @@ interface
         type(c_ptr), value :: irefs, shifts
         real(c_float), intent(out) :: best_cc(*)
         integer(c_int32_t), intent(out) :: best_iref(*), best_ishift(*), best_irot(*)
+    end function
+    integer(c_int) function c_search_mfma(h, np, iptcls, nr, irefs, prec, ntop, best_cc, best_iref, best_irot) &
+        &bind(C, name='pftcc_search_mfma')
+        import :: c_int, c_int32_t, c_ptr, c_float
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: np, nr, prec, ntop
+        integer(c_int32_t), intent(in) :: iptcls(*)
+        type(c_ptr), value :: irefs
+        real(c_float), intent(out) :: best_cc(*)
+        integer(c_int32_t), intent(out) :: best_iref(*), best_irot(*)
     end function
     integer(c_int) function c_grad_batch(h, npairs, irefs, iptcls, shvecs, irots, f, grad) bind(C, name='pftcc_grad_batch')
         import :: c_int, c_int32_t, c_ptr, c_double
@@ -744,6 +757,33 @@ contains
         best_irot = brt
         if( present(best_ishift) ) best_ishift = bis + 1
     end subroutine search
+
+    ! precision form of search without shifts: MFMA contraction per angular frequency, the ntop
+    ! best references per particle rescored exactly in FP64 (ntop = 0: no rescoring)
+    subroutine search_mfma( self, iptcls, best_cc, best_iref, best_irot, prec, ntop, irefs )
+        class(polarft_corrcalc),    intent(inout) :: self
+        integer,                    intent(in)    :: iptcls(:)
+        real(sp),                   intent(out)   :: best_cc(size(iptcls))
+        integer,                    intent(out)   :: best_iref(size(iptcls)), best_irot(size(iptcls))
+        integer,                    intent(in)    :: prec, ntop
+        integer,  optional, target, intent(in)    :: irefs(:)
+        integer(c_int32_t) :: ip(size(iptcls)), bir(size(iptcls)), brt(size(iptcls))
+        integer(c_int32_t), allocatable, target :: ir(:)
+        type(c_ptr) :: pir
+        integer(c_int32_t) :: nr
+        ip  = iptcls
+        pir = c_null_ptr
+        nr  = self%nrefs
+        if( present(irefs) )then
+            allocate(ir(size(irefs)), source=int(irefs, c_int32_t))
+            pir = c_loc(ir)
+            nr  = size(irefs)
+        endif
+        call chk(self, c_search_mfma(self%h, int(size(iptcls),c_int32_t), ip, nr, pir, int(prec,c_int32_t), &
+            &int(ntop,c_int32_t), best_cc, bir, brt), 'search_mfma')
+        best_iref = bir
+        best_irot = brt
+    end subroutine search_mfma
 
     subroutine grad_batch( self, irefs, iptcls, shvecs, irots, f, grad )
         class(polarft_corrcalc), intent(inout) :: self

@@ -69,6 +69,9 @@ def test_fortran_module_parity(tmp_path):
     f8 = take(np.float64, 3)
     g8 = take(np.float64, 6).reshape(3, 2)
     scores = take(np.float32, 3)
+    m_cc = take(np.float32, 4)
+    m_ref = take(np.int32, 4)
+    m_rot = take(np.int32, 4)
     assert off == len(raw)
     # the Python mirror over the same C ABI gives bitwise identical numbers
     lib = to_lib(pf, cps)
@@ -91,4 +94,10 @@ def test_fortran_module_parity(tmp_path):
         assert abs(f8[q] - fo) <= 1e-9 * max(1, abs(fo))
         assert np.all(np.abs(g8[q] - go) <= 1e-9 * np.maximum(1, np.abs(go)))
         assert abs(scores[q] - pf.specscore(q_ref[q] - 1, q_ptcl[q] - 1, q_rot[q])) < 1e-5
+    # precision form through the module (split BF16, top-3 rescored in FP64): the exact argmax
+    for i in range(4):
+        exact = np.stack([pf.gencorrs(rr, i) for rr in range(3)])
+        r_, t_ = np.unravel_index(int(np.argmax(exact)), exact.shape)
+        assert (m_ref[i], m_rot[i]) == (r_ + 1, t_ + 1)
+        assert abs(m_cc[i] - exact[r_, t_]) <= 1e-7
     lib.kill()
