@@ -193,6 +193,16 @@ int pftcc_grad_batch_dev(pftcc_t* h, int32_t npairs, const int32_t* d_irefs,
                          const int32_t* d_iptcls, const double* d_shvecs, const int32_t* d_irots,
                          double* d_f, double* d_grad);
 
+/* prep_matchfilt (simple_polarft_corrcalc.f90:987-1095): with match_filt, whiten particle iptcl
+ * against reference iref at rotation irot (1-based) -- shell powers normalised to optlp^2, the
+ * reference-particle difference spectrum smoothed over shells (Savitzky-Golay, 7 points, order 4,
+ * simple_math.f90:3511), particle scaled to power ssnr/(ssnr+1); with pssnr scaled by
+ * sqrt(k/pw) sqrt(1 + ssnr |CTF|^2) -- store it and re-memoize its spectra.  iref = 0 or
+ * match_filt off: memoize only.  The batch form takes distinct particles. */
+int pftcc_prep_matchfilt(pftcc_t* h, int32_t iptcl, int32_t iref, int32_t irot);
+int pftcc_prep_matchfilt_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, const int32_t* irefs,
+                               const int32_t* irots);
+
 /* -------------------------------------------------------- precision (MFMA) form -- */
 /* Exhaustive search without shifts (the 3D-refinement use of gencorrs_1) computed as one dense
  * complex contraction over shells per angular frequency m on MFMA, followed by the inverse FFT

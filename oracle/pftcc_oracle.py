@@ -22,7 +22,12 @@ Conventions
 Parity status (each function is pinned by tests/test_oracle_*.py; see the test docstrings):
 every function here is pinned by at least one of: two independent formulations that must
 agree, a closed form, brute force on tiny inputs, or central finite differences.
-``prep_matchfilt`` (``:987-1095``) is not restated here.
+``prep_matchfilt`` (``:987-1095``) and its Savitzky-Golay smoother (``simple_math.f90:3511-3588``)
+are pinned by closed forms: the 7-point quartic weights (5, -30, 75, 131, 75, -30, 5)/231, exact
+reproduction of quartic polynomials inside the window, the border rules, the per-shell power of
+the filtered particle (ssnr/(ssnr+1), or k (1 + ssnr <|CTF|^2>) with pssnr) and the identity for
+iref = 0.  The reference's FP32 rounding inside ``prep_matchfilt`` (FP32 ``ludcmp`` weights, FP32
+scaling) is not reproduced bit for bit.
 """
 from __future__ import annotations
 
@@ -391,6 +396,41 @@ def corrs_over_k_fft(q: np.ndarray, pa: np.ndarray, pb: np.ndarray) -> np.ndarra
 
 
 # ---------------------------------------------------------------------------------------------
+# a17: Savitzky-Golay smoothing used by prep_matchfilt (simple_math.f90:3511-3588)
+# ---------------------------------------------------------------------------------------------
+SG_NL, SG_NR, SG_M = 3, 3, 4
+
+
+def savgol_coefficients(nl: int = SG_NL, nr: int = SG_NR, m: int = SG_M) -> dict:
+    """Smoothing weights c[o], o = -nl..nr, of the least-squares polynomial of order m fitted
+    to the window and evaluated at its centre (savgol with ld = 0, :3546-3587): solve the normal
+    equations A a = e_0 with A[i][j] = sum_o o^(i+j) and c[o] = sum_i a_i o^i.  FP64."""
+    offs = np.arange(-nl, nr + 1, dtype=np.float64)
+    A = np.array([[np.sum(offs ** (i + j)) for j in range(m + 1)] for i in range(m + 1)])
+    e = np.zeros(m + 1)
+    e[0] = 1.0
+    a = np.linalg.solve(A, e)
+    c = np.array([np.sum(a * o ** np.arange(m + 1)) for o in offs])
+    return {int(o): float(w) for o, w in zip(offs, c)}
+
+
+def savitzky_golay_filter(y: np.ndarray) -> np.ndarray:
+    """SavitzkyGolay_filter (:3511-3542): y_i <- sum_o c[o] y_{i+o} for i = 1..n-nr, skipping the
+    window points left of the first sample (no renormalisation); the last nr samples keep their
+    values."""
+    c = savgol_coefficients()
+    n = len(y)
+    out = np.array(y, dtype=np.float64)
+    for i in range(n - SG_NR):
+        acc = 0.0
+        for o, w in c.items():
+            if i + o >= 0:
+                acc += w * float(y[i + o])
+        out[i] = acc
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
 # Object state (the parts of type polarft_corrcalc, :68-198, that the math needs)
 # ---------------------------------------------------------------------------------------------
 @dataclass
@@ -458,6 +498,54 @@ class Pftcc:
                     w /= math.sqrt(pw)
             q[kk] *= w
         return q
+
+    def prep_matchfilt(self, i: int, iref: int, irot: int) -> np.ndarray:
+        """prep_matchfilt (:987-1095): the particle rings after the match filter, [nk, n] complex128.
+
+        iref < 0 (the reference's iref = 0) or match_filt off: unchanged.  pssnr: per shell
+        p <- p sqrt(k)/sqrt(pw_p) sqrt(1 + ssnr |CTF|^2) (ssnr = ref_optlp).  Otherwise: particle
+        and reference (x |CTF|) are each scaled to power optlp^2 per shell, the power of their
+        difference at rotation irot is smoothed over shells (Savitzky-Golay, the first 3 shells and
+        non-positive values kept raw), and the particle is scaled to power ssnr/(ssnr+1) with
+        ssnr = pw_ref / pw_diff.  Shells beyond kstop are untouched.  FP64.
+        """
+        p = self.ptcls[i].astype(np.complex128).copy()
+        if iref < 0 or not self.match_filt:
+            return p
+        n = self.pftsz
+        ks = self.ks
+        opt = self.ref_optlp[iref].astype(np.float64)
+        c = self.ctf(i)
+        if self.pssnr:
+            for kk in range(ks):
+                k = self.kfrom + kk
+                pw = float(np.sum(np.abs(p[kk]) ** 2)) / n
+                w = math.sqrt(k)
+                if pw > 1e-12:
+                    w /= math.sqrt(pw)
+                p[kk] = p[kk] * w * np.sqrt(1.0 + opt[kk] * c[kk] ** 2)
+            return p
+        r = self.ref(iref, i) * c
+        for kk in range(ks):
+            pw = float(np.sum(np.abs(p[kk]) ** 2)) / n
+            p[kk] = p[kk] * opt[kk] / math.sqrt(pw) if pw > 1e-12 else p[kk] * opt[kk]
+            pr = float(np.sum(np.abs(r[kk]) ** 2)) / n
+            r[kk] = r[kk] * opt[kk] / math.sqrt(pr) if pr > 1e-12 else r[kk] * opt[kk]
+        P = full_ring(p[:ks])
+        idx = (np.arange(n) + (irot - 1)) % (2 * n)
+        diff = np.sum(np.abs(r[:ks] - P[:, idx]) ** 2, axis=-1) / n
+        fit = savitzky_golay_filter(diff)
+        fit = np.where(fit <= 0.0, diff, fit)
+        fit[:3] = diff[:3]
+        for kk in range(ks):
+            pw = float(np.sum(np.abs(p[kk]) ** 2)) / n
+            w = 1.0 / math.sqrt(pw) if pw > 1e-12 else 1.0
+            pr = float(np.sum(np.abs(r[kk]) ** 2)) / n
+            if fit[kk] > 1e-12:
+                ssnr = pr / fit[kk]
+                w *= math.sqrt(ssnr / (ssnr + 1.0))
+            p[kk] = w * p[kk]
+        return p
 
     def ctf(self, i: int) -> np.ndarray:
         if self.ctfmats is None:

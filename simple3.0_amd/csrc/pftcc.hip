@@ -21,6 +21,7 @@
 #include "pftcc_kernels.hpp"
 #include "corr_fused.hpp"
 #include "mfma_search.hpp"
+#include "matchfilt.hpp"
 
 using namespace pfk;
 
@@ -104,6 +105,8 @@ struct pftcc {
   float2 *d_refs_e = nullptr, *d_refs_o = nullptr, *d_ptcls = nullptr, *d_memo = nullptr;
   float2 *d_tw2 = nullptr, *d_dptcls = nullptr;
   float *d_ctf = nullptr, *d_sqsums = nullptr, *d_sqsums_k = nullptr;
+  float* d_optlp = nullptr;         // [nrefs][kcorr] device copy of ref_optlp (prep_matchfilt)
+  bool optlp_dev_valid = false;
   int8_t* d_iseven = nullptr;
   double *d_argx = nullptr, *d_argy = nullptr, *d_onex = nullptr, *d_oney = nullptr;
   float *d_spafreq = nullptr, *d_angmat = nullptr;
@@ -323,7 +326,7 @@ int upload_iseven(pftcc* h) {
 int update_sqsums(pftcc* h, int first, int count) {
   if (count <= 0) return PFTCC_OK;
   hipLaunchKernelGGL(k_sqsums, dim3(count), dim3(256), 0, h->stream, (const float2*)h->d_ptcls, h->nk,
-                     h->n, h->kcorr, h->d_sqsums, h->d_sqsums_k, first, count);
+                     h->n, h->kcorr, h->d_sqsums, h->d_sqsums_k, first, count, (const int*)nullptr);
   HIPCHK(h, hipGetLastError());
   return PFTCC_OK;
 }
@@ -770,7 +773,8 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, bool fp8, const s
   int F = 8;
   while (F > 1 && sizeof(float2) * ((size_t)2 * F * n + 2 * n) > kMaxLds) F /= 2;
   hipLaunchKernelGGL(k_memoize, dim3((nr * nk + F - 1) / F), dim3(256), sizeof(float2) * ((size_t)2 * F * n + 2 * n),
-                     h->stream, (const float2*)rings, 0, nr, nk, n, h->plan, (const float2*)h->d_tw2, spec, F);
+                     h->stream, (const float2*)rings, 0, nr, nk, n, h->plan, (const float2*)h->d_tw2, spec, F,
+                     (const int*)nullptr);
   if (fp8) {
     if ((rc = shell_ms(h, spec, nullptr, nr, ap.msV))) return rc;
     ap.ak = fp8_shell_scales(ap.msV, *msW);
@@ -1153,6 +1157,7 @@ int pftcc_kill(pftcc_t* h) {
   dfree(h->d_refinv_d);
   dfree(h->d_ofac_d);
   dfree(h->d_refpw);
+  dfree(h->d_optlp);
   for (auto& ap : h->apack) free_apack(ap);
   if (h->d_scratch) (void)hipFree(h->d_scratch);
   delete h;
@@ -1367,7 +1372,108 @@ int pftcc_set_ref_optlp(pftcc_t* h, int32_t iref, const float* optlp) {
   if (h->optlp.empty()) h->optlp.assign((size_t)h->nrefs * h->kcorr, 1.0f);
   std::memcpy(&h->optlp[(size_t)(iref - 1) * h->kcorr], optlp, sizeof(float) * h->kcorr);
   h->refnorm_valid = false;
+  h->optlp_dev_valid = false;
   return PFTCC_OK;
+}
+
+namespace {
+// prep_matchfilt for a list of (particle, reference, rotation); iref 0 or match_filt off: the
+// particle is only memoized, as in the reference.
+int prep_matchfilt_run(pftcc* h, int np, const int32_t* iptcls, const int32_t* irefs, const int32_t* irots) {
+  int rc;
+  std::vector<int> pl(np), rl(np), rh(np);
+  std::vector<char> seen(h->nptcls, 0);
+  bool any = false;
+  for (int i = 0; i < np; ++i) {
+    if ((rc = local_index(h, iptcls[i], &pl[i]))) return rc;
+    if (seen[pl[i]]) return fail(h, PFTCC_E_ARG, "particle %d listed twice", iptcls[i]);
+    seen[pl[i]] = 1;
+    rl[i] = -1;
+    rh[i] = 0;
+    if (h->match_filt && irefs[i] != 0) {
+      if ((rc = check_ref(h, irefs[i])) || (rc = check_rot(h, irots[i]))) return rc;
+      rl[i] = irefs[i] - 1;
+      rh[i] = irots[i] - 1;
+      any = true;
+    }
+  }
+  if (any && h->with_ctf && !h->ctf_ready)
+    return fail(h, PFTCC_E_STATE, "CTF matrices missing: call create_polar_absctfmats");
+  if (h->kcorr > MF_MAXK) return fail(h, PFTCC_E_UNSUPPORTED, "prep_matchfilt supports at most %d shells", MF_MAXK);
+  if (!h->memo_valid) {
+    // the rest of the particles need their spectra too (the reference's arrays always hold them)
+    int F = 8;
+    while (F > 1 && sizeof(float2) * ((size_t)2 * F * h->n + 2 * h->n) > kMaxLds) F /= 2;
+    const size_t lds = sizeof(float2) * ((size_t)2 * F * h->n + 2 * h->n);
+    hipLaunchKernelGGL(k_memoize, dim3((h->nptcls * h->nk + F - 1) / F), dim3(256), lds, h->stream,
+                       (const float2*)h->d_ptcls, 0, h->nptcls, h->nk, h->n, h->plan, (const float2*)h->d_tw2,
+                       h->d_memo, F, (const int*)nullptr);
+    HIPCHK(h, hipGetLastError());
+  }
+  if (any && !h->optlp_dev_valid) {
+    const size_t cnt = (size_t)h->nrefs * h->kcorr;
+    std::vector<float> o = h->optlp.empty() ? std::vector<float>(cnt, 1.0f) : h->optlp;
+    if (!h->d_optlp && (rc = dalloc(h, &h->d_optlp, cnt))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->d_optlp, o.data(), cnt * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->optlp_dev_valid = true;
+  }
+  if ((rc = scratch_reserve(h, 3 * aligned((size_t)np * 4)))) return rc;
+  int* d_pl = take<int>(h, np);
+  int* d_rl = take<int>(h, np);
+  int* d_rh = take<int>(h, np);
+  HIPCHK(h, hipMemcpyAsync(d_pl, pl.data(), np * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_rl, rl.data(), np * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_rh, rh.data(), np * 4, hipMemcpyHostToDevice, h->stream));
+  if (any) {
+    MatchFiltArgs a{};
+    a.n = h->n;
+    a.nk = h->nk;
+    a.kcorr = h->kcorr;
+    a.kfrom = h->kfrom;
+    a.nrefs = h->nrefs;
+    a.pssnr = h->pssnr ? 1 : 0;
+    a.ptcls = h->d_ptcls;
+    a.refs_even = h->d_refs_e;
+    a.refs_odd = h->d_refs_o;
+    a.ctf = h->with_ctf ? h->d_ctf : nullptr;
+    a.optlp = h->d_optlp;
+    a.iseven = h->d_iseven;
+    a.plist = d_pl;
+    a.rlist = d_rl;
+    a.rhos = d_rh;
+    a.count = np;
+    hipLaunchKernelGGL(k_prep_matchfilt, dim3(np), dim3(256), 0, h->stream, a);
+    HIPCHK(h, hipGetLastError());
+    hipLaunchKernelGGL(k_sqsums, dim3(np), dim3(256), 0, h->stream, (const float2*)h->d_ptcls, h->nk, h->n,
+                       h->kcorr, h->d_sqsums, h->d_sqsums_k, 0, np, (const int*)d_pl);
+    HIPCHK(h, hipGetLastError());
+  }
+  int F = 8;
+  while (F > 1 && sizeof(float2) * ((size_t)2 * F * h->n + 2 * h->n) > kMaxLds) F /= 2;
+  const size_t lds = sizeof(float2) * ((size_t)2 * F * h->n + 2 * h->n);
+  hipLaunchKernelGGL(k_memoize, dim3((np * h->nk + F - 1) / F), dim3(256), lds, h->stream, (const float2*)h->d_ptcls, 0,
+                     np, h->nk, h->n, h->plan, (const float2*)h->d_tw2, h->d_memo, F, (const int*)d_pl);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->memo_valid = true;
+  h->dptcls_valid = false;
+  return PFTCC_OK;
+}
+}  // namespace
+
+int pftcc_prep_matchfilt(pftcc_t* h, int32_t iptcl, int32_t iref, int32_t irot) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  return prep_matchfilt_run(h, 1, &iptcl, &iref, &irot);
+}
+
+int pftcc_prep_matchfilt_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, const int32_t* irefs,
+                               const int32_t* irots) {
+  if (!h || np < 0 || (np > 0 && (!iptcls || !irefs || !irots))) return PFTCC_E_ARG;
+  if (np == 0) return PFTCC_OK;
+  std::lock_guard<std::mutex> g(h->mtx);
+  return prep_matchfilt_run(h, np, iptcls, irefs, irots);
 }
 
 int pftcc_assign_sigma2_noise(pftcc_t* h, const float* sigma2) {
@@ -1463,7 +1569,7 @@ int pftcc_memoize_ffts(pftcc_t* h) {
   const int nitems = h->nptcls * h->nk;
   const size_t lds = sizeof(float2) * ((size_t)2 * F * h->n + 2 * h->n);
   hipLaunchKernelGGL(k_memoize, dim3((nitems + F - 1) / F), dim3(256), lds, h->stream, (const float2*)h->d_ptcls, 0,
-                     h->nptcls, h->nk, h->n, h->plan, (const float2*)h->d_tw2, h->d_memo, F);
+                     h->nptcls, h->nk, h->n, h->plan, (const float2*)h->d_tw2, h->d_memo, F, (const int*)nullptr);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->memo_valid = true;

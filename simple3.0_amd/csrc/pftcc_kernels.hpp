@@ -49,9 +49,10 @@ __global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
 
 // memoize_sqsum_ptcl (:841-845) and per-shell sums; one block per particle.
 __global__ void k_sqsums(const float2* __restrict__ ptcls, int nk, int n, int kcount,
-                         float* __restrict__ sqsums, float* __restrict__ sqsums_k, int first, int count) {
-  const int ip = first + blockIdx.x;
+                         float* __restrict__ sqsums, float* __restrict__ sqsums_k, int first, int count,
+                         const int* __restrict__ list) {
   if (blockIdx.x >= count) return;
+  const int ip = list ? list[blockIdx.x] : first + blockIdx.x;
   __shared__ double red[256];
   double tot = 0.0;
   for (int kk = 0; kk < nk; ++kk) {
@@ -172,7 +173,8 @@ __device__ __forceinline__ float2 rfft_split(const float2* __restrict__ zc, int 
 // memoize_ffts (:848-877): W[ip][kk][m] = rfft_2n(w)[m], w = (Re p + Im p | Re p - Im p).
 // One block handles F (particle, shell) items.  Dynamic LDS: 2*F*n + 2n float2.
 __global__ void k_memoize(const float2* __restrict__ ptcls, int first, int count, int nk, int n,
-                          FftPlan plan, const float2* __restrict__ tw2g, float2* __restrict__ memo, int F) {
+                          FftPlan plan, const float2* __restrict__ tw2g, float2* __restrict__ memo, int F,
+                          const int* __restrict__ list) {
   extern __shared__ float2 smem[];
   float2* bufA = smem;
   float2* bufB = bufA + (size_t)F * n;
@@ -187,7 +189,7 @@ __global__ void k_memoize(const float2* __restrict__ ptcls, int first, int count
     const int item = item0 + f;
     float2 a = make_float2(0.f, 0.f), b = a;
     if (item < nitems) {
-      const int ip = first + item / nk, kk = item % nk;
+      const int ip = (list ? list[item / nk] : first + item / nk), kk = item % nk;
       const float2* p = ptcls + ((size_t)ip * nk + kk) * n;
       a = p[2 * u];
       b = p[2 * u + 1];
@@ -200,7 +202,7 @@ __global__ void k_memoize(const float2* __restrict__ ptcls, int first, int count
     const int f = idx / (n + 1), m = idx - f * (n + 1);
     const int item = item0 + f;
     if (item >= nitems) continue;
-    const int ip = first + item / nk, kk = item % nk;
+    const int ip = (list ? list[item / nk] : first + item / nk), kk = item % nk;
     memo[((size_t)ip * nk + kk) * (n + 1) + m] = rfft_split(res + (size_t)f * n, n, m, tw2);
   }
 }

@@ -97,6 +97,7 @@ type :: polarft_corrcalc
     procedure, private :: specscore_2
     generic            :: specscore => specscore_1, specscore_2
     procedure          :: calc_roinv_corrmat
+    procedure          :: prep_matchfilt
     procedure          :: search
     procedure          :: search_mfma
     procedure          :: grad_batch
@@ -332,6 +333,11 @@ interface
         type(c_ptr), value :: irefs, shifts
         real(c_float), intent(out) :: best_cc(*)
         integer(c_int32_t), intent(out) :: best_iref(*), best_ishift(*), best_irot(*)
+    end function
+    integer(c_int) function c_prep_matchfilt(h, iptcl, iref, irot) bind(C, name='pftcc_prep_matchfilt')
+        import :: c_int, c_int32_t, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iptcl, iref, irot
     end function
     integer(c_int) function c_search_mfma(h, np, iptcls, nr, irefs, prec, ntop, best_cc, best_iref, best_irot) &
         &bind(C, name='pftcc_search_mfma')
@@ -720,6 +726,14 @@ contains
         allocate(corrmat(self%nptcls, self%nptcls))
         call chk(self, c_calc_roinv_corrmat(self%h, corrmat), 'calc_roinv_corrmat')
     end function calc_roinv_corrmat
+
+    ! MATCH FILTER (:987-1095): whiten/filter the particle against iref at irot, re-memoize
+    subroutine prep_matchfilt( self, iptcl, iref, irot )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iptcl, iref, irot
+        call chk(self, c_prep_matchfilt(self%h, int(iptcl,c_int32_t), int(iref,c_int32_t), int(irot,c_int32_t)), &
+            &'prep_matchfilt')
+    end subroutine prep_matchfilt
 
     ! BATCHED FAST PATHS (no reference counterpart: the ref loop of strategy3D_greedy%srch,
     ! simple_strategy3D_greedy.f90:60-77, moved onto the device)

@@ -107,6 +107,8 @@ SIGNATURES = [
     ("pftcc_search_dev", _I, [_P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P]),
     ("pftcc_grad_batch", _I, [_P, _I, _PI, _PI, _PD, _PI, _PD, _PD]),
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    ("pftcc_prep_matchfilt", _I, [_P, _I, _I, _I]),
+    ("pftcc_prep_matchfilt_batch", _I, [_P, _I, _PI, _PI, _PI]),
     ("pftcc_search_mfma", _I, [_P, _I, _PI, _I, _PI, _I, _I, _PF, _PI, _PI]),
     ("pftcc_gencorrs_mfma_batch", _I, [_P, _I, _PI, _I, _PI, _I, _PF]),
     ("pftcc_get_timing_channel", _I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
@@ -511,6 +513,18 @@ class PolarFTCorrCalc:
                                                  C.c_void_p(d_irots), C.c_void_p(d_f),
                                                  C.c_void_p(d_grad) if d_grad else None),
                   "grad_batch_dev")
+
+    def prep_matchfilt(self, iptcl, iref, irot):
+        """prep_matchfilt (:987-1095): match-filter particle iptcl against iref at irot (iref 0:
+        memoize only)."""
+        self._chk(self._lib.pftcc_prep_matchfilt(self._h, int(iptcl), int(iref), int(irot)), "prep_matchfilt")
+
+    def prep_matchfilt_batch(self, iptcls, irefs, irots):
+        p = np.ascontiguousarray(iptcls, np.int32)
+        r = np.ascontiguousarray(irefs, np.int32)
+        o = np.ascontiguousarray(irots, np.int32)
+        self._chk(self._lib.pftcc_prep_matchfilt_batch(self._h, p.size, _ip(p), _ip(r), _ip(o)),
+                  "prep_matchfilt_batch")
 
     def search_mfma(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16, ntop=0):
         """Precision form of search() without shifts: (best_cc, best_iref, best_irot).

@@ -310,3 +310,33 @@ def test_fused_and_generic_kernels_agree(n, kto, ldim, monkeypatch):
             for s in range(2):
                 ref = pf.gencorrs(r, p, tuple(shifts[s]) if s else None, method="fft")
                 assert peak_rel_err(fused[p, r, s], ref) < FP32_TOL
+
+
+@pytest.mark.parametrize("pssnr,ctf", [(False, True), (False, False), (True, True)])
+def test_prep_matchfilt_matches_oracle(pssnr, ctf):
+    """prep_matchfilt on the device against the oracle restatement (FP32 on the device, FP64 in the
+    oracle: particle rings within 2e-6 relative per shell); the filtered particles are then searched
+    with the shell-normalised references exactly as the oracle's gencorrs."""
+    say(f"prep_matchfilt pssnr={pssnr} ctf={ctf}")
+    pf, cps = make_case(seed=91, n=48, kfrom=2, kto=16, kstop=14, ctf=ctf, nrefs=3, nptcls=5, ldim=48,
+                        match_filt=True, eo_mixed=True)
+    pf.pssnr = pssnr
+    lib = to_lib(pf, cps, memoize=False)
+    ids = [1, 2, 3, 4, 5]
+    refs = [2, 0, 3, 1, 1]
+    rots = [1, 17, 49, 96, 60]
+    lib.prep_matchfilt(ids[0], refs[0], rots[0])
+    lib.prep_matchfilt_batch(ids[1:], refs[1:], rots[1:])
+    for t, i in enumerate(ids):
+        want = pf.prep_matchfilt(i - 1, refs[t] - 1, rots[t]) if refs[t] else pf.ptcls[i - 1].astype(np.complex128)
+        got = lib.get_ptcl_pft(i).astype(np.complex128)
+        scale = np.max(np.abs(want), axis=-1, keepdims=True)
+        assert np.max(np.abs(got - want) / scale) < 2e-6, t
+        pf.ptcls[i - 1] = got.astype(np.complex64)      # the oracle continues from the device's particles
+        assert abs(lib.get_sqsum_ptcl(i) - pf.sqsum_ptcl(i - 1)) <= 1e-6 * pf.sqsum_ptcl(i - 1)
+    for i in range(5):
+        for r in range(3):
+            assert peak_rel_err(lib.gencorrs(r + 1, i + 1), pf.gencorrs(r, i)) < FP32_TOL
+    with pytest.raises(PftccError):
+        lib.prep_matchfilt_batch([1, 1], [1, 2], [1, 1])
+    lib.kill()
