@@ -365,7 +365,7 @@ def main():
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated; "
                                      "compulsory bytes per launch are in hbm_compulsory_bytes",
                      "hbm_compulsory_bytes": bytes_launch,
-                     "kernel": "k_corr_fused<308,6,256>", "avg_launch_ms": avg_ms, "launches": launches,
+                     "kernel": "k_corr_fused<308,9,256,3>", "avg_launch_ms": avg_ms, "launches": launches,
                      "flops_per_pair": plain_flops_per_pair(pftsz, w.nk), "pairs_per_launch": pairs_launch,
                      "hbm_compulsory_GBs": bytes_launch / (avg_ms * 1e-3) / 1e9,
                      "hbm_frac": bytes_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},

@@ -113,12 +113,17 @@ constexpr int fused_F() {
   return F;
 }
 
-// Pair tile measured fastest on MI355X for the ring lengths of the benchmark shapes (n = 192, 308:
-// a smaller tile keeps the kernel at 3 wavefronts per SIMD); fused_F elsewhere.
+// Pair tile and register budget measured fastest on MI355X for the ring lengths of the benchmark
+// shapes: n = 308, 9 pairs at >= 3 wavefronts per SIMD (167 VGPRs and a 40-byte spill, against
+// 2 wavefronts at ~190 VGPRs unconstrained); n = 192, 5 pairs.  fused_F elsewhere.
 template <int N, int NT>
 constexpr int fused_F_tuned() {
-  if (NT == 256 && (N == 192 || N == 308)) return N == 192 ? 5 : 6;
+  if (NT == 256 && (N == 192 || N == 308)) return N == 192 ? 5 : 9;
   return fused_F<N, NT>();
+}
+template <int N, int NT>
+constexpr int fused_W_tuned() {
+  return (NT == 256 && N == 308) ? 3 : 1;
 }
 
 template <int N, int F>
@@ -149,9 +154,11 @@ __device__ __forceinline__ float2 unsplit_bin_conj(float2 X, float2 Xn, float2 e
   return make_float2(s.x - ed.y, -(s.y + ed.x));
 }
 
-template <int N, int F, int NT>
-__global__ void __launch_bounds__(NT) k_corr_fused(CorrArgs a) {
+// W: minimum wavefronts per SIMD the register allocation must allow (1: no constraint).
+template <int N, int F, int NT, int W>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_corr_fused(CorrArgs a) {
   static_assert(Fp<N>::fusable, "first radix must be 4");
+  static_assert(NT % 64 == 0, "whole wavefronts");
   constexpr int NP = Fp<N>::np;
   constexpr int RL = Fp<N>::R(NP - 1);      // last radix
   constexpr int NsL = N / RL;               // product of the other radices
@@ -172,7 +179,7 @@ __global__ void __launch_bounds__(NT) k_corr_fused(CorrArgs a) {
   float* offs = sqs + F * Q;                 // [F][Q]
   __shared__ float sq[F], off[F];
   __shared__ float pscal;
-  __shared__ unsigned long long red[NT];
+  __shared__ unsigned long long red[(NT + 63) / 64];
 
   const int tid = threadIdx.x;
   const int pi = blockIdx.x / a.ntiles;
@@ -427,13 +434,19 @@ __global__ void __launch_bounds__(NT) k_corr_fused(CorrArgs a) {
     }
   }
   if (a.best) {
-    red[tid] = bestkey;
-    __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
-      if (tid < w && red[tid + w] > red[tid]) red[tid] = red[tid + w];
-      __syncthreads();
+    // wavefront maxima, then the wavefronts' in a fixed order (any NT that is a multiple of 64)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long v = __shfl_xor(bestkey, o);
+      bestkey = v > bestkey ? v : bestkey;
     }
-    if (tid == 0) atomicMax(&a.best[pi], red[0]);
+    if ((tid & 63) == 0) red[tid >> 6] = bestkey;
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long m = red[0];
+      for (int w = 1; w < NT / 64; ++w) m = red[w] > m ? red[w] : m;
+      atomicMax(&a.best[pi], m);
+    }
   }
 }
 

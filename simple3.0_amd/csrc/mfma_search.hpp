@@ -304,13 +304,10 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   constexpr int TPT1 = (T1 + NT - 1) / NT;
   constexpr int RL = Fp<N>::R(NP - 1);
   constexpr int NsL = N / RL;
-  constexpr int TL = F * NsL;
-  constexpr int TPTL = (TL + NT - 1) / NT;
   extern __shared__ float2 smem[];
   float2* buf = smem;                    // [F][N]
   float2* tw2 = buf + F * N;             // [2N]
   __shared__ double den[F];
-  __shared__ unsigned long long bkey[F];
   const int tid = threadIdx.x;
   // 1-D launch of gx * nrefs work-groups; XCD-aware order keeps the work-groups that read one Z
   // group (consecutive particle blocks of one reference) on one XCD
@@ -319,7 +316,6 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   const int r = t / gx;
   const int p0 = (t % gx) * F;
   for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
-  if (tid < F) bkey[tid] = 0ull;
   {
     const int f = tid / TPF, u = tid % TPF;
     const int p = p0 + f;
@@ -383,49 +379,55 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   __syncthreads();
   lds_passes<N, F, NT, 1, NP - 1>(buf, tw2, tid);
   // ---- last pass in registers, then normalisation, output and best keys ----
+  // Each wavefront owns whole transforms (f = wave, wave + NW, ...), so a pair's best key is a
+  // wavefront reduction with no LDS atomics.
   const double inv4n = 1.0 / (4.0 * N);
   const int iref = a.rlist ? a.rlist[r] : r;
   const bool want_key = a.best || a.prbest;
-#pragma unroll
-  for (int c = 0; c < TPTL; ++c) {
-    const int task = tid + c * NT;
-    if (!(TL % NT == 0 || task < TL)) continue;
-    const int f = task / NsL, j = task % NsL;
+  constexpr int NW = NT / 64;
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int f = wave; f < F; f += NW) {
     const int p = p0 + f;
-    float2 v[RL];
-#pragma unroll
-    for (int t = 0; t < RL; ++t) v[t] = buf[f * N + j + t * NsL];
-#pragma unroll
-    for (int t = 1; t < RL; ++t) v[t] = cmul(v[t], tw2[2 * j * t]);
-    Dft<RL>::run(v);
-    if (p >= a.np) continue;
-    const double sc = inv4n / den[f];
+    const bool live = p < a.np;
+    const double sc = live ? inv4n / den[f] : 0.0;
     unsigned long long bk = 0ull;
 #pragma unroll
-    for (int t = 0; t < RL; ++t) {
-      const int jj = j + t * NsL;                   // output index: rotations 2 jj, 2 jj + 1
-      const float x0 = (float)(v[t].x * sc), x1 = (float)(-v[t].y * sc);
-      if (a.cc_out) {
-        float2* oo = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * N));
-        oo[jj] = make_float2(x0, x1);
-      }
-      if (want_key) {
-        const unsigned int lin0 = (unsigned int)iref * (unsigned int)(2 * N) + 2u * jj;
-        const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
-        const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
-        bk = k0 > bk ? k0 : bk;
-        bk = k1 > bk ? k1 : bk;
+    for (int j0 = 0; j0 < NsL; j0 += 64) {
+      const int j = j0 + lane;
+      if (j >= NsL) continue;
+      float2 v[RL];
+#pragma unroll
+      for (int t = 0; t < RL; ++t) v[t] = buf[f * N + j + t * NsL];
+#pragma unroll
+      for (int t = 1; t < RL; ++t) v[t] = cmul(v[t], tw2[2 * j * t]);
+      Dft<RL>::run(v);
+      if (!live) continue;
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int jj = j + t * NsL;                   // output index: rotations 2 jj, 2 jj + 1
+        const float x0 = (float)(v[t].x * sc), x1 = (float)(-v[t].y * sc);
+        if (a.cc_out) {
+          float2* oo = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * N));
+          oo[jj] = make_float2(x0, x1);
+        }
+        if (want_key) {
+          const unsigned int lin0 = (unsigned int)iref * (unsigned int)(2 * N) + 2u * jj;
+          const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
+          const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
+          bk = k0 > bk ? k0 : bk;
+          bk = k1 > bk ? k1 : bk;
+        }
       }
     }
-    if (want_key) atomicMax(&bkey[f], bk);
-  }
-  if (want_key) {
-    __syncthreads();
-    if (tid < F) {
-      const int p = p0 + tid;
-      if (p < a.np) {
-        if (a.best) atomicMax(&a.best[a.outpos[p]], bkey[tid]);
-        if (a.prbest) a.prbest[(size_t)a.outpos[p] * a.nrefs + r] = bkey[tid];
+    if (want_key && live) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(bk, o);
+        bk = w > bk ? w : bk;
+      }
+      if (lane == 0) {
+        if (a.best) atomicMax(&a.best[a.outpos[p]], bk);
+        if (a.prbest) a.prbest[(size_t)a.outpos[p] * a.nrefs + r] = bk;
       }
     }
   }

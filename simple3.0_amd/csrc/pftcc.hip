@@ -423,16 +423,20 @@ int timing_end(pftcc* h, hipEvent_t e1);
 #ifndef PFTCC_FUSED_NT
 #define PFTCC_FUSED_NT 256
 #endif
+#ifndef PFTCC_FUSED_W_OVERRIDE
+#define PFTCC_FUSED_W_OVERRIDE 0    // register budget for tuning builds (0: fused_W_tuned)
+#endif
 template <int N>
 struct FusedLaunch {
   static constexpr int NT = PFTCC_FUSED_NT;
   static constexpr int F = (PFTCC_FUSED_F_OVERRIDE > 0 && fused_lds_bytes<N, PFTCC_FUSED_F_OVERRIDE + 0>() <= 65536)
                                ? PFTCC_FUSED_F_OVERRIDE
                                : fused_F_tuned<N, NT>();
+  static constexpr int W = PFTCC_FUSED_W_OVERRIDE > 0 ? PFTCC_FUSED_W_OVERRIDE : fused_W_tuned<N, NT>();
   static constexpr size_t lds = fused_lds_bytes<N, F>();
   static_assert(lds <= 65536, "fused kernel LDS");
   static void go(unsigned blocks, hipStream_t s, const CorrArgs& a) {
-    hipLaunchKernelGGL((k_corr_fused<N, F, NT>), dim3(blocks), dim3(NT), lds, s, a);
+    hipLaunchKernelGGL((k_corr_fused<N, F, NT, W>), dim3(blocks), dim3(NT), lds, s, a);
   }
 };
 

@@ -1,8 +1,8 @@
 """The gfx950 library against the committed golden fixtures (tests/golden/), through the C ABI.
 
 The library computes its own |CTF| from the stored CTF parameters (FP32, within the argument
-error of the FP64 model the fixture used), so correlations are compared at the FP32 tolerance
-of the plain form (1e-5 relative to the peak) and the FP64 objective at 1e-6 relative.
+error of the FP64 model the fixture used), so correlations, FP64 objectives and gradients are
+compared at the FP32 tolerance of the plain form (1e-5 relative to the largest value).
 """
 import os
 
@@ -46,9 +46,12 @@ def test_library_reproduces_golden(name):
             assert peak_rel_err(lib.gencorrs(r + 1, i + 1), cc[i, r, 0]) < 1e-5
             for s, sh in enumerate(g["shifts"]):
                 assert peak_rel_err(lib.gencorrs(r + 1, i + 1, sh.astype(np.float32)), cc[i, r, s + 1]) < 1e-5
+    # FP64 objective and gradient: the device's FP32 |CTF| differs from the fixture's FP64 model
+    # by the FP32 error of sin(chi) (up to ~1e-6 where chi is large), so compare relative to the
+    # largest value, as for the correlations
     f, gr = lib.grad_batch(g["q_ref"] + 1, g["q_ptcl"] + 1, g["q_sh"], g["q_rot"])
-    assert np.allclose(f, g["q_f"], rtol=1e-6, atol=1e-9)
-    assert np.allclose(gr[:, :2], g["q_grad"], rtol=1e-5, atol=1e-7 * np.max(np.abs(g["q_grad"])))
+    assert np.max(np.abs(f - g["q_f"])) <= 1e-5 * np.max(np.abs(g["q_f"]))
+    assert np.max(np.abs(gr[:, :2] - g["q_grad"])) <= 1e-5 * np.max(np.abs(g["q_grad"]))
     # searches: plain (unshifted and shifted) and the rescored precision form give the fixture's argmax
     ids = np.arange(1, npt + 1)
     pcc, pir, _, prot = lib.search(ids)
