@@ -31,11 +31,11 @@ METRIC = ("ptcl×ref×rot corr evals/s: plain FP32 vs split-BF16 MFMA @ rel err 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
-# HBM bytes per launch of k_corr_fused<308,6,256> at 1000 particles x 200 refs x 169 shifts, from
+# HBM bytes per launch of k_corr_fused<308,9,256,3> at 1000 particles x 200 refs x 169 shifts, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script (--legs plain --copy-calibration):
-# FETCH 1318733024 KiB x 2.000 + WRITE 176062 KiB x 1.000, the factors read from k_copy moving
+# FETCH 1555153275 KiB x 2.000 + WRITE 33919635 KiB x 1.000, the factors read from k_copy moving
 # 1 GiB in the same runs (profiles/k_corr_fused_308_plain_search.md).
-PLAIN_TRAFFIC_BYTES_B1000 = (1318733024.25 * 2.0 + 176062.5) * 1024.0
+PLAIN_TRAFFIC_BYTES_B1000 = (1555153274.75 * 2.0 + 33919635.0) * 1024.0
 FP64_PEAK_TFLOPS = 78.6
 
 
