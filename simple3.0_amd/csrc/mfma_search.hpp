@@ -64,9 +64,15 @@ __device__ __forceinline__ float bf16_float(unsigned short b) {
   return __uint_as_float((unsigned int)b << 16);
 }
 
+// Split-BF16 operand layout, in 16-byte vectors (8 k' each): [m][row / 16][k' / 32][row % 16][4],
+// so that the 64 lanes of one MFMA operand load (16 rows x 32 k') read one contiguous kilobyte.
+__host__ __device__ __forceinline__ size_t split_vec(int m, int row, int vk, int rows_pad, int KP) {
+  return ((((size_t)m * (rows_pad / 16) + row / 16) * (KP / 32) + vk / 4) * 16 + (row % 16)) * 4 + (vk % 4);
+}
+
 // Packs spectra spec[row][kk][m] (float2, m in [0, n]) times an optional per-(row, kk) scale into
-// split-BF16 operand tiles hi/lo[m][row0 + row][k'] (k' = 2 kk + {0: re, 1: im}, zero past 2 kcorr).
-// One thread per (m, row, kk-pair); rows beyond count and k' beyond 2 kcorr are written as zero.
+// split-BF16 operands hi/lo (k' = 2 kk + {0: re, 1: im}, zero past 2 kcorr; layout split_vec).
+// One thread per (m, row, complex k); rows beyond count and k' beyond 2 kcorr are written as zero.
 __global__ void k_pack_split(const float2* __restrict__ spec, const float* __restrict__ scale, int count, int rows_pad,
                              int nk, int kcorr, int n, int KP, unsigned int* __restrict__ hi,
                              unsigned int* __restrict__ lo) {
@@ -86,8 +92,9 @@ __global__ void k_pack_split(const float2* __restrict__ spec, const float* __res
   }
   const unsigned short hr = bf16_bits(re), hiim = bf16_bits(im);
   const unsigned short lr = bf16_bits(re - bf16_float(hr)), li = bf16_bits(im - bf16_float(hiim));
-  hi[idx] = (unsigned int)hr | ((unsigned int)hiim << 16);
-  lo[idx] = (unsigned int)lr | ((unsigned int)li << 16);
+  const size_t w = split_vec(m, row, kk / 4, rows_pad, KP) * 4 + (kk % 4);
+  hi[w] = (unsigned int)hr | ((unsigned int)hiim << 16);
+  lo[w] = (unsigned int)lr | ((unsigned int)li << 16);
 }
 
 // A2 fragment from A1: per complex (re, im) word -> (-im, re).
@@ -109,52 +116,63 @@ __device__ __forceinline__ bf16x8 rot_neg(bf16x8 a) {
 struct ZTile {
   int p0, r0, m;
 };
-__device__ __forceinline__ ZTile ztile(int gx, int gy) {
+__device__ __forceinline__ ZTile ztile(int gx, int gy, int tp, int tr) {
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   ZTile z;
-  z.p0 = (t % gx) * 64;
-  z.r0 = ((t / gx) % gy) * 64;
+  z.p0 = (t % gx) * tp;
+  z.r0 = ((t / gx) % gy) * tr;
   z.m = t / (gx * gy);
   return z;
 }
 
-template <int SPLIT>
+#ifndef PFTCC_ZGEMM_TR
+#define PFTCC_ZGEMM_TR 4
+#endif
+#ifndef PFTCC_ZGEMM_TP
+#define PFTCC_ZGEMM_TP 4
+#endif
+constexpr int ZGEMM_TR = PFTCC_ZGEMM_TR;   // 16-row reference blocks per wavefront tile
+constexpr int ZGEMM_TP = PFTCC_ZGEMM_TP;   // 16-column particle blocks per wavefront tile
+
+// One (16 TR) x (16 TP) (reference x particle) tile per wavefront, 32 k' per MFMA step.
+template <int SPLIT, int TR, int TP>
 __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
                                               const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
                                               int rows_pad, int cols_pad, int KP, int n, float2* __restrict__ Z) {
   const int l = threadIdx.x;
-  const ZTile zt = ztile(cols_pad / 64, rows_pad / 64);
+  const ZTile zt = ztile(cols_pad / (16 * TP), rows_pad / (16 * TR), 16 * TP, 16 * TR);
   const int p0 = zt.p0, r0 = zt.r0, m = zt.m;
   const int KV = KP / 8;                                  // bf16x8 vectors per row
-  const size_t abase = ((size_t)m * rows_pad + r0) * KV;
-  const size_t bbase = ((size_t)m * cols_pad + p0) * KV;
-  f32x4 zr[4][4], zi[4][4];
+  f32x4 zr[TR][TP], zi[TR][TP];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TR; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < TP; ++j) {
       zr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       zi[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
   const int lr = l % 16, lk = l / 16;
   for (int ks = 0; ks < KV; ks += 4) {                    // 32 k' per step, lane takes 8 of them
-    bf16x8 ah[4], al[4], bh[4], bl[4];
+    bf16x8 ah[TR], al[TR], bh[TP], bl[TP];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const size_t ao = abase + (size_t)(i * 16 + lr) * KV + ks + lk;
+    for (int i = 0; i < TR; ++i) {
+      const size_t ao = split_vec(m, r0 + i * 16 + lr, ks + lk, rows_pad, KP);
       ah[i] = Ahi[ao];
       if (SPLIT == 3) al[i] = Alo[ao];
-      const size_t bo = bbase + (size_t)(i * 16 + lr) * KV + ks + lk;
-      bh[i] = Bhi[bo];
-      if (SPLIT == 3) bl[i] = Blo[bo];
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int j = 0; j < TP; ++j) {
+      const size_t bo = split_vec(m, p0 + j * 16 + lr, ks + lk, cols_pad, KP);
+      bh[j] = Bhi[bo];
+      if (SPLIT == 3) bl[j] = Blo[bo];
+    }
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
       const bf16x8 a2h = rot_neg(ah[i]);
       bf16x8 a2l;
       if (SPLIT == 3) a2l = rot_neg(al[i]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < TP; ++j) {
         if (SPLIT == 3) {
           zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], zr[i][j], 0, 0, 0);
           zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], zr[i][j], 0, 0, 0);
@@ -168,9 +186,9 @@ __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, co
   }
   // D layout: lane l, element e -> row 4 (l/16) + e, column l % 16
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < TR; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < TP; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = r0 + i * 16 + 4 * lk + e;
@@ -434,7 +452,7 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
 }
 
 #ifndef PFTCC_ZIFFT_F
-#define PFTCC_ZIFFT_F 8
+#define PFTCC_ZIFFT_F 4
 #endif
 #ifndef PFTCC_ZIFFT_NT
 #define PFTCC_ZIFFT_NT 256
@@ -568,7 +586,7 @@ __global__ void __launch_bounds__(64) k_zgemm8(const i32x8* __restrict__ A, cons
                                                const unsigned int* __restrict__ gmaxA,
                                                const unsigned int* __restrict__ gmaxB) {
   const int l = threadIdx.x;
-  const ZTile zt = ztile(cols_pad / 64, rows_pad / 64);
+  const ZTile zt = ztile(cols_pad / 64, rows_pad / 64, 64, 64);
   const int p0 = zt.p0, r0 = zt.r0, m = zt.m;
   const int KV = KP8 / 32;                                 // i32x8 vectors per row
   const size_t abase = ((size_t)m * rows_pad + r0) * KV;

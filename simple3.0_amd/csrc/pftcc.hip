@@ -877,15 +877,16 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 1, &ev))) return rc;
     const dim3 gg((unsigned)((size_t)(Pc / 64) * (ap.rows_pad / 64) * (n + 1)));
+    const dim3 gz((unsigned)((size_t)(Pc / (16 * ZGEMM_TP)) * (ap.rows_pad / (16 * ZGEMM_TR)) * (n + 1)));
     if (fp8)
       hipLaunchKernelGGL(k_zgemm8, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad, Pc,
                          KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB);
     else if (prec == PFTCC_PREC_BF16)
-      hipLaunchKernelGGL(k_zgemm<1>, gg, dim3(64), 0, h->stream, (const bf16x8*)ap.hi, (const bf16x8*)ap.lo,
-                         (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
+      hipLaunchKernelGGL((k_zgemm<1, ZGEMM_TR, ZGEMM_TP>), gz, dim3(64), 0, h->stream, (const bf16x8*)ap.hi,
+                         (const bf16x8*)ap.lo, (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
     else
-      hipLaunchKernelGGL(k_zgemm<3>, gg, dim3(64), 0, h->stream, (const bf16x8*)ap.hi, (const bf16x8*)ap.lo,
-                         (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
+      hipLaunchKernelGGL((k_zgemm<3, ZGEMM_TR, ZGEMM_TP>), gz, dim3(64), 0, h->stream, (const bf16x8*)ap.hi,
+                         (const bf16x8*)ap.lo, (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 2, &ev))) return rc;
     ZifftArgs za{};
