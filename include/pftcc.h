@@ -204,9 +204,10 @@ int pftcc_prep_matchfilt_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, co
                                const int32_t* irots);
 
 /* -------------------------------------------------------- precision (MFMA) form -- */
-/* Exhaustive search without shifts (the 3D-refinement use of gencorrs_1) computed as one dense
+/* Exhaustive search over references x rotations (gencorrs_1), or references x shifts x rotations
+ * (gencorrs_2: shifts[nshifts][2], the reference norm over all shells), computed as one dense
  * complex contraction over shells per angular frequency m on MFMA, followed by the inverse FFT
- * over m.  Exact for ring-constant |CTF| (ctf = no or no astigmatism); with astigmatism the
+ * over m; each (reference, shift) is one operand row.  Exact for ring-constant |CTF| (ctf = no or no astigmatism); with astigmatism the
  * particle's |CTF| is replaced by its ring mean per shell (the reference applies the particle's
  * angle-dependent |CTF| in the reference frame, :982, which no per-m contraction can express),
  * and ntop > 0 rescores the candidates exactly.
@@ -224,10 +225,11 @@ int pftcc_prep_matchfilt_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, co
  * two per operand maps its largest component to <= 448; meant for ntop > 0 (exact rescoring). */
 #define PFTCC_PREC_FP8 8
 int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
-                      int32_t prec, int32_t ntop, float* best_cc, int32_t* best_iref, int32_t* best_irot);
-/* cc_out[np][nr][nrots] of the same computation (for verification and small batches). */
+                      int32_t nshifts, const float* shifts, int32_t prec, int32_t ntop, float* best_cc,
+                      int32_t* best_iref, int32_t* best_ishift, int32_t* best_irot);
+/* cc_out[np][nr][max(nshifts,1)][nrots] of the same computation (verification, small batches). */
 int pftcc_gencorrs_mfma_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
-                              int32_t prec, float* cc_out);
+                              int32_t nshifts, const float* shifts, int32_t prec, float* cc_out);
 
 /* ------------------------------------------------------------------- instrumentation -- */
 /* No reference counterpart.  When enabled, each launch of the correlation kernels (the plain

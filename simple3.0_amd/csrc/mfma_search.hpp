@@ -199,12 +199,14 @@ __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, co
 
 struct ZifftArgs {
   int n, nrefs, np, rows_pad, cols_pad, kcorr;
+  int ns;                      // shifts per reference: row r belongs to reference r / ns
+  int knorm;                   // shells in the reference norm (kcorr unshifted, nk shifted)
   const float2* Z;             // zidx layout
-  const double* sr;            // [nrefs][kcorr] reference shell sums of squares
+  const double* sr;            // [nrefs / ns][knorm] reference shell sums of squares
   const double* cbar2;         // [np_chunk][nk] ring-mean |CTF|^2 of the chunk's particles (or null)
   int nk;
   const double* sp;            // [np_chunk] particle sums of squares (FP64)
-  const int* rlist;            // 0-based reference ids of the rows (null = row index)
+  const int* rlist;            // key id of each row (global reference * ns + shift; null = row index)
   const int* outpos;           // [np_chunk] output position of each chunk particle
   float* cc_out;               // [npos][nrefs][2n] or null
   unsigned long long* best;    // [npos] or null
@@ -238,8 +240,8 @@ __global__ void __launch_bounds__(256) k_zifft(ZifftArgs a) {
     double d = 0.0;
     if (p < a.np) {
       double sq = 0.0;
-      for (int kk = 0; kk < a.kcorr; ++kk)
-        sq += a.sr[(size_t)r * a.kcorr + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
+      for (int kk = 0; kk < a.knorm; ++kk)
+        sq += a.sr[(size_t)(r / a.ns) * a.knorm + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
       d = sqrt(sq * a.sp[p]);
     }
     den[tid] = d;
@@ -339,8 +341,8 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
     const int p = p0 + f;
     double s = 0.0;
     if (p < a.np)
-      for (int kk = u; kk < a.kcorr; kk += TPF)
-        s += a.sr[(size_t)r * a.kcorr + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
+      for (int kk = u; kk < a.knorm; kk += TPF)
+        s += a.sr[(size_t)(r / a.ns) * a.knorm + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
 #pragma unroll
     for (int o = TPF / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, TPF);
     if (u == 0) den[f] = p < a.np ? sqrt(s * a.sp[p]) : 1.0;
@@ -630,6 +632,18 @@ __global__ void __launch_bounds__(64) k_zgemm8(const i32x8* __restrict__ A, cons
       }
 }
 
+// Shifted reference rings out[r * ns + s][kk][j] = rings[r][kk][j] * shtab[s][kk][j].
+__global__ void k_shift_rings(const float2* __restrict__ rings, const float2* __restrict__ shtab, int nr, int ns,
+                              int nk, int n, float2* __restrict__ out) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t ring = (size_t)nk * n;
+  if (idx >= (size_t)nr * ns * ring) return;
+  const size_t e = idx % ring;
+  const size_t row = idx / ring;
+  const int r = (int)(row / ns), s = (int)(row % ns);
+  out[idx] = cmul(rings[(size_t)r * ring + e], shtab[(size_t)s * ring + e]);
+}
+
 // Gathers rows of a [N][nk][len] float2 array by a list into [count][nk][len].
 __global__ void k_gather_rows(const float2* __restrict__ src, const int* __restrict__ list, int count, size_t rowlen,
                               float2* __restrict__ dst) {
@@ -702,6 +716,11 @@ __global__ void __launch_bounds__(256) k_topt(const unsigned long long* __restri
 
 struct RescoreArgs {
   int n, nk, kcorr, T;
+  int ns;                      // shifts per reference in the keys (1 when unshifted)
+  int knorm;                   // shells of the reference norm
+  const double* shifts;        // [ns][2] or null (unshifted)
+  const double* argx;          // [nk][n] shift-phase arguments (argtransf)
+  const double* argy;
   const float2* refs_even;
   const float2* refs_odd;
   const float2* ptcls;         // [nptcls][nk][n]
@@ -713,14 +732,17 @@ struct RescoreArgs {
   float* best_cc;
   int* best_iref;
   int* best_irot;
+  int* best_ishift;            // or null
   double* best_cc64;           // or null
 };
 
-// Exact rescoring of the candidates in FP64: for each candidate (reference, rho) the rotations
-// rho - 1, rho, rho + 1 are evaluated as in gencorrs (unshifted): q = ref * |CTF| with the
-// particle's own angle-dependent |CTF|, cc = sum_k<kcorr sum_j Re q[k,j] conj(P[k,(j+rho) mod 2n])
-// / sqrt(sum |q|^2 * sum |p|^2).  One work-group of RW wavefronts per particle, each wavefront
-// taking candidates w, w + RW, ...; the best cc wins, ties to the lowest (reference, rotation).
+// Exact rescoring of the candidates in FP64: for each candidate (reference, shift, rho) the
+// rotations rho - 1, rho, rho + 1 are evaluated as in gencorrs: q = ref * |CTF| [* e^{i phi}] with
+// the particle's own angle-dependent |CTF| and phi = argx sx + argy sy, cc = sum_k<kcorr sum_j
+// Re q[k,j] conj(P[k,(j+rho) mod 2n]) / sqrt(sum_k<knorm |q|^2 * sum_k<kcorr |p|^2) (knorm = all
+// shells when shifted, gencorrs_cc_2).  One work-group of RW wavefronts per particle, each
+// wavefront taking candidates w, w + RW, ...; the best cc wins, ties to the lowest
+// (reference, shift, rotation).
 constexpr int RESCORE_MAX = 64;
 constexpr int RESCORE_WAVES = 16;
 __global__ void __launch_bounds__(64 * RESCORE_WAVES) k_rescore64(RescoreArgs a) {
@@ -743,16 +765,27 @@ __global__ void __launch_bounds__(64 * RESCORE_WAVES) k_rescore64(RescoreArgs a)
       continue;
     }
     const unsigned int lin = 0xFFFFFFFFu - (unsigned int)(key & 0xFFFFFFFFull);
-    const int iref = (int)(lin / (unsigned)n2), rho = (int)(lin % (unsigned)n2);
+    const unsigned int row = lin / (unsigned)n2;
+    const int iref = (int)(row / (unsigned)a.ns), ish = (int)(row % (unsigned)a.ns);
+    const int rho = (int)(lin % (unsigned)n2);
     const int rr[3] = {(rho + n2 - 1) % n2, rho, (rho + 1) % n2};
     const float2* ref = (a.iseven[ip] ? a.refs_even : a.refs_odd) + (size_t)iref * a.nk * n;
+    const double sx = a.shifts ? a.shifts[2 * ish] : 0.0, sy = a.shifts ? a.shifts[2 * ish + 1] : 0.0;
     double num0 = 0.0, num1 = 0.0, num2 = 0.0, sq = 0.0, sp = 0.0;
-    for (int e = lane; e < a.kcorr * n; e += 64) {
+    for (int e = lane; e < a.knorm * n; e += 64) {
       const int k = e / n, j = e % n;
       const float2 rv = ref[e];
       const double cv = ct ? (double)ct[e] : 1.0;
-      const double qx = rv.x * cv, qy = rv.y * cv;
+      double qx = rv.x * cv, qy = rv.y * cv;
+      if (a.shifts) {
+        double sn, cs;
+        sincos(a.argx[e] * sx + a.argy[e] * sy, &sn, &cs);
+        const double tx = qx * cs - qy * sn;
+        qy = qx * sn + qy * cs;
+        qx = tx;
+      }
       sq += qx * qx + qy * qy;
+      if (e >= a.kcorr * n) continue;
       const float2 pv = pt[e];
       sp += (double)pv.x * pv.x + (double)pv.y * pv.y;
       const float2* prow = pt + (size_t)k * n;
@@ -787,7 +820,7 @@ __global__ void __launch_bounds__(64 * RESCORE_WAVES) k_rescore64(RescoreArgs a)
       const double nums[3] = {num0, num1, num2};
       for (int t = 0; t < 3; ++t) {
         ccs[c * 3 + t] = nums[t] / d;
-        lins[c * 3 + t] = (unsigned int)iref * (unsigned int)n2 + (unsigned int)rr[t];
+        lins[c * 3 + t] = row * (unsigned int)n2 + (unsigned int)rr[t];
       }
     }
   }
@@ -804,10 +837,13 @@ __global__ void __launch_bounds__(64 * RESCORE_WAVES) k_rescore64(RescoreArgs a)
       a.best_cc[i] = __uint_as_float(0x7FC00000u);
       a.best_iref[i] = 0;
       a.best_irot[i] = 0;
+      if (a.best_ishift) a.best_ishift[i] = 0;
     } else {
+      const unsigned int row = bl / (unsigned)n2;
       a.best_cc[i] = (float)bc;
-      a.best_iref[i] = (int)(bl / (unsigned)n2) + 1;
+      a.best_iref[i] = (int)(row / (unsigned)a.ns) + 1;
       a.best_irot[i] = (int)(bl % (unsigned)n2) + 1;
+      if (a.best_ishift) a.best_ishift[i] = (int)(row % (unsigned)a.ns);
     }
     if (a.best_cc64) a.best_cc64[i] = bc;
   }

@@ -121,9 +121,11 @@ struct pftcc {
     bool valid = false;
     std::vector<int> refs;            // 0-based reference ids of the rows
     int rows_pad = 0, KP = 0, kcorr = 0;
+    std::vector<float> shifts;        // [ns][2]: rows are (reference, shift), reference-major
+    int knorm = 0;                    // shells in sr (kcorr unshifted, nk shifted)
     bool fp8 = false;                 // e4m3 operand (a8) instead of split BF16 (hi, lo)
     unsigned int *hi = nullptr, *lo = nullptr;
-    double* sr = nullptr;             // [nr][kcorr]
+    double* sr = nullptr;             // [nr][knorm]
     // FP8: per-shell mean squares of the reference spectra, shell scales a_k (device copies of
     // a_k and 1/a_k), the operand's largest component and the operand itself
     std::vector<double> msV;
@@ -743,21 +745,25 @@ int shell_ms(pftcc* h, const float2* spec, const int* d_list, int count, std::ve
 
 // Reference operand of eo class eo over the (0-based) reference list rl; cached.  Split BF16, or
 // FP8 e4m3 with shell scales balanced against the particles' mean squares msW.
-int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, bool fp8, const std::vector<double>* msW) {
+int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector<float>& shv, bool fp8,
+                 const std::vector<double>* msW) {
   pftcc::APack& ap = h->apack[eo];
-  if (ap.valid && ap.refs == rl && ap.kcorr == h->kcorr && ap.fp8 == fp8 &&
+  if (ap.valid && ap.refs == rl && ap.shifts == shv && ap.kcorr == h->kcorr && ap.fp8 == fp8 &&
       (!fp8 || fp8_shell_scales(ap.msV, *msW) == ap.ak))
     return PFTCC_OK;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   free_apack(ap);
-  const int nr = (int)rl.size();
+  const int nref = (int)rl.size();
+  const int ns = (int)shv.size() / 2, nse = ns > 0 ? ns : 1;
+  const int nr = nref * nse;                          // operand rows (reference, shift)
+  const int knorm = ns > 0 ? h->nk : h->kcorr;
   const int n = h->n, nk = h->nk;
   ap.rows_pad = round_up(nr, 64);
   ap.KP = round_up(2 * h->kcorr, 32);
   ap.KP8 = round_up(2 * h->kcorr, 128);
   const size_t words = fp8 ? (size_t)(n + 1) * ap.rows_pad * (ap.KP8 / 4) : (size_t)(n + 1) * ap.rows_pad * (ap.KP / 2);
   int rc;
-  if ((rc = dalloc(h, &ap.sr, (size_t)nr * h->kcorr))) return rc;
+  if ((rc = dalloc(h, &ap.sr, (size_t)nref * knorm))) return rc;
   if (fp8) {
     if ((rc = dalloc(h, &ap.a8, words)) || (rc = dalloc(h, &ap.d_ak, (size_t)h->kcorr)) ||
         (rc = dalloc(h, &ap.d_bk, (size_t)h->kcorr)) || (rc = dalloc(h, &ap.gmax, (size_t)1)))
@@ -766,18 +772,32 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, bool fp8, const s
     return rc;
   }
   int* d_list = nullptr;
-  float2 *rings = nullptr, *spec = nullptr;
-  if ((rc = dalloc(h, &d_list, (size_t)nr)) || (rc = dalloc(h, &rings, (size_t)nr * nk * n)) ||
+  float2 *rings = nullptr, *spec = nullptr, *srings = nullptr, *shtab = nullptr;
+  double* d_sh = nullptr;
+  if ((rc = dalloc(h, &d_list, (size_t)nref)) || (rc = dalloc(h, &rings, (size_t)nref * nk * n)) ||
       (rc = dalloc(h, &spec, (size_t)nr * nk * (n + 1))))
     return rc;
-  HIPCHK(h, hipMemcpyAsync(d_list, rl.data(), nr * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_list, rl.data(), nref * sizeof(int), hipMemcpyHostToDevice, h->stream));
   const size_t rowlen = (size_t)nk * n;
-  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((nr * rowlen + 255) / 256)), dim3(256), 0, h->stream,
-                     (const float2*)(eo ? h->d_refs_o : h->d_refs_e), (const int*)d_list, nr, rowlen, rings);
+  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((nref * rowlen + 255) / 256)), dim3(256), 0, h->stream,
+                     (const float2*)(eo ? h->d_refs_o : h->d_refs_e), (const int*)d_list, nref, rowlen, rings);
+  float2* rowrings = rings;
+  if (ns > 0) {
+    // q = ref x shmat, with the same FP32 shift table as the plain search (gen_shmat, :1123-1130)
+    std::vector<double> sh64(shv.begin(), shv.end());
+    if ((rc = dalloc(h, &d_sh, sh64.size())) || (rc = dalloc(h, &shtab, (size_t)ns * rowlen)) ||
+        (rc = dalloc(h, &srings, (size_t)nr * rowlen)))
+      return rc;
+    HIPCHK(h, hipMemcpyAsync(d_sh, sh64.data(), sh64.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((rc = shift_table(h, ns, d_sh, shtab))) return rc;
+    hipLaunchKernelGGL(k_shift_rings, dim3((unsigned)(((size_t)nr * rowlen + 255) / 256)), dim3(256), 0, h->stream,
+                       (const float2*)rings, (const float2*)shtab, nref, ns, nk, n, srings);
+    rowrings = srings;
+  }
   int F = 8;
   while (F > 1 && sizeof(float2) * ((size_t)2 * F * n + 2 * n) > kMaxLds) F /= 2;
   hipLaunchKernelGGL(k_memoize, dim3((nr * nk + F - 1) / F), dim3(256), sizeof(float2) * ((size_t)2 * F * n + 2 * n),
-                     h->stream, (const float2*)rings, 0, nr, nk, n, h->plan, (const float2*)h->d_tw2, spec, F,
+                     h->stream, (const float2*)rowrings, 0, nr, nk, n, h->plan, (const float2*)h->d_tw2, spec, F,
                      (const int*)nullptr);
   if (fp8) {
     if ((rc = shell_ms(h, spec, nullptr, nr, ap.msV))) return rc;
@@ -798,14 +818,19 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, bool fp8, const s
                        (const float2*)spec, (const float*)nullptr, nr, ap.rows_pad, nk, h->kcorr, n, ap.KP, ap.hi,
                        ap.lo);
   }
-  hipLaunchKernelGGL(k_ring_sq64, dim3((nr * h->kcorr + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)rings, nr,
-                     nk, n, h->kcorr, ap.sr);
+  hipLaunchKernelGGL(k_ring_sq64, dim3((nref * knorm + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)rings, nref,
+                     nk, n, knorm, ap.sr);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   dfree(d_list);
   dfree(rings);
   dfree(spec);
+  dfree(srings);
+  dfree(shtab);
+  dfree(d_sh);
   ap.refs = rl;
+  ap.shifts = shv;
+  ap.knorm = knorm;
   ap.kcorr = h->kcorr;
   ap.fp8 = fp8;
   ap.valid = true;
@@ -815,13 +840,17 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, bool fp8, const s
 // Runs the precision form for particles pl (local ids) that share eo class eo, against the cached
 // operand of refs rl.  outpos[i] = position of pl[i] in best/cc_out.
 int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<int>& outpos,
-               const std::vector<int>& rl, int prec, unsigned long long* d_best, float* d_cc,
-               unsigned long long* d_prbest, const std::vector<double>* msW) {
+               const std::vector<int>& rl, const std::vector<float>& shv, int prec, unsigned long long* d_best,
+               float* d_cc, unsigned long long* d_prbest, const std::vector<double>* msW) {
   int rc;
   const bool fp8 = prec == PFTCC_PREC_FP8;
-  if ((rc = ensure_apack(h, eo, rl, fp8, msW))) return rc;
+  if ((rc = ensure_apack(h, eo, rl, shv, fp8, msW))) return rc;
   const pftcc::APack& ap = h->apack[eo];
-  const int n = h->n, nk = h->nk, KP = fp8 ? ap.KP8 : ap.KP, nr = (int)rl.size();
+  const int nse = std::max(1, (int)shv.size() / 2);
+  const int n = h->n, nk = h->nk, KP = fp8 ? ap.KP8 : ap.KP, nr = (int)rl.size() * nse;
+  // key id of each operand row: global reference * nse + shift
+  std::vector<int> rowkey(nr);
+  for (int r = 0; r < nr; ++r) rowkey[r] = rl[r / nse] * nse + r % nse;
   const int total = (int)pl.size();
   // chunk of particles: keep Z (n+1) x rows_pad x Pc float2 within ~4 GiB
   int Pc = 256;
@@ -846,7 +875,7 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
   unsigned int* gmaxB = take<unsigned int>(h, 1);
   HIPCHK(h, hipMemcpyAsync(d_pl, pl.data(), total * 4, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(d_pos, outpos.data(), total * 4, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipMemcpyAsync(d_rl, rl.data(), nr * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_rl, rowkey.data(), nr * 4, hipMemcpyHostToDevice, h->stream));
   // FP64 particle sums of squares over the correlated shells, all particles of the group at once
   hipLaunchKernelGGL(k_ring_sq64_list, dim3((total + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)h->d_ptcls,
                      (const int*)d_pl, total, nk, n, h->kcorr, spall);
@@ -896,6 +925,8 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     za.rows_pad = ap.rows_pad;
     za.cols_pad = Pc;
     za.kcorr = h->kcorr;
+    za.ns = nse;
+    za.knorm = ap.knorm;
     za.Z = Z;
     za.sr = ap.sr;
     za.cbar2 = ctf ? cbar2 : nullptr;
@@ -913,6 +944,7 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
       if ((rc = timing_end(h, ev))) return rc;
       continue;
     }
+    if (nr > 65535) return fail(h, PFTCC_E_UNSUPPORTED, "pftsz %d: at most 65535 operand rows per call", n);
     int Fz = 8;
     while (Fz > 2 && zifft_lds(Fz, n) > kMaxLds) Fz /= 2;
     const size_t lds = zifft_lds(Fz, n);
@@ -929,8 +961,8 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
 }
 
 // Common driver: validates, splits particles by eo class and runs both groups.
-int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* irefs, int prec,
-             unsigned long long* d_best, float* d_cc, unsigned long long* d_prbest = nullptr,
+int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* irefs, const std::vector<float>& shv,
+             int prec, unsigned long long* d_best, float* d_cc, unsigned long long* d_prbest = nullptr,
              std::vector<int>* plocal = nullptr) {
   int rc;
   if (prec != PFTCC_PREC_BF16 && prec != PFTCC_PREC_SPLIT_BF16 && prec != PFTCC_PREC_FP8)
@@ -938,7 +970,9 @@ int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* ire
   if (h->euclid || h->match_filt || h->pssnr)
     return fail(h, PFTCC_E_UNSUPPORTED, "the MFMA form implements objfun cc without match filter");
   if (zifft_lds(2, h->n) > kMaxLds) return fail(h, PFTCC_E_UNSUPPORTED, "pftsz %d too large for the MFMA form", h->n);
-  if (nr > 65535) return fail(h, PFTCC_E_ARG, "the MFMA form takes at most 65535 references per call");
+  const size_t nse = std::max<size_t>(1, shv.size() / 2);
+  if ((size_t)h->nrefs * nse * h->nrots >= 0xFFFFFFFFull)
+    return fail(h, PFTCC_E_ARG, "too many references x shifts x rotations for the search keys");
   if ((rc = need_memo(h))) return rc;
   std::vector<int> rl(nr);
   for (int i = 0; i < nr; ++i) {
@@ -968,9 +1002,13 @@ int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* ire
     if (rc) return rc;
   }
   for (int eo = 0; eo < 2; ++eo)
-    if (!pl[eo].empty() && (rc = mfma_group(h, eo, pl[eo], pos[eo], rl, prec, d_best, d_cc, d_prbest, &msW)))
+    if (!pl[eo].empty() && (rc = mfma_group(h, eo, pl[eo], pos[eo], rl, shv, prec, d_best, d_cc, d_prbest, &msW)))
       return rc;
   return PFTCC_OK;
+}
+
+std::vector<float> shift_vector(int32_t ns, const float* shifts) {
+  return ns > 0 && shifts ? std::vector<float>(shifts, shifts + 2 * (size_t)ns) : std::vector<float>();
 }
 
 // One (iref, iptcl, shvec, irot) evaluation through k_rot8; returns f and grad[3].
@@ -1911,57 +1949,77 @@ int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr
   return search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, d_best_cc, d_best_iref, d_best_ishift, d_best_irot);
 }
 
-int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs, int32_t prec,
-                      int32_t ntop, float* best_cc, int32_t* best_iref, int32_t* best_irot) {
-  if (!h || !iptcls || np < 1 || nr < 1 || !best_cc || !best_iref || !best_irot) return PFTCC_E_ARG;
+int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs, int32_t ns,
+                      const float* shifts, int32_t prec, int32_t ntop, float* best_cc, int32_t* best_iref,
+                      int32_t* best_ishift, int32_t* best_irot) {
+  if (!h || !iptcls || np < 1 || nr < 1 || ns < 0 || (ns > 0 && !shifts) || !best_cc || !best_iref || !best_irot)
+    return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
   if (ntop < 0 || ntop > RESCORE_MAX) return fail(h, PFTCC_E_ARG, "ntop %d outside [0, %d]", ntop, RESCORE_MAX);
+  const std::vector<float> shv = shift_vector(ns, shifts);
+  const int nse = ns > 0 ? ns : 1;
+  const size_t nrows = (size_t)nr * nse;
   unsigned long long *d_best = nullptr, *d_prbest = nullptr, *d_cand = nullptr;
   float* d_cc = nullptr;
-  int *d_ir = nullptr, *d_rot = nullptr, *d_pl = nullptr;
+  int *d_ir = nullptr, *d_rot = nullptr, *d_ish = nullptr, *d_pl = nullptr;
+  double* d_sh = nullptr;
   int rc;
   if ((rc = dalloc(h, &d_best, (size_t)np)) || (rc = dalloc(h, &d_cc, (size_t)np)) || (rc = dalloc(h, &d_ir, (size_t)np)) ||
-      (rc = dalloc(h, &d_rot, (size_t)np)))
+      (rc = dalloc(h, &d_rot, (size_t)np)) || (rc = dalloc(h, &d_ish, (size_t)np)))
     return rc;
-  if (ntop > 0 && ((rc = dalloc(h, &d_prbest, (size_t)np * nr)) || (rc = dalloc(h, &d_cand, (size_t)np * ntop)) ||
-                   (rc = dalloc(h, &d_pl, (size_t)np))))
+  if (ntop > 0 && ((rc = dalloc(h, &d_prbest, (size_t)np * nrows)) || (rc = dalloc(h, &d_cand, (size_t)np * ntop)) ||
+                   (rc = dalloc(h, &d_pl, (size_t)np)) || (ns > 0 && (rc = dalloc(h, &d_sh, (size_t)2 * ns)))))
     return rc;
   hipLaunchKernelGGL(k_fill_u64, dim3((np + 255) / 256), dim3(256), 0, h->stream, d_best, np, 0ull);
   std::vector<int> plocal;
-  rc = mfma_run(h, np, iptcls, nr, irefs, prec, ntop > 0 ? nullptr : d_best, nullptr, d_prbest, &plocal);
-  if (rc == PFTCC_OK && ntop > 0) {
-    hipEvent_t ev;
-    if ((rc = timing_begin(h, 3, &ev))) return rc;
-    HIPCHK(h, hipMemcpyAsync(d_pl, plocal.data(), np * sizeof(int), hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_topt, dim3(np), dim3(256), 0, h->stream, (const unsigned long long*)d_prbest, np, nr, ntop,
-                       d_cand);
-    RescoreArgs ra{};
-    ra.n = h->n;
-    ra.nk = h->nk;
-    ra.kcorr = h->kcorr;
-    ra.T = ntop;
-    ra.refs_even = h->d_refs_e;
-    ra.refs_odd = h->d_refs_o;
-    ra.ptcls = h->d_ptcls;
-    ra.ctf = h->with_ctf ? h->d_ctf : nullptr;
-    ra.iseven = h->d_iseven;
-    ra.plist = d_pl;
-    ra.cand = d_cand;
-    ra.np = np;
-    ra.best_cc = d_cc;
-    ra.best_iref = d_ir;
-    ra.best_irot = d_rot;
-    hipLaunchKernelGGL(k_rescore64, dim3(np), dim3(64 * std::min(ntop, RESCORE_WAVES)), 0, h->stream, ra);
-    HIPCHK(h, hipGetLastError());
-    if ((rc = timing_end(h, ev))) return rc;
+  rc = mfma_run(h, np, iptcls, nr, irefs, shv, prec, ntop > 0 ? nullptr : d_best, nullptr, d_prbest, &plocal);
+  hipEvent_t ev = nullptr;
+  if (rc == PFTCC_OK && ntop > 0 && (rc = timing_begin(h, 3, &ev)) == PFTCC_OK) {
+    std::vector<double> sh64(shv.begin(), shv.end());
+    if (hipMemcpyAsync(d_pl, plocal.data(), np * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        (ns > 0 && hipMemcpyAsync(d_sh, sh64.data(), sh64.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  h->stream) != hipSuccess))
+      rc = fail(h, PFTCC_E_DEVICE, "search_mfma: upload failed");
+    if (rc == PFTCC_OK) {
+      hipLaunchKernelGGL(k_topt, dim3(np), dim3(256), 0, h->stream, (const unsigned long long*)d_prbest, np,
+                         (int)nrows, ntop, d_cand);
+      RescoreArgs ra{};
+      ra.n = h->n;
+      ra.nk = h->nk;
+      ra.kcorr = h->kcorr;
+      ra.T = ntop;
+      ra.ns = nse;
+      ra.knorm = ns > 0 ? h->nk : h->kcorr;
+      ra.shifts = ns > 0 ? d_sh : nullptr;
+      ra.argx = h->d_argx;
+      ra.argy = h->d_argy;
+      ra.refs_even = h->d_refs_e;
+      ra.refs_odd = h->d_refs_o;
+      ra.ptcls = h->d_ptcls;
+      ra.ctf = h->with_ctf ? h->d_ctf : nullptr;
+      ra.iseven = h->d_iseven;
+      ra.plist = d_pl;
+      ra.cand = d_cand;
+      ra.np = np;
+      ra.best_cc = d_cc;
+      ra.best_iref = d_ir;
+      ra.best_irot = d_rot;
+      ra.best_ishift = d_ish;
+      hipLaunchKernelGGL(k_rescore64, dim3(np), dim3(64 * std::min(ntop, RESCORE_WAVES)), 0, h->stream, ra);
+      if (hipGetLastError() != hipSuccess) rc = fail(h, PFTCC_E_DEVICE, "search_mfma: rescoring launch failed");
+    }
+    const int rc2 = timing_end(h, ev);
+    if (rc == PFTCC_OK) rc = rc2;
   } else if (rc == PFTCC_OK) {
-    hipLaunchKernelGGL(k_best_decode_rot, dim3((np + 255) / 256), dim3(256), 0, h->stream,
-                       (const unsigned long long*)d_best, np, h->nrots, d_cc, d_ir, d_rot);
+    hipLaunchKernelGGL(k_best_decode, dim3((np + 255) / 256), dim3(256), 0, h->stream,
+                       (const unsigned long long*)d_best, np, h->nrefs, nse, h->nrots, (const int*)nullptr, d_cc, d_ir,
+                       d_ish, d_rot);
   }
   if (rc == PFTCC_OK) {
     if (hipMemcpyAsync(best_cc, d_cc, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipMemcpyAsync(best_iref, d_ir, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipMemcpyAsync(best_irot, d_rot, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        (best_ishift && hipMemcpyAsync(best_ishift, d_ish, np * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess) ||
         hipStreamSynchronize(h->stream) != hipSuccess)
       rc = fail(h, PFTCC_E_DEVICE, "search_mfma: copy of the results failed");
   }
@@ -1970,21 +2028,24 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
   dfree(d_cc);
   dfree(d_ir);
   dfree(d_rot);
+  dfree(d_ish);
   dfree(d_prbest);
   dfree(d_cand);
   dfree(d_pl);
+  dfree(d_sh);
   return rc;
 }
 
 int pftcc_gencorrs_mfma_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs,
-                              int32_t prec, float* cc_out) {
-  if (!h || !iptcls || np < 1 || nr < 1 || !cc_out) return PFTCC_E_ARG;
+                              int32_t ns, const float* shifts, int32_t prec, float* cc_out) {
+  if (!h || !iptcls || np < 1 || nr < 1 || ns < 0 || (ns > 0 && !shifts) || !cc_out) return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
+  const std::vector<float> shv = shift_vector(ns, shifts);
   float* d_cc = nullptr;
   int rc;
-  const size_t ncc = (size_t)np * nr * h->nrots;
+  const size_t ncc = (size_t)np * nr * (ns > 0 ? ns : 1) * h->nrots;
   if ((rc = dalloc(h, &d_cc, ncc))) return rc;
-  rc = mfma_run(h, np, iptcls, nr, irefs, prec, nullptr, d_cc);
+  rc = mfma_run(h, np, iptcls, nr, irefs, shv, prec, nullptr, d_cc);
   if (rc == PFTCC_OK && (hipMemcpyAsync(cc_out, d_cc, ncc * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
                          hipStreamSynchronize(h->stream) != hipSuccess))
     rc = fail(h, PFTCC_E_DEVICE, "gencorrs_mfma_batch: copy failed");

@@ -339,15 +339,15 @@ interface
         type(c_ptr), value :: h
         integer(c_int32_t), value :: iptcl, iref, irot
     end function
-    integer(c_int) function c_search_mfma(h, np, iptcls, nr, irefs, prec, ntop, best_cc, best_iref, best_irot) &
-        &bind(C, name='pftcc_search_mfma')
+    integer(c_int) function c_search_mfma(h, np, iptcls, nr, irefs, ns, shifts, prec, ntop, best_cc, best_iref, &
+        &best_ishift, best_irot) bind(C, name='pftcc_search_mfma')
         import :: c_int, c_int32_t, c_ptr, c_float
         type(c_ptr), value :: h
-        integer(c_int32_t), value :: np, nr, prec, ntop
+        integer(c_int32_t), value :: np, nr, ns, prec, ntop
         integer(c_int32_t), intent(in) :: iptcls(*)
-        type(c_ptr), value :: irefs
+        type(c_ptr), value :: irefs, shifts
         real(c_float), intent(out) :: best_cc(*)
-        integer(c_int32_t), intent(out) :: best_iref(*), best_irot(*)
+        integer(c_int32_t), intent(out) :: best_iref(*), best_ishift(*), best_irot(*)
     end function
     integer(c_int) function c_grad_batch(h, npairs, irefs, iptcls, shvecs, irots, f, grad) bind(C, name='pftcc_grad_batch')
         import :: c_int, c_int32_t, c_ptr, c_double
@@ -774,17 +774,20 @@ contains
 
     ! precision form of search without shifts: MFMA contraction per angular frequency, the ntop
     ! best references per particle rescored exactly in FP64 (ntop = 0: no rescoring)
-    subroutine search_mfma( self, iptcls, best_cc, best_iref, best_irot, prec, ntop, irefs )
+    subroutine search_mfma( self, iptcls, best_cc, best_iref, best_irot, prec, ntop, irefs, shifts, best_ishift )
         class(polarft_corrcalc),    intent(inout) :: self
         integer,                    intent(in)    :: iptcls(:)
         real(sp),                   intent(out)   :: best_cc(size(iptcls))
         integer,                    intent(out)   :: best_iref(size(iptcls)), best_irot(size(iptcls))
         integer,                    intent(in)    :: prec, ntop
         integer,  optional, target, intent(in)    :: irefs(:)
-        integer(c_int32_t) :: ip(size(iptcls)), bir(size(iptcls)), brt(size(iptcls))
+        real(sp), optional, target, intent(in)    :: shifts(:,:)      ! (2, nshifts)
+        integer,  optional,         intent(out)   :: best_ishift(size(iptcls))
+        integer(c_int32_t) :: ip(size(iptcls)), bir(size(iptcls)), bis(size(iptcls)), brt(size(iptcls))
         integer(c_int32_t), allocatable, target :: ir(:)
-        type(c_ptr) :: pir
-        integer(c_int32_t) :: nr
+        real(c_float),      allocatable, target :: sh(:,:)
+        type(c_ptr) :: pir, psh
+        integer(c_int32_t) :: nr, ns
         ip  = iptcls
         pir = c_null_ptr
         nr  = self%nrefs
@@ -793,10 +796,18 @@ contains
             pir = c_loc(ir)
             nr  = size(irefs)
         endif
-        call chk(self, c_search_mfma(self%h, int(size(iptcls),c_int32_t), ip, nr, pir, int(prec,c_int32_t), &
-            &int(ntop,c_int32_t), best_cc, bir, brt), 'search_mfma')
+        psh = c_null_ptr
+        ns  = 0
+        if( present(shifts) )then
+            allocate(sh(2, size(shifts,2)), source=real(shifts, c_float))
+            psh = c_loc(sh)
+            ns  = size(shifts,2)
+        endif
+        call chk(self, c_search_mfma(self%h, int(size(iptcls),c_int32_t), ip, nr, pir, ns, psh, int(prec,c_int32_t), &
+            &int(ntop,c_int32_t), best_cc, bir, bis, brt), 'search_mfma')
         best_iref = bir
         best_irot = brt
+        if( present(best_ishift) ) best_ishift = bis + 1
     end subroutine search_mfma
 
     subroutine grad_batch( self, irefs, iptcls, shvecs, irots, f, grad )

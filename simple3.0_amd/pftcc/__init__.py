@@ -109,8 +109,8 @@ SIGNATURES = [
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     ("pftcc_prep_matchfilt", _I, [_P, _I, _I, _I]),
     ("pftcc_prep_matchfilt_batch", _I, [_P, _I, _PI, _PI, _PI]),
-    ("pftcc_search_mfma", _I, [_P, _I, _PI, _I, _PI, _I, _I, _PF, _PI, _PI]),
-    ("pftcc_gencorrs_mfma_batch", _I, [_P, _I, _PI, _I, _PI, _I, _PF]),
+    ("pftcc_search_mfma", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _I, _I, _PF, _PI, _PI, _PI]),
+    ("pftcc_gencorrs_mfma_batch", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _I, _PF]),
     ("pftcc_get_timing_channel", _I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
     ("pftcc_set_timing", _I, [_P, _I]),
     ("pftcc_get_timing", _I, [_P, _PD, C.POINTER(C.c_int64)]),
@@ -526,28 +526,37 @@ class PolarFTCorrCalc:
         self._chk(self._lib.pftcc_prep_matchfilt_batch(self._h, p.size, _ip(p), _ip(r), _ip(o)),
                   "prep_matchfilt_batch")
 
-    def search_mfma(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16, ntop=0):
-        """Precision form of search() without shifts: (best_cc, best_iref, best_irot).
+    def search_mfma(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16, ntop=0, shifts=None):
+        """Precision form of search(): (best_cc, best_iref, best_irot), plus best_ishift (0-based)
+        as a fourth value when shifts [ns, 2] are given.
 
-        ntop > 0 rescores the ntop best references per particle exactly in FP64."""
+        ntop > 0 rescores the ntop best (reference, shift) rows per particle exactly in FP64."""
         p = np.ascontiguousarray(iptcls, np.int32)
         r = None if irefs is None else np.ascontiguousarray(irefs, np.int32)
         nr = self.nrefs if r is None else r.size
+        sh = None if shifts is None else np.ascontiguousarray(shifts, np.float32).reshape(-1, 2)
+        ns = 0 if sh is None else sh.shape[0]
         cc = np.zeros(p.size, np.float32)
         ir = np.zeros(p.size, np.int32)
+        ish = np.zeros(p.size, np.int32)
         rot = np.zeros(p.size, np.int32)
-        self._chk(self._lib.pftcc_search_mfma(self._h, p.size, _ip(p), nr, None if r is None else _ip(r), prec,
-                                              ntop, _fp(cc), _ip(ir), _ip(rot)), "search_mfma")
-        return cc, ir, rot
+        self._chk(self._lib.pftcc_search_mfma(self._h, p.size, _ip(p), nr, None if r is None else _ip(r), ns,
+                                              None if sh is None else _fp(sh), prec, ntop, _fp(cc), _ip(ir),
+                                              _ip(ish), _ip(rot)), "search_mfma")
+        return (cc, ir, ish, rot) if sh is not None else (cc, ir, rot)
 
-    def gencorrs_mfma_batch(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16):
-        """cc[np, nr, nrots] of the precision form."""
+    def gencorrs_mfma_batch(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16, shifts=None):
+        """cc[np, nr, nrots] (or [np, nr, ns, nrots] with shifts) of the precision form."""
         p = np.ascontiguousarray(iptcls, np.int32)
         r = None if irefs is None else np.ascontiguousarray(irefs, np.int32)
         nr = self.nrefs if r is None else r.size
-        out = np.zeros((p.size, nr, self.nrots), np.float32)
-        self._chk(self._lib.pftcc_gencorrs_mfma_batch(self._h, p.size, _ip(p), nr, None if r is None else _ip(r),
-                                                      prec, _fp(out)), "gencorrs_mfma_batch")
+        sh = None if shifts is None else np.ascontiguousarray(shifts, np.float32).reshape(-1, 2)
+        ns = 0 if sh is None else sh.shape[0]
+        shape = (p.size, nr, ns, self.nrots) if sh is not None else (p.size, nr, self.nrots)
+        out = np.zeros(shape, np.float32)
+        self._chk(self._lib.pftcc_gencorrs_mfma_batch(self._h, p.size, _ip(p), nr, None if r is None else _ip(r), ns,
+                                                      None if sh is None else _fp(sh), prec, _fp(out)),
+                  "gencorrs_mfma_batch")
         return out
 
     def get_timing_channel(self, ch):

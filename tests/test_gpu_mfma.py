@@ -210,3 +210,44 @@ def test_fp8_coarse_pass_error_and_rescoring():
         assert (ir[i], rot[i]) == (r + 1, t + 1)
         assert abs(cc[i] - exact[i][r, t]) <= 1e-7
     lib.kill()
+
+
+def test_mfma_with_shifts_matches_oracle_and_rescoring():
+    """Shifted precision form (gencorrs_2): each (reference, shift) is one operand row and the
+    reference norm runs over all shells.  Ring-constant |CTF|: within the split-BF16 tolerance of
+    the oracle; with astigmatism and rescoring: the oracle's exact (ref, shift, rot) and cc."""
+    say("mfma with shifts")
+    shifts = np.array([[0.0, 0.0], [1.5, -0.5], [-2.0, 1.0]], np.float32)
+    pf, cps = make_case(seed=49, n=64, kfrom=2, kto=20, kstop=18, ctf=True, astig=0.0, nrefs=4, nptcls=5, ldim=64,
+                        eo_mixed=True)
+    lib = to_lib(pf, cps)
+    got = lib.gencorrs_mfma_batch(np.arange(1, 6), shifts=shifts)
+    assert got.shape == (5, 4, 3, 128)
+    for i in range(5):
+        ref = np.stack([np.stack([pf.gencorrs(r, i, tuple(sh), method="fft") for sh in shifts]) for r in range(4)])
+        e = peak_rel_err(got[i], ref)
+        assert e < MFMA_TOL, (i, e)
+    lib.kill()
+    # astigmatic CTF + planted shifted winners: rescoring returns the exact argmax
+    pf, cps = make_case(seed=50, n=64, kfrom=2, kto=20, ctf=True, astig=0.3, nrefs=6, nptcls=6, ldim=64, eo_mixed=True)
+    rng = np.random.default_rng(51)
+    for i in range(6):
+        r = int(rng.integers(0, 6))
+        s = shifts[int(rng.integers(0, 3))]
+        rho = int(rng.integers(0, 2 * pf.pftsz))
+        src = pf.refs_even[r] if pf.iseven[i] else pf.refs_odd[r]
+        sh = O.shmat_closed(pf.grid, tuple(s.astype(np.float64)))
+        full = O.full_ring(src.astype(np.complex128) * sh)
+        rolled = np.roll(full, rho, axis=-1)[:, :pf.pftsz]
+        noise = 0.5 * (rng.standard_normal(rolled.shape) + 1j * rng.standard_normal(rolled.shape))
+        pf.ptcls[i] = (rolled + noise * np.abs(rolled).mean()).astype(np.complex64)
+    lib = to_lib(pf, cps)
+    cc, ir, ish, rot = lib.search_mfma(np.arange(1, 7), shifts=shifts, ntop=4)
+    pcc, pir, pish, prot = lib.search(np.arange(1, 7), None, shifts)
+    for i in range(6):
+        ref = np.stack([np.stack([pf.gencorrs(r, i, tuple(sh)) for sh in shifts]) for r in range(6)])
+        r_, s_, t_ = np.unravel_index(int(np.argmax(ref)), ref.shape)
+        assert (ir[i], ish[i], rot[i]) == (r_ + 1, s_, t_ + 1)
+        assert abs(cc[i] - ref[r_, s_, t_]) <= 1e-6
+        assert (pir[i], pish[i], prot[i]) == (ir[i], ish[i], rot[i])
+    lib.kill()
