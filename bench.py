@@ -204,6 +204,43 @@ def gradient_form(steps, nptcls=50000, nrefs=500, fd_sample=256, h=1e-3):
     }
 
 
+def precision_same_workload(lib, w, shifts, plain_value, nptcls, sample=1000, ntop=16):
+    """The split-BF16 MFMA form (with the 169-shift grid as extra operand rows and top-4 exact
+    FP64 rescoring) on the main workload's references and particles, next to the plain form."""
+    from pftcc import PFTCC_PREC_SPLIT_BF16
+    from pftcc import synth
+
+    S = synth.synth_lib()
+    nptcls = min(nptcls, w.nptcls)
+    ids = np.arange(1, nptcls + 1, dtype=np.int32)
+    nrots = 2 * w.pftsz
+    lib.search_mfma(ids[:64], prec=PFTCC_PREC_SPLIT_BF16, ntop=ntop, shifts=shifts)   # warm-up
+    # rebuild the reference operand inside the timed region (as for every new reference set)
+    refs_e = [lib.get_ref_pft(r + 1, True) for r in range(w.nrefs)]
+    lib.set_ref_pfts(1, np.stack(refs_e), True)
+    S.synth_device_sync()
+    lib.set_timing(True)
+    t0 = time.perf_counter()
+    cc, ir, ish, rot = lib.search_mfma(ids, prec=PFTCC_PREC_SPLIT_BF16, ntop=ntop, shifts=shifts)
+    S.synth_device_sync()
+    wall = time.perf_counter() - t0
+    ch = {name: lib.get_timing_channel(c) for c, name in ((1, "zgemm"), (2, "zifft"), (3, "pack_rescore"))}
+    lib.set_timing(False)
+    value = nptcls * w.nrefs * len(shifts) * nrots / wall
+    pcc, pir, pish, prot = lib.search(ids[:sample], None, shifts)
+    same = (pir == ir[:sample]) & (pish == ish[:sample]) & (prot == rot[:sample])
+    return {
+        "workload": f"split-BF16 MFMA exhaustive search with the +-3 px shift grid as operand rows, top-{ntop} "
+                    "(reference, shift) rows rescored in FP64 with the exact per-pixel |CTF| and shift phases; "
+                    "same references, particles and shifts as the plain line",
+        "value": value, "unit": "evals/s", "ptcls": nptcls, "refs": w.nrefs, "shifts": len(shifts),
+        "wall_s": wall, "speedup_vs_plain_fp32": value / plain_value,
+        "assignment_agreement_vs_plain_fp32": float(np.mean(same)), "sample_ptcls": sample,
+        "max_abs_best_cc_diff_vs_plain_fp32": float(np.max(np.abs(pcc - cc[:sample]))),
+        "kernel_ms": {k: v[0] for k, v in ch.items()},
+    }
+
+
 def cpu_baseline(pftcc, w, shifts, nthreads, target_s=12.0):
     """Times the reference CPU algorithm (C++/OpenMP port, oracle/pftcc_cpu.cpp) on a bounded
     sample of the same workload and checks the GPU's assignments on that sample."""
@@ -258,6 +295,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the MFMA precision-form line")
     ap.add_argument("--precision-steps", type=int, default=2)
+    ap.add_argument("--mfma-ptcls", type=int, default=10000,
+                    help="particles of the precision-form run on the main workload")
     ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-kernel line")
     ap.add_argument("--legs", default="plain,precision,gradient",
                     help="comma list of plain, precision, gradient (profiling runs select one)")
@@ -376,6 +415,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline(lib, w, shifts, nthreads)
         except Exception as e:  # the GPU figure stands on its own
             out["cpu_baseline"] = {"error": repr(e)}
+    if not args.no_precision and "precision" in legs:
+        out["precision_form_same_workload"] = precision_same_workload(lib, w, shifts, value, args.mfma_ptcls)
     lib.kill()
     if not args.no_precision and "precision" in legs:
         out["precision_form"] = precision_form(args.precision_steps)
