@@ -197,6 +197,95 @@ __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, co
       }
 }
 
+// A-stationary variant: one wavefront holds the reference operand of a 16 TR-row slab at one m
+// (all NSTEP k' steps) in registers and sweeps every 64-column particle tile of the chunk, so the
+// large operand is read once per launch and only the particle operand (prefetched one step ahead)
+// is loaded in the inner loop.  The XCD-aware order keeps the slabs of one m on one XCD (the
+// particle operand of that m stays in its L2).
+template <int SPLIT, int TR, int NSTEP>
+__global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
+                                                 const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
+                                                 int rows_pad, int cols_pad, int n, float2* __restrict__ Z) {
+  constexpr int TP = 4;
+  constexpr int KP = NSTEP * 32;
+  const int l = threadIdx.x;
+  const int gr = rows_pad / (16 * TR);
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int r0 = (t % gr) * 16 * TR, m = t / gr;
+  const int lr = l % 16, lk = l / 16;
+  bf16x8 ah[NSTEP][TR], al[NSTEP][TR];
+#pragma unroll
+  for (int st = 0; st < NSTEP; ++st)
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
+      const size_t ao = split_vec(m, r0 + i * 16 + lr, 4 * st + lk, rows_pad, KP);
+      ah[st][i] = Ahi[ao];
+      if (SPLIT == 3) al[st][i] = Alo[ao];
+    }
+  bf16x8 bh[TP], bl[TP], nh[TP], nl[TP];
+  auto load_b = [&](int p0, int st, bf16x8 (&h)[TP], bf16x8 (&lo)[TP]) {
+#pragma unroll
+    for (int j = 0; j < TP; ++j) {
+      const size_t bo = split_vec(m, p0 + j * 16 + lr, 4 * st + lk, cols_pad, KP);
+      h[j] = Bhi[bo];
+      if (SPLIT == 3) lo[j] = Blo[bo];
+    }
+  };
+  const int ntile = cols_pad / 64;
+  load_b(0, 0, bh, bl);
+  for (int tile = 0; tile < ntile; ++tile) {
+    const int p0 = tile * 64;
+    f32x4 zr[TR][TP], zi[TR][TP];
+#pragma unroll
+    for (int i = 0; i < TR; ++i)
+#pragma unroll
+      for (int j = 0; j < TP; ++j) {
+        zr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        zi[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+    for (int st = 0; st < NSTEP; ++st) {
+      // prefetch the next step (or the next tile's first step)
+      const bool more = st + 1 < NSTEP || tile + 1 < ntile;
+      if (more) load_b(st + 1 < NSTEP ? p0 : p0 + 64, st + 1 < NSTEP ? st + 1 : 0, nh, nl);
+#pragma unroll
+      for (int i = 0; i < TR; ++i) {
+        const bf16x8 a2h = rot_neg(ah[st][i]);
+        bf16x8 a2l;
+        if (SPLIT == 3) a2l = rot_neg(al[st][i]);
+#pragma unroll
+        for (int j = 0; j < TP; ++j) {
+          if (SPLIT == 3) {
+            zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[st][i], bh[j], zr[i][j], 0, 0, 0);
+            zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[st][i], bl[j], zr[i][j], 0, 0, 0);
+            zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l, bh[j], zi[i][j], 0, 0, 0);
+            zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bl[j], zi[i][j], 0, 0, 0);
+          }
+          zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[st][i], bh[j], zr[i][j], 0, 0, 0);
+          zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bh[j], zi[i][j], 0, 0, 0);
+        }
+      }
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < TP; ++j) {
+          bh[j] = nh[j];
+          if (SPLIT == 3) bl[j] = nl[j];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < TR; ++i)
+#pragma unroll
+      for (int j = 0; j < TP; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = r0 + i * 16 + 4 * lk + e;
+          const int p = p0 + j * 16 + lr;
+          Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e], zi[i][j][e]);
+        }
+  }
+}
+
 struct ZifftArgs {
   int n, nrefs, np, rows_pad, cols_pad, kcorr;
   int ns;                      // shifts per reference: row r belongs to reference r / ns

@@ -686,6 +686,29 @@ int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, cons
 // ------------------------------------------------------------------ precision form ---
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+#ifndef PFTCC_ZGEMM_AS
+#define PFTCC_ZGEMM_AS 1       // A-stationary contraction kernel (0: one tile per wavefront)
+#endif
+#ifndef PFTCC_ZGEMM_AS_TR
+#define PFTCC_ZGEMM_AS_TR 2
+#endif
+// A-stationary contraction for KP = 32 NSTEP; false if NSTEP has no instantiation.
+template <int SPLIT>
+bool zgemm_as_go(int KP, int rows_pad, int cols_pad, int n, hipStream_t s, const unsigned int* ahi,
+                 const unsigned int* alo, const unsigned int* bhi, const unsigned int* blo, float2* Z) {
+  constexpr int TR = PFTCC_ZGEMM_AS_TR;
+  const dim3 grid((unsigned)((size_t)(rows_pad / (16 * TR)) * (n + 1)));
+#define X(NS)                                                                                         \
+  if (KP == 32 * NS) {                                                                                \
+    hipLaunchKernelGGL((k_zgemm_as<SPLIT, TR, NS>), grid, dim3(64), 0, s, (const bf16x8*)ahi,          \
+                       (const bf16x8*)alo, (const bf16x8*)bhi, (const bf16x8*)blo, rows_pad, cols_pad, n, Z); \
+    return true;                                                                                      \
+  }
+  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+#undef X
+  return false;
+}
+
 // Compile-time inverse stage for the fused ring lengths; false if n has none.
 bool zifft_ct_go(int n, int cnt, int nr, hipStream_t s, const ZifftArgs& a) {
 #define X(N)                                                                                          \
@@ -910,7 +933,11 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     if (fp8)
       hipLaunchKernelGGL(k_zgemm8, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad, Pc,
                          KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB);
-    else if (prec == PFTCC_PREC_BF16)
+    else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_BF16 &&
+             zgemm_as_go<1>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z)) {
+    } else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_SPLIT_BF16 &&
+               zgemm_as_go<3>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z)) {
+    } else if (prec == PFTCC_PREC_BF16)
       hipLaunchKernelGGL((k_zgemm<1, ZGEMM_TR, ZGEMM_TP>), gz, dim3(64), 0, h->stream, (const bf16x8*)ap.hi,
                          (const bf16x8*)ap.lo, (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
     else
