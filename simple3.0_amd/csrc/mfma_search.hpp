@@ -498,8 +498,10 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   for (int f = wave; f < F; f += NW) {
     const int p = p0 + f;
     const bool live = p < a.np;
-    const double sc = live ? inv4n / den[f] : 0.0;
-    unsigned long long bk = 0ull;
+    // FP64 normalisation factor, applied as one FP32 multiply per value
+    const float sc = live ? (float)(inv4n / den[f]) : 0.f;
+    float bv = -INFINITY;      // this lane's best value and its rotation index (ties: lowest)
+    int bi = -1;
 #pragma unroll
     for (int j0 = 0; j0 < NsL; j0 += 64) {
       const int j = j0 + lane;
@@ -514,21 +516,26 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int jj = j + t * NsL;                   // output index: rotations 2 jj, 2 jj + 1
-        const float x0 = (float)(v[t].x * sc), x1 = (float)(-v[t].y * sc);
+        const float x0 = v[t].x * sc, x1 = -v[t].y * sc;
         if (a.cc_out) {
           float2* oo = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * N));
           oo[jj] = make_float2(x0, x1);
         }
-        if (want_key) {
-          const unsigned int lin0 = (unsigned int)iref * (unsigned int)(2 * N) + 2u * jj;
-          const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
-          const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
-          bk = k0 > bk ? k0 : bk;
-          bk = k1 > bk ? k1 : bk;
+        if (x0 > bv || (x0 == bv && 2 * jj < bi)) {
+          bv = x0;
+          bi = 2 * jj;
+        }
+        if (x1 > bv || (x1 == bv && 2 * jj + 1 < bi)) {
+          bv = x1;
+          bi = 2 * jj + 1;
         }
       }
     }
     if (want_key && live) {
+      unsigned long long bk = 0ull;
+      if (bi >= 0)
+        bk = ((unsigned long long)ordered_bits(bv) << 32) |
+             (0xFFFFFFFFu - ((unsigned int)iref * (unsigned int)(2 * N) + (unsigned int)bi));
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long w = __shfl_xor(bk, o);
@@ -551,10 +558,11 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
 
 template <int N>
 struct ZifftV2 {
-  static constexpr int NT = PFTCC_ZIFFT_NT;
   static constexpr int F0 = PFTCC_ZIFFT_F;
   static constexpr int F = (sizeof(float2) * (F0 * N + 2 * N) <= 65536) ? F0
                          : (sizeof(float2) * (F0 / 2 * N + 2 * N) <= 65536) ? F0 / 2 : F0 / 4;
+  // at most one wavefront per transform
+  static constexpr int NT = PFTCC_ZIFFT_NT < 64 * F ? PFTCC_ZIFFT_NT : 64 * F;
   static constexpr size_t lds = sizeof(float2) * ((size_t)F * N + 2 * N);
 };
 
