@@ -106,7 +106,7 @@ def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
     zg_ms, zg_n = ch["zgemm"]
     zi_ms, zi_n = ch["zifft"]
     pairs = nptcls * nrefs
-    zg_tflops = mfma_flops_per_pair(pftsz, kcorr) * pairs / (zg_ms * 1e-3) / 1e12
+    zg_tflops = mfma_flops_per_pair(pftsz, kcorr) * pairs / (zg_ms / steps * 1e-3) / 1e12
     zbytes = pairs * (pftsz + 1) * 8.0
     return {
         "workload": "split-BF16 MFMA exhaustive search (no shifts), 3D-refinement shape, top-4 references "
@@ -122,11 +122,11 @@ def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
         "unrescored": {"assignment_agreement_vs_plain_fp32": float(np.mean(same_raw)),
                        "max_abs_best_cc_diff_vs_plain_fp32": float(np.max(np.abs(pcc - raw_cc)))},
         "kernels": {
-            "k_zgemm<3>": {"ms_per_step": zg_ms / steps, "launches_per_step": zg_n / steps,
+            "k_zgemm_as<3,2,5>": {"ms_per_step": zg_ms / steps, "launches_per_step": zg_n / steps,
                            "achieved": zg_tflops, "unit": "TFLOP/s (algorithmic complex contraction)",
                            "peak": BF16_PEAK_TFLOPS, "frac": zg_tflops / BF16_PEAK_TFLOPS,
                            "mfma_issued_frac": 3.0 * zg_tflops / BF16_PEAK_TFLOPS},
-            "k_zifft_ct<400>": {"ms_per_step": zi_ms / steps, "launches_per_step": zi_n / steps,
+            "k_zifft_v2<400,4,256>": {"ms_per_step": zi_ms / steps, "launches_per_step": zi_n / steps,
                                 "achieved": zbytes / (zi_ms / steps * 1e-3) / 1e9,
                                 "unit": "GB/s (spectra read)", "peak": HBM_PEAK_GBS,
                                 "frac": zbytes / (zi_ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS},
