@@ -203,6 +203,26 @@ int pftcc_prep_matchfilt(pftcc_t* h, int32_t iptcl, int32_t iref, int32_t irot);
 int pftcc_prep_matchfilt_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, const int32_t* irefs,
                                const int32_t* irots);
 
+/* ----------------------------------------------------------------- polarizer (input side) -- */
+/* init_polarizer (simple_polarizer.f90:41-92): Kaiser-Bessel gridding tables for the handle's
+ * polar grid and box (window half-width KBWINSZ = 1.5, beta 7.4, 3 x 3 components per polar
+ * point, weights divided by their sum unless griddev != 0, params_glob%griddev == 'yes').  The
+ * oversampling alpha does not enter these weights; it only shapes the instrument function that the
+ * caller divides real-space images by before their FFT (div_by_instrfun).  Requires
+ * kfromto(2) <= boxmatch/2. */
+int pftcc_init_polarizer(pftcc_t* h, int32_t griddev);
+/* polarize (simple_polarizer.f90:120-151) for count images at once.  cmats: the images' Fourier
+ * transforms as the image object holds them after fwd_ft (simple_image.f90:5773-5782), i.e.
+ * Fortran cmat(boxmatch/2+1, boxmatch) of complex float per image, images consecutive.
+ * img_inds: distinct 1-based particle ids (isptcl != 0; set_ptcl_pft, sqsums refreshed) or
+ * reference ids (isptcl = 0, into the even or odd set by iseven; set_ref_pft).  mask: NULL, or the
+ * reference's band mask(:) with nmask >= kfromto(2) entries; shell k is zeroed when mask[k-1] == 0.
+ * The _dev form reads the transforms from device memory. */
+int pftcc_polarize(pftcc_t* h, int32_t count, const int32_t* img_inds, const float* cmats, int32_t isptcl,
+                   int32_t iseven, const int8_t* mask, int32_t nmask);
+int pftcc_polarize_dev(pftcc_t* h, int32_t count, const int32_t* img_inds, const void* d_cmats, int32_t isptcl,
+                       int32_t iseven, const int8_t* mask, int32_t nmask);
+
 /* -------------------------------------------------------- precision (MFMA) form -- */
 /* Exhaustive search over references x rotations (gencorrs_1), or references x shifts x rotations
  * (gencorrs_2: shifts[nshifts][2], the reference norm over all shells), computed as one dense

@@ -22,6 +22,7 @@
 #include "corr_fused.hpp"
 #include "mfma_search.hpp"
 #include "matchfilt.hpp"
+#include "polarizer.hpp"
 
 using namespace pfk;
 
@@ -113,6 +114,10 @@ struct pftcc {
   float *d_refinv_f = nullptr, *d_ofac_f = nullptr, *d_shellw = nullptr;
   double *d_refinv_d = nullptr, *d_ofac_d = nullptr, *d_sigma2w = nullptr, *d_refpw = nullptr;
   int* d_optrow = nullptr;
+  // polarizer tables (init_polarizer): weights [9][nk n] and logical addresses [6][nk n]
+  float* d_pz_w = nullptr;
+  int* d_pz_hk = nullptr;
+  bool pz_ready = false;
   bool memo_valid = false, dptcls_valid = false, refnorm_valid = false, have_sigma2 = false;
   bool ctf_ready = false;
   bool use_fused = true;            // PFTCC_GENERIC=1 forces the runtime-size kernel
@@ -1228,6 +1233,8 @@ int pftcc_kill(pftcc_t* h) {
   dfree(h->d_ofac_d);
   dfree(h->d_refpw);
   dfree(h->d_optlp);
+  dfree(h->d_pz_w);
+  dfree(h->d_pz_hk);
   for (auto& ap : h->apack) free_apack(ap);
   if (h->d_scratch) (void)hipFree(h->d_scratch);
   delete h;
@@ -1314,6 +1321,144 @@ int pftcc_set_ptcl_pfts(pftcc_t* h, int32_t first, int32_t count, const float* p
 int pftcc_set_ptcl_pfts_dev(pftcc_t* h, int32_t first, int32_t count, const void* d) {
   if (!h) return PFTCC_E_ARG;
   return set_ptcls(h, first, count, d, hipMemcpyDeviceToDevice);
+}
+
+// init_polarizer (simple_polarizer.f90:41-92) for the handle's polar grid and box.
+int pftcc_init_polarizer(pftcc_t* h, int32_t griddev) {
+  if (!h) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  const int ldim = h->ldim, nk = h->nk, n = h->n, np = nk * n;
+  if (h->kto > ldim / 2)
+    return fail(h, PFTCC_E_ARG, "kfromto(2) = %d beyond the Nyquist index %d of box %d; init_polarizer", h->kto,
+                ldim / 2, ldim);
+  int rc;
+  if (!h->d_pz_w) {
+    if ((rc = dalloc(h, &h->d_pz_w, (size_t)PZ_WLEN * np))) return rc;
+    if ((rc = dalloc(h, &h->d_pz_hk, (size_t)2 * PZ_WDIM * np))) return rc;
+  }
+  const size_t grid = (size_t)nk * h->nrots;
+  if ((rc = scratch_reserve(h, 2 * aligned(grid * 4)))) return rc;
+  float* d_px = take<float>(h, grid);
+  float* d_py = take<float>(h, grid);
+  HIPCHK(h, hipMemcpyAsync(d_px, h->polar_x.data(), grid * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_py, h->polar_y.data(), grid * 4, hipMemcpyHostToDevice, h->stream));
+  // kbinterpol(KBWINSZ = 1.5, alpha): W = 3, beta = 7.4 (half-width <= 1.5); alpha does not enter
+  // the weights (simple_kbinterpol.f90:92-109, simple_defs.f90:131)
+  KbWin kb;
+  kb.whalf = 1.5f;
+  kb.W = 2.0f * kb.whalf;
+  kb.beta = 7.4f;
+  kb.twooW = 2.0f / kb.W;
+  kb.oneoW = 1.0f / kb.W;
+  // loop_lims(3) of the even square image (simple_ftiter.f90:89-114)
+  const int hlo = -ldim / 2, hhi = ldim / 2, klo = -ldim / 2, khi = ldim / 2 - 1;
+  hipLaunchKernelGGL(k_polar_tables, dim3((np + 255) / 256), dim3(256), 0, h->stream, (const float*)d_px,
+                     (const float*)d_py, nk, n, h->nrots, kb, hlo, hhi, klo, khi, griddev ? 1 : 0, h->d_pz_w,
+                     h->d_pz_hk);
+  HIPCHK(h, hipGetLastError());
+  // every address the polarize kernel will read must lie in the half-plane transform
+  std::vector<int> hk((size_t)2 * PZ_WDIM * np);
+  HIPCHK(h, hipMemcpyAsync(hk.data(), h->d_pz_hk, hk.size() * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t c = 0; c < (size_t)2 * PZ_WDIM; ++c) {
+    const int lo = c < PZ_WDIM ? hlo : klo, hi = c < PZ_WDIM ? hhi : khi;
+    for (int p = 0; p < np; ++p) {
+      const int v = hk[c * np + p];
+      if (v < lo || v > hi) return fail(h, PFTCC_E_DEVICE, "polarizer address %d outside [%d, %d]", v, lo, hi);
+    }
+  }
+  h->pz_ready = true;
+  return PFTCC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// polarize (simple_polarizer.f90:120-151) of count transforms into particle or reference slots.
+int polarize_impl(pftcc* h, int count, const int32_t* img_inds, const void* cmats, bool dev, int isptcl,
+                  int iseven, const int8_t* mask, int nmask) {
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!h->pz_ready) return fail(h, PFTCC_E_STATE, "init_polarizer must come before polarize");
+  if (count < 1 || !img_inds || !cmats) return fail(h, PFTCC_E_ARG, "polarize: no images");
+  if (mask && nmask < h->kto)
+    return fail(h, PFTCC_E_ARG, "polarize: mask has %d entries, shells up to %d need indexing", nmask, h->kto);
+  std::vector<int> slot(count);
+  std::vector<char> seen(isptcl ? (size_t)h->nptcls : (size_t)h->nrefs, 0);
+  for (int b = 0; b < count; ++b) {
+    int rc, s;
+    if (isptcl) {
+      if ((rc = local_index(h, img_inds[b], &s))) return rc;
+    } else {
+      if ((rc = check_ref(h, img_inds[b]))) return rc;
+      s = img_inds[b] - 1;
+    }
+    if (seen[s]) return fail(h, PFTCC_E_ARG, "polarize: image index %d given twice", img_inds[b]);
+    seen[s] = 1;
+    slot[b] = s;
+  }
+  const int ldim = h->ldim, nk = h->nk, n = h->n, np = nk * n;
+  const size_t img_bytes = (size_t)ldim * (ldim / 2 + 1) * sizeof(float2);
+  const size_t budget = (size_t)256 << 20;    // host transforms are staged through this much HBM
+  const int chunk = dev ? count : (int)std::max<size_t>(1, std::min<size_t>(count, budget / img_bytes));
+  int rc;
+  if ((rc = scratch_reserve(h, aligned((size_t)count * 4) + aligned(nk) + (dev ? 0 : aligned(chunk * img_bytes)))))
+    return rc;
+  int* d_slot = take<int>(h, count);
+  int8_t* d_mask = take<int8_t>(h, nk);
+  float2* d_img = dev ? nullptr : take<float2>(h, chunk * img_bytes / sizeof(float2));
+  HIPCHK(h, hipMemcpyAsync(d_slot, slot.data(), (size_t)count * 4, hipMemcpyHostToDevice, h->stream));
+  if (mask) {
+    std::vector<int8_t> mk(nk);
+    for (int kk = 0; kk < nk; ++kk) mk[kk] = mask[h->kfrom + kk - 1] ? 1 : 0;   // mask(k), 1-based k
+    HIPCHK(h, hipMemcpyAsync(d_mask, mk.data(), nk, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  float2* out = isptcl ? h->d_ptcls : (iseven ? h->d_refs_e : h->d_refs_o);
+  for (int b0 = 0; b0 < count; b0 += chunk) {
+    const int nb = std::min(chunk, count - b0);
+    const float2* src;
+    if (dev) {
+      src = static_cast<const float2*>(cmats) + (size_t)b0 * (img_bytes / sizeof(float2));
+    } else {
+      HIPCHK(h, hipMemcpyAsync(d_img, static_cast<const char*>(cmats) + (size_t)b0 * img_bytes, nb * img_bytes,
+                               hipMemcpyHostToDevice, h->stream));
+      src = d_img;
+    }
+    hipLaunchKernelGGL(k_polarize, dim3((np + 255) / 256, nb), dim3(256), 0, h->stream, src, ldim,
+                       (const float*)h->d_pz_w, (const int*)h->d_pz_hk, nk, n, (const int*)d_slot + b0,
+                       mask ? (const int8_t*)d_mask : (const int8_t*)nullptr, out);
+    HIPCHK(h, hipGetLastError());
+  }
+  if (isptcl) {
+    // set_ptcl_pft refreshes sqsums_ptcls (:438-446)
+    hipLaunchKernelGGL(k_sqsums, dim3(count), dim3(256), 0, h->stream, (const float2*)h->d_ptcls, nk, n, h->kcorr,
+                       h->d_sqsums, h->d_sqsums_k, 0, count, (const int*)d_slot);
+    HIPCHK(h, hipGetLastError());
+    h->memo_valid = false;
+    h->dptcls_valid = false;
+  } else {
+    h->refnorm_valid = false;
+    h->apack[0].valid = h->apack[1].valid = false;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pftcc_polarize(pftcc_t* h, int32_t count, const int32_t* img_inds, const float* cmats, int32_t isptcl,
+                   int32_t iseven, const int8_t* mask, int32_t nmask) {
+  if (!h) return PFTCC_E_ARG;
+  return polarize_impl(h, count, img_inds, cmats, false, isptcl, iseven, mask, nmask);
+}
+
+int pftcc_polarize_dev(pftcc_t* h, int32_t count, const int32_t* img_inds, const void* d_cmats, int32_t isptcl,
+                       int32_t iseven, const int8_t* mask, int32_t nmask) {
+  if (!h) return PFTCC_E_ARG;
+  return polarize_impl(h, count, img_inds, d_cmats, true, isptcl, iseven, mask, nmask);
 }
 
 int pftcc_set_ref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, float re, float im, int32_t iseven) {

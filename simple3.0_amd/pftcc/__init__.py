@@ -109,6 +109,9 @@ SIGNATURES = [
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     ("pftcc_prep_matchfilt", _I, [_P, _I, _I, _I]),
     ("pftcc_prep_matchfilt_batch", _I, [_P, _I, _PI, _PI, _PI]),
+    ("pftcc_init_polarizer", _I, [_P, _I]),
+    ("pftcc_polarize", _I, [_P, _I, _PI, _PF, _I, _I, C.POINTER(C.c_int8), _I]),
+    ("pftcc_polarize_dev", _I, [_P, _I, _PI, _P, _I, _I, C.POINTER(C.c_int8), _I]),
     ("pftcc_search_mfma", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _I, _I, _PF, _PI, _PI, _PI]),
     ("pftcc_gencorrs_mfma_batch", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _I, _PF]),
     ("pftcc_get_timing_channel", _I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
@@ -525,6 +528,32 @@ class PolarFTCorrCalc:
         o = np.ascontiguousarray(irots, np.int32)
         self._chk(self._lib.pftcc_prep_matchfilt_batch(self._h, p.size, _ip(p), _ip(r), _ip(o)),
                   "prep_matchfilt_batch")
+
+    def init_polarizer(self, griddev=False):
+        """init_polarizer (simple_polarizer.f90:41-92): Kaiser-Bessel gridding tables."""
+        self._chk(self._lib.pftcc_init_polarizer(self._h, 1 if griddev else 0), "init_polarizer")
+
+    def polarize(self, img_inds, cmats, isptcl, iseven=True, mask=None):
+        """polarize (simple_polarizer.f90:120-151) of a batch: cmats [count][box][box/2+1] complex64
+        (the image's cmat, Fortran order), img_inds 1-based particle or reference ids, mask the
+        reference's 1-based band mask (index k-1 for shell k) or None."""
+        ids = np.ascontiguousarray(np.atleast_1d(img_inds), np.int32)
+        c = np.ascontiguousarray(cmats, np.complex64)
+        if c.ndim == 2:
+            c = c[None]
+        m = None if mask is None else np.ascontiguousarray(mask, np.int8)
+        self._chk(self._lib.pftcc_polarize(self._h, ids.size, _ip(ids), c.ctypes.data_as(_PF), 1 if isptcl else 0,
+                                           1 if iseven else 0,
+                                           None if m is None else m.ctypes.data_as(C.POINTER(C.c_int8)),
+                                           0 if m is None else m.size), "polarize")
+
+    def polarize_dev(self, img_inds, d_cmats, isptcl, iseven=True, mask=None):
+        ids = np.ascontiguousarray(np.atleast_1d(img_inds), np.int32)
+        m = None if mask is None else np.ascontiguousarray(mask, np.int8)
+        self._chk(self._lib.pftcc_polarize_dev(self._h, ids.size, _ip(ids), C.c_void_p(d_cmats), 1 if isptcl else 0,
+                                               1 if iseven else 0,
+                                               None if m is None else m.ctypes.data_as(C.POINTER(C.c_int8)),
+                                               0 if m is None else m.size), "polarize_dev")
 
     def search_mfma(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16, ntop=0, shifts=None):
         """Precision form of search(): (best_cc, best_iref, best_irot), plus best_ishift (0-based)
