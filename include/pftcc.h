@@ -223,6 +223,22 @@ int pftcc_polarize(pftcc_t* h, int32_t count, const int32_t* img_inds, const flo
 int pftcc_polarize_dev(pftcc_t* h, int32_t count, const int32_t* img_inds, const void* d_cmats, int32_t isptcl,
                        int32_t iseven, const int8_t* mask, int32_t nmask);
 
+/* ------------------------------------------------------------ gradient shift search -- */
+/* pftcc_shsrch_grad%minimize (simple_pftcc_shsrch_grad.f90:254-338) for n (reference,
+ * particle) pairs at once (set_indices(irefs[i], iptcls[i]) then minimize(irots[i])).  Each pair
+ * runs L-BFGS-B over the shift within lims = Fortran lims(2,2) = {xlo, ylo, xhi, yhi}
+ * (grad_shsrch_new :46-82: m 15, factr 1e7, pgtol 1e-5, line search ftol 0.1 gtol 1e-3, maxits
+ * on the objective evaluations) on gencorr_grad_for_rot_8.  opt_angle != 0: the in-plane rotation
+ * starts at the best of gencorrs at shift 0 and is re-chosen (gencorrs + maxloc) at every new
+ * iterate, for up to 5 shift-search / rotation cycles; opt_angle = 0: irots[i] is the fixed
+ * rotation.  Out: irots[i] = the rotation, or 0 when nothing better than the start was found
+ * (then cxy is not meaningful); cxy[i] = {cc, shift rotated to the reference frame}
+ * (matmul(shift, rotmat2d(get_rot(irot)))).  No direct counterpart entry point: it replaces the
+ * reference's per-particle calls in strategy2D_srch%inpl_srch / strategy3D_srch%inpl_srch
+ * (simple_strategy2D_srch.f90:120-145, simple_strategy3D_srch.f90:190-205). */
+int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const int32_t* iptcls, const float* lims,
+                            int32_t maxits, int32_t opt_angle, int32_t* irots, float* cxy);
+
 /* -------------------------------------------------------- precision (MFMA) form -- */
 /* Exhaustive search over references x rotations (gencorrs_1), or references x shifts x rotations
  * (gencorrs_2: shifts[nshifts][2], the reference norm over all shells), computed as one dense

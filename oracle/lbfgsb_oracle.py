@@ -44,9 +44,9 @@ XTRAPL, XTRAPU = 1.1, 4.0
 # ---------------------------------------------------------------------------------------------
 # Moré-Thuente line search (dcsrch / dcstep, simple_opt_lbfgsb.f90:2532-2830)
 # ---------------------------------------------------------------------------------------------
-def dcstep(stx, fx, dx, sty, fy, dy, stp, fp, dp, brackt, stpmin, stpmax):
+def dcstep(stx, fx, dx, sty, fy, dy, stp, fp, dp, brackt, stpmin, stpmax, zero_tol=0.0):
     """One safeguarded step of the Moré-Thuente interval update; returns the updated
-    (stx, fx, dx, sty, fy, dy, stp, brackt)."""
+    (stx, fx, dx, sty, fy, dy, stp, brackt).  zero_tol > 0: the reference's is_zero(gamma)."""
     sgnd = dp * (dx / abs(dx))
     if fp > fx:
         # higher function value: the minimum is bracketed; cubic vs quadratic step
@@ -84,7 +84,8 @@ def dcstep(stx, fx, dx, sty, fy, dy, stp, fp, dp, brackt, stpmin, stpmax):
         p = (gamma - dp) + theta
         q = (gamma + (dx - dp)) + gamma
         r = p / q
-        if r < 0.0 and gamma != 0.0:
+        gamma_nonzero = abs(gamma) >= zero_tol if zero_tol > 0 else gamma != 0.0
+        if r < 0.0 and gamma_nonzero:
             stpc = stp + r * (stx - stp)
         elif stp > stx:
             stpc = stpmax
@@ -175,14 +176,15 @@ class Dcsrch:
             # modified function psi(t) = f(t) - f(0) - ftol t f'(0) in the first stage
             gt = self.gtest
             r = dcstep(self.stx, self.fx - self.stx * gt, self.gx - gt, self.sty, self.fy - self.sty * gt,
-                       self.gy - gt, stp, f - stp * gt, g - gt, self.brackt, self.stmin, self.stmax)
+                       self.gy - gt, stp, f - stp * gt, g - gt, self.brackt, self.stmin, self.stmax,
+                       self.eq_tol)
             stx, fxm, gxm, sty, fym, gym, stp, self.brackt = r
             self.stx, self.sty = stx, sty
             self.fx, self.fy = fxm + stx * gt, fym + sty * gt
             self.gx, self.gy = gxm + gt, gym + gt
         else:
             r = dcstep(self.stx, self.fx, self.gx, self.sty, self.fy, self.gy, stp, f, g, self.brackt,
-                       self.stmin, self.stmax)
+                       self.stmin, self.stmax, self.eq_tol)
             self.stx, self.fx, self.gx, self.sty, self.fy, self.gy, stp, self.brackt = r
         if self.brackt:
             if abs(self.sty - self.stx) >= P66 * self.width1:

@@ -23,6 +23,7 @@
 #include "mfma_search.hpp"
 #include "matchfilt.hpp"
 #include "polarizer.hpp"
+#include "shsrch.hpp"
 
 using namespace pfk;
 
@@ -411,6 +412,9 @@ size_t corr_lds_bytes(int n, int F) {
   return sizeof(float2) * ((size_t)2 * F * n + (size_t)F * (n + 1) + (n + 1) + 2 * n) +
          sizeof(float) * ((size_t)n + (size_t)F * n);
 }
+
+// item mode of k_corr_generic (F = 1): two FP64 phase arrays of n after the FP32 buffers
+size_t corr_lds_bytes_items(int n) { return corr_lds_bytes(n, 1) + 16 + 2 * sizeof(double2) * (size_t)n; }
 
 int pick_F(int n, int npairs) {
   int F = 8;
@@ -2438,6 +2442,211 @@ int pftcc_calc_roinv_corrmat(pftcc_t* h, float* corrmat) {
       corrmat[(size_t)p * np + r] = m;
       corrmat[(size_t)r * np + p] = m;
     }
+  return PFTCC_OK;
+}
+
+}  // extern "C"
+
+// ===================================================================== gradient shift search ===
+namespace {
+
+// gencorrs at one FP32 shift per item (the in-plane callback of the shift search): best key per
+// item (cc, rotation) from the item mode of k_corr_generic.
+int launch_corr_items(pftcc* h, int n, const int* d_ip, const int* d_ref, const float* d_sh,
+                      unsigned long long* d_best) {
+  CorrArgs a{};
+  a.n = h->n;
+  a.nk = h->nk;
+  a.kcorr = h->kcorr;
+  a.mode = h->euclid ? 1 : 0;
+  a.knorm = h->euclid ? h->kcorr : h->nk;      // the shifted gencorrs (:1531)
+  a.nrefs = h->nrefs;
+  a.plan = h->plan;
+  a.refs_even = h->d_refs_e;
+  a.refs_odd = h->d_refs_o;
+  a.memo = h->d_memo;
+  a.ctf = h->with_ctf ? h->d_ctf : nullptr;
+  a.sqsums = h->d_sqsums;
+  a.sqsums_k = h->d_sqsums_k;
+  a.iseven = h->d_iseven;
+  a.shtab = nullptr;
+  a.ns = 1;
+  const bool norm = (h->match_filt || h->pssnr) && !h->euclid;
+  a.refinv = norm ? h->d_refinv_f : nullptr;
+  a.ofac = norm ? h->d_ofac_f : nullptr;
+  a.optrow = (norm && h->clsfrcs) ? h->d_optrow : nullptr;
+  a.shellw = h->euclid ? h->d_shellw : nullptr;
+  a.plist = d_ip;
+  a.np = n;
+  a.rlist = d_ref;
+  a.nr = 1;
+  a.cc_out = nullptr;
+  a.best = d_best;
+  a.tw2g = h->d_tw2;
+  a.F = 1;
+  a.ntiles = 1;
+  a.item_shift = d_sh;
+  a.argx = h->d_argx;
+  a.argy = h->d_argy;
+  a.onex = h->d_onex;
+  a.oney = h->d_oney;
+  const size_t lds = corr_lds_bytes_items(h->n);
+  if (lds > kMaxLds) return fail(h, PFTCC_E_UNSUPPORTED, "pftsz %d too large for the shift-search callback", h->n);
+  hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, h->stream, d_best, n, 0ull);
+  hipEvent_t e1 = nullptr;
+  int rc;
+  if ((rc = timing_begin(h, 0, &e1))) return rc;
+  hipLaunchKernelGGL(k_corr_generic, dim3((unsigned)n), dim3(256), lds, h->stream, a);
+  HIPCHK(h, hipGetLastError());
+  return timing_end(h, e1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const int32_t* iptcls, const float* lims,
+                            int32_t maxits, int32_t opt_angle, int32_t* irots, float* cxy) {
+  if (!h || n < 0 || (n > 0 && (!irefs || !iptcls || !lims || !irots || !cxy))) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (n == 0) return PFTCC_OK;
+  int rc;
+  if ((rc = need_memo(h))) return rc;
+  if (!(lims[0] <= lims[2]) || !(lims[1] <= lims[3]))
+    return fail(h, PFTCC_E_ARG, "shift limits [%g, %g] x [%g, %g] are empty", lims[0], lims[2], lims[1], lims[3]);
+  if (maxits < 0) return fail(h, PFTCC_E_ARG, "maxits %d < 0", maxits);
+  if (opt_angle && corr_lds_bytes_items(h->n) > kMaxLds)
+    return fail(h, PFTCC_E_UNSUPPORTED, "pftsz %d too large for the shift-search callback", h->n);
+  std::vector<int> ref0(n), ip0(n), rot(n);
+  for (int i = 0; i < n; ++i) {
+    if ((rc = check_ref(h, irefs[i]))) return rc;
+    if ((rc = local_index(h, iptcls[i], &ip0[i]))) return rc;
+    ref0[i] = irefs[i] - 1;
+    rot[i] = irots[i];
+    if (!opt_angle && (rot[i] < 1 || rot[i] > h->nrots))
+      return fail(h, PFTCC_E_ARG, "irot %d outside [1, %d]", rot[i], h->nrots);
+  }
+  // device state: items, two request-list sets, counters
+  ShsItem* d_items = nullptr;
+  int *d_i = nullptr, *d_cnt = nullptr, *d_rot = nullptr;
+  double* d_d = nullptr;
+  float* d_f = nullptr;
+  unsigned long long* d_key = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(h->stream);
+    dfree(d_items);
+    dfree(d_i);
+    dfree(d_cnt);
+    dfree(d_rot);
+    dfree(d_d);
+    dfree(d_f);
+    dfree(d_key);
+  };
+  const size_t N = (size_t)n;
+  // ints per set: fg item, ref, ip, rho; corr item, ref, ip  (7 N); plus 2 x N init ids
+  if ((rc = dalloc(h, &d_items, N)) || (rc = dalloc(h, &d_i, 2 * 7 * N + 2 * N)) || (rc = dalloc(h, &d_cnt, 4)) ||
+      (rc = dalloc(h, &d_rot, N)) || (rc = dalloc(h, &d_d, 2 * 2 * N + 4 * N)) ||
+      (rc = dalloc(h, &d_f, 2 * 2 * N + 3 * N + (size_t)h->nrots)) || (rc = dalloc(h, &d_key, N))) {
+    cleanup();
+    return rc;
+  }
+  float* d_out = d_f + 4 * N;            // cxy [n][3]
+  float* d_angtab = d_out + 3 * N;
+  int* set_i[2] = {d_i, d_i + 7 * N};
+  int* d_ref_init = d_i + 14 * N;
+  int* d_ip_init = d_ref_init + N;
+  double* set_sh[2] = {d_d, d_d + 2 * N};
+  double* d_fv = d_d + 4 * N;            // f [n]
+  double* d_gr = d_fv + N;               // grad [n][3]
+  float* set_csh[2] = {d_f, d_f + 2 * N};
+  auto fail_clean = [&](int code) {
+    cleanup();
+    return code;
+  };
+  if (hipMemcpyAsync(d_ref_init, ref0.data(), N * 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(d_ip_init, ip0.data(), N * 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(d_rot, rot.data(), N * 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(d_angtab, h->angtab_deg.data(), (size_t)h->nrots * 4, hipMemcpyHostToDevice, h->stream) !=
+          hipSuccess)
+    return fail_clean(fail(h, PFTCC_E_DEVICE, "shsrch_grad_batch: upload failed"));
+  ShsArgs a{};
+  a.items = d_items;
+  a.nitems = n;
+  a.angtab_deg = d_angtab;
+  a.max_evals = 5;                        // pftcc_shsrch_grad%max_evals
+  a.nrots = h->nrots;
+  a.lo0 = lims[0];
+  a.lo1 = lims[1];
+  a.hi0 = lims[2];
+  a.hi1 = lims[3];
+  a.maxits = maxits;
+  a.opt_angle = opt_angle ? 1 : 0;
+  auto bind_out = [&](ShsArgs& s, int set) {
+    int* b = set_i[set];
+    s.counters_out = d_cnt + 2 * set;
+    s.fg_item_out = b;
+    s.fg_ref_out = b + N;
+    s.fg_ip_out = b + 2 * N;
+    s.fg_rho_out = b + 3 * N;
+    s.corr_item_out = b + 4 * N;
+    s.corr_ref_out = b + 5 * N;
+    s.corr_ip_out = b + 6 * N;
+    s.fg_sh_out = set_sh[set];
+    s.corr_sh_out = set_csh[set];
+  };
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, h->stream, (unsigned*)d_cnt, 4, 0u);
+  {
+    ShsArgs s = a;
+    s.init = 1;
+    s.irefs = d_ref_init;
+    s.ips = d_ip_init;
+    s.irots = d_rot;
+    bind_out(s, 0);
+    hipLaunchKernelGGL(k_shs_step, dim3((n + 63) / 64), dim3(64), 0, h->stream, s);
+    if (hipGetLastError() != hipSuccess) return fail_clean(fail(h, PFTCC_E_DEVICE, "k_shs_step launch failed"));
+  }
+  int cur = 0;
+  // every item issues at most max_evals x (maxits + line-search trials + callbacks) requests
+  const long max_rounds = 5L * (maxits + 2 * LB_MAXLS + 8) * 4 + 64;
+  long rounds = 0;
+  for (;; ++rounds) {
+    int cnt[2];
+    if (hipMemcpyAsync(cnt, d_cnt + 2 * cur, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      return fail_clean(fail(h, PFTCC_E_DEVICE, "shsrch_grad_batch: device error"));
+    const int nfg = cnt[0], ncorr = cnt[1];
+    if (nfg == 0 && ncorr == 0) break;
+    if (rounds > max_rounds)
+      return fail_clean(fail(h, PFTCC_E_DEVICE, "shsrch_grad_batch: no convergence after %ld rounds", rounds));
+    int* b = set_i[cur];
+    if (nfg > 0 && (rc = launch_rot8(h, nfg, b + N, b + 2 * N, set_sh[cur], b + 3 * N, d_fv, d_gr, false)))
+      return fail_clean(rc);
+    if (ncorr > 0 && (rc = launch_corr_items(h, ncorr, b + 6 * N, b + 5 * N, set_csh[cur], d_key)))
+      return fail_clean(rc);
+    const int nxt = 1 - cur;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, h->stream, (unsigned*)(d_cnt + 2 * nxt), 2, 0u);
+    ShsArgs s = a;
+    s.init = 0;
+    s.nfg_in = nfg;
+    s.ncorr_in = ncorr;
+    s.fg_item_in = b;
+    s.fg_f_in = d_fv;
+    s.fg_grad_in = d_gr;
+    s.corr_item_in = b + 4 * N;
+    s.corr_key_in = d_key;
+    bind_out(s, nxt);
+    const int tot = nfg + ncorr;
+    hipLaunchKernelGGL(k_shs_step, dim3((tot + 63) / 64), dim3(64), 0, h->stream, s);
+    if (hipGetLastError() != hipSuccess) return fail_clean(fail(h, PFTCC_E_DEVICE, "k_shs_step launch failed"));
+    cur = nxt;
+  }
+  hipLaunchKernelGGL(k_shs_out, dim3((n + 255) / 256), dim3(256), 0, h->stream, (const ShsItem*)d_items, n, d_out,
+                     d_rot);
+  if (hipMemcpyAsync(cxy, d_out, N * 12, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipMemcpyAsync(irots, d_rot, N * 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return fail_clean(fail(h, PFTCC_E_DEVICE, "shsrch_grad_batch: download failed"));
+  cleanup();
   return PFTCC_OK;
 }
 

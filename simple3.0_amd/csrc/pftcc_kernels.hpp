@@ -236,6 +236,15 @@ struct CorrArgs {
   const float2* tw2g;          // exp(-i pi a/n), a in [0, 2n)
   int F;
   int ntiles;
+  // item mode (item_shift != null): block b correlates particle plist[b] with reference rlist[b]
+  // at its own FP32 shift item_shift[b] (F = 1, one pair per block); the shift phases follow
+  // gen_shmat_8 (:1098-1120) from argtransf of the first shell (argx/argy row 0) and of shell
+  // one (onex/oney), in FP64, rounded to FP32 as gen_shmat does.
+  const float* item_shift;     // [np][2]
+  const double* argx;          // [nk][n]
+  const double* argy;
+  const double* onex;          // [n]
+  const double* oney;
 };
 
 __device__ __forceinline__ unsigned int ordered_bits(float v) {
@@ -266,6 +275,23 @@ __global__ void __launch_bounds__(256) k_corr_generic(CorrArgs a) {
   const bool even = a.iseven[ip] != 0;
   const float2* refs = even ? a.refs_even : a.refs_odd;
   const int npairs = a.nr * a.ns;
+  const bool items = a.item_shift != nullptr;
+  // item mode: running shift phase of each angle (current shell) and its shell-to-shell step
+  // (after offacc, 16-byte aligned; present only in item mode)
+  float* const fbase = reinterpret_cast<float*>(smem);
+  const size_t shoff = ((size_t)(offacc + (size_t)F * half - fbase) + 3) & ~(size_t)3;
+  double2* shph = reinterpret_cast<double2*>(fbase + shoff);
+  double2* shst = shph + n;
+  if (items) {
+    const double sx = (double)a.item_shift[2 * pi], sy = (double)a.item_shift[2 * pi + 1];
+    for (int j = tid; j < n; j += nt) {
+      double sn, cs;
+      sincos(a.argx[j] * sx + a.argy[j] * sy, &sn, &cs);
+      shph[j] = make_double2(cs, sn);
+      sincos(a.onex[j] * sx + a.oney[j] * sy, &sn, &cs);
+      shst[j] = make_double2(cs, sn);
+    }
+  }
 
   for (int i = tid; i < 2 * n; i += nt) tw2[i] = a.tw2g[i];
   for (int i = tid; i < F * (n + 1); i += nt) acc[i] = make_float2(0.f, 0.f);
@@ -288,13 +314,21 @@ __global__ void __launch_bounds__(256) k_corr_generic(CorrArgs a) {
       float2 q0 = make_float2(0.f, 0.f), q1 = q0;
       if (P < npairs) {
         const int ir = P / a.ns, is = P - (P / a.ns) * a.ns;
-        const int iref = a.rlist ? a.rlist[ir] : ir;
+        const int iref = items ? a.rlist[pi] : (a.rlist ? a.rlist[ir] : ir);
         const float2* r = refs + ((size_t)iref * a.nk + kk) * n;
         const int t = 2 * u;
         float2 r0 = r[t], r1 = r[t + 1];
         q0 = cscale(r0, ctfk[t]);
         q1 = cscale(r1, ctfk[t + 1]);
-        if (a.shtab) {
+        if (items) {
+          // this thread owns angles t, t+1 for every shell: use, then advance, their phases
+          const double2 p0 = shph[t], p1 = shph[t + 1];
+          q0 = cmul(q0, make_float2((float)p0.x, (float)p0.y));
+          q1 = cmul(q1, make_float2((float)p1.x, (float)p1.y));
+          const double2 s0 = shst[t], s1 = shst[t + 1];
+          shph[t] = make_double2(p0.x * s0.x - p0.y * s0.y, p0.x * s0.y + p0.y * s0.x);
+          shph[t + 1] = make_double2(p1.x * s1.x - p1.y * s1.y, p1.x * s1.y + p1.y * s1.x);
+        } else if (a.shtab) {
           const float2* sh = a.shtab + ((size_t)is * a.nk + kk) * n;
           q0 = cmul(q0, sh[t]);
           q1 = cmul(q1, sh[t + 1]);
