@@ -204,6 +204,134 @@ def gradient_form(steps, nptcls=50000, nrefs=500, fd_sample=256, h=1e-3):
     }
 
 
+def shift_search_form(nptcls=50000, nrefs=500, trs=5.0, cpu_sample=12):
+    """Batched gradient shift search (pftcc_shsrch_grad%minimize with the in-plane callback) at the
+    continuous-refinement shape: every particle against its source reference; the oracle's
+    restatement (Python L-BFGS-B + numpy FP64 objective) on a few pairs is the CPU figure and the
+    parity check at full size."""
+    import pftcc
+    from pftcc import Params, PolarFTCorrCalc
+    from pftcc import synth
+
+    box, smpd, kfromto = 200, 1.0, (2, 70)
+    pftsz = pftcc.magic_pftsz(98)
+    w = synth.Workload(box=box, smpd=smpd, kfromto=kfromto, pftsz=pftsz, nrefs=nrefs, nptcls=nptcls, snr=0.1,
+                       seed=20251206, dfrange=(0.8, 3.5), astig=0.1, max_shift=4.0, uniform_dirs=True)
+    synth.make_params(w)
+    lib = PolarFTCorrCalc().new(nrefs, (1, nptcls), Params(boxmatch=box, kfromto=kfromto, pftsz=pftsz, ctf=True))
+    synth.load_into(lib, w, chunk=10000)
+    iptcls = np.arange(1, nptcls + 1, dtype=np.int32)
+    irefs = (w.prm["ref"] + 1).astype(np.int32)
+    S = synth.synth_lib()
+    lib.shsrch_grad_batch(irefs[:64], iptcls[:64], trs=trs)          # warm-up (code objects)
+    S.synth_device_sync()
+    lib.set_timing(True)
+    t0 = time.perf_counter()
+    cxy, rot = lib.shsrch_grad_batch(irefs, iptcls, trs=trs)
+    wall = time.perf_counter() - t0
+    k_ms, k_n = lib.get_timing_channel(4)
+    c_ms, c_n = lib.get_timing_channel(0)
+    lib.set_timing(False)
+    out = {
+        "workload": "batched gradient shift search (L-BFGS-B over the shift, in-plane rotation callback), "
+                    "every particle against its source reference, continuous-refinement shape",
+        "value": nptcls / wall, "unit": "(reference, particle) searches per s", "wall_s": wall,
+        "pairs": nptcls, "refs": nrefs, "box": box, "kfromto": list(kfromto), "pftsz": pftsz, "trs_px": trs,
+        "snr": 0.1, "improved_frac": float(np.mean(rot > 0)),
+        "objective_kernel": {"name": "k_rot8", "launches": int(k_n), "ms": k_ms},
+        "callback_kernel": {"name": "k_corr_generic (item mode)", "launches": int(c_n), "ms": c_ms},
+        "state_machine_and_host_ms": 1e3 * wall - k_ms - c_ms,
+    }
+    # CPU: the oracle's restatement on a few of the same pairs (inputs read back from the library)
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import lbfgsb_oracle as LO
+        import pftcc_oracle as O
+        rng = np.random.default_rng(7)
+        sel = rng.choice(nptcls, cpu_sample, replace=False)
+        grid = O.make_grid(pftsz, kfromto[0], kfromto[1], box)
+        refs = np.stack([lib.get_ref_pft(int(irefs[i]), True) for i in sel])
+        pts = np.stack([lib.get_ptcl_pft(int(iptcls[i])) for i in sel])
+        ctf = np.stack([lib.get_ctfmat(int(iptcls[i])) for i in sel])
+        pf = O.Pftcc(grid=grid, kstop=kfromto[1], refs_even=refs, refs_odd=refs, ptcls=pts, ctfmats=ctf,
+                     iseven=np.ones(cpu_sample, bool))
+        t0 = time.perf_counter()
+        agree, dsh = 0, 0.0
+        for j, i in enumerate(sel):
+            fdf = (lambda sh, r, j=j: pf.gencorr_grad_for_rot_8(j, j, sh, r))
+            gen = (lambda sh, j=j: pf.gencorrs(j, j, tuple(float(v) for v in sh), method="fft"))
+            ocxy, orot = LO.shsrch_minimize(fdf, gen, grid.angtab_deg, trs=trs, opt_angle=True)
+            if orot == rot[i] and (orot == 0 or abs(float(ocxy[0]) - float(cxy[i, 0])) < 1e-5):
+                agree += 1
+            if orot and orot == rot[i]:
+                dsh = max(dsh, float(np.max(np.abs(ocxy[1:] - cxy[i, 1:]))))
+        cpu_wall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_sample / cpu_wall, "unit": "(reference, particle) searches per s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{cpu_sample} pairs of the same workload: the Python restatement "
+                                         "(L-BFGS-B + numpy FP64 objective + FFT gencorrs callback)",
+                               "agreement": agree / cpu_sample, "max_abs_shift_diff_px": dsh}
+    except Exception as e:  # the GPU figure stands on its own
+        out["cpu_baseline"] = {"error": repr(e)}
+    lib.kill()
+    return out
+
+
+def polarizer_form(nimg=10000, box=200, reps=3):
+    """polarize (Kaiser-Bessel gridding, 3 x 3 window) of device-resident Cartesian transforms into
+    particle slots at the main workload's polar grid (box 200, kfromto [2,60], pftsz 308)."""
+    import pftcc
+    from pftcc import Params, PolarFTCorrCalc
+    from pftcc import synth
+
+    kfromto = (2, 60)
+    pftsz = pftcc.magic_pftsz(98)
+    lib = PolarFTCorrCalc().new(1, (1, nimg), Params(boxmatch=box, kfromto=kfromto, pftsz=pftsz, ctf=False))
+    lib.init_polarizer()
+    rng = np.random.default_rng(3)
+    hc = box // 2 + 1
+    chunk = 1000
+    base = (rng.standard_normal((chunk, box, hc)) + 1j * rng.standard_normal((chunk, box, hc))).astype(np.complex64)
+    d = synth.DeviceBuffer(nimg * box * hc * 8)
+    S = synth.synth_lib()
+    for c0 in range(0, nimg, chunk):
+        S.synth_h2d(d.ptr + c0 * box * hc * 8, base.ctypes.data, base.nbytes)
+    ids = np.arange(1, nimg + 1, dtype=np.int32)
+    lib.polarize_dev(ids[:16], d.ptr, isptcl=True)                   # warm-up
+    S.synth_device_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.polarize_dev(ids, d.ptr, isptcl=True)
+    S.synth_device_sync()
+    wall = (time.perf_counter() - t0) / reps
+    d.close()
+    lib.kill()
+    # algorithmic bytes per image: the distinct half-plane components the windows touch (the
+    # polar grid's coordinates k sin, -k cos, windows nint +- 1, Friedel-folded) + the polar output
+    nk = kfromto[1] - kfromto[0] + 1
+    th = np.arange(pftsz) * (np.pi / pftsz)
+    ks = np.arange(kfromto[0], kfromto[1] + 1)[:, None]
+    cx = np.rint(ks * np.sin(th)).astype(int)
+    cy = np.rint(-ks * np.cos(th)).astype(int)
+    pts = set()
+    for dh in (-1, 0, 1):
+        for dk in (-1, 0, 1):
+            h = (cx + dh).ravel()
+            k = (cy + dk).ravel()
+            neg = h < 0
+            pts.update(zip(np.where(neg, -h, h).tolist(), np.where(neg, -k, k).tolist()))
+    bytes_img = 8.0 * len(pts) + 8.0 * nk * pftsz
+    achieved = bytes_img * nimg / wall / 1e9
+    return {
+        "workload": f"polarize {nimg} device-resident {box}x{box} Fourier transforms into particle slots "
+                    f"(kfromto {list(kfromto)}, pftsz {pftsz}, sqsums refreshed)",
+        "value": nimg / wall, "unit": "images per s", "ms_per_call": 1e3 * wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "bytes_per_image": bytes_img,
+                     "note": "whole call (k_polarize + k_sqsums) timed from the host"},
+    }
+
+
 def precision_same_workload(lib, w, shifts, plain_value, nptcls, sample=1000, ntop=16):
     """The split-BF16 MFMA form (with the 169-shift grid as extra operand rows and top-4 exact
     FP64 rescoring) on the main workload's references and particles, next to the plain form."""
@@ -298,8 +426,8 @@ def main():
     ap.add_argument("--mfma-ptcls", type=int, default=10000,
                     help="particles of the precision-form run on the main workload")
     ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-kernel line")
-    ap.add_argument("--legs", default="plain,precision,gradient",
-                    help="comma list of plain, precision, gradient (profiling runs select one)")
+    ap.add_argument("--legs", default="plain,precision,gradient,shsrch,polarizer",
+                    help="comma list of plain, precision, gradient, shsrch, polarizer (profiling runs select one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device copy (k_copy), the reference for HBM traffic counters")
     args = ap.parse_args()
@@ -327,6 +455,10 @@ def main():
             out["precision_form"] = precision_form(args.precision_steps)
         if "gradient" in legs:
             out["gradient_form"] = gradient_form(3)
+        if "shsrch" in legs:
+            out["shift_search"] = shift_search_form()
+        if "polarizer" in legs:
+            out["polarizer"] = polarizer_form()
         print(json.dumps(out), flush=True)
         return 0
 
@@ -422,6 +554,10 @@ def main():
         out["precision_form"] = precision_form(args.precision_steps)
     if not args.no_gradient and "gradient" in legs:
         out["gradient_form"] = gradient_form(3)
+    if "shsrch" in legs:
+        out["shift_search"] = shift_search_form()
+    if "polarizer" in legs:
+        out["polarizer"] = polarizer_form()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -40,6 +40,9 @@ contains
         real(c_float),  allocatable :: shifts(:,:), cc(:), ccall(:,:,:,:), best_cc(:), scores(:), mcc(:)
         integer,        allocatable :: best_iref(:), best_irot(:), best_ishift(:), ids(:), miref(:), mirot(:)
         integer(c_int32_t), allocatable :: qref(:), qptcl(:), qrot(:)
+        integer,        allocatable :: srefs(:), sptcls(:), srots(:)
+        real(c_float),  allocatable :: scxy(:,:)
+        real(c_float)               :: lims(2,2)
         real(c_double), allocatable :: qsh(:,:), f8(:), g8(:,:)
         integer :: nrefs, nptcls, n, kfrom, kto, kstop, box, withctf, nsh, nq, pfrom, nk, nrots
         integer :: u, i, r, s, q
@@ -103,6 +106,20 @@ contains
         ! precision form with exact rescoring of the 3 best references
         allocate(mcc(nptcls), miref(nptcls), mirot(nptcls))
         call pftcc%search_mfma(ids, mcc, miref, mirot, PREC_SPLIT_BF16, 3)
+        ! gradient shift search of every (reference, particle) pair, in-plane callback on
+        allocate(srefs(nrefs*nptcls), sptcls(nrefs*nptcls), srots(nrefs*nptcls), scxy(3, nrefs*nptcls))
+        q = 0
+        do i = 1, nptcls
+            do r = 1, nrefs
+                q = q + 1
+                srefs(q)  = r
+                sptcls(q) = pfrom + i - 1
+            end do
+        end do
+        srots = 0
+        lims(:,1) = -3.
+        lims(:,2) =  3.
+        call pftcc%shsrch_grad_batch(srefs, sptcls, lims, 60, .true., srots, scxy)
         allocate(f8(max(nq,1)), g8(2, max(nq,1)), scores(max(nq,1)))
         do q = 1, nq
             f8(q) = pftcc%gencorr_for_rot_8(qref(q), qptcl(q), qsh(:,q), qrot(q))
@@ -124,6 +141,8 @@ contains
         write(u) mcc
         write(u) int(miref, c_int32_t)
         write(u) int(mirot, c_int32_t)
+        write(u) scxy
+        write(u) int(srots, c_int32_t)
         close(u)
         write(*,'(a,i0,a,i0,a,i0,a)') 'parity run done: ', nptcls, ' ptcls x ', nrefs, ' refs x ', nsh, ' shifts'
     end subroutine run_parity

@@ -101,6 +101,9 @@ type :: polarft_corrcalc
     procedure          :: search
     procedure          :: search_mfma
     procedure          :: grad_batch
+    procedure          :: shsrch_grad_batch
+    procedure          :: init_polarizer
+    procedure          :: polarize
     procedure          :: kill
 end type polarft_corrcalc
 
@@ -333,6 +336,28 @@ interface
         type(c_ptr), value :: irefs, shifts
         real(c_float), intent(out) :: best_cc(*)
         integer(c_int32_t), intent(out) :: best_iref(*), best_ishift(*), best_irot(*)
+    end function
+    integer(c_int) function c_shsrch_grad_batch(h, n, irefs, iptcls, lims, maxits, opt_angle, irots, cxy) &
+        &bind(C, name='pftcc_shsrch_grad_batch')
+        import :: c_int, c_int32_t, c_ptr, c_float
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: n, maxits, opt_angle
+        integer(c_int32_t), intent(in) :: irefs(*), iptcls(*)
+        real(c_float), intent(in) :: lims(*)
+        integer(c_int32_t), intent(inout) :: irots(*)
+        real(c_float), intent(out) :: cxy(*)
+    end function
+    integer(c_int) function c_init_polarizer(h, griddev) bind(C, name='pftcc_init_polarizer')
+        import :: c_int, c_int32_t, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: griddev
+    end function
+    integer(c_int) function c_polarize(h, count, img_inds, cmats, isptcl, iseven, mask, nmask) &
+        &bind(C, name='pftcc_polarize')
+        import :: c_int, c_int32_t, c_ptr
+        type(c_ptr), value :: h, cmats, mask
+        integer(c_int32_t), value :: count, isptcl, iseven, nmask
+        integer(c_int32_t), intent(in) :: img_inds(*)
     end function
     integer(c_int) function c_prep_matchfilt(h, iptcl, iref, irot) bind(C, name='pftcc_prep_matchfilt')
         import :: c_int, c_int32_t, c_ptr
@@ -734,6 +759,64 @@ contains
         call chk(self, c_prep_matchfilt(self%h, int(iptcl,c_int32_t), int(iref,c_int32_t), int(irot,c_int32_t)), &
             &'prep_matchfilt')
     end subroutine prep_matchfilt
+
+    ! GRADIENT SHIFT SEARCH: pftcc_shsrch_grad%minimize (simple_pftcc_shsrch_grad.f90:254-338) for
+    ! n (reference, particle) pairs; lims(2,2) as grad_shsrch_new; irots in (opt_angle = .false.)
+    ! and out (0: no better solution); cxy(3, n) = (cc, shift in the reference frame)
+    subroutine shsrch_grad_batch( self, irefs, iptcls, lims, maxits, opt_angle, irots, cxy )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: irefs(:), iptcls(:)
+        real(sp),                intent(in)    :: lims(2,2)
+        integer,                 intent(in)    :: maxits
+        logical,                 intent(in)    :: opt_angle
+        integer,                 intent(inout) :: irots(size(irefs))
+        real(sp),                intent(out)   :: cxy(3, size(irefs))
+        integer(c_int32_t) :: ir(size(irefs)), ip(size(irefs)), rt(size(irefs))
+        real(c_float) :: lm(4)
+        ir = irefs
+        ip = iptcls
+        rt = irots
+        lm = reshape(lims, [4])
+        call chk(self, c_shsrch_grad_batch(self%h, int(size(irefs),c_int32_t), ir, ip, lm, int(maxits,c_int32_t), &
+            &merge(1_c_int32_t, 0_c_int32_t, opt_angle), rt, cxy), 'shsrch_grad_batch')
+        irots = rt
+    end subroutine shsrch_grad_batch
+
+    ! POLARIZER: init_polarizer / polarize (simple_polarizer.f90:41-151) for the handle's grid;
+    ! cmats(box/2+1, box, n) are the images' Fourier transforms as the image object holds them
+    subroutine init_polarizer( self, griddev )
+        class(polarft_corrcalc), intent(inout) :: self
+        logical, optional,       intent(in)    :: griddev
+        integer(c_int32_t) :: gd
+        gd = 0
+        if( present(griddev) )then
+            if( griddev ) gd = 1
+        endif
+        call chk(self, c_init_polarizer(self%h, gd), 'init_polarizer')
+    end subroutine init_polarizer
+
+    subroutine polarize( self, img_inds, cmats, isptcl, iseven, mask )
+        class(polarft_corrcalc),   intent(inout) :: self
+        integer,                   intent(in)    :: img_inds(:)
+        complex(sp), target,       intent(in)    :: cmats(:,:,:)
+        logical,                   intent(in)    :: isptcl, iseven
+        logical, optional,         intent(in)    :: mask(:)
+        integer(c_int32_t) :: ids(size(img_inds))
+        integer(c_int8_t), allocatable, target :: mk(:)
+        type(c_ptr) :: pm
+        integer(c_int32_t) :: nm
+        ids = img_inds
+        pm  = c_null_ptr
+        nm  = 0
+        if( present(mask) )then
+            allocate(mk(size(mask)))
+            mk = merge(1_c_int8_t, 0_c_int8_t, mask)
+            pm = c_loc(mk)
+            nm = size(mask)
+        endif
+        call chk(self, c_polarize(self%h, int(size(img_inds),c_int32_t), ids, c_loc(cmats), &
+            &merge(1_c_int32_t, 0_c_int32_t, isptcl), merge(1_c_int32_t, 0_c_int32_t, iseven), pm, nm), 'polarize')
+    end subroutine polarize
 
     ! BATCHED FAST PATHS (no reference counterpart: the ref loop of strategy3D_greedy%srch,
     ! simple_strategy3D_greedy.f90:60-77, moved onto the device)

@@ -72,6 +72,8 @@ def test_fortran_module_parity(tmp_path):
     m_cc = take(np.float32, 4)
     m_ref = take(np.int32, 4)
     m_rot = take(np.int32, 4)
+    s_cxy = take(np.float32, 3 * 12).reshape(12, 3)
+    s_rot = take(np.int32, 12)
     assert off == len(raw)
     # the Python mirror over the same C ABI gives bitwise identical numbers
     lib = to_lib(pf, cps)
@@ -100,4 +102,11 @@ def test_fortran_module_parity(tmp_path):
         r_, t_ = np.unravel_index(int(np.argmax(exact)), exact.shape)
         assert (m_ref[i], m_rot[i]) == (r_ + 1, t_ + 1)
         assert abs(m_cc[i] - exact[r_, t_]) <= 1e-7
+    # gradient shift search through the module: identical to the Python mirror's call
+    refs = np.tile(np.arange(1, 4, dtype=np.int32), 4)
+    ptcls = np.repeat(np.arange(1, 5, dtype=np.int32), 3)
+    cxy_py, rot_py = lib.shsrch_grad_batch(refs, ptcls, trs=3.0, maxits=60, opt_angle=True)
+    assert np.array_equal(s_rot, rot_py)
+    assert np.array_equal(s_cxy, cxy_py)
+    assert np.sum(s_rot > 0) >= 6          # random pairs: most improve on the start
     lib.kill()
