@@ -106,50 +106,63 @@ __global__ void k_polar_tables(const float* __restrict__ polar_x, const float* _
     for (int l = 0; l < PZ_WDIM; ++l) wts[(size_t)(l + PZ_WDIM * m) * np + p] = griddev ? w[l][m] : w[l][m] / s;
 }
 
-// polarize (simple_polarizer.f90:120-151), batched: block (x, b) covers polar points of image b.
-// cmats: [nimg][ldim][ldim/2 + 1] complex; out: the handle's ring block, image b written at
-// slot[b] * nk * n; mask: [nk] shell flags or null.  FP32 accumulation in the reference's
-// order (l fastest, then m).
-__global__ void __launch_bounds__(256) k_polarize(const float2* __restrict__ cmats, int ldim,
+// polarize (simple_polarizer.f90:120-151), batched: thread (point p, image group y) reads the
+// point's table once and grids PZ_IMGS consecutive images with it.  cmats: [nimg][ldim][ldim/2 + 1]
+// complex; out: the handle's ring block, image b written at slot[b] * nk * n; mask: [nk] shell
+// flags or null.  FP32 accumulation in the reference's order (l fastest, then m).
+constexpr int PZ_IMGS = 8;
+
+__global__ void __launch_bounds__(256) k_polarize(const float2* __restrict__ cmats, int ldim, int nimg,
                                                   const float* __restrict__ wts, const int* __restrict__ hk,
                                                   int nk, int n, const int* __restrict__ slot,
                                                   const int8_t* __restrict__ mask, float2* __restrict__ out) {
   const int np = nk * n;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const int b = blockIdx.y;
+  const int b0 = blockIdx.y * PZ_IMGS;
   const int kk = p / n;
-  float2* dst = out + (size_t)slot[b] * np + p;
-  if (mask && !mask[kk]) {
-    *dst = make_float2(0.f, 0.f);
-    return;
-  }
+  const bool zero = mask && !mask[kk];
   const int hc = ldim / 2 + 1;
-  const float2* F = cmats + (size_t)b * ldim * hc;
-  int hv[PZ_WDIM], kv[PZ_WDIM];
-#pragma unroll
-  for (int c = 0; c < PZ_WDIM; ++c) {
-    hv[c] = hk[(size_t)c * np + p];
-    kv[c] = hk[(size_t)(PZ_WDIM + c) * np + p];
-  }
-  float re = 0.f, im = 0.f;
+  // the 9 addresses of the window (Friedel-folded, image-relative) and their weights
+  int addr[PZ_WLEN];
+  bool conj[PZ_WLEN];
+  float w[PZ_WLEN];
 #pragma unroll
   for (int m = 0; m < PZ_WDIM; ++m) {
 #pragma unroll
     for (int l = 0; l < PZ_WDIM; ++l) {
-      const int h = hv[l], k = kv[m];
+      const int h = hk[(size_t)l * np + p], k = hk[(size_t)(PZ_WDIM + m) * np + p];
       // get_fcomp2D: h >= 0 reads (h, k); h < 0 reads the conjugate of (-h, -k)
       const bool neg = h < 0;
       const int ph = neg ? -h : h;
       int pk = neg ? -k : k;
       pk += pk < 0 ? ldim : 0;
-      const float2 v = F[(size_t)pk * hc + ph];
-      const float w = wts[(size_t)(l + PZ_WDIM * m) * np + p];
-      re = fmaf(w, v.x, re);
-      im = fmaf(w, neg ? -v.y : v.y, im);
+      const int t = l + PZ_WDIM * m;
+      addr[t] = pk * hc + ph;
+      conj[t] = neg;
+      w[t] = wts[(size_t)t * np + p];
     }
   }
-  *dst = make_float2(re, im);
+  const size_t isz = (size_t)ldim * hc;
+#pragma unroll
+  for (int i = 0; i < PZ_IMGS; ++i) {
+    const int b = b0 + i;
+    if (b >= nimg) break;
+    float2* dst = out + (size_t)slot[b] * np + p;
+    if (zero) {
+      *dst = make_float2(0.f, 0.f);
+      continue;
+    }
+    const float2* F = cmats + (size_t)b * isz;
+    float re = 0.f, im = 0.f;
+#pragma unroll
+    for (int t = 0; t < PZ_WLEN; ++t) {
+      const float2 v = F[addr[t]];
+      re = fmaf(w[t], v.x, re);
+      im = fmaf(w[t], conj[t] ? -v.y : v.y, im);
+    }
+    *dst = make_float2(re, im);
+  }
 }
 
 }  // namespace pfk
