@@ -197,8 +197,12 @@ __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, co
       }
 }
 
+#ifndef PFTCC_ZGEMM_AS_TP
+#define PFTCC_ZGEMM_AS_TP 2   // 32-column tiles: 252 registers, 2 wavefronts per SIMD (64 columns: 1)
+#endif
+
 // A-stationary variant: one wavefront holds the reference operand of a 16 TR-row slab at one m
-// (all NSTEP k' steps) in registers and sweeps every 64-column particle tile of the chunk, so the
+// (all NSTEP k' steps) in registers and sweeps every 16 TP-column particle tile of the chunk, so the
 // large operand is read once per launch and only the particle operand (prefetched one step ahead)
 // is loaded in the inner loop.  The XCD-aware order keeps the slabs of one m on one XCD (the
 // particle operand of that m stays in its L2).
@@ -206,7 +210,7 @@ template <int SPLIT, int TR, int NSTEP>
 __global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
                                                  const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
                                                  int rows_pad, int cols_pad, int n, float2* __restrict__ Z) {
-  constexpr int TP = 4;
+  constexpr int TP = PFTCC_ZGEMM_AS_TP;        // 16-column particle blocks per tile
   constexpr int KP = NSTEP * 32;
   const int l = threadIdx.x;
   const int gr = rows_pad / (16 * TR);
@@ -231,10 +235,10 @@ __global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi,
       if (SPLIT == 3) lo[j] = Blo[bo];
     }
   };
-  const int ntile = cols_pad / 64;
+  const int ntile = cols_pad / (16 * TP);
   load_b(0, 0, bh, bl);
   for (int tile = 0; tile < ntile; ++tile) {
-    const int p0 = tile * 64;
+    const int p0 = tile * 16 * TP;
     f32x4 zr[TR][TP], zi[TR][TP];
 #pragma unroll
     for (int i = 0; i < TR; ++i)
@@ -247,7 +251,7 @@ __global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi,
     for (int st = 0; st < NSTEP; ++st) {
       // prefetch the next step (or the next tile's first step)
       const bool more = st + 1 < NSTEP || tile + 1 < ntile;
-      if (more) load_b(st + 1 < NSTEP ? p0 : p0 + 64, st + 1 < NSTEP ? st + 1 : 0, nh, nl);
+      if (more) load_b(st + 1 < NSTEP ? p0 : p0 + 16 * TP, st + 1 < NSTEP ? st + 1 : 0, nh, nl);
 #pragma unroll
       for (int i = 0; i < TR; ++i) {
         const bf16x8 a2h = rot_neg(ah[st][i]);
