@@ -429,17 +429,6 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   const int r = t / gx;
   const int p0 = (t % gx) * F;
   for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
-  {
-    const int f = tid / TPF, u = tid % TPF;
-    const int p = p0 + f;
-    double s = 0.0;
-    if (p < a.np)
-      for (int kk = u; kk < a.knorm; kk += TPF)
-        s += a.sr[(size_t)(r / a.ns) * a.knorm + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
-#pragma unroll
-    for (int o = TPF / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, TPF);
-    if (u == 0) den[f] = p < a.np ? sqrt(s * a.sp[p]) : 1.0;
-  }
   // ---- pass 1: loads first (they do not depend on LDS), then the butterflies ----
   const float2* Zb = a.Z + zidx(r, p0, 0, N, a.cols_pad);
   float2 za[TPT1][4], zb[TPT1][4], zn[TPT1];
@@ -454,6 +443,18 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
       zb[c][t] = Zb[(size_t)((t + 1) * Q - g) * ZG + f];
     }
     zn[c] = zb[c][3];   // Z[N - g]; for g = 0 this is Z[N]
+  }
+  // FP64 denominators (their loads overlap the spectra's)
+  {
+    const int f = tid / TPF, u = tid % TPF;
+    const int p = p0 + f;
+    double s = 0.0;
+    if (p < a.np)
+      for (int kk = u; kk < a.knorm; kk += TPF)
+        s += a.sr[(size_t)(r / a.ns) * a.knorm + kk] * (a.cbar2 ? a.cbar2[(size_t)p * a.nk + kk] : 1.0);
+#pragma unroll
+    for (int o = TPF / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, TPF);
+    if (u == 0) den[f] = p < a.np ? sqrt(s * a.sp[p]) : 1.0;
   }
   __syncthreads();   // tw2
 #pragma unroll
