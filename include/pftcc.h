@@ -231,13 +231,14 @@ int pftcc_polarize_dev(pftcc_t* h, int32_t count, const int32_t* img_inds, const
  * on the objective evaluations) on gencorr_grad_for_rot_8.  opt_angle != 0: the in-plane rotation
  * starts at the best of gencorrs at shift 0 and is re-chosen (gencorrs + maxloc) at every new
  * iterate, for up to 5 shift-search / rotation cycles; opt_angle = 0: irots[i] is the fixed
- * rotation.  Out: irots[i] = the rotation, or 0 when nothing better than the start was found
+ * rotation.  coarse_init != 0: the start comes from the 5 x 5 grid over lims first
+ * (coarse_search_opt_angle / coarse_search, :340-395).  Out: irots[i] = the rotation, or 0 when nothing better than the start was found
  * (then cxy is not meaningful); cxy[i] = {cc, shift rotated to the reference frame}
  * (matmul(shift, rotmat2d(get_rot(irot)))).  No direct counterpart entry point: it replaces the
  * reference's per-particle calls in strategy2D_srch%inpl_srch / strategy3D_srch%inpl_srch
  * (simple_strategy2D_srch.f90:120-145, simple_strategy3D_srch.f90:190-205). */
 int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const int32_t* iptcls, const float* lims,
-                            int32_t maxits, int32_t opt_angle, int32_t* irots, float* cxy);
+                            int32_t maxits, int32_t opt_angle, int32_t coarse_init, int32_t* irots, float* cxy);
 
 /* -------------------------------------------------------- precision (MFMA) form -- */
 /* Exhaustive search over references x rotations (gencorrs_1), or references x shifts x rotations

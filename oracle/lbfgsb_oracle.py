@@ -646,8 +646,26 @@ def rotmat2d_apply(v, ang_deg):
     return np.array([f(v0 * c + v1 * (-s)), f(v0 * s + v1 * c)], np.float32)
 
 
+COARSE_NUM_STEPS = 5    # simple_pftcc_shsrch_grad.f90:15
+
+
+def _coarse_grid(trs):
+    """The 5 x 5 grid of coarse_search / coarse_search_opt_angle (:340-395): step (hi - lo)/5 in FP64
+    from the FP32 limits, points lo + step/2 + i step, x outer."""
+    f32 = np.float32
+    lo, hi = f32(-trs), f32(trs)
+    step = float(f32(hi - lo)) / float(COARSE_NUM_STEPS)
+    pts = []
+    for ix in range(COARSE_NUM_STEPS):
+        x = float(lo) + step / 2.0 + float(ix) * step
+        for iy in range(COARSE_NUM_STEPS):
+            y = float(lo) + step / 2.0 + float(iy) * step
+            pts.append((x, y))
+    return pts
+
+
 def shsrch_minimize(fdf, gencorrs, angtab_deg, trs, irot=None, opt_angle=True, maxits=SHSRCH_MAXITS,
-                    max_evals=SHSRCH_MAX_EVALS, eq_tol=1e-10, trace=None):
+                    max_evals=SHSRCH_MAX_EVALS, eq_tol=1e-10, trace=None, coarse_init=False):
     """pftcc_shsrch_grad%minimize (simple_pftcc_shsrch_grad.f90:254-338) for one (reference,
     particle) pair.
 
@@ -688,6 +706,18 @@ def shsrch_minimize(fdf, gencorrs, angtab_deg, trs, irot=None, opt_angle=True, m
         corrs = gencorrs(x)
         state["cur"] = int(np.argmax(corrs)) + 1
         lowest_overall = -float(corrs[state["cur"] - 1])
+        if coarse_init:
+            # coarse_search_opt_angle (:368-395): gencorrs at the FP32 grid points, strict minimum
+            best, brot, bxy = np.inf, 0, (0.0, 0.0)
+            for (gx, gy) in _coarse_grid(trs):
+                c = gencorrs(np.array([gx, gy], np.float32))
+                loc = int(np.argmax(c)) + 1
+                cost = -float(c[loc - 1])
+                if cost < best:
+                    best, brot, bxy = cost, loc, (gx, gy)
+            if brot != 0:
+                x = np.array(bxy, np.float32)
+                state["cur"] = brot
         for _ in range(max_evals):
             lowest, x = run_minimize(x)
             corrs = gencorrs(x)
@@ -705,7 +735,18 @@ def shsrch_minimize(fdf, gencorrs, angtab_deg, trs, irot=None, opt_angle=True, m
         return None, 0
     state["cur"] = int(irot)
     lowest_overall = -fdf(np.zeros(2), state["cur"])[0]
-    lowest, x = run_minimize(np.zeros(2, np.float32))
+    x0 = np.zeros(2, np.float32)
+    if coarse_init:
+        # coarse_search (:340-366): the FP64 objective at the grid points, strict minimum
+        best, bxy = np.inf, (0.0, 0.0)
+        for (gx, gy) in _coarse_grid(trs):
+            cost = -fdf(np.array([gx, gy]), state["cur"])[0]
+            if cost < best:
+                best, bxy = cost, (gx, gy)
+        if best < lowest_overall:
+            lowest_overall = best
+            x0 = np.array(bxy, np.float32)
+    lowest, x = run_minimize(x0)
     if float(lowest) < lowest_overall:
         cxy = np.zeros(3, np.float32)
         cxy[0] = f32(-float(lowest))

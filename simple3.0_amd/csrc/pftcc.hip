@@ -2507,7 +2507,7 @@ int launch_corr_items(pftcc* h, int n, const int* d_ip, const int* d_ref, const 
 extern "C" {
 
 int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const int32_t* iptcls, const float* lims,
-                            int32_t maxits, int32_t opt_angle, int32_t* irots, float* cxy) {
+                            int32_t maxits, int32_t opt_angle, int32_t coarse_init, int32_t* irots, float* cxy) {
   if (!h || n < 0 || (n > 0 && (!irefs || !iptcls || !lims || !irots || !cxy))) return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
   if (n == 0) return PFTCC_OK;
@@ -2582,6 +2582,7 @@ int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const i
   a.hi1 = lims[3];
   a.maxits = maxits;
   a.opt_angle = opt_angle ? 1 : 0;
+  a.coarse_init = coarse_init ? 1 : 0;
   auto bind_out = [&](ShsArgs& s, int set) {
     int* b = set_i[set];
     s.counters_out = d_cnt + 2 * set;
@@ -2608,7 +2609,7 @@ int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const i
   }
   int cur = 0;
   // every item issues at most max_evals x (maxits + line-search trials + callbacks) requests
-  const long max_rounds = 5L * (maxits + 2 * LB_MAXLS + 8) * 4 + 64;
+  const long max_rounds = 5L * (maxits + 2 * LB_MAXLS + 8) * 4 + 64 + 2 * SH_COARSE_STEPS * SH_COARSE_STEPS;
   long rounds = 0;
   for (;; ++rounds) {
     int cnt[2];

@@ -36,8 +36,11 @@ enum : int {
   SH_RS_FG0,            // first objective of a minimisation
   SH_RS_LS_FG,          // objective at a line-search trial
   SH_RS_NEWX_CORR,      // callback gencorrs at a new iterate
-  SH_RS_END_CORR        // gencorrs after a minimisation (opt_angle cycle test)
+  SH_RS_END_CORR,       // gencorrs after a minimisation (opt_angle cycle test)
+  SH_RS_COARSE_CORR,    // coarse_search_opt_angle: gencorrs at a grid point
+  SH_RS_COARSE_FG       // coarse_search: objective at a grid point
 };
+constexpr int SH_COARSE_STEPS = 5;   // coarse_num_steps (simple_pftcc_shsrch_grad.f90:15)
 enum : int { LB_FG = 1, LB_NEWX = 2, LB_END = 3 };
 
 struct Dcs {              // Moré-Thuente search state (dcsrch isave/dsave)
@@ -48,7 +51,9 @@ struct Dcs {              // Moré-Thuente search state (dcsrch isave/dsave)
 struct ShsItem {
   // the pair and the search settings
   int iref, ip, opt_angle, cur;        // 0-based reference, local particle, flag, rotation (1-based)
-  int resume, maxits, cycle, pad0;
+  int resume, maxits, cycle, coarse;  // coarse: the 5 x 5 initial grid (coarse_init)
+  int ci, cbest_rot, pad0, pad1;       // grid point, best rotation of the grid (opt_angle)
+  double cbest, cbx, cby;              // best grid cost and point
   double lo[2], hi[2];
   float x32[2];                        // spec%x (FP32 view of the iterate)
   double lowest_overall;
@@ -664,6 +669,16 @@ __device__ inline void shs_rotate(const ShsItem& S, float ang_deg, float v0, flo
   o1 = v0 * s + v1 * c;
 }
 
+// Point c of the coarse grid (coarse_search :340-366): step (hi - lo)/5 in FP64 from the FP32
+// limits, lo + step/2 + i step, x outer.
+__device__ inline void shs_coarse_point(const ShsItem& S, int c, double& x, double& y) {
+  const double stx = (double)((float)S.hi[0] - (float)S.lo[0]) / (double)SH_COARSE_STEPS;
+  const double sty = (double)((float)S.hi[1] - (float)S.lo[1]) / (double)SH_COARSE_STEPS;
+  const int ix = c / SH_COARSE_STEPS, iy = c % SH_COARSE_STEPS;
+  x = (double)(float)S.lo[0] + stx / 2.0 + (double)ix * stx;
+  y = (double)(float)S.lo[1] + sty / 2.0 + (double)iy * sty;
+}
+
 // Result of the last request: f, g (SH_REQ_FG) or the best rotation and its cc (SH_REQ_CORR).
 // Returns the next request.
 __device__ inline int shs_advance(ShsItem& S, double f, double g0, double g1, int corr_rot, float corr_cc,
@@ -685,14 +700,86 @@ __device__ inline int shs_advance(ShsItem& S, double f, double g0, double g1, in
     case SH_RS_INIT_CORR:
       S.cur = corr_rot;
       S.lowest_overall = -(double)corr_cc;
+      if (S.coarse) {
+        S.ci = 0;
+        S.cbest = 1.0e300;
+        S.cbest_rot = 0;
+        S.cbx = S.cby = 0.0;
+        double gx, gy;
+        shs_coarse_point(S, 0, gx, gy);
+        S.x32[0] = (float)gx;
+        S.x32[1] = (float)gy;
+        S.resume = SH_RS_COARSE_CORR;
+        return SH_REQ_CORR;
+      }
       lb_start(S, 0.0, 0.0);
       S.resume = SH_RS_FG0;
       return SH_REQ_FG;
     case SH_RS_INIT_FG:
       S.lowest_overall = -f;
+      if (S.coarse) {
+        S.ci = 0;
+        S.cbest = 1.0e300;
+        S.cbx = S.cby = 0.0;
+        shs_coarse_point(S, 0, S.x[0], S.x[1]);
+        S.resume = SH_RS_COARSE_FG;
+        return SH_REQ_FG;
+      }
       lb_start(S, 0.0, 0.0);
       S.resume = SH_RS_FG0;
       return SH_REQ_FG;
+    case SH_RS_COARSE_CORR: {
+      // coarse_search_opt_angle (:368-395): strict minimum of -max(gencorrs) over the grid
+      const double cost = -(double)corr_cc;
+      if (cost < S.cbest) {
+        double gx, gy;
+        shs_coarse_point(S, S.ci, gx, gy);
+        S.cbest = cost;
+        S.cbest_rot = corr_rot;
+        S.cbx = gx;
+        S.cby = gy;
+      }
+      if (++S.ci < SH_COARSE_STEPS * SH_COARSE_STEPS) {
+        double gx, gy;
+        shs_coarse_point(S, S.ci, gx, gy);
+        S.x32[0] = (float)gx;
+        S.x32[1] = (float)gy;
+        return SH_REQ_CORR;
+      }
+      S.x32[0] = 0.f;
+      S.x32[1] = 0.f;
+      if (S.cbest_rot != 0) {
+        S.x32[0] = (float)S.cbx;
+        S.x32[1] = (float)S.cby;
+        S.cur = S.cbest_rot;
+      }
+      lb_start(S, (double)S.x32[0], (double)S.x32[1]);
+      S.resume = SH_RS_FG0;
+      return SH_REQ_FG;
+    }
+    case SH_RS_COARSE_FG: {
+      // coarse_search (:340-366): strict minimum of -gencorr_for_rot_8 over the grid
+      const double cost = -f;
+      if (cost < S.cbest) {
+        S.cbest = cost;
+        S.cbx = S.x[0];
+        S.cby = S.x[1];
+      }
+      if (++S.ci < SH_COARSE_STEPS * SH_COARSE_STEPS) {
+        shs_coarse_point(S, S.ci, S.x[0], S.x[1]);
+        return SH_REQ_FG;
+      }
+      S.x32[0] = 0.f;
+      S.x32[1] = 0.f;
+      if (S.cbest < S.lowest_overall) {
+        S.lowest_overall = S.cbest;
+        S.x32[0] = (float)S.cbx;
+        S.x32[1] = (float)S.cby;
+      }
+      lb_start(S, (double)S.x32[0], (double)S.x32[1]);
+      S.resume = SH_RS_FG0;
+      return SH_REQ_FG;
+    }
     case SH_RS_FG0:
       S.f = -f;
       S.g[0] = -g0;
@@ -781,7 +868,7 @@ struct ShsArgs {
   const int* ips;
   const int* irots;
   float lo0, lo1, hi0, hi1;
-  int maxits, opt_angle;
+  int maxits, opt_angle, coarse_init;
   // consumed requests of the previous round
   int nfg_in, ncorr_in;
   const int* fg_item_in;
@@ -825,6 +912,7 @@ __global__ void __launch_bounds__(64) k_shs_step(ShsArgs a) {
     S.ip = a.ips[pos];
     S.cur = a.irots ? a.irots[pos] : 0;
     S.opt_angle = a.opt_angle;
+    S.coarse = a.coarse_init;
     S.maxits = a.maxits;
     S.lo[0] = a.lo0;
     S.lo[1] = a.lo1;

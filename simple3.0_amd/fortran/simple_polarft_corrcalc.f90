@@ -337,11 +337,11 @@ interface
         real(c_float), intent(out) :: best_cc(*)
         integer(c_int32_t), intent(out) :: best_iref(*), best_ishift(*), best_irot(*)
     end function
-    integer(c_int) function c_shsrch_grad_batch(h, n, irefs, iptcls, lims, maxits, opt_angle, irots, cxy) &
-        &bind(C, name='pftcc_shsrch_grad_batch')
+    integer(c_int) function c_shsrch_grad_batch(h, n, irefs, iptcls, lims, maxits, opt_angle, coarse_init, irots, &
+        &cxy) bind(C, name='pftcc_shsrch_grad_batch')
         import :: c_int, c_int32_t, c_ptr, c_float
         type(c_ptr), value :: h
-        integer(c_int32_t), value :: n, maxits, opt_angle
+        integer(c_int32_t), value :: n, maxits, opt_angle, coarse_init
         integer(c_int32_t), intent(in) :: irefs(*), iptcls(*)
         real(c_float), intent(in) :: lims(*)
         integer(c_int32_t), intent(inout) :: irots(*)
@@ -763,7 +763,7 @@ contains
     ! GRADIENT SHIFT SEARCH: pftcc_shsrch_grad%minimize (simple_pftcc_shsrch_grad.f90:254-338) for
     ! n (reference, particle) pairs; lims(2,2) as grad_shsrch_new; irots in (opt_angle = .false.)
     ! and out (0: no better solution); cxy(3, n) = (cc, shift in the reference frame)
-    subroutine shsrch_grad_batch( self, irefs, iptcls, lims, maxits, opt_angle, irots, cxy )
+    subroutine shsrch_grad_batch( self, irefs, iptcls, lims, maxits, opt_angle, irots, cxy, coarse_init )
         class(polarft_corrcalc), intent(inout) :: self
         integer,                 intent(in)    :: irefs(:), iptcls(:)
         real(sp),                intent(in)    :: lims(2,2)
@@ -771,14 +771,19 @@ contains
         logical,                 intent(in)    :: opt_angle
         integer,                 intent(inout) :: irots(size(irefs))
         real(sp),                intent(out)   :: cxy(3, size(irefs))
-        integer(c_int32_t) :: ir(size(irefs)), ip(size(irefs)), rt(size(irefs))
+        logical, optional,       intent(in)    :: coarse_init
+        integer(c_int32_t) :: ir(size(irefs)), ip(size(irefs)), rt(size(irefs)), ci
         real(c_float) :: lm(4)
         ir = irefs
         ip = iptcls
         rt = irots
         lm = reshape(lims, [4])
+        ci = 0
+        if( present(coarse_init) )then
+            if( coarse_init ) ci = 1
+        endif
         call chk(self, c_shsrch_grad_batch(self%h, int(size(irefs),c_int32_t), ir, ip, lm, int(maxits,c_int32_t), &
-            &merge(1_c_int32_t, 0_c_int32_t, opt_angle), rt, cxy), 'shsrch_grad_batch')
+            &merge(1_c_int32_t, 0_c_int32_t, opt_angle), ci, rt, cxy), 'shsrch_grad_batch')
         irots = rt
     end subroutine shsrch_grad_batch
 

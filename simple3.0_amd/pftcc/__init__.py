@@ -109,7 +109,7 @@ SIGNATURES = [
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     ("pftcc_prep_matchfilt", _I, [_P, _I, _I, _I]),
     ("pftcc_prep_matchfilt_batch", _I, [_P, _I, _PI, _PI, _PI]),
-    ("pftcc_shsrch_grad_batch", _I, [_P, _I, _PI, _PI, _PF, _I, _I, _PI, _PF]),
+    ("pftcc_shsrch_grad_batch", _I, [_P, _I, _PI, _PI, _PF, _I, _I, _I, _PI, _PF]),
     ("pftcc_init_polarizer", _I, [_P, _I]),
     ("pftcc_polarize", _I, [_P, _I, _PI, _PF, _I, _I, C.POINTER(C.c_int8), _I]),
     ("pftcc_polarize_dev", _I, [_P, _I, _PI, _P, _I, _I, C.POINTER(C.c_int8), _I]),
@@ -530,7 +530,8 @@ class PolarFTCorrCalc:
         self._chk(self._lib.pftcc_prep_matchfilt_batch(self._h, p.size, _ip(p), _ip(r), _ip(o)),
                   "prep_matchfilt_batch")
 
-    def shsrch_grad_batch(self, irefs, iptcls, trs=None, lims=None, maxits=60, opt_angle=True, irots=None):
+    def shsrch_grad_batch(self, irefs, iptcls, trs=None, lims=None, maxits=60, opt_angle=True, irots=None,
+                          coarse_init=False):
         """pftcc_shsrch_grad%minimize for pairs (irefs[i], iptcls[i]): returns (cxy [n][3], irot [n]),
         irot 0 where no better solution than the start was found.  lims: (xlo, ylo, xhi, yhi), or
         +-trs in both directions."""
@@ -542,7 +543,8 @@ class PolarFTCorrCalc:
         rot = np.zeros(r.size, np.int32) if irots is None else np.array(np.atleast_1d(irots), np.int32)
         cxy = np.zeros((r.size, 3), np.float32)
         self._chk(self._lib.pftcc_shsrch_grad_batch(self._h, r.size, _ip(r), _ip(p), _fp(lm), int(maxits),
-                                                    1 if opt_angle else 0, _ip(rot), _fp(cxy)), "shsrch_grad_batch")
+                                                    1 if opt_angle else 0, 1 if coarse_init else 0, _ip(rot),
+                                                    _fp(cxy)), "shsrch_grad_batch")
         return cxy, rot
 
     def init_polarizer(self, griddev=False):

@@ -39,19 +39,22 @@ def _planted_case(seed, ctf, objective="cc"):
     return pf, cps
 
 
-def _oracle_pair(pf, r, p, opt_angle, irot):
+def _oracle_pair(pf, r, p, opt_angle, irot, coarse=False):
     def fdf(sh, rot):
         return pf.gencorr_grad_for_rot_8(r, p, sh, rot)
 
     def gencorrs(sh32):
         return pf.gencorrs(r, p, tuple(float(v) for v in sh32), method="fft")
-    return L.shsrch_minimize(fdf, gencorrs, pf.grid.angtab_deg, trs=TRS, irot=irot, opt_angle=opt_angle)
+    return L.shsrch_minimize(fdf, gencorrs, pf.grid.angtab_deg, trs=TRS, irot=irot, opt_angle=opt_angle,
+                             coarse_init=coarse)
 
 
-@pytest.mark.parametrize("opt_angle,ctf,objective", [(True, False, "cc"), (True, True, "cc"), (False, True, "cc"),
-                                                     (True, True, "euclid"), (True, True, "match_filt")])
-def test_shift_search_matches_oracle(opt_angle, ctf, objective):
-    say(f"shift search parity: opt_angle {opt_angle}, ctf {ctf}, objective {objective}")
+@pytest.mark.parametrize("opt_angle,ctf,objective,coarse",
+                         [(True, False, "cc", False), (True, True, "cc", False), (False, True, "cc", False),
+                          (True, True, "euclid", False), (True, True, "match_filt", False),
+                          (True, True, "cc", True), (False, True, "cc", True)])
+def test_shift_search_matches_oracle(opt_angle, ctf, objective, coarse):
+    say(f"shift search parity: opt_angle {opt_angle}, ctf {ctf}, objective {objective}, coarse grid {coarse}")
     pf, cps = _planted_case(31 + int(ctf), ctf, objective)
     lib = to_lib(pf, cps)
     nr, npt = 3, 5
@@ -60,11 +63,11 @@ def test_shift_search_matches_oracle(opt_angle, ctf, objective):
     for r, p in pairs:
         irots.append(int(np.argmax(pf.gencorrs(r, p, (0.0, 0.0), method="fft"))) + 1)
     cxy, rot = lib.shsrch_grad_batch([r + 1 for r, _ in pairs], [p + 1 for _, p in pairs], trs=TRS, maxits=60,
-                                     opt_angle=opt_angle, irots=irots)
+                                     opt_angle=opt_angle, irots=irots, coarse_init=coarse)
     lib.kill()
     found = 0
     for i, (r, p) in enumerate(pairs):
-        ocxy, orot = _oracle_pair(pf, r, p, opt_angle, irots[i])
+        ocxy, orot = _oracle_pair(pf, r, p, opt_angle, irots[i], coarse)
         assert rot[i] == orot, (r, p, rot[i], orot)
         if orot == 0:
             continue
