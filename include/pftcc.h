@@ -217,7 +217,8 @@ int pftcc_init_polarizer(pftcc_t* h, int32_t griddev);
  * img_inds: distinct 1-based particle ids (isptcl != 0; set_ptcl_pft, sqsums refreshed) or
  * reference ids (isptcl = 0, into the even or odd set by iseven; set_ref_pft).  mask: NULL, or the
  * reference's band mask(:) with nmask >= kfromto(2) entries; shell k is zeroed when mask[k-1] == 0.
- * The _dev form reads the transforms from device memory. */
+ * The _dev form reads the transforms from device memory.  As after set_ptcl_pft, memoize_ffts
+ * must follow before particles are searched. */
 int pftcc_polarize(pftcc_t* h, int32_t count, const int32_t* img_inds, const float* cmats, int32_t isptcl,
                    int32_t iseven, const int8_t* mask, int32_t nmask);
 int pftcc_polarize_dev(pftcc_t* h, int32_t count, const int32_t* img_inds, const void* d_cmats, int32_t isptcl,
@@ -236,7 +237,8 @@ int pftcc_polarize_dev(pftcc_t* h, int32_t count, const int32_t* img_inds, const
  * (then cxy is not meaningful); cxy[i] = {cc, shift rotated to the reference frame}
  * (matmul(shift, rotmat2d(get_rot(irot)))).  No direct counterpart entry point: it replaces the
  * reference's per-particle calls in strategy2D_srch%inpl_srch / strategy3D_srch%inpl_srch
- * (simple_strategy2D_srch.f90:120-145, simple_strategy3D_srch.f90:190-205). */
+ * (simple_strategy2D_srch.f90:120-145, simple_strategy3D_srch.f90:190-205).  With opt_angle the
+ * particles must be memoized (memoize_ffts), as for gencorrs. */
 int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const int32_t* iptcls, const float* lims,
                             int32_t maxits, int32_t opt_angle, int32_t coarse_init, int32_t* irots, float* cxy);
 

@@ -2512,7 +2512,8 @@ int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const i
   std::lock_guard<std::mutex> g(h->mtx);
   if (n == 0) return PFTCC_OK;
   int rc;
-  if ((rc = need_memo(h))) return rc;
+  // the rotation callback runs the FP32 search (memoized spectra); the objective alone does not
+  if ((rc = opt_angle ? need_memo(h) : need_direct(h))) return rc;
   if (!(lims[0] <= lims[2]) || !(lims[1] <= lims[3]))
     return fail(h, PFTCC_E_ARG, "shift limits [%g, %g] x [%g, %g] are empty", lims[0], lims[2], lims[1], lims[3]);
   if (maxits < 0) return fail(h, PFTCC_E_ARG, "maxits %d < 0", maxits);

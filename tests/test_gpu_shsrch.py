@@ -106,6 +106,8 @@ def test_shift_search_errors():
     lib = to_lib(pf, cps, memoize=False)
     with pytest.raises(PftccError, match="memoize"):
         lib.shsrch_grad_batch([1], [1], trs=2.0)
+    # without the rotation callback only the FP64 objective is needed (no memoized spectra)
+    lib.shsrch_grad_batch([1], [1], trs=2.0, opt_angle=False, irots=[1])
     lib.memoize_ffts()
     with pytest.raises(PftccError, match="empty"):
         lib.shsrch_grad_batch([1], [1], lims=(1.0, -1.0, -1.0, 1.0))
