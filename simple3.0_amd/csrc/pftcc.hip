@@ -100,6 +100,7 @@ struct pftcc {
   std::vector<int8_t> iseven;       // host mirror of the eo flags
   std::vector<float> angtab_rad, angtab_deg, polar_x, polar_y, spafreq, angmat, pxls;
   std::vector<double> argx, argy, onex, oney;
+  std::vector<float> sn32, ncn32;   // sin(theta_j), -cos(theta_j) in FP32 (polar = k x these)
   std::vector<float> optlp;         // [nrefs][kcorr]
   std::vector<float> sigma2;        // [nptcls][nk] (local order)
   FftPlan plan{};
@@ -111,6 +112,8 @@ struct pftcc {
   bool optlp_dev_valid = false;
   int8_t* d_iseven = nullptr;
   double *d_argx = nullptr, *d_argy = nullptr, *d_onex = nullptr, *d_oney = nullptr;
+  float *d_sn32 = nullptr, *d_ncn32 = nullptr;
+  double* d_sqp8 = nullptr;         // [nptcls][nk] FP64 shell sums of |P|^2 (built on first use)
   float *d_spafreq = nullptr, *d_angmat = nullptr;
   float *d_refinv_f = nullptr, *d_ofac_f = nullptr, *d_shellw = nullptr;
   double *d_refinv_d = nullptr, *d_ofac_d = nullptr, *d_sigma2w = nullptr, *d_refpw = nullptr;
@@ -120,6 +123,7 @@ struct pftcc {
   int* d_pz_hk = nullptr;
   bool pz_ready = false;
   bool memo_valid = false, dptcls_valid = false, refnorm_valid = false, have_sigma2 = false;
+  bool sqp8_valid = false;
   bool ctf_ready = false;
   bool use_fused = true;            // PFTCC_GENERIC=1 forces the runtime-size kernel
   // precision form: cached split-BF16 reference operands per eo class
@@ -262,7 +266,11 @@ void build_host_grid(pftcc* h) {
   h->argy.resize((size_t)nk * n);
   h->onex.resize(n);
   h->oney.resize(n);
+  h->sn32.resize(n);
+  h->ncn32.resize(n);
   for (int j = 0; j < n; ++j) {
+    h->sn32[j] = sn[j];
+    h->ncn32[j] = -cn[j];
     h->onex[j] = (double)sn[j] * A;                 // argtransf_shellone (:313-314)
     h->oney[j] = (double)(-cn[j]) * A;
     for (int kk = 0; kk < nk; ++kk) {
@@ -318,10 +326,11 @@ void free_ptcl_arrays(pftcc* h) {
   dfree(h->d_iseven);
   dfree(h->d_ctf);
   dfree(h->d_dptcls);
+  dfree(h->d_sqp8);
   dfree(h->d_shellw);
   dfree(h->d_sigma2w);
   dfree(h->d_optrow);
-  h->memo_valid = h->dptcls_valid = h->ctf_ready = h->have_sigma2 = false;
+  h->memo_valid = h->dptcls_valid = h->sqp8_valid = h->ctf_ready = h->have_sigma2 = false;
   h->refnorm_valid = false;
 }
 
@@ -648,6 +657,20 @@ int ensure_dptcls(pftcc* h) {
   return PFTCC_OK;
 }
 
+// FP64 per-shell sums of |P|^2 of every particle (the k-weighted objective's particle norm).
+int ensure_sqp8(pftcc* h) {
+  if (h->sqp8_valid) return PFTCC_OK;
+  int rc;
+  if (!h->d_sqp8 && (rc = dalloc(h, &h->d_sqp8, (size_t)h->nptcls * h->nk))) return rc;
+  const int nitems = h->nptcls * h->nk;
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_shell_sq8, dim3((nitems + 3) / 4), dim3(256), 0, h->stream, (const float2*)h->d_ptcls, nitems,
+                       h->n, h->d_sqp8);
+  HIPCHK(h, hipGetLastError());
+  h->sqp8_valid = true;
+  return PFTCC_OK;
+}
+
 // Launches k_rot8 on device lists.
 int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, const double* d_shifts,
                 const int* d_rhos, double* d_f, double* d_grad, bool angle) {
@@ -671,6 +694,9 @@ int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, cons
   a.optrow = (norm && h->clsfrcs) ? h->d_optrow : nullptr;
   a.argx = h->d_argx;
   a.argy = h->d_argy;
+  a.sn32 = h->d_sn32;
+  a.ncn32 = h->d_ncn32;
+  a.arg_a = kDPi / (double)(h->ldim / 2);
   a.one_x = h->d_onex;
   a.one_y = h->d_oney;
   a.sigma2w = h->d_sigma2w;
@@ -682,12 +708,19 @@ int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, cons
   a.f = d_f;
   a.grad = d_grad;
   if (npairs <= 0) return PFTCC_OK;
-  hipEvent_t ev;
   int rc;
+  if (a.mode == 0) {
+    if ((rc = ensure_sqp8(h))) return rc;
+    a.sqp8 = h->d_sqp8;
+  }
+  hipEvent_t ev;
   if ((rc = timing_begin(h, 4, &ev))) return rc;
-  // one thread per ring position j (rounded up to whole wavefronts)
-  const int nt = std::min(1024, (h->n + 63) / 64 * 64);
-  hipLaunchKernelGGL(k_rot8, dim3(npairs), dim3(nt), 0, h->stream, a);
+  // one thread per ring position j (rounded up to whole wavefronts; k_rot8's bound is 512, larger
+  // rings loop over j)
+  const int nt = std::min(512, (h->n + 63) / 64 * 64);
+  if (a.mode == 0) hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_UNROLL, 0>), dim3(npairs), dim3(nt), 0, h->stream, a);
+  else if (a.mode == 1) hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_UNROLL, 1>), dim3(npairs), dim3(nt), 0, h->stream, a);
+  else hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_UNROLL, 2>), dim3(npairs), dim3(nt), 0, h->stream, a);
   HIPCHK(h, hipGetLastError());
   return timing_end(h, ev);
 }
@@ -1166,7 +1199,8 @@ int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, c
   if ((rc = dalloc(h, &h->d_tw2, tw.size())) || (rc = dalloc(h, &h->d_argx, h->argx.size())) ||
       (rc = dalloc(h, &h->d_argy, h->argy.size())) || (rc = dalloc(h, &h->d_onex, h->onex.size())) ||
       (rc = dalloc(h, &h->d_oney, h->oney.size())) || (rc = dalloc(h, &h->d_spafreq, h->spafreq.size())) ||
-      (rc = dalloc(h, &h->d_angmat, h->angmat.size())))
+      (rc = dalloc(h, &h->d_angmat, h->angmat.size())) || (rc = dalloc(h, &h->d_sn32, h->sn32.size())) ||
+      (rc = dalloc(h, &h->d_ncn32, h->ncn32.size())))
     return bad(rc);
   if (hipMemcpy(h->d_tw2, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->d_argx, h->argx.data(), h->argx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1174,7 +1208,9 @@ int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, c
       hipMemcpy(h->d_onex, h->onex.data(), h->onex.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->d_oney, h->oney.data(), h->oney.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(h->d_spafreq, h->spafreq.data(), h->spafreq.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_angmat, h->angmat.data(), h->angmat.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(h->d_angmat, h->angmat.data(), h->angmat.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_sn32, h->sn32.data(), h->sn32.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_ncn32, h->ncn32.data(), h->ncn32.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     return bad(fail(h, PFTCC_E_DEVICE, "upload of constant tables failed"));
   if (h->match_filt || h->pssnr) h->optlp.assign((size_t)nrefs * h->kcorr, 1.0f);
   *out = h;
@@ -1227,6 +1263,8 @@ int pftcc_kill(pftcc_t* h) {
   dfree(h->d_tw2);
   dfree(h->d_argx);
   dfree(h->d_argy);
+  dfree(h->d_sn32);
+  dfree(h->d_ncn32);
   dfree(h->d_onex);
   dfree(h->d_oney);
   dfree(h->d_spafreq);
@@ -1310,7 +1348,7 @@ static int set_ptcls(pftcc_t* h, int first, int count, const void* src, hipMemcp
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->memo_valid = false;
-  h->dptcls_valid = false;
+  h->dptcls_valid = h->sqp8_valid = false;
   return PFTCC_OK;
 }
 
@@ -1440,7 +1478,7 @@ int polarize_impl(pftcc* h, int count, const int32_t* img_inds, const void* cmat
                        h->d_sqsums, h->d_sqsums_k, 0, count, (const int*)d_slot);
     HIPCHK(h, hipGetLastError());
     h->memo_valid = false;
-    h->dptcls_valid = false;
+    h->dptcls_valid = h->sqp8_valid = false;
   } else {
     h->refnorm_valid = false;
     h->apack[0].valid = h->apack[1].valid = false;
@@ -1492,7 +1530,7 @@ int pftcc_set_ptcl_fcomp(pftcc_t* h, int32_t iptcl, int32_t irot, int32_t k, flo
                       hipMemcpyHostToDevice));
   // set_ptcl_fcomp does not refresh sqsums_ptcls in the reference (:471-476); neither do we.
   h->memo_valid = false;
-  h->dptcls_valid = false;
+  h->dptcls_valid = h->sqp8_valid = false;
   return PFTCC_OK;
 }
 
@@ -1520,7 +1558,7 @@ int pftcc_cp_even_ref2ptcl(pftcc_t* h, int32_t iref, int32_t iptcl) {
   if ((rc = update_sqsums(h, ip, 1))) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->memo_valid = false;
-  h->dptcls_valid = false;
+  h->dptcls_valid = h->sqp8_valid = false;
   return PFTCC_OK;
 }
 
@@ -1560,7 +1598,7 @@ int pftcc_reallocate_ptcls(pftcc_t* h, int32_t nptcls, const int32_t* pinds) {
   h->pto = hi;
   h->pinds.assign(hi - lo + 1, 0);
   for (int i = 0; i < nptcls; ++i) h->pinds[pinds[i] - lo] = i + 1;
-  h->memo_valid = h->dptcls_valid = false;
+  h->memo_valid = h->dptcls_valid = h->sqp8_valid = false;
   h->refnorm_valid = false;
   return PFTCC_OK;
 }
@@ -1676,7 +1714,7 @@ int prep_matchfilt_run(pftcc* h, int np, const int32_t* iptcls, const int32_t* i
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->memo_valid = true;
-  h->dptcls_valid = false;
+  h->dptcls_valid = h->sqp8_valid = false;
   return PFTCC_OK;
 }
 }  // namespace
@@ -1792,7 +1830,7 @@ int pftcc_memoize_ffts(pftcc_t* h) {
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->memo_valid = true;
-  h->dptcls_valid = false;
+  h->dptcls_valid = h->sqp8_valid = false;
   return PFTCC_OK;
 }
 

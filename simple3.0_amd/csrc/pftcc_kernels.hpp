@@ -478,7 +478,11 @@ struct Rot8Args {
   const double* refinv;        // as CorrArgs, in double (shellnorm_and_filter_ref_8, :783-805)
   const double* ofac;
   const int* optrow;
-  const double* argx;          // [nk][n]
+  const float* sn32;           // [n] sin(theta_j), FP32: argtransf(j, k) = (double)(sn32[j] * k) * arg_a
+  const float* ncn32;          // [n] -cos(theta_j), FP32, for the y component
+  double arg_a;                // pi / (ldim / 2)
+  const double* sqp8;          // [nptcls][nk] FP64 shell sums of |P|^2 (k-weighted CC)
+  const double* argx;          // [nk][n]; the first shell's row starts the phase recurrence
   const double* argy;
   const double* one_x;         // [n]
   const double* one_y;
@@ -504,7 +508,100 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
   return x * per + (x < rem ? x : rem) + q;
 }
 
-__global__ void __launch_bounds__(1024) k_rot8(Rot8Args a) {
+// The operands of one shell at ring position j, loaded ahead of their use (raw FP32 values, so
+// that several shells fit in registers).
+struct Rot8Shell {
+  float2 rv, pv, dv;           // reference q_k[j], particle P_k[t], dP_k[t]
+  float cf;                    // |CTF|
+  double fac;                  // filter factors (match_filt / pssnr; uniform over the work-group)
+  double s2w;                  // Euclid shell weight (uniform)
+};
+
+template <int MODE>
+__device__ __forceinline__ void rot8_load(const Rot8Args a, Rot8Shell& s, int kk, int j, int t, int ip, int iref,
+                                          bool even, const float2* ref, const float2* ptc, const float2* dpt,
+                                          const float* ctf) {
+  const int n = a.n;
+  s.rv = ref[(size_t)kk * n + j];
+  s.pv = ptc[(size_t)kk * n + t];
+  s.dv = dpt ? dpt[(size_t)kk * n + t] : make_float2(0.f, 0.f);
+  s.cf = ctf ? ctf[(size_t)kk * n + j] : 1.f;
+  s.fac = 1.0;
+  if (a.refinv) {
+    const int row = a.optrow ? a.optrow[ip] : iref;
+    s.fac = a.ofac[(size_t)row * a.kcorr + kk] * a.refinv[((size_t)(even ? 0 : 1) * a.nrefs + iref) * a.kcorr + kk];
+  }
+  s.s2w = MODE == 2 ? a.sigma2w[(size_t)ip * a.nk + kk] : 0.0;
+}
+
+// The FP64 sums of one thread.
+struct Rot8Acc {
+  double num = 0, gx = 0, gy = 0, gpsi = 0, sqref = 0, sqptcl = 0;
+};
+
+// One shell's contributions; advances the shift phase to the next shell (gen_shmat_8's recurrence).
+template <int MODE>
+__device__ __forceinline__ void rot8_step(const Rot8Args a, const Rot8Shell& s, int kk, bool cj, bool angle,
+                                          float snj, float ncnj, Rot8Acc& acc, double& shr, double& shi, double cs,
+                                          double ss) {
+  const double w = (MODE == 0 || MODE == 2) ? (double)(a.kfrom + kk) : 1.0;
+  double c = (double)s.cf;
+  if (a.refinv) c *= s.fac;
+  // argtransf (:311-317) from the FP32 polar coordinate k sin(theta_j), -k cos(theta_j)
+  const float kf = (float)(a.kfrom + kk);
+  const double ax = (double)(snj * kf) * a.arg_a;
+  const double ay = (double)(ncnj * kf) * a.arg_a;
+  // q = ref * ctf * shmat   ((pft_ref*ctfmats)*shmat, :1709)
+  const double ar = (double)s.rv.x * c, ai = (double)s.rv.y * c;
+  const double qr = ar * shr - ai * shi;
+  const double qi = ar * shi + ai * shr;
+  const double pr = s.pv.x, pim = cj ? -(double)s.pv.y : (double)s.pv.y;
+  if (MODE == 2) {
+    const double dr = qr - pr, di = qi - pim;
+    acc.num -= w * 0.5 * (dr * dr + di * di) * s.s2w;
+    // corrk(i*a*q) - Re sum (i a q) conj(q);  Re((i a q) conj P) = -a Im(q conj P)
+    const double imqp = qi * pr - qr * pim;
+    acc.gx += w * (-ax * imqp) * s.s2w;
+    acc.gy += w * (-ay * imqp) * s.s2w;
+  } else {
+    const double re = qr * pr + qi * pim;            // Re(q conj P)
+    const double im = qi * pr - qr * pim;            // Im(q conj P)
+    acc.num += w * re;
+    acc.gx += w * (-ax * im);
+    acc.gy += w * (-ay * im);
+    if (angle) {
+      const double dr = s.dv.x, di = cj ? -(double)s.dv.y : (double)s.dv.y;
+      acc.gpsi += w * (qr * dr + qi * di);
+    }
+    acc.sqref += w * (qr * qr + qi * qi);
+  }
+  const double r2 = shr * cs - shi * ss;
+  const double i2 = shr * ss + shi * cs;
+  shr = r2;
+  shi = i2;
+}
+
+#ifndef PFTCC_ROT8_UNROLL
+#define PFTCC_ROT8_UNROLL 2
+#endif
+
+// FP64 sums of |P|^2 per (particle, shell): one wavefront per ring.
+__global__ void __launch_bounds__(256) k_shell_sq8(const float2* __restrict__ rings, int nitems, int n,
+                                                   double* __restrict__ out) {
+  const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (it >= nitems) return;
+  const float2* v = rings + (size_t)it * n;
+  double s = 0.0;
+  for (int j = threadIdx.x & 63; j < n; j += 64) s += (double)v[j].x * v[j].x + (double)v[j].y * v[j].y;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) out[it] = s;
+}
+
+// The shells run in groups of U: the loads of a group are issued before its arithmetic, so each
+// thread keeps U shells of operands in flight.  MODE: 0 CC k-weighted, 1 CC match_filt, 2 Euclid
+// (equal to a.mode; a template argument so that each form's sums stay in registers).
+template <int U, int MODE>
+__global__ void __launch_bounds__(512) k_rot8(Rot8Args a) {
   const int item = xcd_remap(blockIdx.x, gridDim.x);
   if (item >= a.npairs) return;
   const int n = a.n;
@@ -517,9 +614,14 @@ __global__ void __launch_bounds__(1024) k_rot8(Rot8Args a) {
   const float2* ptc = a.ptcls + (size_t)ip * a.nk * n;
   const float2* dpt = a.dptcls ? a.dptcls + (size_t)ip * a.nk * n : nullptr;
   const float* ctf = a.ctf ? a.ctf + (size_t)ip * a.nk * n : nullptr;
-  // accumulators: num, gx, gy, gpsi, sqref, sqptcl
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const bool angle = dpt != nullptr;
+  Rot8Acc acc;
+  // k-weighted particle norm from the per-shell FP64 sums, one shell per thread
+  if (MODE == 0)
+    for (int kk = threadIdx.x; kk < a.kcorr; kk += blockDim.x)
+      acc.sqptcl += (double)(a.kfrom + kk) * a.sqp8[(size_t)ip * a.nk + kk];
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const float snj = a.sn32[j], ncnj = a.ncn32[j];
     double sa, ca, ss, cs;
     sincos(a.argx[j] * sx + a.argy[j] * sy, &sa, &ca);
     sincos(a.one_x[j] * sx + a.one_y[j] * sy, &ss, &cs);
@@ -528,58 +630,26 @@ __global__ void __launch_bounds__(1024) k_rot8(Rot8Args a) {
     bool cj = false;
     if (t >= 2 * n) t -= 2 * n;
     if (t >= n) { t -= n; cj = true; }
-    for (int kk = 0; kk < a.kcorr; ++kk) {
-      const double w = (a.mode == 0 || a.mode == 2) ? (double)(a.kfrom + kk) : 1.0;
-      const float2 rv = ref[(size_t)kk * n + j];
-      double c = ctf ? (double)ctf[(size_t)kk * n + j] : 1.0;
-      if (a.refinv) {
-        const int row = a.optrow ? a.optrow[ip] : iref;
-        c *= a.ofac[(size_t)row * a.kcorr + kk] *
-             a.refinv[((size_t)(even ? 0 : 1) * a.nrefs + iref) * a.kcorr + kk];
-      }
-      // q = ref * ctf * shmat   ((pft_ref*ctfmats)*shmat, :1709)
-      const double ar = (double)rv.x * c, ai = (double)rv.y * c;
-      const double qr = ar * shr - ai * shi;
-      const double qi = ar * shi + ai * shr;
-      const float2 pv = ptc[(size_t)kk * n + t];
-      const double pr = pv.x, pim = cj ? -(double)pv.y : (double)pv.y;
-      const double ax = a.argx[(size_t)kk * n + j], ay = a.argy[(size_t)kk * n + j];
-      if (a.mode == 2) {
-        const double s2w = a.sigma2w[(size_t)ip * a.nk + kk];
-        const double dr = qr - pr, di = qi - pim;
-        acc[0] -= w * 0.5 * (dr * dr + di * di) * s2w;
-        // corrk(i*a*q) - Re sum (i a q) conj(q);  Re((i a q) conj P) = -a Im(q conj P)
-        const double imqp = qi * pr - qr * pim;
-        acc[1] += w * (-ax * imqp) * s2w;
-        acc[2] += w * (-ay * imqp) * s2w;
-      } else {
-        const double re = qr * pr + qi * pim;            // Re(q conj P)
-        const double im = qi * pr - qr * pim;            // Im(q conj P)
-        acc[0] += w * re;
-        acc[1] += w * (-ax * im);
-        acc[2] += w * (-ay * im);
-        if (dpt) {
-          const float2 dv = dpt[(size_t)kk * n + t];
-          const double dr = dv.x, di = cj ? -(double)dv.y : (double)dv.y;
-          acc[3] += w * (qr * dr + qi * di);
-        }
-        acc[4] += w * (qr * qr + qi * qi);
-        if (a.mode == 0) {
-          const float2 pj = ptc[(size_t)kk * n + j];
-          acc[5] += w * ((double)pj.x * pj.x + (double)pj.y * pj.y);
-        }
-      }
-      const double r2 = shr * cs - shi * ss;
-      const double i2 = shr * ss + shi * cs;
-      shr = r2;
-      shi = i2;
+    int kk = 0;
+    for (; kk + U <= a.kcorr; kk += U) {
+      Rot8Shell s[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) rot8_load<MODE>(a, s[u], kk + u, j, t, ip, iref, even, ref, ptc, dpt, ctf);
+#pragma unroll
+      for (int u = 0; u < U; ++u) rot8_step<MODE>(a, s[u], kk + u, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
+    }
+    for (; kk < a.kcorr; ++kk) {
+      Rot8Shell s;
+      rot8_load<MODE>(a, s, kk, j, t, ip, iref, even, ref, ptc, dpt, ctf);
+      rot8_step<MODE>(a, s, kk, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
     }
   }
   // wavefront sums, then the waves' partial sums in a fixed order
   __shared__ double red[6][16];
+  const double accv[6] = {acc.num, acc.gx, acc.gy, acc.gpsi, acc.sqref, acc.sqptcl};
 #pragma unroll
   for (int v = 0; v < 6; ++v) {
-    double x = acc[v];
+    double x = accv[v];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
     if ((threadIdx.x & 63) == 0) red[v][threadIdx.x >> 6] = x;
