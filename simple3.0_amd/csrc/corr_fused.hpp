@@ -55,9 +55,9 @@ struct Fp {
   static constexpr bool fusable = np >= 2 && plan.r[0] == 4 && N % 4 == 0;
 };
 
-// In-place Stockham pass P of F transforms held in LDS (stride N); each thread stages its
-// butterflies' inputs in registers between two barriers.
-template <int N, int F, int NT, int P>
+// In-place Stockham pass P of F transforms held in LDS (transform f at buf + f S); each thread
+// stages its butterflies' inputs in registers between two barriers.
+template <int N, int F, int NT, int P, int S = N>
 __device__ __forceinline__ void lds_pass(float2* __restrict__ buf, const float2* __restrict__ tw2, int tid) {
   constexpr int R = Fp<N>::R(P);
   constexpr int Ns = Fp<N>::Ns(P);
@@ -72,7 +72,7 @@ __device__ __forceinline__ void lds_pass(float2* __restrict__ buf, const float2*
     if (TASKS % NT == 0 || task < TASKS) {
       const int f = task / nb, j = task % nb;
 #pragma unroll
-      for (int t = 0; t < R; ++t) v[c][t] = buf[f * N + j + t * nb];
+      for (int t = 0; t < R; ++t) v[c][t] = buf[f * S + j + t * nb];
     }
   }
   __syncthreads();
@@ -89,17 +89,17 @@ __device__ __forceinline__ void lds_pass(float2* __restrict__ buf, const float2*
       Dft<R>::run(v[c]);
       const int base = (j / Ns) * Ns * R + jm;
 #pragma unroll
-      for (int t = 0; t < R; ++t) buf[f * N + base + t * Ns] = v[c][t];
+      for (int t = 0; t < R; ++t) buf[f * S + base + t * Ns] = v[c][t];
     }
   }
   __syncthreads();
 }
 
-template <int N, int F, int NT, int P, int PEND>
+template <int N, int F, int NT, int P, int PEND, int S = N>
 __device__ __forceinline__ void lds_passes(float2* buf, const float2* tw2, int tid) {
   if constexpr (P < PEND) {
-    lds_pass<N, F, NT, P>(buf, tw2, tid);
-    lds_passes<N, F, NT, P + 1, PEND>(buf, tw2, tid);
+    lds_pass<N, F, NT, P, S>(buf, tw2, tid);
+    lds_passes<N, F, NT, P + 1, PEND, S>(buf, tw2, tid);
   }
 }
 
@@ -126,10 +126,34 @@ constexpr int fused_W_tuned() {
   return (NT == 256 && N == 308) ? 3 : 1;
 }
 
+#ifndef PFTCC_FUSED_TWL
+#define PFTCC_FUSED_TWL 1      // last-pass twiddles from a [t][g] table (conflict-free reads)
+#endif
+#ifndef PFTCC_FUSED_P1V4
+#define PFTCC_FUSED_P1V4 1     // pass-1 outputs stored as two 16-byte writes
+#endif
+#ifndef PFTCC_FUSED_PAD
+#define PFTCC_FUSED_PAD 0      // extra float2 between the transforms of a tile (even: 16-byte rows)
+#endif
+
+// LDS row stride of one transform of the fused kernel.
+template <int N>
+constexpr int fused_stride() {
+  return N + PFTCC_FUSED_PAD;
+}
+
+// Entries of the last pass's twiddle table: (RL - 1) x NsL.
+template <int N>
+constexpr int fused_twl_len() {
+  constexpr int RL = Fp<N>::R(Fp<N>::np - 1);
+  return PFTCC_FUSED_TWL ? (RL - 1) * (N / RL) : 0;
+}
+
 template <int N, int F>
 constexpr size_t fused_lds_bytes() {
-  // buf[F N] + Wk[N+1] + tw2[2N] float2;  ctfk[N] + sq/off slots 2 F N/4 floats
-  return sizeof(float2) * ((size_t)F * N + (N + 1) + 2 * N) + sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
+  // buf[F N] + Wk[N+1] + tw2[2N] + twL float2;  ctfk[N] + sq/off slots 2 F N/4 floats
+  return sizeof(float2) * ((size_t)F * fused_stride<N>() + (N + 1) + 2 * N + fused_twl_len<N>()) +
+         sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
 }
 
 // Value select (a conditional on array lvalues would select an address and force the array
@@ -138,10 +162,12 @@ __device__ __forceinline__ float2 vsel(bool c, float2 a, float2 b) {
   return make_float2(c ? a.x : b.x, c ? a.y : b.y);
 }
 
-// V[m] of the real FFT of length 2N from the packed FFT values A = Zc[m], B = Zc[N-m].
-__device__ __forceinline__ float2 split_bin(float2 A, float2 B, float2 w) {
-  const float2 sum = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y - B.y));
-  const float2 dif = make_float2(0.5f * (A.x - B.x), 0.5f * (A.y + B.y));
+// 2 V[m] of the real FFT of length 2N from the packed FFT values A = Zc[m], B = Zc[N-m].  The
+// factor 1/2 of the split is applied to the particle spectrum instead (W_k / 2 when staged):
+// scaling by powers of two is exact, so conj(2V) (W/2) rounds exactly as conj(V) W.
+__device__ __forceinline__ float2 split_bin2(float2 A, float2 B, float2 w) {
+  const float2 sum = make_float2(A.x + B.x, A.y - B.y);
+  const float2 dif = make_float2(A.x - B.x, A.y + B.y);
   const float2 wd = cmul(w, dif);
   return make_float2(sum.x + wd.y, sum.y - wd.x);
 }
@@ -170,11 +196,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   constexpr int T1 = F * Q;
   constexpr int TPT1 = (T1 + NT - 1) / NT;
 
-  extern __shared__ float2 smem[];
+  extern __shared__ __align__(16) float2 smem[];
   float2* buf = smem;                        // [F][N]
-  float2* Wk = buf + F * N;       // [N+1]
+  constexpr int S = fused_stride<N>();       // transform stride in buf
+  static_assert(S % 2 == 0, "16-byte aligned rows");
+  float2* Wk = buf + F * S;                  // [N+1]  W_k / 2
   float2* tw2 = Wk + (N + 1);                // [2N]   exp(-i pi a / N)
-  float* ctfk = reinterpret_cast<float*>(tw2 + 2 * N);   // [N]
+  float2* twL = tw2 + 2 * N;                 // [RL-1][NsL]  twL[(t-1) NsL + g] = tw2[2 g t]
+  float* ctfk = reinterpret_cast<float*>(twL + fused_twl_len<N>());   // [N]
   float* sqs = ctfk + N;                     // [F][Q]
   float* offs = sqs + F * Q;                 // [F][Q]
   __shared__ float sq[F], off[F];
@@ -191,6 +220,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   const int nk = a.nk;
 
   for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
+  if constexpr (PFTCC_FUSED_TWL)
+    for (int i = tid; i < (RL - 1) * NsL; i += NT) {
+      const int t = i / NsL + 1, g = i - (t - 1) * NsL;
+      twL[i] = a.tw2g[2 * g * t];
+    }
 
   // pass-1 slots: pair, reference and shift rows are fixed for all shells
   float sqp[TPT1], offp[TPT1];
@@ -232,7 +266,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     const size_t prow = (size_t)ip * nk + kk;
     for (int j = tid; j < N; j += NT) ctfk[j] = a.ctf ? a.ctf[prow * N + j] : 1.0f;
     if (docorr)
-      for (int m = tid; m <= N; m += NT) Wk[m] = a.memo[prow * (N + 1) + m];
+      for (int m = tid; m <= N; m += NT) {
+        const float2 w = a.memo[prow * (N + 1) + m];
+        Wk[m] = make_float2(0.5f * w.x, 0.5f * w.y);
+      }
     const float wk = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
     __syncthreads();
     // ---- pass 1: z from q = ref * |CTF| [* shift] [* shell scale]; radix-4 DFT (Ns = 1) ----
@@ -283,14 +320,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       }
       if (docorr) {
         Dft<4>::run(y);
+        if constexpr (PFTCC_FUSED_P1V4) {
+          float4* dst = reinterpret_cast<float4*>(buf + f * S + 4 * j);   // 16-byte aligned: S even
+          dst[0] = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
+          dst[1] = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
+        } else {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) buf[f * N + 4 * j + t] = y[t];
+          for (int t = 0; t < 4; ++t) buf[f * S + 4 * j + t] = y[t];
+        }
       }
     }
     if (!docorr) continue;
     __syncthreads();
     // ---- middle passes in LDS ----
-    lds_passes<N, F, NT, 1, NP - 1>(buf, tw2, tid);
+    lds_passes<N, F, NT, 1, NP - 1, S>(buf, tw2, tid);
     // ---- last pass (radix RL, Ns = NsL) on mirrored butterflies + split + conj(V) W_k ----
 #pragma unroll
     for (int c = 0; c < TPTL; ++c) {
@@ -301,15 +344,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       const int j2 = NsL - g;
       float2 y1[RL], y2[RL];
 #pragma unroll
-      for (int t = 0; t < RL; ++t) y1[t] = buf[f * N + g + t * NsL];
+      for (int t = 0; t < RL; ++t) y1[t] = buf[f * S + g + t * NsL];
 #pragma unroll
-      for (int t = 1; t < RL; ++t) y1[t] = cmul(y1[t], tw2[2 * g * t]);
+      for (int t = 1; t < RL; ++t) y1[t] = cmul(y1[t], PFTCC_FUSED_TWL ? twL[(t - 1) * NsL + g] : tw2[2 * g * t]);
       Dft<RL>::run(y1);
       if (pair) {
 #pragma unroll
-        for (int t = 0; t < RL; ++t) y2[t] = buf[f * N + j2 + t * NsL];
+        for (int t = 0; t < RL; ++t) y2[t] = buf[f * S + j2 + t * NsL];
 #pragma unroll
-        for (int t = 1; t < RL; ++t) y2[t] = cmul(y2[t], tw2[2 * j2 * t]);
+        for (int t = 1; t < RL; ++t) y2[t] = cmul(y2[t], PFTCC_FUSED_TWL ? twL[(t - 1) * NsL + j2] : tw2[2 * j2 * t]);
         Dft<RL>::run(y2);
       } else {
 #pragma unroll
@@ -320,7 +363,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
         const int m = g + t * NsL;
         const float2 b0 = y1[(RL - t) % RL], b1 = y2[RL - 1 - t], b2 = y1[RL - 1 - t];
         const float2 B = vsel(g == 0, b0, vsel(pair, b1, b2));
-        const float2 V = split_bin(y1[t], B, tw2[m]);
+        const float2 V = split_bin2(y1[t], B, tw2[m]);
         const float2 p = cmulcj(V, Wk[m]);
         acc[c][t].x = fmaf(wk, p.x, acc[c][t].x);
         acc[c][t].y = fmaf(wk, p.y, acc[c][t].y);
@@ -329,14 +372,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
 #pragma unroll
         for (int t = 0; t < RL; ++t) {
           const int m = j2 + t * NsL;
-          const float2 V = split_bin(y2[t], y1[RL - 1 - t], tw2[m]);
+          const float2 V = split_bin2(y2[t], y1[RL - 1 - t], tw2[m]);
           const float2 p = cmulcj(V, Wk[m]);
           acc[c][RL + t].x = fmaf(wk, p.x, acc[c][RL + t].x);
           acc[c][RL + t].y = fmaf(wk, p.y, acc[c][RL + t].y);
         }
       }
       if (g == 0) {
-        const float2 V = split_bin(y1[0], y1[0], tw2[N]);
+        const float2 V = split_bin2(y1[0], y1[0], tw2[N]);
         const float2 p = cmulcj(V, Wk[N]);
         acc[c][2 * RL].x = fmaf(wk, p.x, acc[c][2 * RL].x);
         acc[c][2 * RL].y = fmaf(wk, p.y, acc[c][2 * RL].y);
@@ -372,13 +415,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
         X.y = 0.f;
         Xn.y = 0.f;
       }
-      buf[f * N + m] = unsplit_bin_conj(X, Xn, cconj(tw2[m]));
+      buf[f * S + m] = unsplit_bin_conj(X, Xn, cconj(tw2[m]));
     }
     if (pair) {
 #pragma unroll
       for (int t = 0; t < RL; ++t) {
         const int m = j2 + t * NsL;
-        buf[f * N + m] = unsplit_bin_conj(acc[c][RL + t], acc[c][RL - 1 - t], cconj(tw2[m]));
+        buf[f * S + m] = unsplit_bin_conj(acc[c][RL + t], acc[c][RL - 1 - t], cconj(tw2[m]));
       }
     }
   }
@@ -403,14 +446,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     }
   }
   // ---- inverse FFT of every pair (all passes in LDS) ----
-  lds_passes<N, F, NT, 0, NP>(buf, tw2, tid);
+  lds_passes<N, F, NT, 0, NP, S>(buf, tw2, tid);
   const float inv4n = 1.0f / (4.0f * (float)N);
   unsigned long long bestkey = 0ull;
   for (int idx = tid; idx < F * N; idx += NT) {
     const int f = idx / N, j = idx % N;
     const int P = tile * F + f;
     if (P >= npairs) continue;
-    const float2 y = buf[f * N + j];
+    const float2 y = buf[f * S + j];
     float x0 = y.x, x1 = -y.y;
     if (a.mode == 1) {
       const float o = off[f] + pscal;
