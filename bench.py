@@ -332,6 +332,71 @@ def polarizer_form(nimg=10000, box=200, reps=3):
     }
 
 
+def prep_form(nimg=10000, box=200, reps=3):
+    """prepimg4align (noise_norm, fft, shift, CTF phase flip, ifft, clip, soft mask, fft) of
+    device-resident 200 x 200 particle images, then polarize into particle slots (box 200,
+    kfromto [2,60], pftsz 308)."""
+    import pftcc
+    from pftcc import Params, PolarFTCorrCalc
+    from pftcc import synth
+
+    kfromto = (2, 60)
+    pftsz = pftcc.magic_pftsz(98)
+    lib = PolarFTCorrCalc().new(1, (1, nimg), Params(boxmatch=box, kfromto=kfromto, pftsz=pftsz, ctf=False))
+    lib.init_polarizer()
+    rng = np.random.default_rng(4)
+    chunk = 1000
+    yy, xx = np.mgrid[0:box, 0:box].astype(np.float32)
+    base = rng.standard_normal((chunk, box, box)).astype(np.float32)
+    base += 3.0 * np.exp(-((xx - box / 2) ** 2 + (yy - box / 2) ** 2) / (2 * 20.0 ** 2)).astype(np.float32)
+    d_img = synth.DeviceBuffer(nimg * box * box * 4)
+    hc = box // 2 + 1
+    d_cm = synth.DeviceBuffer(nimg * box * hc * 8)
+    S = synth.synth_lib()
+    for c0 in range(0, nimg, chunk):
+        S.synth_h2d(d_img.ptr + c0 * box * box * 4, base.ctypes.data, base.nbytes)
+    shifts = rng.uniform(-5.0, 5.0, (nimg, 2)).astype(np.float32)
+
+    class _Ctf:
+        smpd, kv, cs, fraca, angast, phshift, phaseplate = 1.0, 300.0, 2.7, 0.1, 30.0, 0.0, False
+
+        def __init__(self, dfx, dfy):
+            self.dfx, self.dfy = dfx, dfy
+
+    df = rng.uniform(0.8, 3.5, nimg)
+    ctfs = pftcc.ctf_params_array([_Ctf(float(d), float(d) + 0.05) for d in df])
+    ids = np.arange(1, nimg + 1, dtype=np.int32)
+    msk = 80.0
+    lib.prep_imgs_dev(16, d_img.ptr, shifts[:16], box, msk, d_cm.ptr, ctfflag=1, ctfparams=ctfs)   # warm-up
+    S.synth_device_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.prep_imgs_dev(nimg, d_img.ptr, shifts, box, msk, d_cm.ptr, ctfflag=1, ctfparams=ctfs)
+    S.synth_device_sync()
+    wall = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.prep_imgs_dev(nimg, d_img.ptr, shifts, box, msk, d_cm.ptr, ctfflag=1, ctfparams=ctfs)
+        lib.polarize_dev(ids, d_cm.ptr, isptcl=True)
+    S.synth_device_sync()
+    wall_chain = (time.perf_counter() - t0) / reps
+    d_img.close()
+    d_cm.close()
+    lib.kill()
+    bytes_img = 4.0 * box * box + 8.0 * box * hc        # the image in, its transform out
+    achieved = bytes_img * nimg / wall / 1e9
+    return {
+        "workload": f"prepimg4align of {nimg} device-resident {box}x{box} particle images (noise_norm, fft, "
+                    f"shift, CTF phase flip, ifft, clip, soft mask r {msk:.0f}, fft), then polarize",
+        "value": nimg / wall, "unit": "images per s", "ms_per_call": 1e3 * wall,
+        "with_polarize": {"value": nimg / wall_chain, "unit": "images per s", "ms_per_call": 1e3 * wall_chain},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "bytes_per_image": bytes_img,
+                     "note": "algorithmic bytes (image in, transform out) over the whole call timed from the "
+                             "host; the call streams the batch through HBM 8 times (8 kernels)"},
+    }
+
+
 def precision_same_workload(lib, w, shifts, plain_value, nptcls, sample=1000, ntop=16):
     """The split-BF16 MFMA form (with the 169-shift grid as extra operand rows and top-4 exact
     FP64 rescoring) on the main workload's references and particles, next to the plain form."""
@@ -426,8 +491,8 @@ def main():
     ap.add_argument("--mfma-ptcls", type=int, default=10000,
                     help="particles of the precision-form run on the main workload")
     ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-kernel line")
-    ap.add_argument("--legs", default="plain,precision,gradient,shsrch,polarizer",
-                    help="comma list of plain, precision, gradient, shsrch, polarizer (profiling runs select one)")
+    ap.add_argument("--legs", default="plain,precision,gradient,shsrch,polarizer,prep",
+                    help="comma list of plain, precision, gradient, shsrch, polarizer, prep (profiling runs select one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device copy (k_copy), the reference for HBM traffic counters")
     args = ap.parse_args()
@@ -459,6 +524,8 @@ def main():
             out["shift_search"] = shift_search_form()
         if "polarizer" in legs:
             out["polarizer"] = polarizer_form()
+        if "prep" in legs:
+            out["prep"] = prep_form()
         print(json.dumps(out), flush=True)
         return 0
 
@@ -558,6 +625,8 @@ def main():
         out["shift_search"] = shift_search_form()
     if "polarizer" in legs:
         out["polarizer"] = polarizer_form()
+    if "prep" in legs:
+        out["prep"] = prep_form()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -224,6 +224,33 @@ int pftcc_polarize(pftcc_t* h, int32_t count, const int32_t* img_inds, const flo
 int pftcc_polarize_dev(pftcc_t* h, int32_t count, const int32_t* img_inds, const void* d_cmats, int32_t isptcl,
                        int32_t iseven, const int8_t* mask, int32_t nmask);
 
+/* Particle preparation ahead of the polarizer: prepimg4align (simple_strategy2D3D_common.f90:
+ * 316-366) for count 2-D images at once, replacing the per-image host work before
+ * polarize(isptcl = 1).  Per image: noise_norm over the background of the disc of radius msk
+ * (build_glob%lmsk), fft, shift2Dserial(-x, -y) when |x| or |y| > 0.001, CTF phase flip
+ * (ctfflag == CTFFLAG_YES, apply_serial 'flip' with ctfp[i]), ifft, real-space clip to
+ * boxmatch, soft mask (zero_background + cosine edge of radius msk; inner/width with
+ * l_innermsk; focusmsk instead of msk with l_focusmsk), div_by_instrfun when griddev, fft and
+ * the factor boxmatch^2/box^2.  imgs: real images [count][box][box] (Fortran rmat(box, box)
+ * order, x fastest); shifts: [count][2] particle (x, y); ctfp: [count] or NULL when ctfflag !=
+ * CTFFLAG_YES.  cmats: [count] transforms in the layout polarize reads (Fortran
+ * cmat(boxmatch/2+1, boxmatch)).  boxmatch must equal the handle's.  The _dev form takes device
+ * imgs and cmats (shifts and ctfp stay on the host). */
+typedef struct pftcc_prep_cfg {
+  int32_t box;        /* input image size, even                                          */
+  int32_t boxmatch;   /* output transform size, even, <= box; the handle's boxmatch         */
+  float msk;          /* mask radius in pixels (params%msk)                                 */
+  float inner;        /* l_innermsk: inner mask radius (0: off)                             */
+  float width;        /* inner mask width (params%width)                                    */
+  float focusmsk;     /* l_focusmsk: soft-mask radius used instead of msk (0: off)          */
+  int32_t ctfflag;    /* CTFFLAG_NO 0, CTFFLAG_YES 1 (phase flip), CTFFLAG_FLIP 2 (simple_defs) */
+  int32_t griddev;    /* params%griddev == 'yes'                                            */
+} pftcc_prep_cfg;
+int pftcc_prep_imgs(pftcc_t* h, int32_t count, const float* imgs, const pftcc_prep_cfg* cfg, const float* shifts,
+                    const pftcc_ctfparams* ctfp, float* cmats);
+int pftcc_prep_imgs_dev(pftcc_t* h, int32_t count, const void* d_imgs, const pftcc_prep_cfg* cfg,
+                        const float* shifts, const pftcc_ctfparams* ctfp, void* d_cmats);
+
 /* ------------------------------------------------------------ gradient shift search -- */
 /* pftcc_shsrch_grad%minimize (simple_pftcc_shsrch_grad.f90:254-338) for n (reference,
  * particle) pairs at once (set_indices(irefs[i], iptcls[i]) then minimize(irots[i])).  Each pair

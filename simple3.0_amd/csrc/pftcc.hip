@@ -23,6 +23,7 @@
 #include "mfma_search.hpp"
 #include "matchfilt.hpp"
 #include "polarizer.hpp"
+#include "prep.hpp"
 #include "shsrch.hpp"
 
 using namespace pfk;
@@ -1501,6 +1502,194 @@ int pftcc_polarize_dev(pftcc_t* h, int32_t count, const int32_t* img_inds, const
                        int32_t iseven, const int8_t* mask, int32_t nmask) {
   if (!h) return PFTCC_E_ARG;
   return polarize_impl(h, count, img_inds, d_cmats, true, isptcl, iseven, mask, nmask);
+}
+
+}  // extern "C"
+
+namespace {
+
+// constructor / init of the ctf object (simple_ctf.f90:54-81, 522-525) in real(sp) arithmetic;
+// gate_phaseplate: create_polar_absctfmats adds the phase shift only with a phase plate, while
+// apply_serial passes ctfparms%phshift as given.
+KCtfConst make_kctf(const pftcc_ctfparams& p, bool gate_phaseplate) {
+  KCtfConst c{};
+  const float a = 1000.0f * p.kv;
+  const float wl = 12.26f / std::sqrt(a + (0.9784f * (a * a)) / 1.0e6f);
+  c.wl = wl / p.smpd;
+  c.cs = (p.cs * 1.0e7f) / p.smpd;
+  c.ampc = std::atan(p.fraca / std::sqrt(1.0f - p.fraca * p.fraca));
+  c.dfx = (p.dfx * 1.0e4f) / p.smpd;
+  c.dfy = (p.dfy * 1.0e4f) / p.smpd;
+  c.angast = (p.angast / 180.0f) * kPi32;
+  c.phshift = (!gate_phaseplate || p.l_phaseplate) ? p.phshift : 0.0f;
+  c.active = 1;
+  return c;
+}
+
+// tw2 table of a length-L transform: exp(-i pi a / L), a in [0, 2L), FP64 rounded to FP32.
+std::vector<float2> twiddles(int L) {
+  std::vector<float2> t(2 * L);
+  for (int a = 0; a < 2 * L; ++a) {
+    const double ang = -kDPi * a / L;
+    t[a] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  return t;
+}
+
+// Fortran nint of an FP32 value
+int nint32(float x) { return (int)(x >= 0.f ? std::floor(x + 0.5f) : -std::floor(-x + 0.5f)); }
+
+// gen_instrfun_img (simple_gridding.f90:18-55) with the polarizer's window (kbinterpol%instr,
+// simple_kbinterpol.f90:92-109, 211-225): w(i) for i = 1..ldim, sinh(a)/a evaluated in FP64.
+std::vector<float> instrfun_1d(int ldim) {
+  const double whalf = 1.5, W = 2.0 * whalf, beta = 7.4, piW = kDPi * W;
+  const double thresh = beta / piW - 1e-20;
+  std::vector<float> w(ldim);
+  const int center = ldim / 2 + 1;
+  for (int i = 1; i <= ldim; ++i) {
+    const float x = (float)(i - center) / (float)ldim;
+    double r = 1.0;
+    if (std::fabs((double)x) < thresh) {
+      const double a2 = std::sqrt(beta * beta - (piW * x) * (piW * x));
+      r = a2 < 1e-10 ? 1.0 : std::sinh(a2) / a2;
+    }
+    w[i - 1] = (float)r;
+  }
+  return w;
+}
+
+int prep_impl(pftcc* h, int count, const void* imgs, bool dev, const pftcc_prep_cfg* cfg, const float* shifts,
+              const pftcc_ctfparams* ctfp, void* cmats) {
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (count < 1 || !imgs || !cmats || !cfg) return fail(h, PFTCC_E_ARG, "prep_imgs: no images");
+  const int B = cfg->box, M = cfg->boxmatch;
+  if (B < 8 || M < 8 || (B & 1) || (M & 1) || M > B)
+    return fail(h, PFTCC_E_ARG, "prep_imgs: box %d / boxmatch %d must be even, >= 8, boxmatch <= box", B, M);
+  if (M != h->ldim) return fail(h, PFTCC_E_ARG, "prep_imgs: boxmatch %d differs from the handle's %d", M, h->ldim);
+  const FftPlan pB = make_plan(B), pM = make_plan(M);
+  if (pB.np < 0 || pM.np < 0)
+    return fail(h, PFTCC_E_UNSUPPORTED, "prep_imgs: box %d or boxmatch %d has a prime factor above 13", B, M);
+  const size_t lds_rows = sizeof(float2) * ((size_t)2 * PREP_RP * B + 2 * B);
+  const size_t lds_cols = sizeof(float2) * ((size_t)2 * PREP_CP * B + 2 * B);
+  if (lds_rows > kMaxLds || lds_cols > kMaxLds) return fail(h, PFTCC_E_UNSUPPORTED, "prep_imgs: box %d too large", B);
+  if (!(cfg->msk > 0.f)) return fail(h, PFTCC_E_ARG, "prep_imgs: mask radius must be positive");
+  if (cfg->ctfflag < 0 || cfg->ctfflag > 2) return fail(h, PFTCC_E_ARG, "prep_imgs: unsupported CTF flag %d", cfg->ctfflag);
+  const bool flip = cfg->ctfflag == 1;
+  if (flip && !ctfp) return fail(h, PFTCC_E_ARG, "prep_imgs: CTF parameters required for the phase flip");
+  if (!shifts) return fail(h, PFTCC_E_ARG, "prep_imgs: shifts required");
+  // soft-mask radius and window (mask, simple_image.f90:6208-6240; prepimg4align :354-360)
+  const float soft = cfg->inner > 0.f ? cfg->msk : (cfg->focusmsk > 0.f ? cfg->focusmsk : cfg->msk);
+  const int minlen = std::min(nint32(2.0f * (soft + 6.0f)), M);
+  const float shconst = kPi32 / ((float)B / 2.0f);
+  const float ratio = (float)(M * M) / (float)(B * B);
+  const size_t nhB = B / 2 + 1, nhM = M / 2 + 1;
+  const size_t in_img = (size_t)B * B, out_img = (size_t)M * nhM;
+  // chunk so that the work buffers stay within ~512 MB
+  const size_t per = in_img * 4 * (dev ? 0 : 1) + (size_t)B * nhB * 8 + (size_t)M * M * 4 + out_img * 8 * (dev ? 0 : 1) +
+                     8 + 8 + sizeof(KCtfConst);
+  const int C = (int)std::max<size_t>(1, std::min<size_t>(count, ((size_t)512 << 20) / per));
+  std::vector<float2> twB = twiddles(B), twM = twiddles(M);
+  std::vector<float> wI;
+  if (cfg->griddev) wI = instrfun_1d(M);
+  int rc;
+  if ((rc = scratch_reserve(h, 2 * aligned(twB.size() * 8) + aligned(twM.size() * 8) + aligned((size_t)M * 4) +
+                                   aligned((size_t)C * 8) * 2 + aligned((size_t)C * sizeof(KCtfConst)) +
+                                   aligned((size_t)C * B * nhB * 8) + aligned((size_t)C * M * M * 4) +
+                                   (dev ? 0 : aligned((size_t)C * in_img * 4) + aligned((size_t)C * out_img * 8)))))
+    return rc;
+  float2* d_twB = take<float2>(h, twB.size());
+  float2* d_twM = take<float2>(h, twM.size());
+  float* d_wI = take<float>(h, M);
+  float2* d_stats = take<float2>(h, C);
+  float* d_sh = take<float>(h, 2 * (size_t)C);
+  KCtfConst* d_ctf = take<KCtfConst>(h, C);
+  float2* d_work = take<float2>(h, (size_t)C * B * nhB);
+  float* d_img2 = take<float>(h, (size_t)C * M * M);
+  float* d_in = dev ? nullptr : take<float>(h, (size_t)C * in_img);
+  float2* d_out = dev ? nullptr : take<float2>(h, (size_t)C * out_img);
+  HIPCHK(h, hipMemcpyAsync(d_twB, twB.data(), twB.size() * 8, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d_twM, twM.data(), twM.size() * 8, hipMemcpyHostToDevice, h->stream));
+  if (cfg->griddev) HIPCHK(h, hipMemcpyAsync(d_wI, wI.data(), (size_t)M * 4, hipMemcpyHostToDevice, h->stream));
+  std::vector<KCtfConst> kc;
+  for (int b0 = 0; b0 < count; b0 += C) {
+    const int nb = std::min(C, count - b0);
+    const float* src;
+    float2* dst;
+    if (dev) {
+      src = static_cast<const float*>(imgs) + (size_t)b0 * in_img;
+      dst = static_cast<float2*>(cmats) + (size_t)b0 * out_img;
+    } else {
+      HIPCHK(h, hipMemcpyAsync(d_in, static_cast<const float*>(imgs) + (size_t)b0 * in_img, (size_t)nb * in_img * 4,
+                               hipMemcpyHostToDevice, h->stream));
+      src = d_in;
+      dst = d_out;
+    }
+    HIPCHK(h, hipMemcpyAsync(d_sh, shifts + 2 * (size_t)b0, (size_t)nb * 8, hipMemcpyHostToDevice, h->stream));
+    if (flip) {
+      kc.resize(nb);
+      for (int i = 0; i < nb; ++i) kc[i] = make_kctf(ctfp[b0 + i], false);
+      HIPCHK(h, hipMemcpyAsync(d_ctf, kc.data(), (size_t)nb * sizeof(KCtfConst), hipMemcpyHostToDevice, h->stream));
+    }
+    hipLaunchKernelGGL(k_prep_stats, dim3(nb), dim3(256), 0, h->stream, src, B, cfg->msk, d_stats);
+    hipLaunchKernelGGL(k_prep_rows_fwd, dim3((B / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256), lds_rows, h->stream, src, B,
+                       (const float2*)d_stats, (const float2*)d_twB, pB, d_work);
+    PrepColArgs ca{};
+    ca.data = d_work;
+    ca.L = B;
+    ca.inverse = 0;
+    ca.post = 1;
+    ca.tw2g = d_twB;
+    ca.plan = pB;
+    ca.shifts = d_sh;
+    ca.shconst = shconst;
+    ca.ctf = flip ? d_ctf : nullptr;
+    ca.ratio = 1.0f;
+    const dim3 gcB((unsigned)((nhB + PREP_CP - 1) / PREP_CP), nb);
+    hipLaunchKernelGGL(k_prep_cols, gcB, dim3(256), lds_cols, h->stream, ca);
+    ca.inverse = 1;
+    ca.post = 0;
+    hipLaunchKernelGGL(k_prep_cols, gcB, dim3(256), lds_cols, h->stream, ca);
+    hipLaunchKernelGGL(k_prep_rows_inv, dim3((M / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256), lds_rows, h->stream,
+                       (const float2*)d_work, B, M, (const float2*)d_twB, pB, d_img2);
+    hipLaunchKernelGGL(k_prep_mask, dim3(nb), dim3(256), sizeof(float) * 4 * M, h->stream, d_img2, M, soft,
+                       cfg->inner, cfg->width, minlen, cfg->griddev ? (const float*)d_wI : nullptr);
+    const size_t lds_rows_m = sizeof(float2) * ((size_t)2 * PREP_RP * M + 2 * M);
+    const size_t lds_cols_m = sizeof(float2) * ((size_t)2 * PREP_CP * M + 2 * M);
+    hipLaunchKernelGGL(k_prep_rows_fwd, dim3((M / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256), lds_rows_m, h->stream,
+                       (const float*)d_img2, M, (const float2*)nullptr, (const float2*)d_twM, pM, dst);
+    PrepColArgs cm{};
+    cm.data = dst;
+    cm.L = M;
+    cm.inverse = 0;
+    cm.post = 2;
+    cm.tw2g = d_twM;
+    cm.plan = pM;
+    cm.ratio = ratio;
+    hipLaunchKernelGGL(k_prep_cols, dim3((unsigned)((nhM + PREP_CP - 1) / PREP_CP), nb), dim3(256), lds_cols_m, h->stream,
+                       cm);
+    HIPCHK(h, hipGetLastError());
+    if (!dev)
+      HIPCHK(h, hipMemcpyAsync(static_cast<float2*>(cmats) + (size_t)b0 * out_img, d_out, (size_t)nb * out_img * 8,
+                               hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pftcc_prep_imgs(pftcc_t* h, int32_t count, const float* imgs, const pftcc_prep_cfg* cfg, const float* shifts,
+                    const pftcc_ctfparams* ctfp, float* cmats) {
+  if (!h) return PFTCC_E_ARG;
+  return prep_impl(h, count, imgs, false, cfg, shifts, ctfp, cmats);
+}
+
+int pftcc_prep_imgs_dev(pftcc_t* h, int32_t count, const void* d_imgs, const pftcc_prep_cfg* cfg,
+                        const float* shifts, const pftcc_ctfparams* ctfp, void* d_cmats) {
+  if (!h) return PFTCC_E_ARG;
+  return prep_impl(h, count, d_imgs, true, cfg, shifts, ctfp, d_cmats);
 }
 
 int pftcc_set_ref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, float re, float im, int32_t iseven) {

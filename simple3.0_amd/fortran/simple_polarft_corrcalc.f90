@@ -16,7 +16,7 @@ module simple_polarft_corrcalc
 use, intrinsic :: iso_c_binding
 implicit none
 
-public :: polarft_corrcalc, pftcc_glob, pftcc_params, ctfparams, OBJFUN_CC, OBJFUN_EUCLID
+public :: polarft_corrcalc, pftcc_glob, pftcc_params, ctfparams, prep_cfg, OBJFUN_CC, OBJFUN_EUCLID
 public :: PREC_BF16, PREC_SPLIT_BF16, PREC_FP8
 private
 
@@ -49,6 +49,13 @@ type, bind(C) :: ctfparams
     real(c_float)      :: dfx = 0., dfy = 0., angast = 0., phshift = 0.
     integer(c_int32_t) :: l_phaseplate = 0
 end type ctfparams
+
+! pftcc_prep_cfg (prepimg4align settings, params_glob fields of simple_strategy2D3D_common.f90:316-366)
+type, bind(C) :: prep_cfg
+    integer(c_int32_t) :: box = 0, boxmatch = 0
+    real(c_float)      :: msk = 0., inner = 0., width = 10., focusmsk = 0.
+    integer(c_int32_t) :: ctfflag = 0, griddev = 0
+end type prep_cfg
 
 type :: polarft_corrcalc
     private
@@ -104,6 +111,7 @@ type :: polarft_corrcalc
     procedure          :: shsrch_grad_batch
     procedure          :: init_polarizer
     procedure          :: polarize
+    procedure          :: prep_imgs
     procedure          :: kill
 end type polarft_corrcalc
 
@@ -358,6 +366,12 @@ interface
         type(c_ptr), value :: h, cmats, mask
         integer(c_int32_t), value :: count, isptcl, iseven, nmask
         integer(c_int32_t), intent(in) :: img_inds(*)
+    end function
+    integer(c_int) function c_prep_imgs(h, count, imgs, cfg, shifts, ctfp, cmats) bind(C, name='pftcc_prep_imgs')
+        import :: c_int, c_int32_t, c_ptr, prep_cfg
+        type(c_ptr), value :: h, imgs, shifts, ctfp, cmats
+        integer(c_int32_t), value :: count
+        type(prep_cfg), intent(in) :: cfg
     end function
     integer(c_int) function c_prep_matchfilt(h, iptcl, iref, irot) bind(C, name='pftcc_prep_matchfilt')
         import :: c_int, c_int32_t, c_ptr
@@ -822,6 +836,24 @@ contains
         call chk(self, c_polarize(self%h, int(size(img_inds),c_int32_t), ids, c_loc(cmats), &
             &merge(1_c_int32_t, 0_c_int32_t, isptcl), merge(1_c_int32_t, 0_c_int32_t, iseven), pm, nm), 'polarize')
     end subroutine polarize
+
+    ! prepimg4align (simple_strategy2D3D_common.f90:316-366) for n particle images at once:
+    ! imgs(box, box, n) real space, shifts(2, n) the particles' (x, y), ctfp(n) their CTF
+    ! parameters (read when cfg%ctfflag == CTFFLAG_YES); cmats(boxmatch/2+1, boxmatch, n) the
+    ! transforms, ready for polarize
+    subroutine prep_imgs( self, imgs, shifts, cfg, cmats, ctfp )
+        class(polarft_corrcalc),   intent(inout) :: self
+        real(sp), target,          intent(in)    :: imgs(:,:,:)
+        real(sp), target,          intent(in)    :: shifts(:,:)
+        type(prep_cfg),            intent(in)    :: cfg
+        complex(sp), target,       intent(inout) :: cmats(:,:,:)
+        type(ctfparams), optional, target, intent(in) :: ctfp(:)
+        type(c_ptr) :: pc
+        pc = c_null_ptr
+        if( present(ctfp) ) pc = c_loc(ctfp)
+        call chk(self, c_prep_imgs(self%h, int(size(imgs,3),c_int32_t), c_loc(imgs), cfg, c_loc(shifts), pc, &
+            &c_loc(cmats)), 'prep_imgs')
+    end subroutine prep_imgs
 
     ! BATCHED FAST PATHS (no reference counterpart: the ref loop of strategy3D_greedy%srch,
     ! simple_strategy3D_greedy.f90:60-77, moved onto the device)

@@ -46,6 +46,26 @@ class CtfParamsC(C.Structure):
                 ("phshift", C.c_float), ("l_phaseplate", C.c_int32)]
 
 
+class PrepCfgC(C.Structure):
+    _fields_ = [("box", C.c_int32), ("boxmatch", C.c_int32), ("msk", C.c_float), ("inner", C.c_float),
+                ("width", C.c_float), ("focusmsk", C.c_float), ("ctfflag", C.c_int32), ("griddev", C.c_int32)]
+
+
+def _ctf_array(ctfparams):
+    """ctypes array of pftcc_ctfparams from objects with smpd, kv, cs, fraca, dfx, dfy, angast[,
+    phshift, phaseplate] attributes (an array built here passes through unchanged)."""
+    if isinstance(ctfparams, C.Array):
+        return ctfparams
+    arr = (CtfParamsC * len(ctfparams))()
+    for i, p in enumerate(ctfparams):
+        arr[i] = CtfParamsC(p.smpd, p.kv, p.cs, p.fraca, p.dfx, p.dfy, p.angast,
+                            getattr(p, "phshift", 0.0), int(getattr(p, "phaseplate", False)))
+    return arr
+
+
+ctf_params_array = _ctf_array
+
+
 # (name, restype, argtypes) of every entry point declared in include/pftcc.h
 _P = C.c_void_p
 _I = C.c_int32
@@ -113,6 +133,8 @@ SIGNATURES = [
     ("pftcc_init_polarizer", _I, [_P, _I]),
     ("pftcc_polarize", _I, [_P, _I, _PI, _PF, _I, _I, C.POINTER(C.c_int8), _I]),
     ("pftcc_polarize_dev", _I, [_P, _I, _PI, _P, _I, _I, C.POINTER(C.c_int8), _I]),
+    ("pftcc_prep_imgs", _I, [_P, _I, _PF, C.POINTER(PrepCfgC), _PF, C.POINTER(CtfParamsC), _PF]),
+    ("pftcc_prep_imgs_dev", _I, [_P, _I, _P, C.POINTER(PrepCfgC), _PF, C.POINTER(CtfParamsC), _P]),
     ("pftcc_search_mfma", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _I, _I, _PF, _PI, _PI, _PI]),
     ("pftcc_gencorrs_mfma_batch", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _I, _PF]),
     ("pftcc_get_timing_channel", _I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
@@ -318,11 +340,7 @@ class PolarFTCorrCalc:
     def create_polar_absctfmats(self, ctfparams):
         """ctfparams: sequence (length pto-pfrom+1) of objects with smpd, kv, cs, fraca, dfx,
         dfy, angast[, phshift, phaseplate] attributes."""
-        arr = (CtfParamsC * len(ctfparams))()
-        for i, p in enumerate(ctfparams):
-            arr[i] = CtfParamsC(p.smpd, p.kv, p.cs, p.fraca, p.dfx, p.dfy, p.angast,
-                                getattr(p, "phshift", 0.0), int(getattr(p, "phaseplate", False)))
-        self._chk(self._lib.pftcc_create_polar_absctfmats(self._h, arr), "create_polar_absctfmats")
+        self._chk(self._lib.pftcc_create_polar_absctfmats(self._h, _ctf_array(ctfparams)), "create_polar_absctfmats")
 
     def memoize_ffts(self):
         self._chk(self._lib.pftcc_memoize_ffts(self._h), "memoize_ffts")
@@ -572,6 +590,39 @@ class PolarFTCorrCalc:
                                                1 if iseven else 0,
                                                None if m is None else m.ctypes.data_as(C.POINTER(C.c_int8)),
                                                0 if m is None else m.size), "polarize_dev")
+
+    @staticmethod
+    def _prep_cfg(box, boxmatch, msk, inner, width, focusmsk, ctfflag, griddev):
+        return PrepCfgC(int(box), int(boxmatch), float(msk), float(inner), float(width), float(focusmsk),
+                        int(ctfflag), 1 if griddev else 0)
+
+    def prep_imgs(self, imgs, shifts, box, msk, ctfflag=0, ctfparams=None, inner=0.0, width=10.0, focusmsk=0.0,
+                  griddev=False, boxmatch=None):
+        """prepimg4align (simple_strategy2D3D_common.f90:316-366) of a batch of real images
+        imgs [count][box][box] (float32, Fortran rmat order: x fastest); shifts [count][2] the
+        particles' (x, y); ctfflag 1 (CTFFLAG_YES) phase-flips with ctfparams (one per image).
+        Returns the transforms [count][boxmatch][boxmatch/2+1] complex64 for polarize."""
+        im = np.ascontiguousarray(imgs, np.float32)
+        if im.ndim == 2:
+            im = im[None]
+        n = im.shape[0]
+        sh = np.ascontiguousarray(np.asarray(shifts, np.float32).reshape(n, 2))
+        M = self.params.boxmatch if boxmatch is None else int(boxmatch)
+        out = np.zeros((n, M, M // 2 + 1), np.complex64)
+        cfg = self._prep_cfg(box, M, msk, inner, width, focusmsk, ctfflag, griddev)
+        arr = _ctf_array(ctfparams) if ctfparams is not None else None
+        self._chk(self._lib.pftcc_prep_imgs(self._h, n, im.ctypes.data_as(_PF), C.byref(cfg), _fp(sh), arr,
+                                            out.ctypes.data_as(_PF)), "prep_imgs")
+        return out
+
+    def prep_imgs_dev(self, count, d_imgs, shifts, box, msk, d_cmats, ctfflag=0, ctfparams=None, inner=0.0,
+                      width=10.0, focusmsk=0.0, griddev=False):
+        """prep_imgs on device-resident images and transforms (addresses d_imgs, d_cmats)."""
+        sh = np.ascontiguousarray(np.asarray(shifts, np.float32).reshape(count, 2))
+        cfg = self._prep_cfg(box, self.params.boxmatch, msk, inner, width, focusmsk, ctfflag, griddev)
+        arr = _ctf_array(ctfparams) if ctfparams is not None else None
+        self._chk(self._lib.pftcc_prep_imgs_dev(self._h, int(count), C.c_void_p(d_imgs), C.byref(cfg), _fp(sh), arr,
+                                                C.c_void_p(d_cmats)), "prep_imgs_dev")
 
     def search_mfma(self, iptcls, irefs=None, prec=PFTCC_PREC_SPLIT_BF16, ntop=0, shifts=None):
         """Precision form of search(): (best_cc, best_iref, best_irot), plus best_ishift (0-based)
