@@ -404,6 +404,13 @@ __global__ void __launch_bounds__(256) k_zifft(ZifftArgs a) {
 //  * middle passes in LDS;
 //  * the last pass leaves its outputs in registers, where they are normalised, written (cc_out)
 //    and reduced to per-particle best keys (LDS 64-bit max, order independent).
+// Four consecutive float2 as two 16-byte LDS stores (o 16-byte aligned).
+__device__ __forceinline__ void store4(float2* o, const float2 (&y)[4]) {
+  float4* d = reinterpret_cast<float4*>(o);
+  d[0] = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
+  d[1] = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
+}
+
 template <int N, int F, int NT>
 __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   static_assert(ZG % F == 0, "F divides the Z group");
@@ -417,7 +424,7 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   constexpr int TPT1 = (T1 + NT - 1) / NT;
   constexpr int RL = Fp<N>::R(NP - 1);
   constexpr int NsL = N / RL;
-  extern __shared__ float2 smem[];
+  extern __shared__ __align__(16) float2 smem[];
   float2* buf = smem;                    // [F][N]
   float2* tw2 = buf + F * N;             // [2N]
   __shared__ double den[F];
@@ -473,20 +480,17 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
       y[2] = unsplit_bin_conj(za[c][2], za[c][2], cconj(tw2[2 * Q]));
       y[3] = unsplit_bin_conj(za[c][3], za[c][1], cconj(tw2[3 * Q]));
       Dft<4>::run(y);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) o[t] = y[t];
+      store4(o, y);
     } else {
 #pragma unroll
       for (int t = 0; t < 4; ++t) y[t] = unsplit_bin_conj(za[c][t], zb[c][3 - t], cconj(tw2[g + t * Q]));
       Dft<4>::run(y);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) o[4 * g + t] = y[t];
+      store4(o + 4 * g, y);
       if (2 * g != Q) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) y[t] = unsplit_bin_conj(zb[c][t], za[c][3 - t], cconj(tw2[(t + 1) * Q - g]));
         Dft<4>::run(y);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) o[4 * (Q - g) + t] = y[t];
+        store4(o + 4 * (Q - g), y);
       }
     }
   }
