@@ -239,7 +239,7 @@ def shift_search_form(nptcls=50000, nrefs=500, trs=5.0, cpu_sample=12):
         "pairs": nptcls, "refs": nrefs, "box": box, "kfromto": list(kfromto), "pftsz": pftsz, "trs_px": trs,
         "snr": 0.1, "improved_frac": float(np.mean(rot > 0)),
         "objective_kernel": {"name": "k_rot8", "launches": int(k_n), "ms": k_ms},
-        "callback_kernel": {"name": "k_corr_generic (item mode)", "launches": int(c_n), "ms": c_ms},
+        "callback_kernel": {"name": "k_corr_fused_items", "launches": int(c_n), "ms": c_ms},
         "state_machine_and_host_ms": 1e3 * wall - k_ms - c_ms,
     }
     # CPU: the oracle's restatement on a few of the same pairs (inputs read back from the library)

@@ -494,3 +494,344 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
 }
 
 }  // namespace pfk
+
+namespace pfk {
+
+// ------------------------------------------------------------------- item mode (callback) ---
+// The shift search's in-plane callback: every item is its own (reference, particle, shift)
+// triple, gencorrs(iref, iptcl, shvec) (:1502-1534) + maxloc.  Same arithmetic as
+// k_corr_fused, with per-item particle data: the particle's |CTF| and memoized spectra are read
+// from global memory in pass 1 and the last pass, and each item's shift phases follow
+// gen_shmat_8's FP64 recurrence over the shells (:1098-1120) in LDS, rounded to FP32 as
+// gen_shmat does.  F items per work-group; the best (cc, rotation) key of item P goes to best[P].
+#ifndef PFTCC_ITEMS_FMAX
+#define PFTCC_ITEMS_FMAX 4
+#endif
+#ifndef PFTCC_ITEMS_W
+#define PFTCC_ITEMS_W 1        // minimum wavefronts per SIMD imposed on the register allocation
+#endif
+
+template <int N>
+constexpr int items_F() {
+  // LDS: buf[F N] + tw2[2N] + twL float2, sq/off slots 2 F N/4 floats, phases 2 F N double2
+  for (int F = PFTCC_ITEMS_FMAX; F > 1; F /= 2)
+    if (sizeof(float2) * ((size_t)F * N + 2 * N + fused_twl_len<N>()) + sizeof(float) * 2 * F * (N / 4) + 16 +
+            sizeof(double2) * 2 * (size_t)F * N <= 65536)
+      return F;
+  return 1;
+}
+template <int N, int F>
+constexpr size_t items_lds_bytes() {
+  return sizeof(float2) * ((size_t)F * N + 2 * N + fused_twl_len<N>()) + sizeof(float) * 2 * F * (N / 4) + 16 +
+         sizeof(double2) * 2 * (size_t)F * N;
+}
+
+template <int N, int F, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PFTCC_ITEMS_W))) k_corr_fused_items(CorrArgs a) {
+  static_assert(Fp<N>::fusable, "first radix must be 4");
+  constexpr int NP = Fp<N>::np;
+  constexpr int RL = Fp<N>::R(NP - 1);
+  constexpr int NsL = N / RL;
+  constexpr int G = NsL / 2 + 1;
+  constexpr int TL = F * G;
+  constexpr int TPTL = (TL + NT - 1) / NT;
+  constexpr int NB = 2 * RL + 1;
+  constexpr int Q = N / 4;
+  constexpr int T1 = F * Q;
+  constexpr int TPT1 = (T1 + NT - 1) / NT;
+
+  extern __shared__ __align__(16) float2 smem[];
+  float2* buf = smem;                        // [F][N]
+  float2* tw2 = buf + F * N;                 // [2N]
+  float2* twL = tw2 + 2 * N;                 // [RL-1][NsL]
+  float* sqs = reinterpret_cast<float*>(twL + fused_twl_len<N>());   // [F][Q]
+  float* offs = sqs + F * Q;                                          // [F][Q]
+  float* const fbase = reinterpret_cast<float*>(smem);
+  const size_t shoff = ((size_t)(offs + F * Q - fbase) + 3) & ~(size_t)3;    // 16-byte aligned
+  double2* shph = reinterpret_cast<double2*>(fbase + shoff);                 // [F][N] current phase
+  double2* shst = shph + F * N;                                             // [F][N] shell step
+  __shared__ float sq[F], off[F], pscal[F];
+  __shared__ unsigned long long bestf[F];
+
+  const int tid = threadIdx.x;
+  const int item0 = blockIdx.x * F;
+  const int nk = a.nk;
+
+  for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
+  if constexpr (PFTCC_FUSED_TWL)
+    for (int i = tid; i < (RL - 1) * NsL; i += NT) {
+      const int t = i / NsL + 1, g = i - (t - 1) * NsL;
+      twL[i] = a.tw2g[2 * g * t];
+    }
+  for (int i = tid; i < F * N; i += NT) {
+    const int f = i / N, j = i - f * N;
+    const int P = item0 + f;
+    double2 ph = make_double2(1.0, 0.0), st = make_double2(1.0, 0.0);
+    if (P < a.np) {
+      const double sx = (double)a.item_shift[2 * P], sy = (double)a.item_shift[2 * P + 1];
+      double sn, cs;
+      sincos(a.argx[j] * sx + a.argy[j] * sy, &sn, &cs);
+      ph = make_double2(cs, sn);
+      sincos(a.onex[j] * sx + a.oney[j] * sy, &sn, &cs);
+      st = make_double2(cs, sn);
+    }
+    shph[i] = ph;
+    shst[i] = st;
+  }
+  if (tid < F) bestf[tid] = 0ull;
+
+  // pass-1 slots: item, particle and reference rows
+  float sqp[TPT1], offp[TPT1];
+  const float2* rrow[TPT1];
+  const float* crow[TPT1];
+  int slotref[TPT1], slotip[TPT1];
+  bool slotok[TPT1];
+#pragma unroll
+  for (int c = 0; c < TPT1; ++c) {
+    const int task = tid + c * NT;
+    const int f = task / Q;
+    const int P = item0 + f;
+    slotok[c] = task < T1 && P < a.np;
+    sqp[c] = 0.f;
+    offp[c] = 0.f;
+    rrow[c] = a.refs_even;
+    crow[c] = nullptr;
+    slotref[c] = 0;
+    slotip[c] = 0;
+    if (slotok[c]) {
+      const int ip = a.plist[P], iref = a.rlist[P];
+      slotip[c] = ip;
+      slotref[c] = iref;
+      rrow[c] = (a.iseven[ip] ? a.refs_even : a.refs_odd) + (size_t)iref * nk * N;
+      crow[c] = a.ctf ? a.ctf + (size_t)ip * nk * N : nullptr;
+    }
+  }
+  // last-pass tasks: the item's memoized spectra
+  int lip[TPTL];
+  bool lok[TPTL];
+#pragma unroll
+  for (int c = 0; c < TPTL; ++c) {
+    const int task = tid + c * NT;
+    const int P = item0 + task / G;
+    lok[c] = task < TL && P < a.np;
+    lip[c] = lok[c] ? a.plist[P] : 0;
+  }
+
+  float2 acc[TPTL][NB];
+#pragma unroll
+  for (int c = 0; c < TPTL; ++c)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[c][b] = make_float2(0.f, 0.f);
+
+  const int kloop = a.kcorr > a.knorm ? a.kcorr : a.knorm;
+  for (int kk = 0; kk < kloop; ++kk) {
+    const bool docorr = kk < a.kcorr;
+    const bool donorm = kk < a.knorm;
+    __syncthreads();   // the previous shell's last pass has finished with buf; phases initialised
+    // ---- pass 1: q = ref * |CTF| * shmat (* shell scale); radix-4 DFT ----
+#pragma unroll
+    for (int c = 0; c < TPT1; ++c) {
+      const int task = tid + c * NT;
+      if (!(T1 % NT == 0 || task < T1)) continue;
+      const int f = task / Q, j = task % Q;
+      float2 y[4];
+      if (slotok[c]) {
+        const int ip = slotip[c];
+        const float2* r = rrow[c] + (size_t)kk * N;
+        const float4 ra = *reinterpret_cast<const float4*>(r + 2 * j);
+        const float4 rb = *reinterpret_cast<const float4*>(r + 2 * j + N / 2);
+        float2 c01 = make_float2(1.f, 1.f), c23 = c01;
+        if (crow[c]) {
+          c01 = *reinterpret_cast<const float2*>(crow[c] + (size_t)kk * N + 2 * j);
+          c23 = *reinterpret_cast<const float2*>(crow[c] + (size_t)kk * N + 2 * j + N / 2);
+        }
+        float2 q0 = make_float2(ra.x * c01.x, ra.y * c01.x), q1 = make_float2(ra.z * c01.y, ra.w * c01.y);
+        float2 q2 = make_float2(rb.x * c23.x, rb.y * c23.x), q3 = make_float2(rb.z * c23.y, rb.w * c23.y);
+        // shift phases of angles 2j, 2j+1, 2j+N/2, 2j+N/2+1 (owned by this task): use, then advance
+        double2* ph = shph + f * N;
+        const double2* sp = shst + f * N;
+        const int ang[4] = {2 * j, 2 * j + 1, 2 * j + N / 2, 2 * j + N / 2 + 1};
+        float2* qs[4] = {&q0, &q1, &q2, &q3};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double2 p = ph[ang[u]], s = sp[ang[u]];
+          *qs[u] = cmul(*qs[u], make_float2((float)p.x, (float)p.y));
+          ph[ang[u]] = make_double2(p.x * s.x - p.y * s.y, p.x * s.y + p.y * s.x);
+        }
+        if (a.refinv && docorr) {
+          const int iref = slotref[c];
+          const int row = a.optrow ? a.optrow[ip] : iref;
+          const float sc = a.ofac[(size_t)row * a.kcorr + kk] *
+                           a.refinv[((size_t)(a.iseven[ip] ? 0 : 1) * a.nrefs + iref) * a.kcorr + kk];
+          q0 = cscale(q0, sc);
+          q1 = cscale(q1, sc);
+          q2 = cscale(q2, sc);
+          q3 = cscale(q3, sc);
+        }
+        const float e2 = q0.x * q0.x + q0.y * q0.y + q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y +
+                         q3.x * q3.x + q3.y * q3.y;
+        if (donorm) sqp[c] += e2;
+        if (a.mode == 1 && docorr) offp[c] = fmaf(a.shellw[(size_t)ip * nk + kk], e2, offp[c]);
+        y[0] = make_float2(q0.x + q0.y, q1.x + q1.y);
+        y[1] = make_float2(q2.x + q2.y, q3.x + q3.y);
+        y[2] = make_float2(q0.x - q0.y, q1.x - q1.y);
+        y[3] = make_float2(q2.x - q2.y, q3.x - q3.y);
+      } else {
+        y[0] = y[1] = y[2] = y[3] = make_float2(0.f, 0.f);
+      }
+      if (docorr) {
+        Dft<4>::run(y);
+        float4* dst = reinterpret_cast<float4*>(buf + f * N + 4 * j);
+        dst[0] = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
+        dst[1] = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
+      }
+    }
+    if (!docorr) continue;
+    __syncthreads();
+    lds_passes<N, F, NT, 1, NP - 1>(buf, tw2, tid);
+    // ---- last pass + split + conj(V) (W_k / 2) accumulated in registers ----
+#pragma unroll
+    for (int c = 0; c < TPTL; ++c) {
+      const int task = tid + c * NT;
+      if (!(TL % NT == 0 || task < TL)) continue;
+      const int f = task / G, g = task % G;
+      const bool pair = g != 0 && 2 * g != NsL;
+      const int j2 = NsL - g;
+      const float2* wrow = a.memo + ((size_t)lip[c] * nk + kk) * (N + 1);
+      const float wk = (a.mode == 1 && lok[c]) ? a.shellw[(size_t)lip[c] * nk + kk] : 1.0f;
+      auto W = [&](int m) {
+        const float2 w = lok[c] ? wrow[m] : make_float2(0.f, 0.f);
+        return make_float2(0.5f * w.x, 0.5f * w.y);
+      };
+      float2 y1[RL], y2[RL];
+#pragma unroll
+      for (int t = 0; t < RL; ++t) y1[t] = buf[f * N + g + t * NsL];
+#pragma unroll
+      for (int t = 1; t < RL; ++t) y1[t] = cmul(y1[t], PFTCC_FUSED_TWL ? twL[(t - 1) * NsL + g] : tw2[2 * g * t]);
+      Dft<RL>::run(y1);
+      if (pair) {
+#pragma unroll
+        for (int t = 0; t < RL; ++t) y2[t] = buf[f * N + j2 + t * NsL];
+#pragma unroll
+        for (int t = 1; t < RL; ++t) y2[t] = cmul(y2[t], PFTCC_FUSED_TWL ? twL[(t - 1) * NsL + j2] : tw2[2 * j2 * t]);
+        Dft<RL>::run(y2);
+      } else {
+#pragma unroll
+        for (int t = 0; t < RL; ++t) y2[t] = make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int m = g + t * NsL;
+        const float2 b0 = y1[(RL - t) % RL], b1 = y2[RL - 1 - t], b2 = y1[RL - 1 - t];
+        const float2 B = vsel(g == 0, b0, vsel(pair, b1, b2));
+        const float2 V = split_bin2(y1[t], B, tw2[m]);
+        const float2 p = cmulcj(V, W(m));
+        acc[c][t].x = fmaf(wk, p.x, acc[c][t].x);
+        acc[c][t].y = fmaf(wk, p.y, acc[c][t].y);
+      }
+      if (pair) {
+#pragma unroll
+        for (int t = 0; t < RL; ++t) {
+          const int m = j2 + t * NsL;
+          const float2 V = split_bin2(y2[t], y1[RL - 1 - t], tw2[m]);
+          const float2 p = cmulcj(V, W(m));
+          acc[c][RL + t].x = fmaf(wk, p.x, acc[c][RL + t].x);
+          acc[c][RL + t].y = fmaf(wk, p.y, acc[c][RL + t].y);
+        }
+      }
+      if (g == 0) {
+        const float2 V = split_bin2(y1[0], y1[0], tw2[N]);
+        const float2 p = cmulcj(V, W(N));
+        acc[c][2 * RL].x = fmaf(wk, p.x, acc[c][2 * RL].x);
+        acc[c][2 * RL].y = fmaf(wk, p.y, acc[c][2 * RL].y);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- normalisation sums (fixed order) ----
+#pragma unroll
+  for (int c = 0; c < TPT1; ++c) {
+    const int task = tid + c * NT;
+    if (task < T1) {
+      sqs[task] = sqp[c];
+      offs[task] = offp[c];
+    }
+  }
+  // ---- Y = pre-processed inverse input, conj'd, from the register spectra ----
+#pragma unroll
+  for (int c = 0; c < TPTL; ++c) {
+    const int task = tid + c * NT;
+    if (!(TL % NT == 0 || task < TL)) continue;
+    const int f = task / G, g = task % G;
+    const bool pair = g != 0 && 2 * g != NsL;
+    const int j2 = NsL - g;
+#pragma unroll
+    for (int t = 0; t < RL; ++t) {
+      const int m = g + t * NsL;
+      float2 X = acc[c][t];
+      const float2 x0 = (t == 0) ? acc[c][2 * RL] : acc[c][(RL - t) % RL];
+      const float2 x1 = acc[c][2 * RL - 1 - t], x2 = acc[c][RL - 1 - t];
+      float2 Xn = vsel(g == 0, x0, vsel(pair, x1, x2));
+      if (m == 0) {
+        X.y = 0.f;
+        Xn.y = 0.f;
+      }
+      buf[f * N + m] = unsplit_bin_conj(X, Xn, cconj(tw2[m]));
+    }
+    if (pair) {
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        const int m = j2 + t * NsL;
+        buf[f * N + m] = unsplit_bin_conj(acc[c][RL + t], acc[c][RL - 1 - t], cconj(tw2[m]));
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < F) {
+    float s0 = 0.f, s1 = 0.f;
+    for (int u = 0; u < Q; ++u) {
+      s0 += sqs[tid * Q + u];
+      s1 += offs[tid * Q + u];
+    }
+    sq[tid] = s0;
+    off[tid] = s1;
+    const int P = item0 + tid;
+    float ps = 1.f;
+    if (P < a.np) {
+      const int ip = a.plist[P];
+      if (a.mode == 1) {
+        ps = 0.f;
+        for (int kk = 0; kk < a.kcorr; ++kk) ps += a.shellw[(size_t)ip * nk + kk] * a.sqsums_k[(size_t)ip * nk + kk];
+      } else {
+        ps = a.sqsums[ip];
+      }
+    }
+    pscal[tid] = ps;
+  }
+  // ---- inverse FFT of every item (all passes in LDS) ----
+  lds_passes<N, F, NT, 0, NP>(buf, tw2, tid);
+  const float inv4n = 1.0f / (4.0f * (float)N);
+  for (int idx = tid; idx < F * N; idx += NT) {
+    const int f = idx / N, j = idx % N;
+    if (item0 + f >= a.np) continue;
+    const float2 y = buf[f * N + j];
+    float x0 = y.x, x1 = -y.y;
+    if (a.mode == 1) {
+      const float o = off[f] + pscal[f];
+      x0 = 2.f * x0 * inv4n - o;
+      x1 = 2.f * x1 * inv4n - o;
+    } else {
+      const float sc = inv4n / sqrtf(sq[f] * pscal[f]);
+      x0 *= sc;
+      x1 *= sc;
+    }
+    const unsigned int lin0 = 2u * j;
+    const unsigned long long k0 = ((unsigned long long)ordered_bits(x0) << 32) | (0xFFFFFFFFu - lin0);
+    const unsigned long long k1 = ((unsigned long long)ordered_bits(x1) << 32) | (0xFFFFFFFFu - lin0 - 1u);
+    atomicMax(&bestf[f], k0 > k1 ? k0 : k1);
+  }
+  __syncthreads();
+  if (tid < F && item0 + tid < a.np) a.best[item0 + tid] = bestf[tid];
+}
+
+}  // namespace pfk

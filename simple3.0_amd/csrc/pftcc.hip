@@ -2679,6 +2679,25 @@ namespace {
 
 // gencorrs at one FP32 shift per item (the in-plane callback of the shift search): best key per
 // item (cc, rotation) from the item mode of k_corr_generic.
+#ifndef PFTCC_ITEMS_FUSED
+#define PFTCC_ITEMS_FUSED 1       // shift-search callback on k_corr_fused_items for fused ring lengths
+#endif
+
+// The callback's correlations with the compile-time kernel for the fused ring lengths; false if
+// n has none.
+bool items_fused_go(int n, int nitems, hipStream_t s, const CorrArgs& a) {
+#define X(N)                                                                                        \
+  if (n == N) {                                                                                     \
+    constexpr int F = items_F<N>();                                                                 \
+    hipLaunchKernelGGL((k_corr_fused_items<N, F, 256>), dim3((unsigned)((nitems + F - 1) / F)), dim3(256), \
+                       (items_lds_bytes<N, F>()), s, a);                                            \
+    return true;                                                                                    \
+  }
+  PFTCC_FUSED_SIZES(X)
+#undef X
+  return false;
+}
+
 int launch_corr_items(pftcc* h, int n, const int* d_ip, const int* d_ref, const float* d_sh,
                       unsigned long long* d_best) {
   CorrArgs a{};
@@ -2723,8 +2742,10 @@ int launch_corr_items(pftcc* h, int n, const int* d_ip, const int* d_ref, const 
   hipEvent_t e1 = nullptr;
   int rc;
   if ((rc = timing_begin(h, 0, &e1))) return rc;
-  // one pair per work-group of 256 (64 and 128 were measured 1.6x and 1.2x slower)
-  hipLaunchKernelGGL(k_corr_generic, dim3((unsigned)n), dim3(256), lds, h->stream, a);
+  if (!(PFTCC_ITEMS_FUSED && items_fused_go(h->n, n, h->stream, a))) {
+    // one pair per work-group of 256 (64 and 128 were measured 1.6x and 1.2x slower)
+    hipLaunchKernelGGL(k_corr_generic, dim3((unsigned)n), dim3(256), lds, h->stream, a);
+  }
   HIPCHK(h, hipGetLastError());
   return timing_end(h, e1);
 }
