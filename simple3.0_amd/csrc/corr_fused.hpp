@@ -132,6 +132,9 @@ constexpr int fused_W_tuned() {
 #ifndef PFTCC_FUSED_P1V4
 #define PFTCC_FUSED_P1V4 1     // pass-1 outputs stored as two 16-byte writes
 #endif
+#ifndef PFTCC_FUSED_W12
+#define PFTCC_FUSED_W12 1      // split folded into two staged spectra W1, W2 (see k_corr_fused)
+#endif
 #ifndef PFTCC_FUSED_PAD
 #define PFTCC_FUSED_PAD 0      // extra float2 between the transforms of a tile (even: 16-byte rows)
 #endif
@@ -151,8 +154,9 @@ constexpr int fused_twl_len() {
 
 template <int N, int F>
 constexpr size_t fused_lds_bytes() {
-  // buf[F N] + Wk[N+1] + tw2[2N] + twL float2;  ctfk[N] + sq/off slots 2 F N/4 floats
-  return sizeof(float2) * ((size_t)F * fused_stride<N>() + (N + 1) + 2 * N + fused_twl_len<N>()) +
+  // buf[F N] + Wk[N+1] (W1, W2: 2 (N+1)) + tw2[2N] + twL float2;  ctfk[N] + sq/off slots 2 F N/4 floats
+  return sizeof(float2) * ((size_t)F * fused_stride<N>() + (PFTCC_FUSED_W12 ? 2 : 1) * (N + 1) + 2 * N +
+                           fused_twl_len<N>()) +
          sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
 }
 
@@ -160,6 +164,19 @@ constexpr size_t fused_lds_bytes() {
 // out of registers).
 __device__ __forceinline__ float2 vsel(bool c, float2 a, float2 b) {
   return make_float2(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+// acc += conj(Z) w1 + Zn w2: the split and the product conj(V) W of one bin with the split's
+// m-dependent factors folded into w1 = conj(A') W / 2 and w2 = conj(B') W / 2.
+__device__ __forceinline__ void acc_w12(float2& acc, float2 Z, float2 Zn, float2 w1, float2 w2) {
+  acc.x = fmaf(Z.x, w1.x, acc.x);
+  acc.x = fmaf(Z.y, w1.y, acc.x);
+  acc.y = fmaf(Z.x, w1.y, acc.y);
+  acc.y = fmaf(-Z.y, w1.x, acc.y);
+  acc.x = fmaf(Zn.x, w2.x, acc.x);
+  acc.x = fmaf(-Zn.y, w2.y, acc.x);
+  acc.y = fmaf(Zn.x, w2.y, acc.y);
+  acc.y = fmaf(Zn.y, w2.x, acc.y);
 }
 
 // 2 V[m] of the real FFT of length 2N from the packed FFT values A = Zc[m], B = Zc[N-m].  The
@@ -200,8 +217,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   float2* buf = smem;                        // [F][N]
   constexpr int S = fused_stride<N>();       // transform stride in buf
   static_assert(S % 2 == 0, "16-byte aligned rows");
-  float2* Wk = buf + F * S;                  // [N+1]  W_k / 2
-  float2* tw2 = Wk + (N + 1);                // [2N]   exp(-i pi a / N)
+  float2* Wk = buf + F * S;                  // [N+1]  W_k / 2, or W1 [N+1] then W2 [N+1]
+  float2* tw2 = Wk + (PFTCC_FUSED_W12 ? 2 : 1) * (N + 1);   // [2N]   exp(-i pi a / N)
   float2* twL = tw2 + 2 * N;                 // [RL-1][NsL]  twL[(t-1) NsL + g] = tw2[2 g t]
   float* ctfk = reinterpret_cast<float*>(twL + fused_twl_len<N>());   // [N]
   float* sqs = ctfk + N;                     // [F][Q]
@@ -265,12 +282,23 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     __syncthreads();   // the previous shell's last pass has finished with buf and Wk
     const size_t prow = (size_t)ip * nk + kk;
     for (int j = tid; j < N; j += NT) ctfk[j] = a.ctf ? a.ctf[prow * N + j] : 1.0f;
+    const float wk = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
     if (docorr)
       for (int m = tid; m <= N; m += NT) {
         const float2 w = a.memo[prow * (N + 1) + m];
-        Wk[m] = make_float2(0.5f * w.x, 0.5f * w.y);
+        float2 wh = make_float2(0.5f * w.x, 0.5f * w.y);
+        if constexpr (PFTCC_FUSED_W12) {
+          // conj(V) W = conj(Z[m]) conj(A) W + Z[N-m] conj(B) W with 2V = A' Z[m] + B' conj(Z[N-m]),
+          // A' = 1 - i w, B' = 1 + i w, w = exp(-i pi m / N): both factors depend on m only, so they
+          // are applied to the staged spectrum (with the Euclid shell weight) once per shell
+          if (a.mode == 1) wh = make_float2(wk * wh.x, wk * wh.y);
+          const float2 t = tw2[m];
+          Wk[m] = cmul(make_float2(1.f + t.y, t.x), wh);               // conj(A') W / 2
+          Wk[N + 1 + m] = cmul(make_float2(1.f - t.y, -t.x), wh);      // conj(B') W / 2
+        } else {
+          Wk[m] = wh;
+        }
       }
-    const float wk = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
     __syncthreads();
     // ---- pass 1: z from q = ref * |CTF| [* shift] [* shell scale]; radix-4 DFT (Ns = 1) ----
 #pragma unroll
@@ -363,26 +391,38 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
         const int m = g + t * NsL;
         const float2 b0 = y1[(RL - t) % RL], b1 = y2[RL - 1 - t], b2 = y1[RL - 1 - t];
         const float2 B = vsel(g == 0, b0, vsel(pair, b1, b2));
-        const float2 V = split_bin2(y1[t], B, tw2[m]);
-        const float2 p = cmulcj(V, Wk[m]);
-        acc[c][t].x = fmaf(wk, p.x, acc[c][t].x);
-        acc[c][t].y = fmaf(wk, p.y, acc[c][t].y);
+        if constexpr (PFTCC_FUSED_W12) {
+          acc_w12(acc[c][t], y1[t], B, Wk[m], Wk[N + 1 + m]);
+        } else {
+          const float2 V = split_bin2(y1[t], B, tw2[m]);
+          const float2 p = cmulcj(V, Wk[m]);
+          acc[c][t].x = fmaf(wk, p.x, acc[c][t].x);
+          acc[c][t].y = fmaf(wk, p.y, acc[c][t].y);
+        }
       }
       if (pair) {
 #pragma unroll
         for (int t = 0; t < RL; ++t) {
           const int m = j2 + t * NsL;
-          const float2 V = split_bin2(y2[t], y1[RL - 1 - t], tw2[m]);
-          const float2 p = cmulcj(V, Wk[m]);
-          acc[c][RL + t].x = fmaf(wk, p.x, acc[c][RL + t].x);
-          acc[c][RL + t].y = fmaf(wk, p.y, acc[c][RL + t].y);
+          if constexpr (PFTCC_FUSED_W12) {
+            acc_w12(acc[c][RL + t], y2[t], y1[RL - 1 - t], Wk[m], Wk[N + 1 + m]);
+          } else {
+            const float2 V = split_bin2(y2[t], y1[RL - 1 - t], tw2[m]);
+            const float2 p = cmulcj(V, Wk[m]);
+            acc[c][RL + t].x = fmaf(wk, p.x, acc[c][RL + t].x);
+            acc[c][RL + t].y = fmaf(wk, p.y, acc[c][RL + t].y);
+          }
         }
       }
       if (g == 0) {
-        const float2 V = split_bin2(y1[0], y1[0], tw2[N]);
-        const float2 p = cmulcj(V, Wk[N]);
-        acc[c][2 * RL].x = fmaf(wk, p.x, acc[c][2 * RL].x);
-        acc[c][2 * RL].y = fmaf(wk, p.y, acc[c][2 * RL].y);
+        if constexpr (PFTCC_FUSED_W12) {
+          acc_w12(acc[c][2 * RL], y1[0], y1[0], Wk[N], Wk[N + 1 + N]);
+        } else {
+          const float2 V = split_bin2(y1[0], y1[0], tw2[N]);
+          const float2 p = cmulcj(V, Wk[N]);
+          acc[c][2 * RL].x = fmaf(wk, p.x, acc[c][2 * RL].x);
+          acc[c][2 * RL].y = fmaf(wk, p.y, acc[c][2 * RL].y);
+        }
       }
     }
   }
