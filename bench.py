@@ -33,9 +33,9 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch of k_corr_fused<308,9,256,3> at 1000 particles x 200 refs x 169 shifts, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script (--legs plain --copy-calibration):
-# FETCH 1510802641 KiB x 2.000 + WRITE 7627974 KiB x 1.000, the factors read from k_copy moving
+# FETCH 1475997373 KiB x 2.000 + WRITE 117375 KiB x 1.000, the factors read from k_copy moving
 # 1 GiB in the same runs (profiles/k_corr_fused_308_plain_search.md).
-PLAIN_TRAFFIC_BYTES_B1000 = (1510802641.19 * 2.0 + 7627973.88) * 1024.0
+PLAIN_TRAFFIC_BYTES_B1000 = (1475997372.63 * 2.0 + 117375.0) * 1024.0
 FP64_PEAK_TFLOPS = 78.6
 
 
