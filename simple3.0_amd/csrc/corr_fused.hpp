@@ -156,7 +156,7 @@ template <int N, int F>
 constexpr size_t fused_lds_bytes() {
   // buf[F N] + Wk[N+1] (W1, W2: 2 (N+1)) + tw2[2N] + twL float2;  ctfk[N] + sq/off slots 2 F N/4 floats
   return sizeof(float2) * ((size_t)F * fused_stride<N>() + (PFTCC_FUSED_W12 ? 2 : 1) * (N + 1) + 2 * N +
-                           fused_twl_len<N>()) +
+                           fused_twl_len<N>() + 2 * N) +
          sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
 }
 
@@ -198,7 +198,11 @@ __device__ __forceinline__ float2 unsplit_bin_conj(float2 X, float2 Xn, float2 e
 }
 
 // W: minimum wavefronts per SIMD the register allocation must allow (1: no constraint).
-template <int N, int F, int NT, int W>
+// RST: the tile's pairs use at most two references (shifts >= F per reference), CC objective, no
+// shell normalisation: the CTF-weighted reference rings of the shell are staged once per
+// work-group in LDS and each reference's norm is summed from them (|ref x CTF x shmat|^2 =
+// |ref x CTF|^2), instead of per pair.
+template <int N, int F, int NT, int W, bool RST>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_corr_fused(CorrArgs a) {
   static_assert(Fp<N>::fusable, "first radix must be 4");
   static_assert(NT % 64 == 0, "whole wavefronts");
@@ -220,11 +224,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   float2* Wk = buf + F * S;                  // [N+1]  W_k / 2, or W1 [N+1] then W2 [N+1]
   float2* tw2 = Wk + (PFTCC_FUSED_W12 ? 2 : 1) * (N + 1);   // [2N]   exp(-i pi a / N)
   float2* twL = tw2 + 2 * N;                 // [RL-1][NsL]  twL[(t-1) NsL + g] = tw2[2 g t]
-  float* ctfk = reinterpret_cast<float*>(twL + fused_twl_len<N>());   // [N]
+  float2* rck = twL + fused_twl_len<N>();    // [2][N] ref x |CTF| of the tile's references (RST)
+  float* ctfk = reinterpret_cast<float*>(rck + 2 * N);   // [N]
   float* sqs = ctfk + N;                     // [F][Q]
   float* offs = sqs + F * Q;                 // [F][Q]
   __shared__ float sq[F], off[F];
   __shared__ float pscal;
+  __shared__ double rsq[2][NT / 64];
   __shared__ unsigned long long red[(NT + 63) / 64];
 
   const int tid = threadIdx.x;
@@ -243,6 +249,17 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       twL[i] = a.tw2g[2 * g * t];
     }
 
+  // RST: the tile's references and their rows
+  int rr0 = 0;
+  const float2* rs0 = refs;
+  const float2* rs1 = refs;
+  if constexpr (RST) {
+    rr0 = (tile * F) / a.ns;
+    const int rr1 = min(rr0 + 1, a.nr - 1);
+    rs0 = refs + (size_t)(a.rlist ? a.rlist[rr0] : rr0) * nk * N;
+    rs1 = refs + (size_t)(a.rlist ? a.rlist[rr1] : rr1) * nk * N;
+  }
+  double rsum0 = 0.0, rsum1 = 0.0;     // this thread's share of the two references' norms
   // pass-1 slots: pair, reference and shift rows are fixed for all shells
   float sqp[TPT1], offp[TPT1];
   const float2* rrow[TPT1];
@@ -281,7 +298,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     const bool donorm = kk < a.knorm;
     __syncthreads();   // the previous shell's last pass has finished with buf and Wk
     const size_t prow = (size_t)ip * nk + kk;
-    for (int j = tid; j < N; j += NT) ctfk[j] = a.ctf ? a.ctf[prow * N + j] : 1.0f;
+    if constexpr (RST) {
+      for (int j = tid; j < N; j += NT) {
+        const float cv = a.ctf ? a.ctf[prow * N + j] : 1.0f;
+        const float2 r0 = rs0[(size_t)kk * N + j], r1 = rs1[(size_t)kk * N + j];
+        const float2 q0 = make_float2(r0.x * cv, r0.y * cv), q1 = make_float2(r1.x * cv, r1.y * cv);
+        rck[j] = q0;
+        rck[N + j] = q1;
+        if (donorm) {
+          rsum0 += (double)(q0.x * q0.x + q0.y * q0.y);
+          rsum1 += (double)(q1.x * q1.x + q1.y * q1.y);
+        }
+      }
+    } else {
+      for (int j = tid; j < N; j += NT) ctfk[j] = a.ctf ? a.ctf[prow * N + j] : 1.0f;
+    }
     const float wk = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
     if (docorr)
       for (int m = tid; m <= N; m += NT) {
@@ -308,13 +339,26 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       const int f = task / Q, j = task % Q;
       float2 y[4];
       if (slotok[c]) {
-        const float2* r = rrow[c] + (size_t)kk * N;
-        const float4 ra = *reinterpret_cast<const float4*>(r + 2 * j);
-        const float4 rb = *reinterpret_cast<const float4*>(r + 2 * j + N / 2);
-        const float2 c01 = *reinterpret_cast<const float2*>(ctfk + 2 * j);
-        const float2 c23 = *reinterpret_cast<const float2*>(ctfk + 2 * j + N / 2);
-        float2 q0 = make_float2(ra.x * c01.x, ra.y * c01.x), q1 = make_float2(ra.z * c01.y, ra.w * c01.y);
-        float2 q2 = make_float2(rb.x * c23.x, rb.y * c23.x), q3 = make_float2(rb.z * c23.y, rb.w * c23.y);
+        float2 q0, q1, q2, q3;
+        if constexpr (RST) {
+          const float2* rc = rck + (((tile * F + f) / a.ns) - rr0) * N;
+          const float4 ra = *reinterpret_cast<const float4*>(rc + 2 * j);
+          const float4 rb = *reinterpret_cast<const float4*>(rc + 2 * j + N / 2);
+          q0 = make_float2(ra.x, ra.y);
+          q1 = make_float2(ra.z, ra.w);
+          q2 = make_float2(rb.x, rb.y);
+          q3 = make_float2(rb.z, rb.w);
+        } else {
+          const float2* r = rrow[c] + (size_t)kk * N;
+          const float4 ra = *reinterpret_cast<const float4*>(r + 2 * j);
+          const float4 rb = *reinterpret_cast<const float4*>(r + 2 * j + N / 2);
+          const float2 c01 = *reinterpret_cast<const float2*>(ctfk + 2 * j);
+          const float2 c23 = *reinterpret_cast<const float2*>(ctfk + 2 * j + N / 2);
+          q0 = make_float2(ra.x * c01.x, ra.y * c01.x);
+          q1 = make_float2(ra.z * c01.y, ra.w * c01.y);
+          q2 = make_float2(rb.x * c23.x, rb.y * c23.x);
+          q3 = make_float2(rb.z * c23.y, rb.w * c23.y);
+        }
         if (srow[c]) {
           const float2* s = srow[c] + (size_t)kk * N;
           const float4 sa = *reinterpret_cast<const float4*>(s + 2 * j);
@@ -324,7 +368,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           q2 = cmul(q2, make_float2(sb.x, sb.y));
           q3 = cmul(q3, make_float2(sb.z, sb.w));
         }
-        if (a.refinv && docorr) {
+        if (!RST && a.refinv && docorr) {
           const int iref = slotref[c];
           const int row = a.optrow ? a.optrow[ip] : iref;
           const float sc = a.ofac[(size_t)row * a.kcorr + kk] *
@@ -334,10 +378,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           q2 = cscale(q2, sc);
           q3 = cscale(q3, sc);
         }
-        const float e2 = q0.x * q0.x + q0.y * q0.y + q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y +
-                         q3.x * q3.x + q3.y * q3.y;
-        if (donorm) sqp[c] += e2;
-        if (a.mode == 1 && docorr) offp[c] = fmaf(wk, e2, offp[c]);
+        if constexpr (!RST) {
+          const float e2 = q0.x * q0.x + q0.y * q0.y + q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y +
+                           q3.x * q3.x + q3.y * q3.y;
+          if (donorm) sqp[c] += e2;
+          if (a.mode == 1 && docorr) offp[c] = fmaf(wk, e2, offp[c]);
+        }
         // z[j], z[j+Q], z[j+2Q], z[j+3Q]
         y[0] = make_float2(q0.x + q0.y, q1.x + q1.y);
         y[1] = make_float2(q2.x + q2.y, q3.x + q3.y);
@@ -466,7 +512,26 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     }
   }
   __syncthreads();
-  if (tid < F) {
+  if constexpr (RST) {
+    // the two references' norms: wavefront sums, then the wavefronts in order
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      rsum0 += __shfl_xor(rsum0, o);
+      rsum1 += __shfl_xor(rsum1, o);
+    }
+    if ((tid & 63) == 0) {
+      rsq[0][tid >> 6] = rsum0;
+      rsq[1][tid >> 6] = rsum1;
+    }
+    __syncthreads();
+    if (tid < F) {
+      double s0 = 0.0;
+      const int row = ((tile * F + tid) / a.ns) - rr0;
+      for (int w = 0; w < NT / 64; ++w) s0 += rsq[row < 1 ? 0 : 1][w];
+      sq[tid] = (float)s0;
+      off[tid] = 0.f;
+    }
+  } else if (tid < F) {
     float s0 = 0.f, s1 = 0.f;
     for (int u = 0; u < Q; ++u) {
       s0 += sqs[tid * Q + u];

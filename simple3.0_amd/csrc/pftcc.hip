@@ -444,6 +444,9 @@ int timing_end(pftcc* h, hipEvent_t e1);
 #ifndef PFTCC_FUSED_NT
 #define PFTCC_FUSED_NT 256
 #endif
+#ifndef PFTCC_FUSED_RST
+#define PFTCC_FUSED_RST 1           // staged reference rows (k_corr_fused RST) where a tile allows it
+#endif
 #ifndef PFTCC_FUSED_W_OVERRIDE
 #define PFTCC_FUSED_W_OVERRIDE 0    // register budget for tuning builds (0: fused_W_tuned)
 #endif
@@ -457,7 +460,12 @@ struct FusedLaunch {
   static constexpr size_t lds = fused_lds_bytes<N, F>();
   static_assert(lds <= 65536, "fused kernel LDS");
   static void go(unsigned blocks, hipStream_t s, const CorrArgs& a) {
-    hipLaunchKernelGGL((k_corr_fused<N, F, NT, W>), dim3(blocks), dim3(NT), lds, s, a);
+    // reference rows staged per work-group when a tile spans at most two references (CC, no
+    // shell normalisation)
+    if (PFTCC_FUSED_RST && a.ns >= F && a.mode == 0 && !a.refinv)
+      hipLaunchKernelGGL((k_corr_fused<N, F, NT, W, true>), dim3(blocks), dim3(NT), lds, s, a);
+    else
+      hipLaunchKernelGGL((k_corr_fused<N, F, NT, W, false>), dim3(blocks), dim3(NT), lds, s, a);
   }
 };
 
