@@ -182,6 +182,42 @@ int pftcc_search(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, cons
 int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr,
                      const int32_t* d_irefs, int32_t ns, const float* d_shifts, float* d_best_cc,
                      int32_t* d_best_iref, int32_t* d_best_ishift, int32_t* d_best_irot);
+/* The reference-loop search strategies for a batch of particles, each particle against all nrefs
+ * references with gencorrs (:1594, unshifted) and maxloc over the rotations, then:
+ *   PFTCC_SRCH_GREEDY2D  strategy2D_greedy%srch (simple_strategy2D_greedy.f90:33-58): references
+ *                        1..nrefs with ref_ok, "inpl_corr >= corr" (the last of equal maxima);
+ *   PFTCC_SRCH_SNHC2D    strategy2D_snhc%srch (simple_strategy2D_snhc.f90:33-93): srch_order,
+ *                        neighbourhood max(2, min(nrefs, nint(nrefs (1 - stoch_bound)))), empty
+ *                        classes skipped but counted, exit at the first reference where shcloc
+ *                        (simple_rnd.f90:273-295) finds a rotation with corrs >= prev_corr,
+ *                        else the best of the neighbourhood in search order;
+ *   PFTCC_SRCH_GREEDY3D  strategy3D_greedy%srch (simple_strategy3D_greedy.f90:40-77);
+ *   PFTCC_SRCH_SHC3D     strategy3D_shc%srch (simple_strategy3D_shc.f90:36-83): srch_order, exit
+ *                        after the first reference whose best exceeds prev_corr;
+ * the 3-D kinds then order the searched references by best correlation (sort_corrs,
+ * simple_strategy3D_utils.f90:100-124; unsearched at -huge, ties to the lower index).
+ * srch_order[np][nrefs]: 1-based permutations (the stochastic kinds; ignored by the greedy ones).
+ * ref_ok[nrefs]: cls_pops > 0 (2-D) / state_exists (3-D), NULL = all.  prev_corr[np]: the
+ * threshold prep4srch computes (NULL = -huge for the greedy kinds).  rnd[np]: one uniform draw in
+ * [0, 1) per particle (snhc): shcloc's choice is the floor(rnd count)-th rotation, in rotation
+ * order, of the count with corrs >= prev_corr, the same uniform distribution as its rejection
+ * sampling.  Out [np]: best_corr, best_ref (1-based; 0 if no reference was searched, when
+ * best_corr = prev_corr), best_rot (1-based), nrefs_eval (as the strategy counts it),
+ * found_better (the stochastic exit was taken).  Optional: ref_corrs/ref_rots [np][nrefs]
+ * (proj_space_corrs / inplinds: -FLT_MAX and 0 where unsearched); sorted_refs [np][cfg->ntop]
+ * (3-D kinds: 1-based references, best first).  No counterpart entry point: it replaces the
+ * per-particle srch procedures above. */
+enum { PFTCC_SRCH_GREEDY2D = 0, PFTCC_SRCH_SNHC2D = 1, PFTCC_SRCH_GREEDY3D = 2, PFTCC_SRCH_SHC3D = 3 };
+typedef struct {
+  int32_t kind;        /* PFTCC_SRCH_* */
+  float stoch_bound;   /* snhc: spec%stoch_bound, in [0, 1] */
+  int32_t ntop;        /* 3-D kinds: length of each sorted_refs row */
+} pftcc_strategy_cfg;
+int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, const int32_t* iptcls,
+                        const int32_t* srch_order, const int8_t* ref_ok, const float* prev_corr,
+                        const float* rnd, float* best_corr, int32_t* best_ref, int32_t* best_rot,
+                        int32_t* nrefs_eval, int32_t* found_better, float* ref_corrs,
+                        int32_t* ref_rots, int32_t* sorted_refs);
 /* Batched gencorr_grad_for_rot_8 over npairs (iref, iptcl, shvec, irot) tuples (the callers are
  * pftcc_shsrch_grad's cost function, simple_pftcc_shsrch_grad.f90:114-131, per particle and
  * local reference).  f[np], grad[np][3]: d/dshift_x, d/dshift_y and d/dpsi, psi the continuous

@@ -114,12 +114,19 @@ constexpr int fused_F() {
 }
 
 // Pair tile and register budget measured fastest on MI355X for the ring lengths of the benchmark
-// shapes: n = 308, 9 pairs at >= 3 wavefronts per SIMD (167 VGPRs and a 40-byte spill, against
-// 2 wavefronts at ~190 VGPRs unconstrained); n = 192, 5 pairs.  fused_F elsewhere.
+// shapes: n = 308, 9 pairs at >= 3 wavefronts per SIMD (167 VGPRs, no spills, against 2
+// wavefronts at ~190 VGPRs unconstrained); n = 192, 5 pairs.  fused_F elsewhere.
 template <int N, int NT>
 constexpr int fused_F_tuned() {
   if (NT == 256 && (N == 192 || N == 308)) return N == 192 ? 5 : 9;
   return fused_F<N, NT>();
+}
+// Tile of the staged-reference variant: its pass 1 holds no per-pair reference pointers or norm
+// sums, which leaves the registers for two more pairs at N = 308 (no spills at F = 11).
+template <int N, int NT>
+constexpr int fused_F_rst_tuned() {
+  if (NT == 256 && N == 308) return 11;
+  return fused_F_tuned<N, NT>();
 }
 template <int N, int NT>
 constexpr int fused_W_tuned() {

@@ -25,6 +25,7 @@
 #include "polarizer.hpp"
 #include "prep.hpp"
 #include "shsrch.hpp"
+#include "strategy.hpp"
 
 using namespace pfk;
 
@@ -220,6 +221,9 @@ size_t aligned(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
 
 // Dynamic LDS per work-group kept within 64 KiB so several groups share a CU.
 constexpr size_t kMaxLds = 64 * 1024;
+// Correlation vectors held in HBM per call of the search strategies (a chunk of particles x all
+// references x all rotations).
+constexpr size_t kStrategyCcBytes = (size_t)1 << 30;
 
 int local_index(pftcc* h, int iptcl, int* ip) {
   if (iptcl < h->pfrom || iptcl > h->pto)
@@ -453,17 +457,16 @@ int timing_end(pftcc* h, hipEvent_t e1);
 template <int N>
 struct FusedLaunch {
   static constexpr int NT = PFTCC_FUSED_NT;
-  static constexpr int F = (PFTCC_FUSED_F_OVERRIDE > 0 && fused_lds_bytes<N, PFTCC_FUSED_F_OVERRIDE + 0>() <= 65536)
-                               ? PFTCC_FUSED_F_OVERRIDE
-                               : fused_F_tuned<N, NT>();
+  static constexpr bool OVR = PFTCC_FUSED_F_OVERRIDE > 0 && fused_lds_bytes<N, PFTCC_FUSED_F_OVERRIDE + 0>() <= 65536;
+  static constexpr int F = OVR ? PFTCC_FUSED_F_OVERRIDE : fused_F_tuned<N, NT>();
+  static constexpr int FR = OVR ? PFTCC_FUSED_F_OVERRIDE : fused_F_rst_tuned<N, NT>();   // staged variant
   static constexpr int W = PFTCC_FUSED_W_OVERRIDE > 0 ? PFTCC_FUSED_W_OVERRIDE : fused_W_tuned<N, NT>();
   static constexpr size_t lds = fused_lds_bytes<N, F>();
-  static_assert(lds <= 65536, "fused kernel LDS");
-  static void go(unsigned blocks, hipStream_t s, const CorrArgs& a) {
-    // reference rows staged per work-group when a tile spans at most two references (CC, no
-    // shell normalisation)
-    if (PFTCC_FUSED_RST && a.ns >= F && a.mode == 0 && !a.refinv)
-      hipLaunchKernelGGL((k_corr_fused<N, F, NT, W, true>), dim3(blocks), dim3(NT), lds, s, a);
+  static constexpr size_t lds_r = fused_lds_bytes<N, FR>();
+  static_assert(lds <= 65536 && lds_r <= 65536, "fused kernel LDS");
+  static void go(unsigned blocks, hipStream_t s, const CorrArgs& a, bool rst) {
+    if (rst)
+      hipLaunchKernelGGL((k_corr_fused<N, FR, NT, W, true>), dim3(blocks), dim3(NT), lds_r, s, a);
     else
       hipLaunchKernelGGL((k_corr_fused<N, F, NT, W, false>), dim3(blocks), dim3(NT), lds, s, a);
   }
@@ -471,16 +474,23 @@ struct FusedLaunch {
 
 #define PFTCC_FUSED_SIZES(X) X(16) X(24) X(32) X(40) X(48) X(64) X(100) X(192) X(240) X(308) X(400) X(500)
 
-// F of the fused kernel for ring length n, 0 if n has no fused instantiation
-int fused_tile(int n) {
-#define X(N) if (n == N) return FusedLaunch<N>::F;
+// Tile (pairs per work-group) of the fused kernel for ring length n, 0 if n has no fused
+// instantiation.  *rst: the staged-reference variant runs, which needs a tile to span at most two
+// references (ns >= its tile), the CC objective and no shell normalisation (rst_ok).
+int fused_tile(int n, bool rst_ok, int ns, bool* rst) {
+#define X(N)                                                                   \
+  if (n == N) {                                                                \
+    *rst = PFTCC_FUSED_RST && rst_ok && ns >= FusedLaunch<N>::FR;              \
+    return *rst ? FusedLaunch<N>::FR : FusedLaunch<N>::F;                      \
+  }
   PFTCC_FUSED_SIZES(X)
 #undef X
+  *rst = false;
   return 0;
 }
 
-void fused_go(int n, unsigned blocks, hipStream_t s, const CorrArgs& a) {
-#define X(N) if (n == N) { FusedLaunch<N>::go(blocks, s, a); return; }
+void fused_go(int n, unsigned blocks, hipStream_t s, const CorrArgs& a, bool rst) {
+#define X(N) if (n == N) { FusedLaunch<N>::go(blocks, s, a, rst); return; }
   PFTCC_FUSED_SIZES(X)
 #undef X
 }
@@ -520,7 +530,8 @@ int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist
   a.best = d_best;
   a.tw2g = h->d_tw2;
   const int npairs = nr * a.ns;
-  const int Ff = h->use_fused ? fused_tile(h->n) : 0;
+  bool rst = false;
+  const int Ff = h->use_fused ? fused_tile(h->n, a.mode == 0 && !a.refinv, a.ns, &rst) : 0;
   a.F = Ff > 0 ? Ff : pick_F(h->n, npairs);
   a.ntiles = (npairs + a.F - 1) / a.F;
   const size_t lds = corr_lds_bytes(h->n, a.F);
@@ -530,7 +541,7 @@ int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist
   hipEvent_t e1 = nullptr;
   int rc;
   if ((rc = timing_begin(h, 0, &e1))) return rc;
-  if (Ff > 0) fused_go(h->n, (unsigned)blocks, h->stream, a);
+  if (Ff > 0) fused_go(h->n, (unsigned)blocks, h->stream, a, rst);
   else hipLaunchKernelGGL(k_corr_generic, dim3((unsigned)blocks), dim3(256), lds, h->stream, a);
   HIPCHK(h, hipGetLastError());
   return timing_end(h, e1);
@@ -2358,6 +2369,133 @@ int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (bad) return fail(h, PFTCC_E_ARG, "search_dev: particle or reference index out of range");
   return search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, d_best_cc, d_best_iref, d_best_ishift, d_best_irot);
+}
+
+int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, const int32_t* iptcls,
+                        const int32_t* srch_order, const int8_t* ref_ok, const float* prev_corr, const float* rnd,
+                        float* best_corr, int32_t* best_ref, int32_t* best_rot, int32_t* nrefs_eval,
+                        int32_t* found_better, float* ref_corrs, int32_t* ref_rots, int32_t* sorted_refs) {
+  if (!h || !cfg || !iptcls || np < 1 || !best_corr || !best_ref || !best_rot || !nrefs_eval || !found_better)
+    return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  const int kind = cfg->kind;
+  if (kind < PFTCC_SRCH_GREEDY2D || kind > PFTCC_SRCH_SHC3D) return fail(h, PFTCC_E_ARG, "unknown strategy %d", kind);
+  const bool stoch = kind == PFTCC_SRCH_SNHC2D || kind == PFTCC_SRCH_SHC3D;
+  const bool is3d = kind == PFTCC_SRCH_GREEDY3D || kind == PFTCC_SRCH_SHC3D;
+  const int nrefs = h->nrefs, nrots = h->nrots;
+  if (stoch && (!srch_order || !prev_corr)) return fail(h, PFTCC_E_ARG, "the stochastic strategies need srch_order and prev_corr");
+  if (kind == PFTCC_SRCH_SNHC2D && !rnd) return fail(h, PFTCC_E_ARG, "snhc needs one uniform draw per particle (rnd)");
+  if ((ref_corrs == nullptr) != (ref_rots == nullptr)) return fail(h, PFTCC_E_ARG, "ref_corrs and ref_rots go together");
+  const int ntop = is3d && sorted_refs ? cfg->ntop : 0;
+  if (is3d && sorted_refs && (ntop < 1 || ntop > nrefs)) return fail(h, PFTCC_E_ARG, "ntop %d outside [1, %d]", cfg->ntop, nrefs);
+  int sortlen = 1;
+  while (sortlen < nrefs) sortlen <<= 1;
+  if (is3d && (size_t)sortlen * 8 > kMaxLds)
+    return fail(h, PFTCC_E_UNSUPPORTED, "the 3-D strategies sort at most %zu references per particle", kMaxLds / 8);
+  // snhc's neighbourhood: max(2, min(nrefs, nint(real(nrefs) * (1 - stoch_bound)))) in default real
+  int bound = nrefs;
+  if (kind == PFTCC_SRCH_SNHC2D) {
+    if (!(cfg->stoch_bound >= 0.f && cfg->stoch_bound <= 1.f))
+      return fail(h, PFTCC_E_ARG, "stoch_bound %g outside [0, 1]", cfg->stoch_bound);
+    const float x = (float)nrefs * (1.0f - cfg->stoch_bound);
+    bound = std::max(2, std::min(nrefs, (int)lroundf(x)));
+  }
+  int rc;
+  std::vector<int> pl(np);
+  for (int i = 0; i < np; ++i)
+    if ((rc = local_index(h, iptcls[i], &pl[i]))) return rc;
+  std::vector<int> ord;
+  if (stoch) {
+    // each row a permutation of 1..nrefs (srch_order of prep_strategy2D_batch / prep_strategy3D)
+    ord.resize((size_t)np * nrefs);
+    std::vector<int> seen(nrefs, -1);
+    for (int i = 0; i < np; ++i)
+      for (int j = 0; j < nrefs; ++j) {
+        const int r = srch_order[(size_t)i * nrefs + j];
+        if (r < 1 || r > nrefs || seen[r - 1] == i)
+          return fail(h, PFTCC_E_ARG, "srch_order row %d is not a permutation of 1..%d", i + 1, nrefs);
+        seen[r - 1] = i;
+        ord[(size_t)i * nrefs + j] = r - 1;
+      }
+  }
+  if (kind == PFTCC_SRCH_SNHC2D)
+    for (int i = 0; i < np; ++i)
+      if (!(rnd[i] >= 0.f && rnd[i] < 1.f)) return fail(h, PFTCC_E_ARG, "rnd[%d] = %g outside [0, 1)", i, rnd[i]);
+  if ((rc = need_memo(h))) return rc;
+  // correlation vectors of a chunk of particles x all references (gencorrs, :1594) stay in HBM
+  const size_t per_ptcl = (size_t)nrefs * nrots * 4;
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)np, kStrategyCcBytes / per_ptcl));
+  const size_t nref_all = (size_t)np * nrefs;
+  if ((rc = scratch_reserve(h, aligned(np * 4) + (stoch ? aligned(nref_all * 4) : 0) + aligned(nrefs) +
+                                   2 * aligned(np * 4) + aligned((size_t)chunk * per_ptcl) +
+                                   2 * aligned((size_t)chunk * nrefs * 4) + 5 * aligned(np * 4) +
+                                   (ref_corrs ? 2 * aligned(nref_all * 4) : 0) + aligned((size_t)np * ntop * 4) +
+                                   aligned(4))))
+    return rc;
+  int* d_pl = take<int>(h, np);
+  int* d_ord = stoch ? take<int>(h, nref_all) : nullptr;
+  int8_t* d_ok = ref_ok ? take<int8_t>(h, nrefs) : nullptr;
+  float* d_prev = prev_corr ? take<float>(h, np) : nullptr;
+  float* d_rnd = rnd ? take<float>(h, np) : nullptr;
+  float* d_cc = take<float>(h, (size_t)chunk * nrefs * nrots);
+  float* d_vb = take<float>(h, (size_t)chunk * nrefs);
+  int* d_vr = take<int>(h, (size_t)chunk * nrefs);
+  float* d_bc = take<float>(h, np);
+  int* d_br = take<int>(h, np);
+  int* d_bro = take<int>(h, np);
+  int* d_ne = take<int>(h, np);
+  int* d_fb = take<int>(h, np);
+  float* d_rc = ref_corrs ? take<float>(h, nref_all) : nullptr;
+  int* d_rr = ref_corrs ? take<int>(h, nref_all) : nullptr;
+  int* d_so = ntop ? take<int>(h, (size_t)np * ntop) : nullptr;
+  HIPCHK(h, hipMemcpyAsync(d_pl, pl.data(), (size_t)np * 4, hipMemcpyHostToDevice, h->stream));
+  if (d_ord) HIPCHK(h, hipMemcpyAsync(d_ord, ord.data(), nref_all * 4, hipMemcpyHostToDevice, h->stream));
+  if (d_ok) HIPCHK(h, hipMemcpyAsync(d_ok, ref_ok, nrefs, hipMemcpyHostToDevice, h->stream));
+  if (d_prev) HIPCHK(h, hipMemcpyAsync(d_prev, prev_corr, (size_t)np * 4, hipMemcpyHostToDevice, h->stream));
+  if (d_rnd) HIPCHK(h, hipMemcpyAsync(d_rnd, rnd, (size_t)np * 4, hipMemcpyHostToDevice, h->stream));
+  for (int p0 = 0; p0 < np; p0 += chunk) {
+    const int c = std::min(chunk, np - p0);
+    if ((rc = launch_corr(h, c, d_pl + p0, nrefs, nullptr, 1, nullptr, false, d_cc, nullptr))) return rc;
+    const long long nvec = (long long)c * nrefs;
+    hipLaunchKernelGGL(k_ref_maxloc, dim3((unsigned)((nvec + 3) / 4)), dim3(256), 0, h->stream, (const float*)d_cc,
+                       nvec, nrots, d_vb, d_vr);
+    StrategyArgs a{};
+    a.kind = kind;
+    a.nrefs = nrefs;
+    a.nrots = nrots;
+    a.bound = bound;
+    a.ntop = ntop;
+    a.sortlen = sortlen;
+    a.order = d_ord ? d_ord + (size_t)p0 * nrefs : nullptr;
+    a.ref_ok = d_ok;
+    a.prev_corr = d_prev ? d_prev + p0 : nullptr;
+    a.rnd = d_rnd ? d_rnd + p0 : nullptr;
+    a.cc = d_cc;
+    a.vbest = d_vb;
+    a.vrot = d_vr;
+    a.best_corr = d_bc + p0;
+    a.best_ref = d_br + p0;
+    a.best_rot = d_bro + p0;
+    a.nrefs_eval = d_ne + p0;
+    a.found = d_fb + p0;
+    a.ref_corrs = d_rc ? d_rc + (size_t)p0 * nrefs : nullptr;
+    a.ref_rots = d_rr ? d_rr + (size_t)p0 * nrefs : nullptr;
+    a.sorted = d_so ? d_so + (size_t)p0 * ntop : nullptr;
+    hipLaunchKernelGGL(k_strategy, dim3((unsigned)c), dim3(256), is3d ? (size_t)sortlen * 8 : 0, h->stream, a);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipMemcpyAsync(best_corr, d_bc, (size_t)np * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(best_ref, d_br, (size_t)np * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(best_rot, d_bro, (size_t)np * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(nrefs_eval, d_ne, (size_t)np * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(found_better, d_fb, (size_t)np * 4, hipMemcpyDeviceToHost, h->stream));
+  if (d_rc) {
+    HIPCHK(h, hipMemcpyAsync(ref_corrs, d_rc, nref_all * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ref_rots, d_rr, nref_all * 4, hipMemcpyDeviceToHost, h->stream));
+  }
+  if (d_so) HIPCHK(h, hipMemcpyAsync(sorted_refs, d_so, (size_t)np * ntop * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
 }
 
 int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs, int32_t ns,

@@ -51,6 +51,14 @@ class PrepCfgC(C.Structure):
                 ("width", C.c_float), ("focusmsk", C.c_float), ("ctfflag", C.c_int32), ("griddev", C.c_int32)]
 
 
+class StrategyCfgC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("stoch_bound", C.c_float), ("ntop", C.c_int32)]
+
+
+# search strategies of pftcc_strategy_srch
+SRCH_GREEDY2D, SRCH_SNHC2D, SRCH_GREEDY3D, SRCH_SHC3D = 0, 1, 2, 3
+
+
 def _ctf_array(ctfparams):
     """ctypes array of pftcc_ctfparams from objects with smpd, kv, cs, fraca, dfx, dfy, angast[,
     phshift, phaseplate] attributes (an array built here passes through unchanged)."""
@@ -124,6 +132,8 @@ SIGNATURES = [
     ("pftcc_calc_roinv_corrmat", _I, [_P, _PF]),
     ("pftcc_gencorrs_batch", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _PF]),
     ("pftcc_search", _I, [_P, _I, _PI, _I, _PI, _I, _PF, _PF, _PI, _PI, _PI]),
+    ("pftcc_strategy_srch", _I, [_P, C.POINTER(StrategyCfgC), _I, _PI, _PI, _P, _PF, _PF, _PF, _PI, _PI, _PI, _PI,
+                                 _PF, _PI, _PI]),
     ("pftcc_search_dev", _I, [_P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P]),
     ("pftcc_grad_batch", _I, [_P, _I, _PI, _PI, _PD, _PI, _PD, _PD]),
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
@@ -510,6 +520,35 @@ class PolarFTCorrCalc:
                                          ns, None if s is None else _fp(s), _fp(cc), _ip(ir),
                                          _ip(ish), _ip(rot)), "search")
         return cc, ir, ish, rot
+
+    def strategy_srch(self, kind, iptcls, srch_order=None, ref_ok=None, prev_corr=None, rnd=None,
+                      stoch_bound=0.0, ntop=0, ref_corrs=False):
+        """pftcc_strategy_srch: returns a dict of best_corr, best_ref, best_rot, nrefs_eval,
+        found_better [np], plus ref_corrs/ref_rots [np][nrefs] (ref_corrs=True) and sorted_refs
+        [np][ntop] (3-D kinds, ntop > 0)."""
+        p = np.ascontiguousarray(iptcls, np.int32)
+        n = p.size
+        o = None if srch_order is None else np.ascontiguousarray(srch_order, np.int32).reshape(n, -1)
+        ok = None if ref_ok is None else np.ascontiguousarray(np.asarray(ref_ok) != 0, np.int8)
+        pc = None if prev_corr is None else np.ascontiguousarray(prev_corr, np.float32)
+        u = None if rnd is None else np.ascontiguousarray(rnd, np.float32)
+        out = {"best_corr": np.zeros(n, np.float32), "best_ref": np.zeros(n, np.int32),
+               "best_rot": np.zeros(n, np.int32), "nrefs_eval": np.zeros(n, np.int32),
+               "found_better": np.zeros(n, np.int32)}
+        rc = rr = so = None
+        if ref_corrs:
+            rc = out["ref_corrs"] = np.zeros((n, self.nrefs), np.float32)
+            rr = out["ref_rots"] = np.zeros((n, self.nrefs), np.int32)
+        if kind in (SRCH_GREEDY3D, SRCH_SHC3D) and ntop > 0:
+            so = out["sorted_refs"] = np.zeros((n, ntop), np.int32)
+        cfg = StrategyCfgC(kind, stoch_bound, ntop)
+        self._chk(self._lib.pftcc_strategy_srch(
+            self._h, C.byref(cfg), n, _ip(p), None if o is None else _ip(o),
+            None if ok is None else ok.ctypes.data_as(C.c_void_p), None if pc is None else _fp(pc),
+            None if u is None else _fp(u), _fp(out["best_corr"]), _ip(out["best_ref"]), _ip(out["best_rot"]),
+            _ip(out["nrefs_eval"]), _ip(out["found_better"]), None if rc is None else _fp(rc),
+            None if rr is None else _ip(rr), None if so is None else _ip(so)), "strategy_srch")
+        return out
 
     def search_dev(self, np_, d_iptcls, nr, d_irefs, ns, d_shifts, d_cc, d_iref, d_ishift, d_irot):
         self._chk(self._lib.pftcc_search_dev(self._h, np_, C.c_void_p(d_iptcls), nr,
