@@ -434,6 +434,60 @@ def precision_same_workload(lib, w, shifts, plain_value, nptcls, sample=1000, nt
     }
 
 
+def strategy_form(lib, w, nptcls=10000, reps=2):
+    """The reference-loop search strategies (pftcc_strategy_srch) over the main line's references
+    and particles: every particle against all references at shift 0 (gencorrs), then the greedy /
+    snhc 2-D and greedy / shc 3-D choices.  The stochastic kinds get a random search order with
+    the previous class last (put_last) and prev_corr at 0.9 of the greedy best."""
+    from pftcc import SRCH_GREEDY2D, SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC2D
+
+    n = min(nptcls, w.nptcls)
+    ids = np.arange(1, n + 1, dtype=np.int32)
+    lib.strategy_srch(SRCH_GREEDY2D, ids[:64])                 # warm-up
+    base = lib.strategy_srch(SRCH_GREEDY2D, ids)
+    rng = np.random.default_rng(7)
+    order = np.empty((n, w.nrefs), np.int32)
+    for p in range(n):
+        o = rng.permutation(w.nrefs) + 1
+        o = np.concatenate([o[o != base["best_ref"][p]], [base["best_ref"][p]]])
+        order[p] = o
+    prev = (0.9 * base["best_corr"]).astype(np.float32)
+    rnd = rng.random(n).astype(np.float32)
+    kinds = {"greedy2d": dict(kind=SRCH_GREEDY2D),
+             "snhc2d": dict(kind=SRCH_SNHC2D, srch_order=order, prev_corr=prev, rnd=rnd, stoch_bound=0.0),
+             "greedy3d": dict(kind=SRCH_GREEDY3D, ntop=10),
+             "shc3d": dict(kind=SRCH_SHC3D, srch_order=order, prev_corr=prev, ntop=10)}
+    nrots = 2 * w.pftsz
+    res = {}
+    for name, kw in kinds.items():
+        lib.set_timing(True)
+        t0 = time.perf_counter()
+        args = {k: v for k, v in kw.items() if k != "kind"}
+        for _ in range(reps):
+            out = lib.strategy_srch(kw["kind"], ids, **args)
+        wall = (time.perf_counter() - t0) / reps
+        kern_ms, launches = lib.get_timing()
+        lib.set_timing(False)
+        if name == "greedy3d":
+            _, ir0, _, rot0 = lib.search(ids)
+            same = bool(np.array_equal(out["best_ref"], ir0) and np.array_equal(out["best_rot"], rot0))
+        res[name] = {"ms_per_call": 1e3 * wall, "evals_per_s": n * w.nrefs * nrots / wall,
+                     "correlation_kernel_ms": kern_ms / reps,
+                     "mean_nrefs_eval": float(np.mean(out["nrefs_eval"])),
+                     "found_better_frac": float(np.mean(out["found_better"]))}
+    kern = res["greedy2d"]["correlation_kernel_ms"]
+    achieved = plain_flops_per_pair(w.pftsz, w.nk) * n * w.nrefs / (kern * 1e-3) / 1e12
+    return {"workload": f"search strategies: {n} particles x {w.nrefs} references, all rotations at shift 0 "
+                        "(gencorrs), then the strategy's choice on the device",
+            "value": res["greedy2d"]["evals_per_s"], "unit": "evals/s (greedy 2-D, whole call)",
+            "ptcls": n, "refs": w.nrefs, "nrots": nrots, "kinds": res,
+            "greedy3d_best_equals_pftcc_search": same,
+            "roofline": {"bound": "mfma", "pipe": "VALU fp32", "kernel": "k_corr_fused<308,9,256,3,false>",
+                         "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_PEAK_TFLOPS,
+                         "note": "unshifted search: one pair per reference, so the staged-reference tile does not apply"}}
+
+
 def cpu_baseline(pftcc, w, shifts, nthreads, target_s=12.0):
     """Times the reference CPU algorithm (C++/OpenMP port, oracle/pftcc_cpu.cpp) on a bounded
     sample of the same workload and checks the GPU's assignments on that sample."""
@@ -491,8 +545,8 @@ def main():
     ap.add_argument("--mfma-ptcls", type=int, default=10000,
                     help="particles of the precision-form run on the main workload")
     ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-kernel line")
-    ap.add_argument("--legs", default="plain,precision,gradient,shsrch,polarizer,prep",
-                    help="comma list of plain, precision, gradient, shsrch, polarizer, prep (profiling runs select one)")
+    ap.add_argument("--legs", default="plain,precision,gradient,shsrch,polarizer,prep,strategy",
+                    help="comma list of plain, precision, gradient, shsrch, polarizer, prep, strategy (profiling runs select one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device copy (k_copy), the reference for HBM traffic counters")
     args = ap.parse_args()
@@ -616,6 +670,8 @@ def main():
             out["cpu_baseline"] = {"error": repr(e)}
     if not args.no_precision and "precision" in legs:
         out["precision_form_same_workload"] = precision_same_workload(lib, w, shifts, value, args.mfma_ptcls)
+    if "strategy" in legs:
+        out["search_strategies"] = strategy_form(lib, w)
     lib.kill()
     if not args.no_precision and "precision" in legs:
         out["precision_form"] = precision_form(args.precision_steps)

@@ -3,7 +3,8 @@
 !   pftcc_fortran_driver parity <in.bin> <out.bin>
 !       reads references, particles, CTF parameters, shifts and single-rotation queries, runs
 !       new / set_ref_pft / set_ptcl_pft / create_polar_absctfmats / memoize_ffts / gencorrs /
-!       search / gencorr_for_rot_8 / gencorr_grad_for_rot_8 / specscore and writes the results
+!       search / search_mfma / shsrch_grad_batch / strategy_srch / gencorr_for_rot_8 /
+!       gencorr_grad_for_rot_8 / specscore and writes the results
 !       (raw little-endian; layout in tests/test_gpu_fortran.py).
 !   pftcc_fortran_driver shiftsrch
 !       the invariant of production/tests/test_shiftsrch (simple_test_shiftsrch.f90:40-118):
@@ -44,6 +45,11 @@ contains
         real(c_float),  allocatable :: scxy(:,:)
         real(c_float)               :: lims(2,2)
         real(c_double), allocatable :: qsh(:,:), f8(:), g8(:,:)
+        type(strategy_cfg)          :: scfg
+        real(c_float),      allocatable :: gbc(:), hbc(:), hprev(:)
+        integer(c_int32_t), allocatable :: gbr(:), gbrot(:), gne(:), gfb(:), gsorted(:,:)
+        integer(c_int32_t), allocatable :: hbr(:), hbrot(:), hne(:), hfb(:), hsorted(:,:)
+        integer,            allocatable :: horder(:,:)
         integer :: nrefs, nptcls, n, kfrom, kto, kstop, box, withctf, nsh, nq, pfrom, nk, nrots
         integer :: u, i, r, s, q
         open(newunit=u, file=fin, access='stream', form='unformatted', status='old')
@@ -120,6 +126,21 @@ contains
         lims(:,1) = -3.
         lims(:,2) =  3.
         call pftcc%shsrch_grad_batch(srefs, sptcls, lims, 60, .true., srots, scxy)
+        ! search strategies: greedy 3-D with every reference sorted, and shc 3-D over the
+        ! references in reversed order with prev_corr 0
+        allocate(gbc(nptcls), gbr(nptcls), gbrot(nptcls), gne(nptcls), gfb(nptcls), gsorted(nrefs, nptcls))
+        allocate(hbc(nptcls), hbr(nptcls), hbrot(nptcls), hne(nptcls), hfb(nptcls), hsorted(nrefs, nptcls))
+        allocate(horder(nptcls, nrefs), hprev(nptcls))
+        scfg%kind = SRCH_GREEDY3D
+        scfg%ntop = nrefs
+        call pftcc%strategy_srch(scfg, ids, gbc, gbr, gbrot, gne, gfb, sorted_refs=gsorted)
+        do i = 1, nptcls
+            horder(i,:) = [(nrefs - r + 1, r = 1, nrefs)]
+        end do
+        hprev = 0.
+        scfg%kind = SRCH_SHC3D
+        call pftcc%strategy_srch(scfg, ids, hbc, hbr, hbrot, hne, hfb, srch_order=horder, prev_corr=hprev, &
+            &sorted_refs=hsorted)
         allocate(f8(max(nq,1)), g8(2, max(nq,1)), scores(max(nq,1)))
         do q = 1, nq
             f8(q) = pftcc%gencorr_for_rot_8(qref(q), qptcl(q), qsh(:,q), qrot(q))
@@ -143,6 +164,15 @@ contains
         write(u) int(mirot, c_int32_t)
         write(u) scxy
         write(u) int(srots, c_int32_t)
+        write(u) gbc
+        write(u) gbr
+        write(u) gbrot
+        write(u) gsorted
+        write(u) hbc
+        write(u) hbr
+        write(u) hne
+        write(u) hfb
+        write(u) hsorted
         close(u)
         write(*,'(a,i0,a,i0,a,i0,a)') 'parity run done: ', nptcls, ' ptcls x ', nrefs, ' refs x ', nsh, ' shifts'
     end subroutine run_parity

@@ -17,6 +17,7 @@ use, intrinsic :: iso_c_binding
 implicit none
 
 public :: polarft_corrcalc, pftcc_glob, pftcc_params, ctfparams, prep_cfg, OBJFUN_CC, OBJFUN_EUCLID
+public :: strategy_cfg, SRCH_GREEDY2D, SRCH_SNHC2D, SRCH_GREEDY3D, SRCH_SHC3D
 public :: PREC_BF16, PREC_SPLIT_BF16, PREC_FP8
 private
 
@@ -56,6 +57,15 @@ type, bind(C) :: prep_cfg
     real(c_float)      :: msk = 0., inner = 0., width = 10., focusmsk = 0.
     integer(c_int32_t) :: ctfflag = 0, griddev = 0
 end type prep_cfg
+
+! pftcc_strategy_cfg: which reference-loop strategy, snhc's spec%stoch_bound, the length of each
+! sorted reference list (3-D kinds)
+integer, parameter :: SRCH_GREEDY2D = 0, SRCH_SNHC2D = 1, SRCH_GREEDY3D = 2, SRCH_SHC3D = 3
+type, bind(C) :: strategy_cfg
+    integer(c_int32_t) :: kind = SRCH_GREEDY2D
+    real(c_float)      :: stoch_bound = 0.
+    integer(c_int32_t) :: ntop = 0
+end type strategy_cfg
 
 type :: polarft_corrcalc
     private
@@ -107,6 +117,7 @@ type :: polarft_corrcalc
     procedure          :: prep_matchfilt
     procedure          :: search
     procedure          :: search_mfma
+    procedure          :: strategy_srch
     procedure          :: grad_batch
     procedure          :: shsrch_grad_batch
     procedure          :: init_polarizer
@@ -372,6 +383,15 @@ interface
         type(c_ptr), value :: h, imgs, shifts, ctfp, cmats
         integer(c_int32_t), value :: count
         type(prep_cfg), intent(in) :: cfg
+    end function
+    integer(c_int) function c_strategy_srch(h, cfg, np, iptcls, order, ok, prev, rnd, bc, br, bro, ne, fb, rc, rr, so) &
+        &bind(C, name='pftcc_strategy_srch')
+        import :: c_int, c_int32_t, c_ptr, strategy_cfg
+        type(c_ptr), value :: h, order, ok, prev, rnd, rc, rr, so
+        type(strategy_cfg), intent(in) :: cfg
+        integer(c_int32_t), value :: np
+        integer(c_int32_t), intent(in) :: iptcls(*)
+        type(c_ptr), value :: bc, br, bro, ne, fb
     end function
     integer(c_int) function c_prep_matchfilt(h, iptcl, iref, irot) bind(C, name='pftcc_prep_matchfilt')
         import :: c_int, c_int32_t, c_ptr
@@ -891,6 +911,57 @@ contains
         best_irot = brt
         if( present(best_ishift) ) best_ishift = bis + 1
     end subroutine search
+
+    ! The reference-loop strategies (strategy2D_greedy/snhc%srch, strategy3D_greedy/shc%srch with
+    ! sort_corrs) for a batch of particles against all references.  srch_order(np, nrefs) as
+    ! s2D%srch_order / s3D%srch_order; ref_ok(nrefs) cls_pops > 0 / state_exists; prev_corr(np) the
+    ! prep4srch threshold; rnd(np) one uniform draw per particle for shcloc (snhc).  Out: the
+    ! chosen reference and rotation, nrefs_eval and whether the stochastic exit was taken;
+    ! optionally proj_space_corrs / inplinds (nrefs, np) and the sorted references (ntop, np).
+    subroutine strategy_srch( self, cfg, iptcls, best_corr, best_ref, best_rot, nrefs_eval, found_better, &
+            &srch_order, ref_ok, prev_corr, rnd, ref_corrs, ref_rots, sorted_refs )
+        class(polarft_corrcalc),    intent(inout) :: self
+        type(strategy_cfg),         intent(in)    :: cfg
+        integer,                    intent(in)    :: iptcls(:)
+        real(sp),           target, intent(out)   :: best_corr(size(iptcls))
+        integer(c_int32_t), target, intent(out)   :: best_ref(size(iptcls)), best_rot(size(iptcls))
+        integer(c_int32_t), target, intent(out)   :: nrefs_eval(size(iptcls)), found_better(size(iptcls))
+        integer,  optional,         intent(in)    :: srch_order(:,:)
+        logical,  optional,         intent(in)    :: ref_ok(:)
+        real(sp), optional, target, intent(in)    :: prev_corr(:), rnd(:)
+        real(sp), optional, target, intent(out)   :: ref_corrs(:,:)
+        integer(c_int32_t), optional, target, intent(out) :: ref_rots(:,:), sorted_refs(:,:)
+        integer(c_int32_t) :: ip(size(iptcls))
+        integer(c_int32_t), allocatable, target :: ord(:,:)
+        integer(c_int8_t),  allocatable, target :: ok(:)
+        type(c_ptr) :: pord, pok, pprev, prnd, prc, prr, pso
+        ip   = iptcls
+        pord = c_null_ptr
+        if( present(srch_order) )then
+            allocate(ord(size(srch_order,2), size(srch_order,1)), source=int(transpose(srch_order), c_int32_t))
+            pord = c_loc(ord)
+        endif
+        pok = c_null_ptr
+        if( present(ref_ok) )then
+            allocate(ok(size(ref_ok)), source=int(merge(1, 0, ref_ok), c_int8_t))
+            pok = c_loc(ok)
+        endif
+        pprev = c_null_ptr
+        if( present(prev_corr) ) pprev = c_loc(prev_corr)
+        prnd = c_null_ptr
+        if( present(rnd) ) prnd = c_loc(rnd)
+        prc = c_null_ptr
+        prr = c_null_ptr
+        if( present(ref_corrs) .and. present(ref_rots) )then
+            prc = c_loc(ref_corrs)
+            prr = c_loc(ref_rots)
+        endif
+        pso = c_null_ptr
+        if( present(sorted_refs) ) pso = c_loc(sorted_refs)
+        call chk(self, c_strategy_srch(self%h, cfg, int(size(iptcls),c_int32_t), ip, pord, pok, pprev, prnd, &
+            &c_loc(best_corr), c_loc(best_ref), c_loc(best_rot), c_loc(nrefs_eval), c_loc(found_better), &
+            &prc, prr, pso), 'strategy_srch')
+    end subroutine strategy_srch
 
     ! precision form of search without shifts: MFMA contraction per angular frequency, the ntop
     ! best references per particle rescored exactly in FP64 (ntop = 0: no rescoring)

@@ -11,6 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import strategy_oracle as S
 from cases import make_case
 from gpu_helpers import peak_rel_err, say, to_lib
 
@@ -74,6 +75,10 @@ def test_fortran_module_parity(tmp_path):
     m_rot = take(np.int32, 4)
     s_cxy = take(np.float32, 3 * 12).reshape(12, 3)
     s_rot = take(np.int32, 12)
+    g_bc, g_br, g_brot = take(np.float32, 4), take(np.int32, 4), take(np.int32, 4)
+    g_sorted = take(np.int32, 4 * 3).reshape(4, 3)
+    h_bc, h_br, h_ne, h_fb = take(np.float32, 4), take(np.int32, 4), take(np.int32, 4), take(np.int32, 4)
+    h_sorted = take(np.int32, 4 * 3).reshape(4, 3)
     assert off == len(raw)
     # the Python mirror over the same C ABI gives bitwise identical numbers
     lib = to_lib(pf, cps)
@@ -108,5 +113,13 @@ def test_fortran_module_parity(tmp_path):
     cxy_py, rot_py = lib.shsrch_grad_batch(refs, ptcls, trs=3.0, maxits=60, opt_angle=True)
     assert np.array_equal(s_rot, rot_py)
     assert np.array_equal(s_cxy, cxy_py)
+    # search strategies through the module against the oracle loops on the unshifted vectors
+    for i in range(4):
+        g = S.greedy_3d(cc[i, :, 0, :], [True] * 3)
+        assert list(g_sorted[i]) == [r + 1 for r in g[3]]
+        assert (g_br[i], g_brot[i], g_bc[i]) == (g[3][0] + 1, g[1][g[3][0]] + 1, g[0][g[3][0]])
+        h = S.shc_3d(cc[i, :, 0, :], [2, 1, 0], [True] * 3, np.float32(0.0))
+        assert list(h_sorted[i]) == [r + 1 for r in h[3]]
+        assert (h_br[i], h_bc[i], h_ne[i], bool(h_fb[i])) == (h[3][0] + 1, h[0][h[3][0]], h[4], h[5])
     assert np.sum(s_rot > 0) >= 6          # random pairs: most improve on the start
     lib.kill()
