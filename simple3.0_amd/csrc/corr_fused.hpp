@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     slotref[c] = 0;
     if (slotok[c]) {
       const int ir = P / a.ns, is = P % a.ns;
-      const int iref = a.rlist ? a.rlist[ir] : ir;
+      const int iref = a.rlist ? a.rlist[(size_t)pi * a.rlist_ld + ir] : ir;
       slotref[c] = iref;
       rrow[c] = refs + (size_t)iref * nk * N;
       srow[c] = a.shtab ? a.shtab + (size_t)is * nk * N : nullptr;

@@ -224,6 +224,8 @@ constexpr size_t kMaxLds = 64 * 1024;
 // Correlation vectors held in HBM per call of the search strategies (a chunk of particles x all
 // references x all rotations).
 constexpr size_t kStrategyCcBytes = (size_t)1 << 30;
+// First round of the stochastic scans, in search-order positions (doubling per round).
+constexpr int kStrategyRound0 = 32;
 
 int local_index(pftcc* h, int iptcl, int* ip) {
   if (iptcl < h->pfrom || iptcl > h->pto)
@@ -496,8 +498,9 @@ void fused_go(int n, unsigned blocks, hipStream_t s, const CorrArgs& a, bool rst
 }
 
 // Launches the plain FP32 correlation for np particles x nr refs x ns shifts.
+// rlist_ld > 0: d_rlist holds one list of nr references per particle slot (row stride rlist_ld).
 int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist, int ns,
-                const float2* d_shtab, bool shifted, float* d_cc, unsigned long long* d_best) {
+                const float2* d_shtab, bool shifted, float* d_cc, unsigned long long* d_best, int rlist_ld = 0) {
   CorrArgs a{};
   a.n = h->n;
   a.nk = h->nk;
@@ -526,12 +529,13 @@ int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist
   a.np = np;
   a.rlist = d_rlist;
   a.nr = nr;
+  a.rlist_ld = rlist_ld;
   a.cc_out = d_cc;
   a.best = d_best;
   a.tw2g = h->d_tw2;
   const int npairs = nr * a.ns;
   bool rst = false;
-  const int Ff = h->use_fused ? fused_tile(h->n, a.mode == 0 && !a.refinv, a.ns, &rst) : 0;
+  const int Ff = h->use_fused ? fused_tile(h->n, a.mode == 0 && !a.refinv && rlist_ld == 0, a.ns, &rst) : 0;
   a.F = Ff > 0 ? Ff : pick_F(h->n, npairs);
   a.ntiles = (npairs + a.F - 1) / a.F;
   const size_t lds = corr_lds_bytes(h->n, a.F);
@@ -2422,24 +2426,30 @@ int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, c
     for (int i = 0; i < np; ++i)
       if (!(rnd[i] >= 0.f && rnd[i] < 1.f)) return fail(h, PFTCC_E_ARG, "rnd[%d] = %g outside [0, 1)", i, rnd[i]);
   if ((rc = need_memo(h))) return rc;
-  // correlation vectors of a chunk of particles x all references (gencorrs, :1594) stay in HBM
-  const size_t per_ptcl = (size_t)nrefs * nrots * 4;
+  // Correlation vectors (gencorrs, :1594) of a chunk of particles x references stay in HBM.  The
+  // greedy kinds correlate every particle with all references.  The stochastic kinds scan their
+  // search order in rounds of kStrategyRound0, 2 kStrategyRound0, ... positions, and only the
+  // particles whose scan has not ended (no exit, limit not reached) take part in the next round.
+  const int lim = kind == PFTCC_SRCH_SNHC2D ? std::min(bound, nrefs) : nrefs;
+  const size_t per_ptcl = (size_t)(stoch ? lim : nrefs) * nrots * 4;
   const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)np, kStrategyCcBytes / per_ptcl));
+  const size_t ccap = std::max((size_t)chunk * per_ptcl, (size_t)lim * nrots * 4);
   const size_t nref_all = (size_t)np * nrefs;
+  const size_t ntab = stoch ? nref_all : (size_t)chunk * nrefs;   // per-reference maxloc tables
   if ((rc = scratch_reserve(h, aligned(np * 4) + (stoch ? aligned(nref_all * 4) : 0) + aligned(nrefs) +
-                                   2 * aligned(np * 4) + aligned((size_t)chunk * per_ptcl) +
-                                   2 * aligned((size_t)chunk * nrefs * 4) + 5 * aligned(np * 4) +
-                                   (ref_corrs ? 2 * aligned(nref_all * 4) : 0) + aligned((size_t)np * ntop * 4) +
-                                   aligned(4))))
+                                   2 * aligned(np * 4) + aligned(ccap) + 2 * aligned(ntab * 4) +
+                                   5 * aligned(np * 4) + (ref_corrs ? 2 * aligned(nref_all * 4) : 0) +
+                                   aligned((size_t)np * ntop * 4) +
+                                   (stoch ? 6 * aligned(np * 4) + aligned((size_t)np * lim * 4) : 0) + aligned(4))))
     return rc;
   int* d_pl = take<int>(h, np);
   int* d_ord = stoch ? take<int>(h, nref_all) : nullptr;
   int8_t* d_ok = ref_ok ? take<int8_t>(h, nrefs) : nullptr;
   float* d_prev = prev_corr ? take<float>(h, np) : nullptr;
   float* d_rnd = rnd ? take<float>(h, np) : nullptr;
-  float* d_cc = take<float>(h, (size_t)chunk * nrefs * nrots);
-  float* d_vb = take<float>(h, (size_t)chunk * nrefs);
-  int* d_vr = take<int>(h, (size_t)chunk * nrefs);
+  float* d_cc = take<float>(h, ccap / 4);
+  float* d_vb = take<float>(h, ntab);
+  int* d_vr = take<int>(h, ntab);
   float* d_bc = take<float>(h, np);
   int* d_br = take<int>(h, np);
   int* d_bro = take<int>(h, np);
@@ -2453,35 +2463,94 @@ int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, c
   if (d_ok) HIPCHK(h, hipMemcpyAsync(d_ok, ref_ok, nrefs, hipMemcpyHostToDevice, h->stream));
   if (d_prev) HIPCHK(h, hipMemcpyAsync(d_prev, prev_corr, (size_t)np * 4, hipMemcpyHostToDevice, h->stream));
   if (d_rnd) HIPCHK(h, hipMemcpyAsync(d_rnd, rnd, (size_t)np * 4, hipMemcpyHostToDevice, h->stream));
-  for (int p0 = 0; p0 < np; p0 += chunk) {
-    const int c = std::min(chunk, np - p0);
-    if ((rc = launch_corr(h, c, d_pl + p0, nrefs, nullptr, 1, nullptr, false, d_cc, nullptr))) return rc;
-    const long long nvec = (long long)c * nrefs;
-    hipLaunchKernelGGL(k_ref_maxloc, dim3((unsigned)((nvec + 3) / 4)), dim3(256), 0, h->stream, (const float*)d_cc,
-                       nvec, nrots, d_vb, d_vr);
-    StrategyArgs a{};
-    a.kind = kind;
-    a.nrefs = nrefs;
-    a.nrots = nrots;
-    a.bound = bound;
-    a.ntop = ntop;
-    a.sortlen = sortlen;
-    a.order = d_ord ? d_ord + (size_t)p0 * nrefs : nullptr;
-    a.ref_ok = d_ok;
-    a.prev_corr = d_prev ? d_prev + p0 : nullptr;
-    a.rnd = d_rnd ? d_rnd + p0 : nullptr;
-    a.cc = d_cc;
+  StrategyArgs a{};
+  a.kind = kind;
+  a.nrefs = nrefs;
+  a.nrots = nrots;
+  a.bound = bound;
+  a.ntop = ntop;
+  a.sortlen = sortlen;
+  a.ref_ok = d_ok;
+  if (!stoch) {
+    for (int p0 = 0; p0 < np; p0 += chunk) {
+      const int c = std::min(chunk, np - p0);
+      if ((rc = launch_corr(h, c, d_pl + p0, nrefs, nullptr, 1, nullptr, false, d_cc, nullptr))) return rc;
+      const long long nvec = (long long)c * nrefs;
+      hipLaunchKernelGGL(k_ref_maxloc, dim3((unsigned)((nvec + 3) / 4)), dim3(256), 0, h->stream, (const float*)d_cc,
+                         nvec, nrots, d_vb, d_vr);
+      a.order = nullptr;
+      a.prev_corr = d_prev ? d_prev + p0 : nullptr;
+      a.rnd = nullptr;
+      a.cc = d_cc;
+      a.vbest = d_vb;
+      a.vrot = d_vr;
+      a.best_corr = d_bc + p0;
+      a.best_ref = d_br + p0;
+      a.best_rot = d_bro + p0;
+      a.nrefs_eval = d_ne + p0;
+      a.found = d_fb + p0;
+      a.ref_corrs = d_rc ? d_rc + (size_t)p0 * nrefs : nullptr;
+      a.ref_rots = d_rr ? d_rr + (size_t)p0 * nrefs : nullptr;
+      a.sorted = d_so ? d_so + (size_t)p0 * ntop : nullptr;
+      hipLaunchKernelGGL(k_strategy, dim3((unsigned)c), dim3(256), is3d ? (size_t)sortlen * 8 : 0, h->stream, a);
+      HIPCHK(h, hipGetLastError());
+    }
+  } else {
+    int* d_act = take<int>(h, np);
+    int* d_rpl = take<int>(h, np);
+    int* d_istar = take<int>(h, np);
+    int* d_srot = take<int>(h, np);
+    float* d_scorr = take<float>(h, np);
+    int* d_done = take<int>(h, np);
+    int* d_rl = take<int>(h, (size_t)np * lim);
+    a.order = d_ord;
+    a.prev_corr = d_prev;
+    a.rnd = d_rnd;
     a.vbest = d_vb;
     a.vrot = d_vr;
-    a.best_corr = d_bc + p0;
-    a.best_ref = d_br + p0;
-    a.best_rot = d_bro + p0;
-    a.nrefs_eval = d_ne + p0;
-    a.found = d_fb + p0;
-    a.ref_corrs = d_rc ? d_rc + (size_t)p0 * nrefs : nullptr;
-    a.ref_rots = d_rr ? d_rr + (size_t)p0 * nrefs : nullptr;
-    a.sorted = d_so ? d_so + (size_t)p0 * ntop : nullptr;
-    hipLaunchKernelGGL(k_strategy, dim3((unsigned)c), dim3(256), is3d ? (size_t)sortlen * 8 : 0, h->stream, a);
+    std::vector<int> act(np), done(np);
+    for (int i = 0; i < np; ++i) act[i] = i;
+    int i0 = 0, w = kStrategyRound0;
+    while (!act.empty() && i0 < lim) {
+      const int i1 = std::min(i0 + w, lim), wr = i1 - i0;
+      const int nact = (int)act.size();
+      HIPCHK(h, hipMemcpyAsync(d_act, act.data(), (size_t)nact * 4, hipMemcpyHostToDevice, h->stream));
+      const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)nact, ccap / ((size_t)wr * nrots * 4)));
+      for (int a0 = 0; a0 < nact; a0 += sub) {
+        const int c = std::min(sub, nact - a0);
+        const long long nv = (long long)c * wr;
+        hipLaunchKernelGGL(k_round_lists, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, h->stream,
+                           (const int*)d_ord, (const int*)(d_act + a0), (const int*)d_pl, c, nrefs, i0, wr, d_rpl, d_rl);
+        if ((rc = launch_corr(h, c, d_rpl, wr, d_rl, 1, nullptr, false, d_cc, nullptr, wr))) return rc;
+        hipLaunchKernelGGL(k_round_maxloc, dim3((unsigned)((nv + 3) / 4)), dim3(256), 0, h->stream,
+                           (const float*)d_cc, c, wr, nrots, (const int*)(d_act + a0), (const int*)d_rl, nrefs, d_vb,
+                           d_vr);
+        hipLaunchKernelGGL(k_strategy_round, dim3((unsigned)c), dim3(256), 0, h->stream, a,
+                           (const int*)(d_act + a0), i0, i1, (const float*)d_cc, wr, d_istar, d_srot, d_scorr, d_done);
+        HIPCHK(h, hipGetLastError());
+      }
+      HIPCHK(h, hipMemcpyAsync(done.data(), d_done, (size_t)np * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      std::vector<int> next;
+      for (int p : act)
+        if (!done[p]) next.push_back(p);
+      act.swap(next);
+      i0 = i1;
+      w *= 2;
+    }
+    a.cc = nullptr;
+    a.istar_in = d_istar;
+    a.srot_in = d_srot;
+    a.scorr_in = d_scorr;
+    a.best_corr = d_bc;
+    a.best_ref = d_br;
+    a.best_rot = d_bro;
+    a.nrefs_eval = d_ne;
+    a.found = d_fb;
+    a.ref_corrs = d_rc;
+    a.ref_rots = d_rr;
+    a.sorted = d_so;
+    hipLaunchKernelGGL(k_strategy, dim3((unsigned)np), dim3(256), is3d ? (size_t)sortlen * 8 : 0, h->stream, a);
     HIPCHK(h, hipGetLastError());
   }
   HIPCHK(h, hipMemcpyAsync(best_corr, d_bc, (size_t)np * 4, hipMemcpyDeviceToHost, h->stream));

@@ -231,6 +231,7 @@ struct CorrArgs {
   int np;
   const int* rlist;            // 0-based reference indices or null (identity)
   int nr;
+  int rlist_ld;                // 0: one list for all particles; else particle slot pi uses rlist[pi * rlist_ld ..]
   float* cc_out;               // [np][nr][ns][2n] or null
   unsigned long long* best;    // [np] packed (cc, -lin) or null
   const float2* tw2g;          // exp(-i pi a/n), a in [0, 2n)
@@ -314,7 +315,7 @@ __global__ void __launch_bounds__(256) k_corr_generic(CorrArgs a) {
       float2 q0 = make_float2(0.f, 0.f), q1 = q0;
       if (P < npairs) {
         const int ir = P / a.ns, is = P - (P / a.ns) * a.ns;
-        const int iref = items ? a.rlist[pi] : (a.rlist ? a.rlist[ir] : ir);
+        const int iref = items ? a.rlist[pi] : (a.rlist ? a.rlist[(size_t)pi * a.rlist_ld + ir] : ir);
         const float2* r = refs + ((size_t)iref * a.nk + kk) * n;
         const int t = 2 * u;
         float2 r0 = r[t], r1 = r[t + 1];

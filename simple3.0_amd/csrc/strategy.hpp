@@ -22,12 +22,8 @@ enum { SRCH_GREEDY2D = 0, SRCH_SNHC2D = 1, SRCH_GREEDY3D = 2, SRCH_SHC3D = 3 };
 
 // maxloc over the rotations of each correlation vector (one wavefront per vector): the largest
 // value, ties to the lowest rotation as Fortran's maxloc.  cc [nvec][nrots].
-__global__ void __launch_bounds__(256) k_ref_maxloc(const float* __restrict__ cc, long long nvec, int nrots,
-                                                    float* __restrict__ vbest, int* __restrict__ vrot) {
-  const long long v = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (v >= nvec) return;
+__device__ __forceinline__ void wave_maxloc(const float* __restrict__ row, int nrots, float& bo, int& io) {
   const int lane = threadIdx.x & 63;
-  const float* row = cc + v * nrots;
   float b = -FLT_MAX;
   int bi = INT_MAX;
   for (int r = lane; r < nrots; r += 64) {
@@ -46,9 +42,50 @@ __global__ void __launch_bounds__(256) k_ref_maxloc(const float* __restrict__ cc
       bi = i2;
     }
   }
-  if (lane == 0) {
+  bo = b;
+  io = bi;
+}
+
+__global__ void __launch_bounds__(256) k_ref_maxloc(const float* __restrict__ cc, long long nvec, int nrots,
+                                                    float* __restrict__ vbest, int* __restrict__ vrot) {
+  const long long v = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= nvec) return;
+  float b;
+  int bi;
+  wave_maxloc(cc + v * nrots, nrots, b, bi);
+  if ((threadIdx.x & 63) == 0) {
     vbest[v] = b;
     vrot[v] = bi;
+  }
+}
+
+// A round of the stochastic scans: active particle a (global index act[a]) correlates with the
+// w references at positions [i0, i0 + w) of its search order.  Builds the launch's particle and
+// per-particle reference lists.
+__global__ void k_round_lists(const int* __restrict__ order, const int* __restrict__ act,
+                              const int* __restrict__ plocal, int nact, int nrefs, int i0, int w,
+                              int* __restrict__ rpl, int* __restrict__ rl) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nact * w) return;
+  const int a = (int)(t / w), c = (int)(t - (long long)a * w);
+  const int p = act[a];
+  rl[t] = order[(size_t)p * nrefs + i0 + c];
+  if (c == 0) rpl[a] = plocal[p];
+}
+
+// maxloc of a round's vectors [nact][w][nrots] into the particles' per-reference tables.
+__global__ void __launch_bounds__(256) k_round_maxloc(const float* __restrict__ cc, int nact, int w, int nrots,
+                                                      const int* __restrict__ act, const int* __restrict__ rl,
+                                                      int nrefs, float* __restrict__ vbest, int* __restrict__ vrot) {
+  const long long v = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= (long long)nact * w) return;
+  float b;
+  int bi;
+  wave_maxloc(cc + v * nrots, nrots, b, bi);
+  if ((threadIdx.x & 63) == 0) {
+    const size_t o = (size_t)act[v / w] * nrefs + rl[v];
+    vbest[o] = b;
+    vrot[o] = bi;
   }
 }
 
@@ -73,6 +110,11 @@ struct StrategyArgs {
   float* ref_corrs;            // [np][nrefs] or null: proj_space_corrs, -FLT_MAX where unsearched
   int* ref_rots;               // [np][nrefs] or null: 1-based, 0 where unsearched
   int* sorted;                 // [np][ntop] or null: 1-based references, best first
+  // scans resolved by the rounds (k_strategy_round): exit position (INT_MAX: none), and for snhc
+  // shcloc's rotation and its correlation; null: k_strategy scans itself
+  const int* istar_in;
+  const int* srot_in;
+  const float* scorr_in;
 };
 
 __device__ __forceinline__ unsigned int strat_bits(float v) {
@@ -91,6 +133,48 @@ __device__ __forceinline__ T block_reduce(T v, T* sh, Op op) {
   T r = sh[0];
   for (int i = 1; i < nw; ++i) r = op(r, sh[i]);
   return r;
+}
+
+// shcloc (simple_rnd.f90:273-295): a uniformly drawn rotation among those with
+// corrs - prev_corr >= 0, here the floor(u count)-th of them in rotation order.  Whole work-group;
+// -1 if none.
+__device__ int shcloc_pick(const float* __restrict__ row, int nrots, float prev, float u, int* scnt, int* srot) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int seg = (nrots + nt - 1) / nt;
+  const int j0 = min(tid * seg, nrots), j1 = min(j0 + seg, nrots);
+  int cnt = 0;
+  for (int j = j0; j < j1; ++j) cnt += row[j] >= prev;
+  __syncthreads();
+  scnt[tid] = cnt;
+  if (tid == 0) *srot = -1;
+  __syncthreads();
+  if (tid == 0) {
+    int s = 0;
+    for (int t = 0; t < nt; ++t) {
+      const int c = scnt[t];
+      scnt[t] = s;
+      s += c;
+    }
+    scnt[0] = s;   // thread 0's exclusive prefix is 0; the total is kept in its slot
+  }
+  __syncthreads();
+  const int total = scnt[0];
+  const int excl = tid == 0 ? 0 : scnt[tid];
+  int k = (int)floor((double)u * (double)total);
+  k = max(0, min(k, total - 1));
+  if (total > 0 && k >= excl && k < excl + cnt) {
+    int c = excl;
+    for (int j = j0; j < j1; ++j)
+      if (row[j] >= prev) {
+        if (c == k) {
+          *srot = j;
+          break;
+        }
+        ++c;
+      }
+  }
+  __syncthreads();
+  return *srot;
 }
 
 // One work-group per particle.  The scan limit, the first improving position, the strategy's
@@ -114,7 +198,9 @@ __global__ void __launch_bounds__(256) k_strategy(StrategyArgs a) {
   // positions [0, lim) may be searched; the stochastic kinds stop at the first improving one
   const int lim = a.kind == SRCH_SNHC2D ? min(a.bound, nrefs) : nrefs;
   int istar = INT_MAX;
-  if (stoch) {
+  if (stoch && a.istar_in) {
+    istar = a.istar_in[p];
+  } else if (stoch) {
     int mine = INT_MAX;
     for (int i = tid; i < lim; i += nt) {
       const int r = ref_at(i);
@@ -165,48 +251,22 @@ __global__ void __launch_bounds__(256) k_strategy(StrategyArgs a) {
       a.best_corr[p] = bref >= 0 ? vb[bref] : prev;
     }
   } else if (a.kind == SRCH_SNHC2D) {
-    // shcloc (simple_rnd.f90:273-295): a uniformly drawn rotation among those with
-    // corrs - prev_corr >= 0, here the floor(u count)-th of them in rotation order
     bref = ref_at(istar);
-    const float* row = a.cc + ((size_t)p * nrefs + bref) * a.nrots;
-    const int seg = (a.nrots + nt - 1) / nt;
-    const int j0 = min(tid * seg, a.nrots), j1 = min(j0 + seg, a.nrots);
-    int cnt = 0;
-    for (int j = j0; j < j1; ++j) cnt += row[j] >= prev;
-    scnt[tid] = cnt;
-    if (tid == 0) srot = -1;
-    __syncthreads();
-    if (tid == 0) {
-      int s = 0;
-      for (int t = 0; t < nt; ++t) {
-        const int c = scnt[t];
-        scnt[t] = s;
-        s += c;
-      }
-      scnt[0] = s;   // thread 0's exclusive prefix is 0; the total is kept in its slot
+    int rot;
+    float corr;
+    if (a.srot_in) {
+      rot = a.srot_in[p];
+      corr = a.scorr_in[p];
+    } else {
+      const float* row = a.cc + ((size_t)p * nrefs + bref) * a.nrots;
+      rot = shcloc_pick(row, a.nrots, prev, a.rnd[p], scnt, &srot);
+      if (rot < 0) rot = vr[bref];
+      corr = row[rot];
     }
-    __syncthreads();
-    const int total = scnt[0];
-    const int excl = tid == 0 ? 0 : scnt[tid];
-    int k = (int)floor((double)a.rnd[p] * (double)total);
-    k = max(0, min(k, total - 1));
-    if (total > 0 && k >= excl && k < excl + cnt) {
-      int c = excl;
-      for (int j = j0; j < j1; ++j)
-        if (row[j] >= prev) {
-          if (c == k) {
-            srot = j;
-            break;
-          }
-          ++c;
-        }
-    }
-    __syncthreads();
     if (tid == 0) {
-      const int rot = srot >= 0 ? srot : vr[bref];
       a.best_ref[p] = bref + 1;
       a.best_rot[p] = rot + 1;
-      a.best_corr[p] = row[rot];
+      a.best_corr[p] = corr;
     }
   } else {
     // sort_corrs: the searched references by best correlation, unsearched at -huge; ties to the
@@ -260,6 +320,51 @@ __global__ void __launch_bounds__(256) k_strategy(StrategyArgs a) {
   if (tid == 0) {
     a.nrefs_eval[p] = ne;
     a.found[p] = found ? 1 : 0;
+  }
+}
+
+// One work-group per active particle of a round: the first improving position in [i0, i1) of
+// the scan limit, with shcloc's rotation for snhc (the round's vectors cc [nact][w][nrots] hold
+// the exit reference's); done once the exit is found or the scan limit is reached.
+__global__ void __launch_bounds__(256) k_strategy_round(StrategyArgs a, const int* __restrict__ act, int i0, int i1,
+                                                        const float* __restrict__ cc, int w, int* __restrict__ istar,
+                                                        int* __restrict__ srot_out, float* __restrict__ scorr,
+                                                        int* __restrict__ done) {
+  __shared__ int sh32[4];
+  __shared__ int scnt[256];
+  __shared__ int srot;
+  const int ia = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int p = act[ia];
+  const int nrefs = a.nrefs;
+  const int* ord = a.order + (size_t)p * nrefs;
+  const float* vb = a.vbest + (size_t)p * nrefs;
+  const float prev = a.prev_corr[p];
+  const int lim = a.kind == SRCH_SNHC2D ? min(a.bound, nrefs) : nrefs;
+  const int iend = min(i1, lim);
+  int mine = INT_MAX;
+  for (int i = i0 + tid; i < iend; i += nt) {
+    const int r = ord[i];
+    if (a.ref_ok == nullptr || a.ref_ok[r] != 0) {
+      const float c = vb[r];
+      if (a.kind == SRCH_SNHC2D ? (c >= prev) : (c > prev)) {
+        mine = i;
+        break;
+      }
+    }
+  }
+  const int is = block_reduce(mine, sh32, [](int x, int y) { return min(x, y); });
+  if (is != INT_MAX && a.kind == SRCH_SNHC2D) {
+    const float* row = cc + ((size_t)ia * w + (is - i0)) * a.nrots;
+    int rot = shcloc_pick(row, a.nrots, prev, a.rnd[p], scnt, &srot);
+    if (rot < 0) rot = a.vrot[(size_t)p * nrefs + ord[is]];
+    if (tid == 0) {
+      srot_out[p] = rot;
+      scorr[p] = row[rot];
+    }
+  }
+  if (tid == 0) {
+    istar[p] = is;
+    done[p] = (is != INT_MAX || iend >= lim) ? 1 : 0;
   }
 }
 
