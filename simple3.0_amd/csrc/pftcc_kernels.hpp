@@ -582,8 +582,8 @@ __device__ __forceinline__ void rot8_step(const Rot8Args a, const Rot8Shell& s, 
   shi = i2;
 }
 
-#ifndef PFTCC_ROT8_UNROLL
-#define PFTCC_ROT8_UNROLL 2
+#ifndef PFTCC_ROT8_AHEAD
+#define PFTCC_ROT8_AHEAD 1     // shells of loads in flight ahead of the arithmetic in k_rot8
 #endif
 
 // FP64 sums of |P|^2 per (particle, shell): one wavefront per ring.
@@ -598,10 +598,11 @@ __global__ void __launch_bounds__(256) k_shell_sq8(const float2* __restrict__ ri
   if ((threadIdx.x & 63) == 0) out[it] = s;
 }
 
-// The shells run in groups of U: the loads of a group are issued before its arithmetic, so each
-// thread keeps U shells of operands in flight.  MODE: 0 CC k-weighted, 1 CC match_filt, 2 Euclid
-// (equal to a.mode; a template argument so that each form's sums stay in registers).
-template <int U, int MODE>
+// Threads own angles j and run the shells in order.  The loads of shell k + D are issued before
+// the arithmetic of shell k (a software pipeline over a ring of D + 1 shells), so a thread always
+// has loads in flight.  MODE: 0 CC k-weighted, 1 CC match_filt, 2 Euclid (equal to a.mode; a
+// template argument so that each form's sums stay in registers).
+template <int D, int MODE>
 __global__ void __launch_bounds__(512) k_rot8(Rot8Args a) {
   const int item = xcd_remap(blockIdx.x, gridDim.x);
   if (item >= a.npairs) return;
@@ -631,19 +632,27 @@ __global__ void __launch_bounds__(512) k_rot8(Rot8Args a) {
     bool cj = false;
     if (t >= 2 * n) t -= 2 * n;
     if (t >= n) { t -= n; cj = true; }
+    // a ring of D + 1 shells: shell m lives in slot m % (D + 1); the loads of shell m + D are
+    // issued before the arithmetic of shell m
+    const int kc = a.kcorr;
+    Rot8Shell s[D + 1];
+#pragma unroll
+    for (int u = 0; u < D; ++u)
+      if (u < kc) rot8_load<MODE>(a, s[u], u, j, t, ip, iref, even, ref, ptc, dpt, ctf);
     int kk = 0;
-    for (; kk + U <= a.kcorr; kk += U) {
-      Rot8Shell s[U];
+    for (; kk + D + 1 <= kc; kk += D + 1) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) rot8_load<MODE>(a, s[u], kk + u, j, t, ip, iref, even, ref, ptc, dpt, ctf);
+      for (int u = 0; u <= D; ++u) {
+        if (kk + u + D < kc) rot8_load<MODE>(a, s[(u + D) % (D + 1)], kk + u + D, j, t, ip, iref, even, ref, ptc, dpt, ctf);
+        rot8_step<MODE>(a, s[u], kk + u, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
+      }
+    }
 #pragma unroll
-      for (int u = 0; u < U; ++u) rot8_step<MODE>(a, s[u], kk + u, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
-    }
-    for (; kk < a.kcorr; ++kk) {
-      Rot8Shell s;
-      rot8_load<MODE>(a, s, kk, j, t, ip, iref, even, ref, ptc, dpt, ctf);
-      rot8_step<MODE>(a, s, kk, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
-    }
+    for (int u = 0; u <= D; ++u)
+      if (kk + u < kc) {
+        if (kk + u + D < kc) rot8_load<MODE>(a, s[(u + D) % (D + 1)], kk + u + D, j, t, ip, iref, even, ref, ptc, dpt, ctf);
+        rot8_step<MODE>(a, s[u], kk + u, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
+      }
   }
   // wavefront sums, then the waves' partial sums in a fixed order
   __shared__ double red[6][16];
