@@ -31,11 +31,11 @@ METRIC = ("ptcl×ref×rot corr evals/s: plain FP32 vs split-BF16 MFMA @ rel err 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
-# HBM bytes per launch of k_corr_fused<308,11,256,3,true> at 1000 particles x 200 refs x 169 shifts,
+# HBM bytes per launch of k_corr_fused<308,9,256,3,true> at 1000 particles x 200 refs x 169 shifts,
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script (--legs plain --copy-calibration):
-# FETCH 1292696794 KiB x 2.000 + WRITE 96031 KiB x 1.000, the factors read from k_copy moving
+# FETCH 1319768133 KiB x 2.000 + WRITE 117375 KiB x 1.000, the factors read from k_copy moving
 # 1 GiB in the same runs (profiles/k_corr_fused_308_plain_search.md).
-PLAIN_TRAFFIC_BYTES_B1000 = (1292696793.63 * 2.0 + 96031.25) * 1024.0
+PLAIN_TRAFFIC_BYTES_B1000 = (1319768132.5 * 2.0 + 117375.0) * 1024.0
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -667,7 +667,7 @@ def main():
                      "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated; "
                                      "compulsory bytes per launch are in hbm_compulsory_bytes",
                      "hbm_compulsory_bytes": bytes_launch,
-                     "kernel": "k_corr_fused<308,11,256,3,true>", "avg_launch_ms": avg_ms, "launches": launches,
+                     "kernel": "k_corr_fused<308,9,256,3,true>", "avg_launch_ms": avg_ms, "launches": launches,
                      "flops_per_pair": plain_flops_per_pair(pftsz, w.nk), "pairs_per_launch": pairs_launch,
                      "hbm_compulsory_GBs": bytes_launch / (avg_ms * 1e-3) / 1e9,
                      "hbm_frac": bytes_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},

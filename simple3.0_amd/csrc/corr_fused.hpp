@@ -121,11 +121,16 @@ constexpr int fused_F_tuned() {
   if (NT == 256 && (N == 192 || N == 308)) return N == 192 ? 5 : 9;
   return fused_F<N, NT>();
 }
-// Tile of the staged-reference variant: its pass 1 holds no per-pair reference pointers or norm
-// sums, which leaves the registers for two more pairs at N = 308 (no spills at F = 11).
+#ifndef PFTCC_FUSED_PREFETCH
+#define PFTCC_FUSED_PREFETCH 1 // the next shell's staged rows (|CTF|, reference rows, spectrum) loaded ahead
+#endif
+
+// Tile of the staged-reference variant.  Its pass 1 holds no per-pair reference pointers or norm
+// sums: at N = 308 that leaves room for 11 pairs without the staging prefetch, and for 9 pairs
+// with it (160 VGPRs; 11 pairs then spill), which was measured faster.
 template <int N, int NT>
 constexpr int fused_F_rst_tuned() {
-  if (NT == 256 && N == 308) return 11;
+  if (NT == 256 && N == 308) return PFTCC_FUSED_PREFETCH ? 9 : 11;
   return fused_F_tuned<N, NT>();
 }
 template <int N, int NT>
@@ -300,12 +305,76 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     for (int b = 0; b < NB; ++b) acc[c][b] = make_float2(0.f, 0.f);
 
   const int kloop = a.kcorr > a.knorm ? a.kcorr : a.knorm;
+  // the staging operands of shell kk + 1, loaded while shell kk runs
+  constexpr int JP = (N + NT - 1) / NT, MP = (N + 1 + NT - 1) / NT;
+  float pcv[JP];
+  float2 pr0[JP], pr1[JP], pwm[MP];
+  auto fetch_stage = [&](int k) {
+    const size_t pr = (size_t)ip * nk + k;
+#pragma unroll
+    for (int u = 0; u < JP; ++u) {
+      const int j = tid + u * NT;
+      if (j < N) {
+        pcv[u] = a.ctf ? a.ctf[pr * N + j] : 1.0f;
+        if constexpr (RST) {
+          pr0[u] = rs0[(size_t)k * N + j];
+          pr1[u] = rs1[(size_t)k * N + j];
+        }
+      }
+    }
+    if (k < a.kcorr)
+#pragma unroll
+      for (int u = 0; u < MP; ++u) {
+        const int m = tid + u * NT;
+        if (m <= N) pwm[u] = a.memo[pr * (N + 1) + m];
+      }
+  };
+  if constexpr (PFTCC_FUSED_PREFETCH)
+    if (kloop > 0) fetch_stage(0);
   for (int kk = 0; kk < kloop; ++kk) {
     const bool docorr = kk < a.kcorr;
     const bool donorm = kk < a.knorm;
     __syncthreads();   // the previous shell's last pass has finished with buf and Wk
     const size_t prow = (size_t)ip * nk + kk;
-    if constexpr (RST) {
+    if constexpr (PFTCC_FUSED_PREFETCH) {
+      // the prefetched operands go to LDS, then the next shell's loads are issued
+      const float wkp = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
+#pragma unroll
+      for (int u = 0; u < JP; ++u) {
+        const int j = tid + u * NT;
+        if (j < N) {
+          if constexpr (RST) {
+            const float cv = pcv[u];
+            const float2 q0 = make_float2(pr0[u].x * cv, pr0[u].y * cv), q1 = make_float2(pr1[u].x * cv, pr1[u].y * cv);
+            rck[j] = q0;
+            rck[N + j] = q1;
+            if (donorm) {
+              rsum0 += (double)(q0.x * q0.x + q0.y * q0.y);
+              rsum1 += (double)(q1.x * q1.x + q1.y * q1.y);
+            }
+          } else {
+            ctfk[j] = pcv[u];
+          }
+        }
+      }
+      if (docorr)
+#pragma unroll
+        for (int u = 0; u < MP; ++u) {
+          const int m = tid + u * NT;
+          if (m <= N) {
+            float2 wh = make_float2(0.5f * pwm[u].x, 0.5f * pwm[u].y);
+            if constexpr (PFTCC_FUSED_W12) {
+              if (a.mode == 1) wh = make_float2(wkp * wh.x, wkp * wh.y);
+              const float2 t = tw2[m];
+              Wk[m] = cmul(make_float2(1.f + t.y, t.x), wh);
+              Wk[N + 1 + m] = cmul(make_float2(1.f - t.y, -t.x), wh);
+            } else {
+              Wk[m] = wh;
+            }
+          }
+        }
+      if (kk + 1 < kloop) fetch_stage(kk + 1);
+    } else if constexpr (RST) {
       for (int j = tid; j < N; j += NT) {
         const float cv = a.ctf ? a.ctf[prow * N + j] : 1.0f;
         const float2 r0 = rs0[(size_t)kk * N + j], r1 = rs1[(size_t)kk * N + j];
@@ -321,7 +390,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       for (int j = tid; j < N; j += NT) ctfk[j] = a.ctf ? a.ctf[prow * N + j] : 1.0f;
     }
     const float wk = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
-    if (docorr)
+    if (!PFTCC_FUSED_PREFETCH && docorr)
       for (int m = tid; m <= N; m += NT) {
         const float2 w = a.memo[prow * (N + 1) + m];
         float2 wh = make_float2(0.5f * w.x, 0.5f * w.y);
@@ -622,6 +691,12 @@ namespace pfk {
 #ifndef PFTCC_ITEMS_W
 #define PFTCC_ITEMS_W 1        // minimum wavefronts per SIMD imposed on the register allocation
 #endif
+#ifndef PFTCC_ITEMS_PREFETCH
+#define PFTCC_ITEMS_PREFETCH 1 // pass 1 loads the next shell's reference and |CTF| values ahead
+#endif
+#ifndef PFTCC_ITEMS_PREFETCH_W
+#define PFTCC_ITEMS_PREFETCH_W 1   // the last pass loads the next shell's memoized spectra ahead
+#endif
 
 template <int N>
 constexpr int items_F() {
@@ -736,6 +811,44 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PFTCC_I
     for (int b = 0; b < NB; ++b) acc[c][b] = make_float2(0.f, 0.f);
 
   const int kloop = a.kcorr > a.knorm ? a.kcorr : a.knorm;
+  // pass-1 operands of the next shell, loaded while the current shell's passes run
+  float4 pra[TPT1], prb[TPT1];
+  float2 pc01[TPT1], pc23[TPT1];
+  auto fetch1 = [&](int c, int k) {
+    const int j = (tid + c * NT) % Q;
+    const float2* r = rrow[c] + (size_t)k * N;
+    pra[c] = *reinterpret_cast<const float4*>(r + 2 * j);
+    prb[c] = *reinterpret_cast<const float4*>(r + 2 * j + N / 2);
+    if (crow[c]) {
+      pc01[c] = *reinterpret_cast<const float2*>(crow[c] + (size_t)k * N + 2 * j);
+      pc23[c] = *reinterpret_cast<const float2*>(crow[c] + (size_t)k * N + 2 * j + N / 2);
+    } else {
+      pc01[c] = pc23[c] = make_float2(1.f, 1.f);
+    }
+  };
+  if constexpr (PFTCC_ITEMS_PREFETCH) {
+#pragma unroll
+    for (int c = 0; c < TPT1; ++c)
+      if (slotok[c] && kloop > 0) fetch1(c, 0);
+  }
+  // last-pass spectra of the next corr shell (bins g + t NsL, NsL - g + t NsL and N of task c)
+  float2 pw[TPTL][NB];
+  auto fetchw = [&](int c, int k) {
+    const int task = tid + c * NT;
+    const int g = task % G;
+    const bool pair = g != 0 && 2 * g != NsL;
+    const float2* wrow = a.memo + ((size_t)lip[c] * nk + k) * (N + 1);
+#pragma unroll
+    for (int t = 0; t < RL; ++t) pw[c][t] = lok[c] ? wrow[g + t * NsL] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int t = 0; t < RL; ++t) pw[c][RL + t] = (lok[c] && pair) ? wrow[NsL - g + t * NsL] : make_float2(0.f, 0.f);
+    pw[c][2 * RL] = (lok[c] && g == 0) ? wrow[N] : make_float2(0.f, 0.f);
+  };
+  if constexpr (PFTCC_ITEMS_PREFETCH_W) {
+#pragma unroll
+    for (int c = 0; c < TPTL; ++c)
+      if ((TL % NT == 0 || tid + c * NT < TL) && a.kcorr > 0) fetchw(c, 0);
+  }
   for (int kk = 0; kk < kloop; ++kk) {
     const bool docorr = kk < a.kcorr;
     const bool donorm = kk < a.knorm;
@@ -749,13 +862,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PFTCC_I
       float2 y[4];
       if (slotok[c]) {
         const int ip = slotip[c];
-        const float2* r = rrow[c] + (size_t)kk * N;
-        const float4 ra = *reinterpret_cast<const float4*>(r + 2 * j);
-        const float4 rb = *reinterpret_cast<const float4*>(r + 2 * j + N / 2);
+        float4 ra, rb;
         float2 c01 = make_float2(1.f, 1.f), c23 = c01;
-        if (crow[c]) {
-          c01 = *reinterpret_cast<const float2*>(crow[c] + (size_t)kk * N + 2 * j);
-          c23 = *reinterpret_cast<const float2*>(crow[c] + (size_t)kk * N + 2 * j + N / 2);
+        if constexpr (PFTCC_ITEMS_PREFETCH) {
+          ra = pra[c];
+          rb = prb[c];
+          c01 = pc01[c];
+          c23 = pc23[c];
+          if (kk + 1 < kloop) fetch1(c, kk + 1);
+        } else {
+          const float2* r = rrow[c] + (size_t)kk * N;
+          ra = *reinterpret_cast<const float4*>(r + 2 * j);
+          rb = *reinterpret_cast<const float4*>(r + 2 * j + N / 2);
+          if (crow[c]) {
+            c01 = *reinterpret_cast<const float2*>(crow[c] + (size_t)kk * N + 2 * j);
+            c23 = *reinterpret_cast<const float2*>(crow[c] + (size_t)kk * N + 2 * j + N / 2);
+          }
         }
         float2 q0 = make_float2(ra.x * c01.x, ra.y * c01.x), q1 = make_float2(ra.z * c01.y, ra.w * c01.y);
         float2 q2 = make_float2(rb.x * c23.x, rb.y * c23.x), q3 = make_float2(rb.z * c23.y, rb.w * c23.y);
@@ -811,8 +933,16 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PFTCC_I
       const int j2 = NsL - g;
       const float2* wrow = a.memo + ((size_t)lip[c] * nk + kk) * (N + 1);
       const float wk = (a.mode == 1 && lok[c]) ? a.shellw[(size_t)lip[c] * nk + kk] : 1.0f;
-      auto W = [&](int m) {
-        const float2 w = lok[c] ? wrow[m] : make_float2(0.f, 0.f);
+      float2 wcur[NB];
+      if constexpr (PFTCC_ITEMS_PREFETCH_W) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) wcur[b] = pw[c][b];
+        if (kk + 1 < a.kcorr) fetchw(c, kk + 1);
+      }
+      // the spectrum at bin m, held in slot t (m = g + t NsL), RL + t (m = NsL - g + t NsL) or
+      // 2 RL (m = N)
+      auto Wb = [&](int slot, int m) {
+        const float2 w = PFTCC_ITEMS_PREFETCH_W ? wcur[slot] : (lok[c] ? wrow[m] : make_float2(0.f, 0.f));
         return make_float2(0.5f * w.x, 0.5f * w.y);
       };
       float2 y1[RL], y2[RL];
@@ -837,7 +967,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PFTCC_I
         const float2 b0 = y1[(RL - t) % RL], b1 = y2[RL - 1 - t], b2 = y1[RL - 1 - t];
         const float2 B = vsel(g == 0, b0, vsel(pair, b1, b2));
         const float2 V = split_bin2(y1[t], B, tw2[m]);
-        const float2 p = cmulcj(V, W(m));
+        const float2 p = cmulcj(V, Wb(t, m));
         acc[c][t].x = fmaf(wk, p.x, acc[c][t].x);
         acc[c][t].y = fmaf(wk, p.y, acc[c][t].y);
       }
@@ -846,14 +976,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PFTCC_I
         for (int t = 0; t < RL; ++t) {
           const int m = j2 + t * NsL;
           const float2 V = split_bin2(y2[t], y1[RL - 1 - t], tw2[m]);
-          const float2 p = cmulcj(V, W(m));
+          const float2 p = cmulcj(V, Wb(RL + t, m));
           acc[c][RL + t].x = fmaf(wk, p.x, acc[c][RL + t].x);
           acc[c][RL + t].y = fmaf(wk, p.y, acc[c][RL + t].y);
         }
       }
       if (g == 0) {
         const float2 V = split_bin2(y1[0], y1[0], tw2[N]);
-        const float2 p = cmulcj(V, W(N));
+        const float2 p = cmulcj(V, Wb(2 * RL, N));
         acc[c][2 * RL].x = fmaf(wk, p.x, acc[c][2 * RL].x);
         acc[c][2 * RL].y = fmaf(wk, p.y, acc[c][2 * RL].y);
       }
