@@ -2432,7 +2432,11 @@ int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, c
   // particles whose scan has not ended (no exit, limit not reached) take part in the next round.
   const int lim = kind == PFTCC_SRCH_SNHC2D ? std::min(bound, nrefs) : nrefs;
   const size_t per_ptcl = (size_t)(stoch ? lim : nrefs) * nrots * 4;
-  const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)np, kStrategyCcBytes / per_ptcl));
+  // PFTCC_STRATEGY_CC_BYTES overrides the budget of the correlation vectors held at once (tests
+  // use a small one to exercise the chunking)
+  size_t budget = kStrategyCcBytes;
+  if (const char* e = std::getenv("PFTCC_STRATEGY_CC_BYTES")) budget = std::max<size_t>(1, std::strtoull(e, nullptr, 10));
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)np, budget / per_ptcl));
   const size_t ccap = std::max((size_t)chunk * per_ptcl, (size_t)lim * nrots * 4);
   const size_t nref_all = (size_t)np * nrefs;
   const size_t ntab = stoch ? nref_all : (size_t)chunk * nrefs;   // per-reference maxloc tables
@@ -2515,7 +2519,7 @@ int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, c
       const int i1 = std::min(i0 + w, lim), wr = i1 - i0;
       const int nact = (int)act.size();
       HIPCHK(h, hipMemcpyAsync(d_act, act.data(), (size_t)nact * 4, hipMemcpyHostToDevice, h->stream));
-      const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)nact, ccap / ((size_t)wr * nrots * 4)));
+      const int sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)nact, std::min(ccap, budget) / ((size_t)wr * nrots * 4)));
       for (int a0 = 0; a0 < nact; a0 += sub) {
         const int c = std::min(sub, nact - a0);
         const long long nv = (long long)c * wr;

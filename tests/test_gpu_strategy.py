@@ -6,6 +6,8 @@ strategy call runs (their values are checked against the FP64 oracle in test_gpu
 every output here is integer or selection work on identical FP32 values and must agree exactly:
 references, rotations, correlations, counts, the searched set and the sorted order.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -107,4 +109,29 @@ def test_greedy_3d_equals_device_search_and_errors():
         lib.strategy_srch(SRCH_SNHC2D, ids, srch_order=order + 1, prev_corr=prev)
     with pytest.raises(PftccError, match="outside"):
         lib.strategy_srch(SRCH_SNHC2D, ids, srch_order=order + 1, prev_corr=prev, rnd=np.ones(len(ids)))
+    lib.kill()
+
+
+def test_chunked_calls_equal_one_chunk():
+    """A correlation budget of a few vectors splits the greedy kinds into many particle chunks and
+    the rounds of the stochastic kinds into sub-chunks: every output is unchanged."""
+    shape = SHAPES[1]
+    say("strategy chunking: budget of 3 particles' vectors")
+    lib, ids, cc, ok, order, prev, rnd = _setup(shape)
+    nrots = cc.shape[2]
+    runs = [dict(kind=SRCH_GREEDY2D, ref_ok=ok, ref_corrs=True),
+            dict(kind=SRCH_SNHC2D, srch_order=order + 1, ref_ok=ok, prev_corr=prev, rnd=rnd, stoch_bound=0.3,
+                 ref_corrs=True),
+            dict(kind=SRCH_GREEDY3D, ref_ok=ok, ntop=7, ref_corrs=True),
+            dict(kind=SRCH_SHC3D, srch_order=order + 1, ref_ok=ok, prev_corr=prev, ntop=7, ref_corrs=True)]
+    base = [lib.strategy_srch(r["kind"], ids, **{k: v for k, v in r.items() if k != "kind"}) for r in runs]
+    os.environ["PFTCC_STRATEGY_CC_BYTES"] = str(3 * 40 * nrots * 4)
+    try:
+        small = [lib.strategy_srch(r["kind"], ids, **{k: v for k, v in r.items() if k != "kind"}) for r in runs]
+    finally:
+        del os.environ["PFTCC_STRATEGY_CC_BYTES"]
+    for b, s_ in zip(base, small):
+        assert b.keys() == s_.keys()
+        for k in b:
+            np.testing.assert_array_equal(b[k], s_[k], err_msg=k)
     lib.kill()
