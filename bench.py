@@ -437,9 +437,10 @@ def precision_same_workload(lib, w, shifts, plain_value, nptcls, sample=1000, nt
 def strategy_form(lib, w, nptcls=10000, reps=2):
     """The reference-loop search strategies (pftcc_strategy_srch) over the main line's references
     and particles: every particle against all references at shift 0 (gencorrs), then the greedy /
-    snhc 2-D and greedy / shc 3-D choices.  The stochastic kinds get a random search order with
-    the previous class last (put_last) and prev_corr at 0.9 of the greedy best."""
-    from pftcc import SRCH_GREEDY2D, SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC2D
+    snhc 2-D and greedy / shc / snhc 3-D choices.  The stochastic kinds get a random search order
+    with the previous class last (put_last) and prev_corr at 0.9 of the greedy best; snhc 3-D
+    searches the first fifth of the order."""
+    from pftcc import SRCH_GREEDY2D, SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC2D, SRCH_SNHC3D
 
     n = min(nptcls, w.nptcls)
     ids = np.arange(1, n + 1, dtype=np.int32)
@@ -456,7 +457,8 @@ def strategy_form(lib, w, nptcls=10000, reps=2):
     kinds = {"greedy2d": dict(kind=SRCH_GREEDY2D),
              "snhc2d": dict(kind=SRCH_SNHC2D, srch_order=order, prev_corr=prev, rnd=rnd, stoch_bound=0.0),
              "greedy3d": dict(kind=SRCH_GREEDY3D, ntop=10),
-             "shc3d": dict(kind=SRCH_SHC3D, srch_order=order, prev_corr=prev, ntop=10)}
+             "shc3d": dict(kind=SRCH_SHC3D, srch_order=order, prev_corr=prev, ntop=10),
+             "snhc3d": dict(kind=SRCH_SNHC3D, srch_order=order, ntop=10, szsn=w.nrefs // 5)}
     nrots = 2 * w.pftsz
     res = {}
     for name, kw in kinds.items():
@@ -476,6 +478,8 @@ def strategy_form(lib, w, nptcls=10000, reps=2):
         if name in ("snhc2d", "shc3d"):
             ends = np.cumsum(32 * 2 ** np.arange(16))
             pos = np.minimum(ends[np.searchsorted(ends, out["nrefs_eval"])], w.nrefs)
+        elif name == "snhc3d":
+            pos = out["nrefs_eval"].astype(np.int64)
         else:
             pos = np.full(n, w.nrefs)
         res[name] = {"ms_per_call": 1e3 * wall, "evals_per_s": float(pos.sum()) * nrots / wall,

@@ -194,6 +194,8 @@ int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr
  *   PFTCC_SRCH_GREEDY3D  strategy3D_greedy%srch (simple_strategy3D_greedy.f90:40-77);
  *   PFTCC_SRCH_SHC3D     strategy3D_shc%srch (simple_strategy3D_shc.f90:36-83): srch_order, exit
  *                        after the first reference whose best exceeds prev_corr;
+ *   PFTCC_SRCH_SNHC3D    strategy3D_snhc%srch (simple_strategy3D_snhc.f90:37-60): the first
+ *                        cfg->szsn references of srch_order, every one searched (no state test);
  * the 3-D kinds then order the searched references by best correlation (sort_corrs,
  * simple_strategy3D_utils.f90:100-124; unsearched at -huge, ties to the lower index).
  * srch_order[np][nrefs]: 1-based permutations (the stochastic kinds; ignored by the greedy ones).
@@ -207,11 +209,13 @@ int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr
  * (proj_space_corrs / inplinds: -FLT_MAX and 0 where unsearched); sorted_refs [np][cfg->ntop]
  * (3-D kinds: 1-based references, best first).  No counterpart entry point: it replaces the
  * per-particle srch procedures above. */
-enum { PFTCC_SRCH_GREEDY2D = 0, PFTCC_SRCH_SNHC2D = 1, PFTCC_SRCH_GREEDY3D = 2, PFTCC_SRCH_SHC3D = 3 };
+enum { PFTCC_SRCH_GREEDY2D = 0, PFTCC_SRCH_SNHC2D = 1, PFTCC_SRCH_GREEDY3D = 2, PFTCC_SRCH_SHC3D = 3,
+       PFTCC_SRCH_SNHC3D = 4 };
 typedef struct {
   int32_t kind;        /* PFTCC_SRCH_* */
-  float stoch_bound;   /* snhc: spec%stoch_bound, in [0, 1] */
+  float stoch_bound;   /* snhc 2-D: spec%stoch_bound, in [0, 1] */
   int32_t ntop;        /* 3-D kinds: length of each sorted_refs row */
+  int32_t szsn;        /* snhc 3-D: spec%szsn, the neighbourhood size, in [1, nrefs] */
 } pftcc_strategy_cfg;
 int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, const int32_t* iptcls,
                         const int32_t* srch_order, const int8_t* ref_ok, const float* prev_corr,

@@ -11,6 +11,7 @@ state_exists), prev_corr and one uniform draw u for shcloc.
   snhc_2d     simple_strategy2D_snhc.f90:33-93 (+ shcloc, simple_rnd.f90:273-295)
   greedy_3d   simple_strategy3D_greedy.f90:40-77 (+ sort_corrs)
   shc_3d      simple_strategy3D_shc.f90:36-83 (+ sort_corrs)
+  snhc_3d     simple_strategy3D_snhc.f90:37-60 (+ sort_corrs)
   sort_corrs  simple_strategy3D_utils.f90:100-124
 
 shcloc draws irnd_uni until it hits an available rotation, so its choice is uniform over the
@@ -145,3 +146,16 @@ def shc_3d(corrs, order, ok, prev_corr):
         if nbetter > 0:
             break
     return rc, rr, searched, sort_corrs(rc, searched), nrefs_eval, nbetter > 0
+
+
+def snhc_3d(corrs, order, szsn):
+    """srch_snhc (3D): the first szsn references of the search order, every one searched (the loop
+    has no state test), then sort_corrs.  Returns (ref_corrs, ref_rots, searched, sorted_refs,
+    nrefs_eval)."""
+    nrefs = len(corrs)
+    rc, rr, searched = [-HUGE] * nrefs, [-1] * nrefs, [False] * nrefs
+    for isample in range(szsn):
+        iref = order[isample]
+        inpl = maxloc(corrs[iref])
+        rc[iref], rr[iref], searched[iref] = corrs[iref][inpl], inpl, True
+    return rc, rr, searched, sort_corrs(rc, searched), szsn

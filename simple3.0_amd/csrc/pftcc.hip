@@ -2383,11 +2383,13 @@ int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, c
     return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
   const int kind = cfg->kind;
-  if (kind < PFTCC_SRCH_GREEDY2D || kind > PFTCC_SRCH_SHC3D) return fail(h, PFTCC_E_ARG, "unknown strategy %d", kind);
-  const bool stoch = kind == PFTCC_SRCH_SNHC2D || kind == PFTCC_SRCH_SHC3D;
-  const bool is3d = kind == PFTCC_SRCH_GREEDY3D || kind == PFTCC_SRCH_SHC3D;
+  if (kind < PFTCC_SRCH_GREEDY2D || kind > PFTCC_SRCH_SNHC3D) return fail(h, PFTCC_E_ARG, "unknown strategy %d", kind);
+  const bool stoch = kind == PFTCC_SRCH_SNHC2D || kind == PFTCC_SRCH_SHC3D || kind == PFTCC_SRCH_SNHC3D;
+  const bool is3d = kind == PFTCC_SRCH_GREEDY3D || kind == PFTCC_SRCH_SHC3D || kind == PFTCC_SRCH_SNHC3D;
   const int nrefs = h->nrefs, nrots = h->nrots;
-  if (stoch && (!srch_order || !prev_corr)) return fail(h, PFTCC_E_ARG, "the stochastic strategies need srch_order and prev_corr");
+  if (stoch && !srch_order) return fail(h, PFTCC_E_ARG, "the stochastic strategies need srch_order");
+  if ((kind == PFTCC_SRCH_SNHC2D || kind == PFTCC_SRCH_SHC3D) && !prev_corr)
+    return fail(h, PFTCC_E_ARG, "snhc 2-D and shc 3-D need prev_corr");
   if (kind == PFTCC_SRCH_SNHC2D && !rnd) return fail(h, PFTCC_E_ARG, "snhc needs one uniform draw per particle (rnd)");
   if ((ref_corrs == nullptr) != (ref_rots == nullptr)) return fail(h, PFTCC_E_ARG, "ref_corrs and ref_rots go together");
   const int ntop = is3d && sorted_refs ? cfg->ntop : 0;
@@ -2403,6 +2405,10 @@ int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, c
       return fail(h, PFTCC_E_ARG, "stoch_bound %g outside [0, 1]", cfg->stoch_bound);
     const float x = (float)nrefs * (1.0f - cfg->stoch_bound);
     bound = std::max(2, std::min(nrefs, (int)lroundf(x)));
+  }
+  if (kind == PFTCC_SRCH_SNHC3D) {
+    if (cfg->szsn < 1 || cfg->szsn > nrefs) return fail(h, PFTCC_E_ARG, "szsn %d outside [1, %d]", cfg->szsn, nrefs);
+    bound = cfg->szsn;
   }
   int rc;
   std::vector<int> pl(np);
@@ -2430,7 +2436,7 @@ int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, c
   // greedy kinds correlate every particle with all references.  The stochastic kinds scan their
   // search order in rounds of kStrategyRound0, 2 kStrategyRound0, ... positions, and only the
   // particles whose scan has not ended (no exit, limit not reached) take part in the next round.
-  const int lim = kind == PFTCC_SRCH_SNHC2D ? std::min(bound, nrefs) : nrefs;
+  const int lim = (kind == PFTCC_SRCH_SNHC2D || kind == PFTCC_SRCH_SNHC3D) ? std::min(bound, nrefs) : nrefs;
   const size_t per_ptcl = (size_t)(stoch ? lim : nrefs) * nrots * 4;
   // PFTCC_STRATEGY_CC_BYTES overrides the budget of the correlation vectors held at once (tests
   // use a small one to exercise the chunking)

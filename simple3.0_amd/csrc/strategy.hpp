@@ -6,6 +6,7 @@
 //   greedy 3-D  simple_strategy3D_greedy.f90:40-77   (all projections, then sort_corrs)
 //   shc 3-D     simple_strategy3D_shc.f90:36-83      (stochastic order, stop after the first
 //                                                     reference whose best beats prev_corr)
+//   snhc 3-D    simple_strategy3D_snhc.f90:37-60     (the first szsn references of the order)
 //   sort_corrs  simple_strategy3D_utils.f90:100-124  (references ordered by their best
 //                                                     correlation, unsearched ones at -huge)
 #pragma once
@@ -18,7 +19,7 @@
 
 namespace pfk {
 
-enum { SRCH_GREEDY2D = 0, SRCH_SNHC2D = 1, SRCH_GREEDY3D = 2, SRCH_SHC3D = 3 };
+enum { SRCH_GREEDY2D = 0, SRCH_SNHC2D = 1, SRCH_GREEDY3D = 2, SRCH_SHC3D = 3, SRCH_SNHC3D = 4 };
 
 // maxloc over the rotations of each correlation vector (one wavefront per vector): the largest
 // value, ties to the lowest rotation as Fortran's maxloc.  cc [nvec][nrots].
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(256) k_round_maxloc(const float* __restrict__ 
 struct StrategyArgs {
   int kind;
   int nrefs, nrots;
-  int bound;                   // snhc: positions of the search order inside the neighbourhood
+  int bound;                   // snhc 2-D / 3-D: positions of the search order inside the neighbourhood
   int ntop;                    // sorted references written per particle (3-D kinds)
   int sortlen;                 // power of two >= nrefs (3-D kinds): LDS keys of the sort
   const int* order;            // [np][nrefs] 0-based references in search order, or null
@@ -187,18 +188,21 @@ __global__ void __launch_bounds__(256) k_strategy(StrategyArgs a) {
   __shared__ int srot;
   const int p = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int nrefs = a.nrefs;
-  const bool stoch = a.kind == SRCH_SNHC2D || a.kind == SRCH_SHC3D;
+  const bool stoch = a.kind == SRCH_SNHC2D || a.kind == SRCH_SHC3D || a.kind == SRCH_SNHC3D;
+  const bool noexit = a.kind == SRCH_SNHC3D;   // a fixed neighbourhood, every reference searched
   const int* ord = stoch ? a.order + (size_t)p * nrefs : nullptr;
   const float* vb = a.vbest + (size_t)p * nrefs;
   const int* vr = a.vrot + (size_t)p * nrefs;
   const float prev = a.prev_corr ? a.prev_corr[p] : -FLT_MAX;
   auto ref_at = [&](int i) { return ord ? ord[i] : i; };
-  auto ok = [&](int r) { return a.ref_ok == nullptr || a.ref_ok[r] != 0; };
+  auto ok = [&](int r) { return noexit || a.ref_ok == nullptr || a.ref_ok[r] != 0; };
 
   // positions [0, lim) may be searched; the stochastic kinds stop at the first improving one
-  const int lim = a.kind == SRCH_SNHC2D ? min(a.bound, nrefs) : nrefs;
+  const int lim = (a.kind == SRCH_SNHC2D || noexit) ? min(a.bound, nrefs) : nrefs;
   int istar = INT_MAX;
-  if (stoch && a.istar_in) {
+  if (noexit) {
+    istar = INT_MAX;
+  } else if (stoch && a.istar_in) {
     istar = a.istar_in[p];
   } else if (stoch) {
     int mine = INT_MAX;
@@ -312,6 +316,7 @@ __global__ void __launch_bounds__(256) k_strategy(StrategyArgs a) {
   // before the bound test); shc the searched ones with an existing state
   int ne = nrefs;
   if (a.kind == SRCH_SNHC2D) ne = found ? istar + 1 : (a.bound < nrefs ? a.bound + 1 : nrefs);
+  if (noexit) ne = lim;
   if (a.kind == SRCH_SHC3D) {
     int c = 0;
     for (int i = tid; i <= last; i += nt) c += ok(ref_at(i));
@@ -338,11 +343,12 @@ __global__ void __launch_bounds__(256) k_strategy_round(StrategyArgs a, const in
   const int nrefs = a.nrefs;
   const int* ord = a.order + (size_t)p * nrefs;
   const float* vb = a.vbest + (size_t)p * nrefs;
-  const float prev = a.prev_corr[p];
-  const int lim = a.kind == SRCH_SNHC2D ? min(a.bound, nrefs) : nrefs;
+  const float prev = a.prev_corr ? a.prev_corr[p] : -FLT_MAX;
+  const bool noexit = a.kind == SRCH_SNHC3D;
+  const int lim = (a.kind == SRCH_SNHC2D || noexit) ? min(a.bound, nrefs) : nrefs;
   const int iend = min(i1, lim);
   int mine = INT_MAX;
-  for (int i = i0 + tid; i < iend; i += nt) {
+  for (int i = i0 + tid; i < (noexit ? i0 : iend); i += nt) {
     const int r = ord[i];
     if (a.ref_ok == nullptr || a.ref_ok[r] != 0) {
       const float c = vb[r];

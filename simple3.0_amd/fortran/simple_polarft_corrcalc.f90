@@ -17,7 +17,7 @@ use, intrinsic :: iso_c_binding
 implicit none
 
 public :: polarft_corrcalc, pftcc_glob, pftcc_params, ctfparams, prep_cfg, OBJFUN_CC, OBJFUN_EUCLID
-public :: strategy_cfg, SRCH_GREEDY2D, SRCH_SNHC2D, SRCH_GREEDY3D, SRCH_SHC3D
+public :: strategy_cfg, SRCH_GREEDY2D, SRCH_SNHC2D, SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC3D
 public :: PREC_BF16, PREC_SPLIT_BF16, PREC_FP8
 private
 
@@ -58,13 +58,14 @@ type, bind(C) :: prep_cfg
     integer(c_int32_t) :: ctfflag = 0, griddev = 0
 end type prep_cfg
 
-! pftcc_strategy_cfg: which reference-loop strategy, snhc's spec%stoch_bound, the length of each
-! sorted reference list (3-D kinds)
-integer, parameter :: SRCH_GREEDY2D = 0, SRCH_SNHC2D = 1, SRCH_GREEDY3D = 2, SRCH_SHC3D = 3
+! pftcc_strategy_cfg: which reference-loop strategy, snhc 2-D's spec%stoch_bound, the length of
+! each sorted reference list (3-D kinds), snhc 3-D's spec%szsn
+integer, parameter :: SRCH_GREEDY2D = 0, SRCH_SNHC2D = 1, SRCH_GREEDY3D = 2, SRCH_SHC3D = 3, SRCH_SNHC3D = 4
 type, bind(C) :: strategy_cfg
     integer(c_int32_t) :: kind = SRCH_GREEDY2D
     real(c_float)      :: stoch_bound = 0.
     integer(c_int32_t) :: ntop = 0
+    integer(c_int32_t) :: szsn = 0
 end type strategy_cfg
 
 type :: polarft_corrcalc
@@ -912,8 +913,8 @@ contains
         if( present(best_ishift) ) best_ishift = bis + 1
     end subroutine search
 
-    ! The reference-loop strategies (strategy2D_greedy/snhc%srch, strategy3D_greedy/shc%srch with
-    ! sort_corrs) for a batch of particles against all references.  srch_order(np, nrefs) as
+    ! The reference-loop strategies (strategy2D_greedy/snhc%srch, strategy3D_greedy/shc/snhc%srch
+    ! with sort_corrs) for a batch of particles against all references.  srch_order(np, nrefs) as
     ! s2D%srch_order / s3D%srch_order; ref_ok(nrefs) cls_pops > 0 / state_exists; prev_corr(np) the
     ! prep4srch threshold; rnd(np) one uniform draw per particle for shcloc (snhc).  Out: the
     ! chosen reference and rotation, nrefs_eval and whether the stochastic exit was taken;

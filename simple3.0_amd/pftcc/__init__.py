@@ -52,11 +52,11 @@ class PrepCfgC(C.Structure):
 
 
 class StrategyCfgC(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("stoch_bound", C.c_float), ("ntop", C.c_int32)]
+    _fields_ = [("kind", C.c_int32), ("stoch_bound", C.c_float), ("ntop", C.c_int32), ("szsn", C.c_int32)]
 
 
 # search strategies of pftcc_strategy_srch
-SRCH_GREEDY2D, SRCH_SNHC2D, SRCH_GREEDY3D, SRCH_SHC3D = 0, 1, 2, 3
+SRCH_GREEDY2D, SRCH_SNHC2D, SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC3D = 0, 1, 2, 3, 4
 
 
 def _ctf_array(ctfparams):
@@ -522,7 +522,7 @@ class PolarFTCorrCalc:
         return cc, ir, ish, rot
 
     def strategy_srch(self, kind, iptcls, srch_order=None, ref_ok=None, prev_corr=None, rnd=None,
-                      stoch_bound=0.0, ntop=0, ref_corrs=False):
+                      stoch_bound=0.0, ntop=0, ref_corrs=False, szsn=0):
         """pftcc_strategy_srch: returns a dict of best_corr, best_ref, best_rot, nrefs_eval,
         found_better [np], plus ref_corrs/ref_rots [np][nrefs] (ref_corrs=True) and sorted_refs
         [np][ntop] (3-D kinds, ntop > 0)."""
@@ -539,9 +539,9 @@ class PolarFTCorrCalc:
         if ref_corrs:
             rc = out["ref_corrs"] = np.zeros((n, self.nrefs), np.float32)
             rr = out["ref_rots"] = np.zeros((n, self.nrefs), np.int32)
-        if kind in (SRCH_GREEDY3D, SRCH_SHC3D) and ntop > 0:
+        if kind in (SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC3D) and ntop > 0:
             so = out["sorted_refs"] = np.zeros((n, ntop), np.int32)
-        cfg = StrategyCfgC(kind, stoch_bound, ntop)
+        cfg = StrategyCfgC(kind, stoch_bound, ntop, szsn)
         self._chk(self._lib.pftcc_strategy_srch(
             self._h, C.byref(cfg), n, _ip(p), None if o is None else _ip(o),
             None if ok is None else ok.ctypes.data_as(C.c_void_p), None if pc is None else _fp(pc),

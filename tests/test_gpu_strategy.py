@@ -14,7 +14,7 @@ import pytest
 import strategy_oracle as S
 from cases import make_case
 from gpu_helpers import say, to_lib
-from pftcc import SRCH_GREEDY2D, SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC2D, PftccError
+from pftcc import SRCH_GREEDY2D, SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC2D, SRCH_SNHC3D, PftccError
 
 pytestmark = pytest.mark.gpu
 
@@ -88,6 +88,16 @@ def test_strategies_match_oracle(shape):
         np.testing.assert_array_equal(out["ref_rots"][p], np.array(rr) + 1)
         assert (out["nrefs_eval"][p], bool(out["found_better"][p])) == (ne, found), p
         assert (out["best_ref"][p], out["best_corr"][p]) == (srt[0] + 1, rc[srt[0]])
+    szsn = max(1, (2 * nrefs) // 3)
+    say(f"  snhc 3-D szsn {szsn}")
+    out = lib.strategy_srch(SRCH_SNHC3D, ids, srch_order=order + 1, ref_ok=ok, ntop=nrefs, ref_corrs=True,
+                            szsn=szsn)
+    for p in range(npt):
+        rc, rr, searched, srt, ne = S.snhc_3d(cc[p], list(order[p]), szsn)
+        np.testing.assert_array_equal(out["sorted_refs"][p], np.array(srt) + 1)
+        np.testing.assert_array_equal(out["ref_corrs"][p], np.array(rc, np.float32))
+        assert (out["nrefs_eval"][p], out["found_better"][p]) == (ne, 0), p
+        assert (out["best_ref"][p], out["best_rot"][p], out["best_corr"][p]) == (srt[0] + 1, rr[srt[0]] + 1, rc[srt[0]])
     lib.kill()
 
 
@@ -123,7 +133,8 @@ def test_chunked_calls_equal_one_chunk():
             dict(kind=SRCH_SNHC2D, srch_order=order + 1, ref_ok=ok, prev_corr=prev, rnd=rnd, stoch_bound=0.3,
                  ref_corrs=True),
             dict(kind=SRCH_GREEDY3D, ref_ok=ok, ntop=7, ref_corrs=True),
-            dict(kind=SRCH_SHC3D, srch_order=order + 1, ref_ok=ok, prev_corr=prev, ntop=7, ref_corrs=True)]
+            dict(kind=SRCH_SHC3D, srch_order=order + 1, ref_ok=ok, prev_corr=prev, ntop=7, ref_corrs=True),
+            dict(kind=SRCH_SNHC3D, srch_order=order + 1, ntop=7, ref_corrs=True, szsn=150)]
     base = [lib.strategy_srch(r["kind"], ids, **{k: v for k, v in r.items() if k != "kind"}) for r in runs]
     os.environ["PFTCC_STRATEGY_CC_BYTES"] = str(3 * 40 * nrots * 4)
     try:

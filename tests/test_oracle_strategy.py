@@ -123,3 +123,15 @@ def test_sort_corrs_against_argsort():
     assert got == list(np.argsort(-v, kind="stable"))
     # the unsearched trail in index order
     assert got[-17:] == [i for i in range(50) if not searched[i]]
+
+
+def test_snhc_3d_prefix_and_full_neighbourhood():
+    rng = np.random.default_rng(5)
+    c = _corrs(rng, 14, 20)
+    order = list(rng.permutation(14))
+    full = S.snhc_3d(c, order, 14)
+    g = S.greedy_3d(c, [True] * 14)
+    assert full[:4] == g[:4] and full[4] == 14
+    rc, rr, searched, srt, ne = S.snhc_3d(c, order, 5)
+    assert ne == 5 and [i for i in range(14) if searched[i]] == sorted(order[:5])
+    assert srt[:5] == sorted(order[:5], key=lambda r: (-float(c[r].max()), r))
