@@ -434,7 +434,34 @@ def precision_same_workload(lib, w, shifts, plain_value, nptcls, sample=1000, nt
     }
 
 
-def strategy_form(lib, w, nptcls=10000, reps=2):
+def strategy_cpu_baseline(lib, w, nthreads):
+    """The port (oracle/pftcc_cpu.cpp, OpenMP over particles) correlating 2 particles per thread
+    with all references without shifts (gencorrs), timed on the host, then the greedy choice over
+    references; checked against the device's greedy 3-D choice for the same particles."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cpu_port
+    from pftcc import SRCH_GREEDY3D
+
+    n, nk = w.pftsz, w.nk
+    npt = 2 * nthreads
+    pt = np.stack([lib.get_ptcl_pft(i + 1) for i in range(npt)])
+    ctf = np.stack([lib.get_ctfmat(i + 1) for i in range(npt)])
+    sq = np.array([lib.get_sqsum_ptcl(i + 1) for i in range(npt)], np.float32)
+    pa, pb = cpu_port.memoize(pt, nthreads)
+    refs = np.stack([lib.get_ref_pft(r + 1, True) for r in range(w.nrefs)])
+    t0 = time.perf_counter()
+    ccc, cir, _, crot = cpu_port.search_fp32(refs, ctf, sq, pa, pb, None, nk, nthreads)
+    dt = time.perf_counter() - t0
+    g = lib.strategy_srch(SRCH_GREEDY3D, np.arange(1, npt + 1, dtype=np.int32), ntop=1)
+    same = (g["best_ref"] == cir) & (g["best_rot"] == crot)
+    return {"value": npt * w.nrefs * 2 * n / dt, "unit": "evals/s", "cores": nthreads, "kind": "port",
+            "sample": f"{npt} ptcls x {w.nrefs} refs x {2 * n} rots, no shift ({dt:.1f} s): the reference's FP32 "
+                      "FFT route, OpenMP over particles, then the greedy choice",
+            "gpu_assignment_agreement": float(np.mean(same)),
+            "max_abs_best_cc_diff": float(np.max(np.abs(g["best_corr"] - ccc)))}
+
+
+def strategy_form(lib, w, nptcls=10000, reps=2, with_cpu=True):
     """The reference-loop search strategies (pftcc_strategy_srch) over the main line's references
     and particles: every particle against all references at shift 0 (gencorrs), then the greedy /
     snhc 2-D and greedy / shc / snhc 3-D choices.  The stochastic kinds get a random search order
@@ -487,6 +514,12 @@ def strategy_form(lib, w, nptcls=10000, reps=2):
                      "correlation_kernel_ms": kern_ms / reps, "mean_positions_correlated": float(pos.mean()),
                      "mean_nrefs_eval": float(np.mean(out["nrefs_eval"])),
                      "found_better_frac": float(np.mean(out["found_better"]))}
+    cpu = None
+    if with_cpu:
+        try:
+            cpu = strategy_cpu_baseline(lib, w, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        except Exception as e:  # the GPU figure stands on its own
+            cpu = {"error": repr(e)}
     kern = res["greedy2d"]["correlation_kernel_ms"]
     achieved = plain_flops_per_pair(w.pftsz, w.nk) * n * w.nrefs / (kern * 1e-3) / 1e12
     return {"workload": f"search strategies: {n} particles x {w.nrefs} references, all rotations at shift 0 "
@@ -496,6 +529,7 @@ def strategy_form(lib, w, nptcls=10000, reps=2):
                     "the evaluations made, full_scan_equivalent_evals_per_s the greedy scan's",
             "ptcls": n, "refs": w.nrefs, "nrots": nrots, "kinds": res,
             "greedy3d_best_equals_pftcc_search": same,
+            "cpu_baseline": cpu,
             "roofline": {"bound": "mfma", "pipe": "VALU fp32", "kernel": "k_corr_fused<308,9,256,3,false>",
                          "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP32_PEAK_TFLOPS,
@@ -685,7 +719,7 @@ def main():
     if not args.no_precision and "precision" in legs:
         out["precision_form_same_workload"] = precision_same_workload(lib, w, shifts, value, args.mfma_ptcls)
     if "strategy" in legs:
-        out["search_strategies"] = strategy_form(lib, w)
+        out["search_strategies"] = strategy_form(lib, w, with_cpu=not args.no_cpu)
     lib.kill()
     if not args.no_precision and "precision" in legs:
         out["precision_form"] = precision_form(args.precision_steps)
