@@ -111,6 +111,18 @@ def test_greedy_3d_equals_device_search_and_errors():
     np.testing.assert_array_equal(out["best_rot"], rot0)
     np.testing.assert_array_equal(out["best_corr"], c0)
     np.testing.assert_array_equal(out["sorted_refs"][:, 0], ir0)
+    # no class with members / no state: nothing searched, the previous parameters stay
+    none = np.zeros(cc.shape[1], bool)
+    for kind, kw in ((SRCH_GREEDY2D, {}), (SRCH_SNHC2D, dict(srch_order=order + 1, rnd=rnd)),
+                     (SRCH_GREEDY3D, dict(ntop=3)), (SRCH_SHC3D, dict(srch_order=order + 1, ntop=3))):
+        out = lib.strategy_srch(kind, ids, ref_ok=none, prev_corr=prev, ref_corrs=True, **kw)
+        np.testing.assert_array_equal(out["best_ref"], 0)
+        np.testing.assert_array_equal(out["best_rot"], 0)
+        np.testing.assert_array_equal(out["best_corr"], prev)
+        np.testing.assert_array_equal(out["found_better"], 0)
+        assert np.all(out["ref_corrs"] == -S.HUGE) and np.all(out["ref_rots"] == 0)
+        if "sorted_refs" in out:
+            np.testing.assert_array_equal(out["sorted_refs"], np.tile(np.arange(1, 4), (len(ids), 1)))
     bad = order.copy() + 1
     bad[1, 3] = bad[1, 4]
     with pytest.raises(PftccError, match="permutation"):
