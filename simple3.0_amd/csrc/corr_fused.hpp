@@ -121,6 +121,9 @@ constexpr int fused_F_tuned() {
   if (NT == 256 && (N == 192 || N == 308)) return N == 192 ? 5 : 9;
   return fused_F<N, NT>();
 }
+#ifndef PFTCC_FUSED_GROUP
+#define PFTCC_FUSED_GROUP 16   // > 0: work-groups ordered tile-major over groups of this many particles, per XCD
+#endif
 #ifndef PFTCC_FUSED_PREFETCH
 #define PFTCC_FUSED_PREFETCH 1 // the next shell's staged rows (|CTF|, reference rows, spectrum) loaded ahead
 #endif
@@ -246,8 +249,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   __shared__ unsigned long long red[(NT + 63) / 64];
 
   const int tid = threadIdx.x;
-  const int pi = blockIdx.x / a.ntiles;
-  const int tile = blockIdx.x - pi * a.ntiles;
+  int pi, tile;
+  if constexpr (PFTCC_FUSED_GROUP > 0) {
+    // each XCD takes a contiguous range of the logical order; within it, the tiles of a group of
+    // particles are interleaved, so that the work-groups resident together share reference and
+    // shift rows in their L2
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int grp = t / (PFTCC_FUSED_GROUP * a.ntiles);
+    const int base = grp * PFTCC_FUSED_GROUP;
+    const int gsz = min(PFTCC_FUSED_GROUP, a.np - base);
+    const int r = t - grp * PFTCC_FUSED_GROUP * a.ntiles;
+    tile = r / gsz;
+    pi = base + (r - tile * gsz);
+  } else {
+    pi = blockIdx.x / a.ntiles;
+    tile = blockIdx.x - pi * a.ntiles;
+  }
   const int ip = a.plist[pi];
   const bool even = a.iseven[ip] != 0;
   const float2* refs = even ? a.refs_even : a.refs_odd;
