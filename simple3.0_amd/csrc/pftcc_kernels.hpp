@@ -585,6 +585,9 @@ __device__ __forceinline__ void rot8_step(const Rot8Args a, const Rot8Shell& s, 
 #ifndef PFTCC_ROT8_AHEAD
 #define PFTCC_ROT8_AHEAD 1     // shells of loads in flight ahead of the arithmetic in k_rot8
 #endif
+#ifndef PFTCC_ROT8_W
+#define PFTCC_ROT8_W 6         // wavefronts per SIMD the register allocation of k_rot8 must allow
+#endif
 
 // FP64 sums of |P|^2 per (particle, shell): one wavefront per ring.
 __global__ void __launch_bounds__(256) k_shell_sq8(const float2* __restrict__ rings, int nitems, int n,
@@ -603,7 +606,7 @@ __global__ void __launch_bounds__(256) k_shell_sq8(const float2* __restrict__ ri
 // has loads in flight.  MODE: 0 CC k-weighted, 1 CC match_filt, 2 Euclid (equal to a.mode; a
 // template argument so that each form's sums stay in registers).
 template <int D, int MODE>
-__global__ void __launch_bounds__(512) k_rot8(Rot8Args a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PFTCC_ROT8_W))) k_rot8(Rot8Args a) {
   const int item = xcd_remap(blockIdx.x, gridDim.x);
   if (item >= a.npairs) return;
   const int n = a.n;
