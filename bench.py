@@ -530,7 +530,7 @@ def strategy_form(lib, w, nptcls=10000, reps=2, with_cpu=True):
             "ptcls": n, "refs": w.nrefs, "nrots": nrots, "kinds": res,
             "greedy3d_best_equals_pftcc_search": same,
             "cpu_baseline": cpu,
-            "roofline": {"bound": "mfma", "pipe": "VALU fp32", "kernel": "k_corr_fused<308,9,256,3,false>",
+            "roofline": {"bound": "valu", "pipe": "VALU fp32", "kernel": "k_corr_fused<308,9,256,3,false>",
                          "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP32_PEAK_TFLOPS,
                          "note": "unshifted search: one pair per reference, so the staged-reference tile does not apply"}}
@@ -698,7 +698,7 @@ def main():
                    "smpd": smpd, "kfromto": list(kfromto), "pftsz": pftsz, "nrots": nrots,
                    "shifts": ns, "shift_grid": "+-3 px step 0.5", "ctf": "astigmatic, defocus U(0.8,3.5) um",
                    "snr": 0.03, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "pipe": "VALU fp32 (no MFMA; packed-fp32 disabled, issue peak 78.6)",
+        "roofline": {"bound": "valu", "pipe": "VALU fp32 (no MFMA; packed-fp32 disabled, issue peak 78.6)",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": PLAIN_TRAFFIC_BYTES_B1000 if (B == 1000 and w.nrefs == 200) else None,

@@ -349,6 +349,12 @@ int pftcc_get_timing(pftcc_t* h, double* total_ms, int64_t* launches);
  * adjoint kernel (gencorr_for_rot_8, grad_batch). */
 int pftcc_get_timing_channel(pftcc_t* h, int32_t channel, double* total_ms, int64_t* launches);
 
+/* Tile of the plain FP32 correlation kernel for an exhaustive search with ns shifts (0 =
+ * unshifted): *tile = (reference, shift) pairs per work-group of the compile-time kernel (0: the
+ * runtime-size kernel runs), *staged = 1 when the staged-reference variant runs (a tile then spans
+ * at most two references).  No reference counterpart (instrumentation). */
+int pftcc_get_corr_tile(const pftcc_t* h, int32_t ns, int32_t* tile, int32_t* staged);
+
 /* ------------------------------------------------------------------------- utilities -- */
 /* magic_pftsz (simple_magic_boxes.f90:25-37). */
 int32_t pftcc_magic_pftsz(int32_t ring2);

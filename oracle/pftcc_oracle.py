@@ -293,6 +293,22 @@ def shmat_8(grid: Grid, shvec) -> np.ndarray:
     return out
 
 
+def shmat_8_many(grid: Grid, shifts) -> np.ndarray:
+    """gen_shmat_8 (:1098-1120) for a list of shifts at once: the same first-shell phase and de
+    Moivre recurrence over k, vectorised over the shifts.  shifts [ns, 2] -> complex128 [ns, nk, n].
+    """
+    sh = np.asarray(shifts, dtype=np.float64).reshape(-1, 2)
+    sx, sy = sh[:, 0:1], sh[:, 1:2]
+    out = np.empty((sh.shape[0], grid.nk, grid.pftsz), dtype=np.complex128)
+    arg0 = grid.argtransf_x[0][None, :] * sx + grid.argtransf_y[0][None, :] * sy
+    out[:, 0] = np.cos(arg0) + 1j * np.sin(arg0)
+    arg1 = grid.shellone_x[None, :] * sx + grid.shellone_y[None, :] * sy
+    step = np.cos(arg1) + 1j * np.sin(arg1)
+    for i in range(1, grid.nk):
+        out[:, i] = out[:, i - 1] * step
+    return out
+
+
 def shmat_closed(grid: Grid, shvec) -> np.ndarray:
     """Closed form exp(i*(ax*sx + ay*sy)) that gen_shmat_8's recurrence approximates."""
     arg = grid.argtransf_x * float(shvec[0]) + grid.argtransf_y * float(shvec[1])
@@ -586,6 +602,31 @@ class Pftcc:
             c = corrs_over_k_fft(q[:ks], pa, pb).sum(axis=0)
         return c / math.sqrt(sq * self.sqsum_ptcl(i))
 
+    def gencorrs_shifts(self, iref: int, i: int, shifts) -> np.ndarray:
+        """gencorrs_cc_2 (:1502-1534) for a list of shifts at once through the FFT route
+        (memoize_fft + calc_corrs_over_k): cc[ns, nrots] (FP64).  The same arithmetic as
+        gencorrs(iref, i, shvec, method="fft") for each shift, vectorised over the shifts (the
+        particle spectra are taken once); the reference norm runs over all shells (:1531)."""
+        if self.euclid or self.match_filt or self.pssnr:
+            raise NotImplementedError("gencorrs_shifts restates the CC objective without match filter")
+        ks = self.ks
+        n = self.pftsz
+        sh = shmat_8_many(self.grid, shifts)                       # [ns, nk, n]
+        q = (self.ref(iref, i) * self.ctf(i))[None] * sh           # [ns, nk, n]
+        sq = np.sum(np.abs(q) ** 2, axis=(1, 2))                   # all shells
+        pa, pb = memoize_fft(self.ptcls[i, :ks].astype(np.complex128))
+        qk = q[:, :ks]
+        a = np.fft.rfft(np.real(qk), axis=-1)
+        b = np.fft.fft(np.imag(qk) * fft_factors(n), axis=-1)
+        h = n // 2
+        prod = np.zeros(qk.shape[:2] + (2 * n + 2,), dtype=np.complex128)
+        prod[..., 0:2 * h + 1:2] = 4.0 * np.conj(a[..., :h + 1]) * pa[None, :, :h + 1]
+        prod[..., 1:2 * h + 2:2] = 4.0 * np.conj(b[..., :h + 1]) * pb[None, :, :h + 1]
+        # c2r_2n of bins 0..n per shell, summed over shells (taken as one c2r of the shell sum:
+        # the transform is linear)
+        back = np.fft.irfft(prod[..., :n + 1].sum(axis=1), n=2 * n, axis=-1) * (2 * n) / (2.0 * 2 * n)
+        return back / np.sqrt(sq * self.sqsum_ptcl(i))[:, None]
+
     def gencorrs_per_shell(self, iref: int, i: int, shvec=None) -> np.ndarray:
         """Per-shell C_k(rho) of the prepared reference: [ks, nrots] (calc_k_corrs :1164)."""
         q = self.prep_ref(iref, i, shvec)
@@ -734,8 +775,28 @@ class Pftcc:
         return np.where(den < 1e-12, 0.0, ck / np.where(den < 1e-12, 1.0, den))
 
     def specscore(self, iref: int, i: int, irot: int, shvec=None) -> float:
-        """specscore_1/2 (:2190-2205): max(0, median(frc)) (median_nocopy, simple_math.f90:2111)."""
-        return max(0.0, float(np.median(self.genfrc(iref, i, irot, shvec))))
+        """specscore_1/2 (:2190-2205): max(0, median_nocopy(frc)) on the FP32 FRC values."""
+        frc = self.genfrc(iref, i, irot, shvec).astype(np.float32)
+        return max(0.0, float(median_nocopy(frc)))
+
+
+def median_nocopy(arr: np.ndarray) -> np.float32:
+    """median_nocopy (simple_math.f90:2111-2130): pos1 = n/2, pos2 = pos1 + 1 for even n (both
+    selected by two successive selec calls on the same, already permuted array, then averaged in
+    FP32), pos1 = pos2 = nint(n/2.) for odd n.  selec (:2204-2261) is the reference's quickselect
+    with its final two-element test, restated in prep_oracle.  arr is permuted in place."""
+    from prep_oracle import selec
+    a = arr.astype(np.float32) if arr.dtype != np.float32 else arr
+    n = a.size
+    if n % 2 == 0:
+        p1, p2 = n // 2, n // 2 + 1
+    else:
+        p1 = p2 = fortran_nint(float(f32(n) / f32(2.0)))
+    if p1 == p2:
+        return selec(p1, a)
+    v1 = selec(p1, a)
+    v2 = selec(p2, a)
+    return f32(f32(v1 + v2) / f32(2.0))
 
 
 def rotate_ring_continuous(p: np.ndarray, psi: float, deriv: bool = False) -> np.ndarray:

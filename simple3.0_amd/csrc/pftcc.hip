@@ -195,6 +195,19 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// Host <-> device copies and fills issued on the handle's stream and completed before returning.
+// The stream may be non-blocking (a pool stream of the caller), which the null stream's implicit
+// ordering would not cover: a synchronous null-stream copy could overtake pending kernels that
+// still read the destination.
+hipError_t sync_copy(pftcc* h, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, h->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(h->stream);
+}
+hipError_t sync_memset(pftcc* h, void* dst, int value, size_t bytes) {
+  const hipError_t e = hipMemsetAsync(dst, value, bytes, h->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(h->stream);
+}
+
 // Scratch arena: reserve() before a call, take() pieces, all stream ordered.
 int scratch_reserve(pftcc* h, size_t bytes) {
   h->scratch_off = 0;
@@ -317,11 +330,11 @@ int alloc_ptcl_arrays(pftcc* h) {
   if ((rc = dalloc(h, &h->d_sqsums, (size_t)h->nptcls))) return rc;
   if ((rc = dalloc(h, &h->d_sqsums_k, (size_t)h->nptcls * h->nk))) return rc;
   if ((rc = dalloc(h, &h->d_iseven, (size_t)h->nptcls))) return rc;
-  HIPCHK(h, hipMemset(h->d_ptcls, 0, (size_t)h->nptcls * ring * sizeof(float2)));
-  HIPCHK(h, hipMemset(h->d_sqsums, 0, (size_t)h->nptcls * sizeof(float)));
-  HIPCHK(h, hipMemset(h->d_sqsums_k, 0, (size_t)h->nptcls * h->nk * sizeof(float)));
+  HIPCHK(h, sync_memset(h, h->d_ptcls, 0, (size_t)h->nptcls * ring * sizeof(float2)));
+  HIPCHK(h, sync_memset(h, h->d_sqsums, 0, (size_t)h->nptcls * sizeof(float)));
+  HIPCHK(h, sync_memset(h, h->d_sqsums_k, 0, (size_t)h->nptcls * h->nk * sizeof(float)));
   h->iseven.assign(h->nptcls, 1);
-  HIPCHK(h, hipMemcpy(h->d_iseven, h->iseven.data(), h->nptcls, hipMemcpyHostToDevice));
+  HIPCHK(h, sync_copy(h, h->d_iseven, h->iseven.data(), h->nptcls, hipMemcpyHostToDevice));
   return PFTCC_OK;
 }
 
@@ -400,10 +413,10 @@ int ensure_refnorm(pftcc* h) {
     if ((rc = dalloc(h, &h->d_ofac_d, (size_t)rows * kc))) return rc;
     if ((rc = dalloc(h, &h->d_ofac_f, (size_t)rows * kc))) return rc;
   }
-  HIPCHK(h, hipMemcpy(h->d_refinv_d, inv.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(h->d_refinv_f, invf.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(h->d_ofac_d, od.data(), od.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(h->d_ofac_f, of.data(), of.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, sync_copy(h, h->d_refinv_d, inv.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, sync_copy(h, h->d_refinv_f, invf.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, sync_copy(h, h->d_ofac_d, od.data(), od.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, sync_copy(h, h->d_ofac_f, of.data(), of.size() * sizeof(float), hipMemcpyHostToDevice));
   if (h->clsfrcs) {
     // ref_optlp(k, iptcl) with the global particle id (:975-979)
     std::vector<int> row(h->nptcls, 0);
@@ -418,7 +431,7 @@ int ensure_refnorm(pftcc* h) {
       int rc;
       if ((rc = dalloc(h, &h->d_optrow, (size_t)h->nptcls))) return rc;
     }
-    HIPCHK(h, hipMemcpy(h->d_optrow, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(h, sync_copy(h, h->d_optrow, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   h->refnorm_valid = true;
   return PFTCC_OK;
@@ -1200,8 +1213,8 @@ int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, c
   const size_t ring = (size_t)h->nk * h->n;
   if ((rc = dalloc(h, &h->d_refs_e, (size_t)nrefs * ring)) || (rc = dalloc(h, &h->d_refs_o, (size_t)nrefs * ring)))
     return bad(rc);
-  if (hipMemset(h->d_refs_e, 0, (size_t)nrefs * ring * sizeof(float2)) != hipSuccess ||
-      hipMemset(h->d_refs_o, 0, (size_t)nrefs * ring * sizeof(float2)) != hipSuccess)
+  if (sync_memset(h, h->d_refs_e, 0, (size_t)nrefs * ring * sizeof(float2)) != hipSuccess ||
+      sync_memset(h, h->d_refs_o, 0, (size_t)nrefs * ring * sizeof(float2)) != hipSuccess)
     return bad(fail(h, PFTCC_E_DEVICE, "hipMemset failed"));
   if ((rc = alloc_ptcl_arrays(h))) return bad(rc);
   // eo assignment (:291-307)
@@ -1211,7 +1224,7 @@ int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, c
     if (!allm1)
       for (int i = 0; i < pto - pfrom + 1; ++i)
         if (h->pinds[i] > 0) h->iseven[h->pinds[i] - 1] = eoarr[i] == 0 ? 1 : 0;
-    if (hipMemcpy(h->d_iseven, h->iseven.data(), h->nptcls, hipMemcpyHostToDevice) != hipSuccess)
+    if (sync_copy(h, h->d_iseven, h->iseven.data(), h->nptcls, hipMemcpyHostToDevice) != hipSuccess)
       return bad(fail(h, PFTCC_E_DEVICE, "copy eo"));
   }
   // constant tables
@@ -1226,15 +1239,15 @@ int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, c
       (rc = dalloc(h, &h->d_angmat, h->angmat.size())) || (rc = dalloc(h, &h->d_sn32, h->sn32.size())) ||
       (rc = dalloc(h, &h->d_ncn32, h->ncn32.size())))
     return bad(rc);
-  if (hipMemcpy(h->d_tw2, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_argx, h->argx.data(), h->argx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_argy, h->argy.data(), h->argy.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_onex, h->onex.data(), h->onex.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_oney, h->oney.data(), h->oney.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_spafreq, h->spafreq.data(), h->spafreq.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_angmat, h->angmat.data(), h->angmat.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_sn32, h->sn32.data(), h->sn32.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->d_ncn32, h->ncn32.data(), h->ncn32.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+  if (sync_copy(h, h->d_tw2, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess ||
+      sync_copy(h, h->d_argx, h->argx.data(), h->argx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      sync_copy(h, h->d_argy, h->argy.data(), h->argy.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      sync_copy(h, h->d_onex, h->onex.data(), h->onex.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      sync_copy(h, h->d_oney, h->oney.data(), h->oney.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      sync_copy(h, h->d_spafreq, h->spafreq.data(), h->spafreq.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      sync_copy(h, h->d_angmat, h->angmat.data(), h->angmat.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      sync_copy(h, h->d_sn32, h->sn32.data(), h->sn32.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      sync_copy(h, h->d_ncn32, h->ncn32.data(), h->ncn32.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     return bad(fail(h, PFTCC_E_DEVICE, "upload of constant tables failed"));
   if (h->match_filt || h->pssnr) h->optlp.assign((size_t)nrefs * h->kcorr, 1.0f);
   *out = h;
@@ -1271,6 +1284,18 @@ int pftcc_get_timing_channel(pftcc_t* h, int32_t ch, double* total_ms, int64_t* 
   if ((rc = collect_timing(h))) return rc;
   *total_ms = h->timed_ms[ch];
   *launches = h->timed_launches[ch];
+  return PFTCC_OK;
+}
+
+int pftcc_get_corr_tile(const pftcc_t* h, int32_t ns, int32_t* tile, int32_t* staged) {
+  if (!h || !tile || !staged || ns < 0) return PFTCC_E_ARG;
+  // the same selection as launch_corr for an exhaustive list of references (rlist_ld = 0)
+  const bool shifted = ns > 0;
+  const bool norm = (h->match_filt || h->pssnr) && !(h->euclid && shifted);
+  bool rst = false;
+  const int Ff = h->use_fused ? fused_tile(h->n, !h->euclid && !norm, std::max(ns, 1), &rst) : 0;
+  *tile = Ff;
+  *staged = rst ? 1 : 0;
   return PFTCC_OK;
 }
 
@@ -1724,7 +1749,7 @@ int pftcc_set_ref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, float
     return fail(h, PFTCC_E_ARG, "component (%d, %d) outside the ring block", irot, k);
   const float2 v = make_float2(re, im);
   float2* dst = (iseven ? h->d_refs_e : h->d_refs_o) + ((size_t)(iref - 1) * h->nk + (k - h->kfrom)) * h->n + (irot - 1);
-  HIPCHK(h, hipMemcpy(dst, &v, sizeof v, hipMemcpyHostToDevice));
+  HIPCHK(h, sync_copy(h, dst, &v, sizeof v, hipMemcpyHostToDevice));
   h->refnorm_valid = false;
   h->apack[0].valid = h->apack[1].valid = false;
   return PFTCC_OK;
@@ -1738,7 +1763,7 @@ int pftcc_set_ptcl_fcomp(pftcc_t* h, int32_t iptcl, int32_t irot, int32_t k, flo
   if (irot < 1 || irot > h->n || k < h->kfrom || k > h->kto)
     return fail(h, PFTCC_E_ARG, "component (%d, %d) outside the ring block", irot, k);
   const float2 v = make_float2(re, im);
-  HIPCHK(h, hipMemcpy(h->d_ptcls + ((size_t)ip * h->nk + (k - h->kfrom)) * h->n + (irot - 1), &v, sizeof v,
+  HIPCHK(h, sync_copy(h, h->d_ptcls + ((size_t)ip * h->nk + (k - h->kfrom)) * h->n + (irot - 1), &v, sizeof v,
                       hipMemcpyHostToDevice));
   // set_ptcl_fcomp does not refresh sqsums_ptcls in the reference (:471-476); neither do we.
   h->memo_valid = false;
@@ -1752,7 +1777,7 @@ int pftcc_cp_even2odd_ref(pftcc_t* h, int32_t iref) {
   int rc;
   if ((rc = check_ref(h, iref))) return rc;
   const size_t ring = (size_t)h->nk * h->n;
-  HIPCHK(h, hipMemcpy(h->d_refs_o + (size_t)(iref - 1) * ring, h->d_refs_e + (size_t)(iref - 1) * ring,
+  HIPCHK(h, sync_copy(h, h->d_refs_o + (size_t)(iref - 1) * ring, h->d_refs_e + (size_t)(iref - 1) * ring,
                       ring * sizeof(float2), hipMemcpyDeviceToDevice));
   h->refnorm_valid = false;
   h->apack[0].valid = h->apack[1].valid = false;
@@ -1799,9 +1824,9 @@ int pftcc_reallocate_ptcls(pftcc_t* h, int32_t nptcls, const int32_t* pinds) {
   } else {
     // (:543-546) particles, square sums and eo flags are reset in both branches
     const size_t ring = (size_t)h->nk * h->n;
-    HIPCHK(h, hipMemset(h->d_ptcls, 0, (size_t)h->nptcls * ring * sizeof(float2)));
-    HIPCHK(h, hipMemset(h->d_sqsums, 0, (size_t)h->nptcls * sizeof(float)));
-    HIPCHK(h, hipMemset(h->d_sqsums_k, 0, (size_t)h->nptcls * h->nk * sizeof(float)));
+    HIPCHK(h, sync_memset(h, h->d_ptcls, 0, (size_t)h->nptcls * ring * sizeof(float2)));
+    HIPCHK(h, sync_memset(h, h->d_sqsums, 0, (size_t)h->nptcls * sizeof(float)));
+    HIPCHK(h, sync_memset(h, h->d_sqsums_k, 0, (size_t)h->nptcls * h->nk * sizeof(float)));
     h->iseven.assign(h->nptcls, 1);
     int rc;
     if ((rc = upload_iseven(h))) return rc;
@@ -1964,8 +1989,8 @@ int pftcc_assign_sigma2_noise(pftcc_t* h, const float* sigma2) {
   int rc;
   if (!h->d_shellw && ((rc = dalloc(h, &h->d_shellw, shellw.size())) || (rc = dalloc(h, &h->d_sigma2w, s2w.size()))))
     return rc;
-  HIPCHK(h, hipMemcpy(h->d_shellw, shellw.data(), shellw.size() * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(h->d_sigma2w, s2w.data(), s2w.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, sync_copy(h, h->d_shellw, shellw.data(), shellw.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, sync_copy(h, h->d_sigma2w, s2w.data(), s2w.size() * sizeof(double), hipMemcpyHostToDevice));
   h->have_sigma2 = true;
   return PFTCC_OK;
 }
@@ -2103,7 +2128,7 @@ int pftcc_get_ref_pft(pftcc_t* h, int32_t iref, int32_t iseven, float* pft) {
   int rc;
   if ((rc = check_ref(h, iref))) return rc;
   const size_t ring = (size_t)h->nk * h->n;
-  HIPCHK(h, hipMemcpy(pft, (iseven ? h->d_refs_e : h->d_refs_o) + (size_t)(iref - 1) * ring, ring * sizeof(float2),
+  HIPCHK(h, sync_copy(h, pft, (iseven ? h->d_refs_e : h->d_refs_o) + (size_t)(iref - 1) * ring, ring * sizeof(float2),
                       hipMemcpyDeviceToHost));
   return 0;
 }
@@ -2113,7 +2138,7 @@ int pftcc_get_ptcl_pft(pftcc_t* h, int32_t iptcl, float* pft) {
   int rc, ip;
   if ((rc = local_index(h, iptcl, &ip))) return rc;
   const size_t ring = (size_t)h->nk * h->n;
-  HIPCHK(h, hipMemcpy(pft, h->d_ptcls + (size_t)ip * ring, ring * sizeof(float2), hipMemcpyDeviceToHost));
+  HIPCHK(h, sync_copy(h, pft, h->d_ptcls + (size_t)ip * ring, ring * sizeof(float2), hipMemcpyDeviceToHost));
   return 0;
 }
 int pftcc_get_ctfmat(pftcc_t* h, int32_t iptcl, float* out) {
@@ -2123,7 +2148,7 @@ int pftcc_get_ctfmat(pftcc_t* h, int32_t iptcl, float* out) {
   if ((rc = local_index(h, iptcl, &ip))) return rc;
   if (!h->ctf_ready) return fail(h, PFTCC_E_STATE, "CTF matrices not created");
   const size_t ring = (size_t)h->nk * h->n;
-  HIPCHK(h, hipMemcpy(out, h->d_ctf + (size_t)ip * ring, ring * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(h, sync_copy(h, out, h->d_ctf + (size_t)ip * ring, ring * sizeof(float), hipMemcpyDeviceToHost));
   return 0;
 }
 int pftcc_get_memo(pftcc_t* h, int32_t iptcl, float* out) {
@@ -2133,7 +2158,7 @@ int pftcc_get_memo(pftcc_t* h, int32_t iptcl, float* out) {
   if ((rc = local_index(h, iptcl, &ip))) return rc;
   if (!h->memo_valid) return fail(h, PFTCC_E_STATE, "memo not computed");
   const size_t per = (size_t)h->nk * (h->n + 1);
-  HIPCHK(h, hipMemcpy(out, h->d_memo + (size_t)ip * per, per * sizeof(float2), hipMemcpyDeviceToHost));
+  HIPCHK(h, sync_copy(h, out, h->d_memo + (size_t)ip * per, per * sizeof(float2), hipMemcpyDeviceToHost));
   return 0;
 }
 int pftcc_get_sqsum_ptcl(pftcc_t* h, int32_t iptcl, float* out) {
@@ -2141,7 +2166,7 @@ int pftcc_get_sqsum_ptcl(pftcc_t* h, int32_t iptcl, float* out) {
   std::lock_guard<std::mutex> g(h->mtx);
   int rc, ip;
   if ((rc = local_index(h, iptcl, &ip))) return rc;
-  HIPCHK(h, hipMemcpy(out, h->d_sqsums + ip, sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(h, sync_copy(h, out, h->d_sqsums + ip, sizeof(float), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -2810,7 +2835,7 @@ static int shell_terms(pftcc_t* h, int iref, int iptcl, const float* shvec, int 
   if (use_scale) {
     // per-shell scale of this (eo, ref): ofac[row] * refinv, computed on the host from the tables
     std::vector<float> inv((size_t)2 * h->nrefs * h->kcorr), of;
-    HIPCHK(h, hipMemcpy(inv.data(), h->d_refinv_f, inv.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, sync_copy(h, inv.data(), h->d_refinv_f, inv.size() * 4, hipMemcpyDeviceToHost));
     std::vector<float> sc(h->nk, 1.0f);
     const int row = h->clsfrcs ? (iptcl - 1) : (iref - 1);
     for (int kk = 0; kk < h->kcorr; ++kk) {
@@ -2859,11 +2884,17 @@ int pftcc_specscore(pftcc_t* h, int32_t iref, int32_t iptcl, int32_t irot, const
     const float den = (float)std::sqrt(t[4 * kk + 2] * t[4 * kk + 1]);
     frc[kk] = den < 1e-12f ? 0.0f : (float)(t[4 * kk] / std::sqrt(t[4 * kk + 2] * t[4 * kk + 1]));
   }
-  // median_nocopy (simple_math.f90:2111-2130)
-  std::vector<float> s = frc;
-  std::sort(s.begin(), s.end());
-  const int nf = (int)s.size();
-  float med = (nf % 2 == 0) ? (s[nf / 2 - 1] + s[nf / 2]) / 2.0f : s[nf / 2];
+  // median_nocopy (simple_math.f90:2111-2130): the reference's quickselect selec on the array in
+  // place, two successive calls for even sizes (the second sees the first's permutation)
+  const int nf = (int)frc.size();
+  float med;
+  if (nf % 2 == 0) {
+    const float v1 = prep_selec(nf / 2, nf, frc.data());
+    const float v2 = prep_selec(nf / 2 + 1, nf, frc.data());
+    med = (v1 + v2) / 2.0f;
+  } else {
+    med = prep_selec(nint32((float)nf / 2.0f), nf, frc.data());
+  }
   *score = std::max(0.0f, med);
   return PFTCC_OK;
 }

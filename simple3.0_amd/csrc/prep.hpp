@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(256) k_prep_rows_inv(const float2* __restrict_
 
 // selec (simple_math.f90:2204-2261), restated step for step on an LDS array (1-based indices as
 // the reference's; arr permuted in place).  Its final test compares ir - 1 with 1.
-__device__ float prep_selec(int k, int n, float* arr) {
+__host__ __device__ float prep_selec(int k, int n, float* arr) {
   int l = 1, ir = n;
   auto A = [&](int i) -> float& { return arr[i - 1]; };
   for (;;) {

@@ -150,6 +150,7 @@ SIGNATURES = [
     ("pftcc_get_timing_channel", _I, [_P, _I, _PD, C.POINTER(C.c_int64)]),
     ("pftcc_set_timing", _I, [_P, _I]),
     ("pftcc_get_timing", _I, [_P, _PD, C.POINTER(C.c_int64)]),
+    ("pftcc_get_corr_tile", _I, [_P, _I, _PI, _PI]),
     ("pftcc_magic_pftsz", _I, [_I]),
 ]
 
@@ -711,6 +712,14 @@ class PolarFTCorrCalc:
         n = C.c_int64()
         self._chk(self._lib.pftcc_get_timing(self._h, C.byref(ms), C.byref(n)), "get_timing")
         return ms.value, n.value
+
+    def get_corr_tile(self, nshifts=0):
+        """(pairs per work-group of the plain kernel, staged-reference variant on) for an
+        exhaustive search with nshifts shifts."""
+        t = C.c_int32()
+        st = C.c_int32()
+        self._chk(self._lib.pftcc_get_corr_tile(self._h, int(nshifts), C.byref(t), C.byref(st)), "get_corr_tile")
+        return t.value, bool(st.value)
 
     def set_stream(self, stream_ptr):
         self._chk(self._lib.pftcc_set_stream(self._h, C.c_void_p(stream_ptr)), "set_stream")

@@ -210,3 +210,22 @@ def test_shift_search_invariants_on_random_pairs():
             # undo the rotation to get the search-frame shift back
             sh = L.rotmat2d_apply(cxy[1:], -rot)
             assert np.all(np.abs(sh) <= trs + 1e-5)
+
+
+def test_shift_search_returns_shift_rotated_to_reference_frame():
+    """The returned shift is matmul(shift, rotmat2d(get_rot(irot))) (simple_pftcc_shsrch_grad.f90:
+    295-297; rotmat2d simple_math.f90:1549-1559: mat = [[c, s], [-s, c]]).  A noiseless particle
+    that is the reference shifted by s* and rotated by 90 degrees (irot = nrots/4 + 1) is found at
+    that rotation and shift, and matmul((sx, sy), rotmat2d(90)) = (-sy, sx) is returned."""
+    n = 48
+    pf, _ = make_case(seed=5, n=n, kfrom=2, kto=16, nrefs=1, nptcls=1, ctf=False, ldim=64)
+    star = np.array([0.85, -1.2])
+    rho = n // 2                                   # 90 degrees of 2n = 96 rotations
+    q = pf.refs_even[0].astype(np.complex128) * O.shmat_8(pf.grid, star)
+    pf.ptcls[0] = np.roll(O.full_ring(q), rho, axis=-1)[:, :n].astype(np.complex64)
+    fdf, gencorrs = _pair_objective(pf, 0, 0)
+    cxy, irot = L.shsrch_minimize(fdf, gencorrs, pf.grid.angtab_deg, trs=3.0, opt_angle=True)
+    assert irot == rho + 1
+    assert abs(float(pf.grid.angtab_deg[irot - 1]) - 90.0) < 1e-4
+    assert cxy[0] > 0.9999
+    np.testing.assert_allclose(cxy[1:], [-star[1], star[0]], atol=2e-3)

@@ -288,3 +288,153 @@ def test_specscore_planted_and_bounds():
     frc = pf2.genfrc(0, 0, 5)
     assert np.all(np.abs(frc) <= 1 + 1e-12)
     assert pf2.specscore(0, 0, 5) >= 0.0
+
+
+# ------------------------------------------------- parity hazards: closed-form pins -----------
+def _single_bin_case(n=16, kfrom=2, kto=8, ldim=32):
+    """A case with no CTF whose reference holds one non-zero value on each of two shells."""
+    pf, _ = make_case(seed=61, n=n, kfrom=kfrom, kto=kto, ctf=False, nrefs=1, nptcls=1, ldim=ldim)
+    pf.refs_even[:] = 0
+    return pf
+
+
+def _phase(k, j, n, ldim, d):
+    """Closed-form shift phase of ring position j on shell k: (x dx + y dy) pi/(ldim/2) with
+    x = k sin(pi j/n), y = -k cos(pi j/n) (new :261-317, gen_shmat_8 :1098-1120)."""
+    th = math.pi * j / n
+    return (k * math.sin(th) * d[0] - k * math.cos(th) * d[1]) * math.pi / (ldim / 2)
+
+
+@pytest.mark.parametrize("s", [(0.0, 0.0), (0.4, -0.7), (-1.3, 0.9)])
+def test_k_weighted_objective_and_gradient_closed_form(s):
+    """gencorr_cc_for_rot_8 / gencorr_cc_grad_for_rot_8 without match filter weight shell k by k
+    in the numerator and both denominators (:1717-1727, :1902-1924).  With the reference non-zero
+    only at (k1, j1) = a and (k2, j2) = b, and the particle equal to the reference shifted by s0 on
+    k1 and to its negative on k2, at irot = 1 and shift s (d = s - s0):
+        f = (k1 |a|^2 cos(phi1(d)) - k2 |b|^2 cos(phi2(d))) / (k1 |a|^2 + k2 |b|^2)
+        df/dsx = (-k1 |a|^2 x1 sin(phi1(d)) + k2 |b|^2 x2 sin(phi2(d))) / (k1 |a|^2 + k2 |b|^2)
+    (x = k sin(theta) pi/(ldim/2), y likewise).  Unit weights, a sign or a dropped weight in either
+    denominator changes f at d = 0 already: (k1|a|^2 - k2|b|^2)/(k1|a|^2 + k2|b|^2)."""
+    n, kfrom, ldim = 16, 2, 32
+    pf = _single_bin_case(n=n, kfrom=kfrom, ldim=ldim)
+    k1, j1, a = 3, 5, 0.8 - 0.3j
+    k2, j2, b = 6, 11, -0.2 + 1.1j
+    pf.refs_even[0, k1 - kfrom, j1] = a
+    pf.refs_even[0, k2 - kfrom, j2] = b
+    s0 = (0.6, -0.2)
+    ph = O.shmat_closed(pf.grid, s0)
+    p = np.zeros_like(pf.refs_even[0], dtype=np.complex128)
+    p[k1 - kfrom, j1] = a * ph[k1 - kfrom, j1]
+    p[k2 - kfrom, j2] = -b * ph[k2 - kfrom, j2]
+    pf.ptcls[0] = p.astype(np.complex64)
+    # the closed form on the FP32-rounded values the oracle correlates
+    A2 = abs(complex(pf.refs_even[0, k1 - kfrom, j1])) ** 2
+    B2 = abs(complex(pf.refs_even[0, k2 - kfrom, j2])) ** 2
+    d = (s[0] - s0[0], s[1] - s0[1])
+    p1, p2 = _phase(k1, j1, n, ldim, d), _phase(k2, j2, n, ldim, d)
+    den = k1 * A2 + k2 * B2
+    f_cf = (k1 * A2 * math.cos(p1) - k2 * B2 * math.cos(p2)) / den
+    g_cf = []
+    for ax in (0, 1):
+        e = [0.0, 0.0]
+        e[ax] = 1.0
+        x1, x2 = _phase(k1, j1, n, ldim, e), _phase(k2, j2, n, ldim, e)
+        g_cf.append((-k1 * A2 * x1 * math.sin(p1) + k2 * B2 * x2 * math.sin(p2)) / den)
+    # FP32 inputs (particle phase, polar grid) leave ~1e-7 relative
+    f = pf.gencorr_for_rot_8(0, 0, s, 1)
+    assert abs(f - f_cf) < 2e-6
+    f2, g = pf.gencorr_grad_for_rot_8(0, 0, s, 1)
+    assert abs(f2 - f_cf) < 2e-6
+    for ax in range(2):
+        assert abs(g[ax] - g_cf[ax]) < 2e-6 * max(1.0, abs(g_cf[ax]))
+    if s == (0.0, 0.0):
+        # at d = -s0 the closed form is not the unweighted value
+        f_unw = (A2 * math.cos(p1) - B2 * math.cos(p2)) / (A2 + B2)
+        assert abs(f - f_unw) > 1e-2
+
+
+def test_shifted_normaliser_uses_all_shells_closed_form():
+    """gencorrs_cc_2 normalises by S_q over ALL shells kfrom..kto (:1531), gencorrs_cc_1 over
+    kfrom..kstop (prep_ref4corr :984).  At shift (0, 0) the numerators agree, so with a reference of
+    constant magnitude a_k on shell k (no CTF) the ratio is, at every rotation,
+        gencorrs(shvec = (0, 0)) / gencorrs(no shift) = sqrt(sum_{k<=kstop} a_k^2 / sum_{k<=kto} a_k^2).
+    Normalising the shifted form over kfrom..kstop as well would give 1."""
+    n, kfrom, kto, kstop = 16, 2, 10, 6
+    pf, _ = make_case(seed=62, n=n, kfrom=kfrom, kto=kto, kstop=kstop, ctf=False, nrefs=1, nptcls=1)
+    rng = np.random.default_rng(3)
+    amp = np.linspace(1.0, 0.3, kto - kfrom + 1)
+    pf.refs_even[0] = (amp[:, None] * np.exp(2j * np.pi * rng.random((kto - kfrom + 1, n)))).astype(np.complex64)
+    a2 = np.abs(pf.refs_even[0].astype(np.complex128)) ** 2        # FP32-rounded magnitudes
+    ratio_cf = math.sqrt(a2[:kstop - kfrom + 1].sum() / a2.sum())
+    assert abs(ratio_cf - math.sqrt((amp[:kstop - kfrom + 1] ** 2).sum() / (amp ** 2).sum())) < 1e-6
+    c1 = pf.gencorrs(0, 0, None)
+    c2 = pf.gencorrs(0, 0, (0.0, 0.0))
+    sel = np.abs(c1) > 1e-3 * np.max(np.abs(c1))
+    np.testing.assert_allclose(c2[sel] / c1[sel], ratio_cf, rtol=1e-12)
+    assert abs(ratio_cf - 1.0) > 0.05
+
+
+def test_gencorrs_shifts_equals_per_shift():
+    """The vectorised shift list form equals gencorrs_cc_2 shift by shift (FFT and direct routes)."""
+    pf, _ = make_case(seed=63, n=24, kfrom=2, kto=9, kstop=8, ctf=True, nrefs=2, nptcls=2, eo_mixed=True)
+    shifts = np.array([[0.0, 0.0], [0.5, -1.0], [-2.25, 1.5]])
+    for r in range(2):
+        for i in range(2):
+            many = pf.gencorrs_shifts(r, i, shifts)
+            for s in range(3):
+                one = pf.gencorrs(r, i, tuple(shifts[s]), method="direct")
+                assert np.max(np.abs(many[s] - one)) < 1e-12
+
+
+@pytest.mark.parametrize("irot", [1, 5, 16, 17, 20, 32])
+def test_sigma_contrib_identity_and_planted_energy(irot):
+    """gencorr_sigma_contrib (:2150-2188) = 0.5/n sum_j |q_k[j] - P_k[j + rho]|^2 on every shell
+    kfrom..kto, with q = ref x |CTF| x shmat (no shell normalisation).
+    (1) Identity: 0.5/n (S_q,k + S_p,k - 2 C_k(rho)), C_k from the reference's FFT route
+        (memoize_fft + calc_k_corrs), at irot = 1, <= n, n + 1 and > n + 1.
+    (2) Planted offset: a particle equal to the reference rotated by rho plus an offset e of
+        magnitude c_k on shell k (as a Hermitian full ring) gives exactly 0.5 c_k^2 on shell k: the
+        0.5/n scale and the rotation index are both fixed by it."""
+    n = 16
+    pf, _ = make_case(seed=64, n=n, kfrom=2, kto=8, kstop=7, ctf=True, nrefs=1, nptcls=1, euclid=True)
+    sh = (0.5, -0.25)
+    q = pf.prep_ref(0, 0, sh, shellnorm=False)
+    p = pf.ptcls[0].astype(np.complex128)
+    pa, pb = O.memoize_fft(p)
+    ck = O.corrs_over_k_fft(q, pa, pb)[:, irot - 1]
+    ident = 0.5 / n * (np.sum(np.abs(q) ** 2, -1) + np.sum(np.abs(p) ** 2, -1) - 2.0 * ck)
+    np.testing.assert_allclose(pf.sigma_contrib(0, 0, sh, irot), ident, rtol=1e-10, atol=1e-12)
+    # planted offset (no CTF so that q is the shifted reference itself)
+    pf.ctfmats = None
+    q = pf.prep_ref(0, 0, sh, shellnorm=False)
+    rng = np.random.default_rng(irot)
+    c = np.linspace(0.05, 0.4, pf.grid.nk)
+    e = c[:, None] * np.exp(2j * np.pi * rng.random((pf.grid.nk, n)))
+    planted = np.roll(O.full_ring(q), irot - 1, axis=-1)[:, :n] + e
+    pf.ptcls[0] = planted.astype(np.complex64)
+    # the FP32 rounding of the planted particle moves the result by ~1e-8
+    np.testing.assert_allclose(pf.sigma_contrib(0, 0, sh, irot), 0.5 * c ** 2, rtol=2e-6, atol=1e-8)
+
+
+def test_median_nocopy_positions_and_selec():
+    """median_nocopy (simple_math.f90:2111-2130).  With an exact order statistic in place of selec
+    it is np.median for both parities (positions n/2 and n/2 + 1 averaged, nint(n/2.) for odd n);
+    with the reference's selec each call stays within one rank of the exact order statistic."""
+    import prep_oracle
+    rng = np.random.default_rng(5)
+    orig = prep_oracle.selec
+    try:
+        prep_oracle.selec = lambda k, a: np.float32(np.sort(a)[k - 1])
+        for size in (1, 2, 5, 8, 39, 40):
+            x = rng.standard_normal(size).astype(np.float32)
+            assert abs(float(O.median_nocopy(x.copy())) - float(np.median(x))) <= 1e-7
+    finally:
+        prep_oracle.selec = orig
+    for size in (7, 8, 39, 59):
+        for _ in range(50):
+            x = rng.standard_normal(size).astype(np.float32)
+            s = np.sort(x)
+            v = float(O.median_nocopy(x.copy()))
+            lo = s[max(0, size // 2 - 2)] if size % 2 == 0 else s[max(0, size // 2 - 1)]
+            hi = s[min(size - 1, size // 2 + 1)]
+            assert lo <= v <= hi
