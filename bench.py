@@ -102,6 +102,10 @@ def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
     plain_value = plain_sample * nrefs * nrots / pwall
     same = (pir == ir[:plain_sample]) & (prot == rot[:plain_sample])
     same_raw = (pir == raw_ir) & (prot == raw_rot)
+    log("precision form: FP64 checker on a sample")
+    cpu = precision_cpu_baseline(lib, nrefs, kcorr, PFTCC_PREC_SPLIT_BF16, ntop,
+                                 (cc[:plain_sample], ir[:plain_sample], rot[:plain_sample]), (pir, prot),
+                                 ids[:plain_sample])
     lib.kill()
     zg_ms, zg_n = ch["zgemm"]
     zi_ms, zi_n = ch["zifft"]
@@ -132,6 +136,91 @@ def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
                                 "frac": zbytes / (zi_ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS},
             "pack_topk_rescore": {"ms_per_step": ch["pack_rescore"][0] / steps},
         },
+        "cpu_baseline": cpu,
+    }
+
+
+def precision_cpu_baseline(lib, nrefs, kcorr, prec, ntop, best, plain, sample_ids, nfp64=6):
+    """CPU baseline of the precision form, used as its FP64 checker: the reference formula
+    (gencorrs_cc_1, FFT route, FP64 numpy restatement of oracle/pftcc_oracle.py) of nfp64 particles
+    against all references, timed on one host core, and the errors of the GPU's results against it.
+
+    * contraction: the precision form's correlations (no rescoring) against the FP64 formula with the
+      particle's ring-mean |CTF| (the quantity the per-m contraction computes): its arithmetic error;
+    * ring_mean_ctf: the FP64 formula with the ring-mean |CTF| against the exact per-pixel |CTF|: the
+      model error that the rescoring removes;
+    * unrescored_total: the precision form's correlations against the exact FP64 formula;
+    * assignments: best (reference, rotation) of the rescored and unrescored searches against the
+      exact FP64 best, counted where the FP64 top-2 gap exceeds 1e-5;
+    * on the plain-FP32 sample: every particle where the rescored precision form and the plain FP32
+      search disagree is rescored at both answers with the exact FP64 formula.
+    Errors are absolute, in units of cc (|cc| <= 1), and per vector relative to its FP64 peak."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pftcc_oracle as O
+
+    refs = np.stack([lib.get_ref_pft(r + 1, True) for r in range(nrefs)])
+    ids = np.asarray(sample_ids[:nfp64])
+    err = {"contraction": [], "ring_mean_ctf": [], "unrescored_total": []}
+    rel = {"contraction": [], "ring_mean_ctf": [], "unrescored_total": []}
+    agree_res = agree_raw = counted = 0
+    cpu_s = 0.0
+    dev_rs = lib.search_mfma(ids, prec=prec, ntop=ntop)
+    dev_raw = lib.search_mfma(ids, prec=prec, ntop=0)
+    for q, i in enumerate(ids):
+        pt = lib.get_ptcl_pft(int(i))
+        ctf = lib.get_ctfmat(int(i)).astype(np.float64)
+        ctf_rm = np.repeat(ctf.mean(axis=1, keepdims=True), ctf.shape[1], axis=1)
+        t0 = time.perf_counter()
+        exact = O.gencorrs_refs_fft(refs, pt, ctf, kcorr)
+        cpu_s += time.perf_counter() - t0
+        ringm = O.gencorrs_refs_fft(refs, pt, ctf_rm, kcorr)
+        dev = lib.gencorrs_mfma_batch([int(i)], prec=prec)[0].astype(np.float64)
+        for key, a, b in (("contraction", dev, ringm), ("ring_mean_ctf", ringm, exact), ("unrescored_total", dev, exact)):
+            d = np.abs(a - b)
+            err[key].append(d.ravel())
+            rel[key].append(np.max(d, axis=1) / np.maximum(np.max(np.abs(b), axis=1), 1e-30))
+        flat = np.sort(exact.ravel())
+        if flat[-1] - flat[-2] > 1e-5:
+            counted += 1
+            r, t = np.unravel_index(int(np.argmax(exact)), exact.shape)
+            agree_res += int((dev_rs[1][q], dev_rs[2][q]) == (r + 1, t + 1))
+            agree_raw += int((dev_raw[1][q], dev_raw[2][q]) == (r + 1, t + 1))
+
+    def summary(key):
+        a = np.concatenate(err[key])
+        v = np.concatenate(rel[key])
+        return {"max_abs": float(a.max()), "p99_abs": float(np.quantile(a, 0.99)),
+                "max_peak_rel": float(v.max()), "p99_peak_rel": float(np.quantile(v, 0.99))}
+
+    # disagreements between the rescored precision form and the plain FP32 search, settled in FP64
+    (bcc, bir, brot), (pir, prot) = best, plain
+    worse = ties = 0
+    nd = 0
+    for q in np.where((bir != pir) | (brot != prot))[0]:
+        nd += 1
+        i = int(sample_ids[q])
+        pt = lib.get_ptcl_pft(i).astype(np.complex128)
+        ctf = lib.get_ctfmat(i).astype(np.float64)
+
+        def cc64(iref, irot):
+            qr = refs[iref - 1, :kcorr].astype(np.complex128) * ctf[:kcorr]
+            c = O.corrk_for_rot(qr, pt[:kcorr], irot).sum()
+            return c / math.sqrt(np.sum(np.abs(qr) ** 2) * np.sum(np.abs(pt[:kcorr]) ** 2))
+        a, b = cc64(int(bir[q]), int(brot[q])), cc64(int(pir[q]), int(prot[q]))
+        if abs(a - b) <= 1e-6:
+            ties += 1
+        elif a < b:
+            worse += 1
+    return {
+        "value": len(ids) * nrefs * 2 * refs.shape[-1] / cpu_s, "unit": "evals/s", "cores": 1, "kind": "port",
+        "sample": f"{len(ids)} ptcls x {nrefs} refs x {2 * refs.shape[-1]} rots: the reference formula in FP64 "
+                  f"(numpy FFT route, one core), used as the checker of the precision form",
+        "error_vs_fp64": {k: summary(k) for k in err},
+        "assignments_vs_fp64_best": {"particles_with_gap_gt_1e-5": counted,
+                                     "rescored_identical": agree_res, "unrescored_identical": agree_raw},
+        "plain_fp32_sample_disagreements": {"count": nd, "fp64_ties_within_1e-6": ties,
+                                            "precision_form_worse_in_fp64": worse,
+                                            "identical_or_fp64_tie_frac": 1.0 - worse / max(1, len(pir))},
     }
 
 

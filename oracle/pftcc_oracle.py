@@ -411,6 +411,36 @@ def corrs_over_k_fft(q: np.ndarray, pa: np.ndarray, pb: np.ndarray) -> np.ndarra
     return back / (2.0 * 2 * n)
 
 
+def gencorrs_refs_fft(refs: np.ndarray, ptcl: np.ndarray, ctf, kcorr: int, chunk: int = 256) -> np.ndarray:
+    """gencorrs_cc_1 (:1486-1500) of one particle against many references, FP64, through the
+    reference's FFT route (prep_ref4corr :961-985 without shell normalisation, memoize_fft
+    :864-877, calc_corrs_over_k :1132-1162), vectorised over the references.
+
+    refs [nr, nk, n] complex, ptcl [nk, n] complex, ctf [nk, n] real (the particle's |CTF|) or
+    None -> cc [nr, 2n].  S_q and S_p over the first kcorr shells (kfrom..kstop)."""
+    n = refs.shape[-1]
+    h = n // 2
+    p = ptcl[:kcorr].astype(np.complex128)
+    pa, pb = memoize_fft(p)
+    sp = float(np.sum(np.abs(p) ** 2))
+    c = None if ctf is None else np.asarray(ctf, np.float64)[:kcorr]
+    ff = fft_factors(n)
+    out = np.empty((refs.shape[0], 2 * n))
+    for r0 in range(0, refs.shape[0], chunk):
+        q = refs[r0:r0 + chunk, :kcorr].astype(np.complex128)
+        if c is not None:
+            q = q * c[None]
+        sq = np.sum(np.abs(q) ** 2, axis=(1, 2))
+        a = np.fft.rfft(np.real(q), axis=-1)
+        b = np.fft.fft(np.imag(q) * ff, axis=-1)
+        prod = np.zeros(q.shape[:2] + (2 * n + 2,), dtype=np.complex128)
+        prod[..., 0:2 * h + 1:2] = 4.0 * np.conj(a[..., :h + 1]) * pa[None, :, :h + 1]
+        prod[..., 1:2 * h + 2:2] = 4.0 * np.conj(b[..., :h + 1]) * pb[None, :, :h + 1]
+        back = np.fft.irfft(prod[..., :n + 1].sum(axis=1), n=2 * n, axis=-1) * (2 * n) / (2.0 * 2 * n)
+        out[r0:r0 + chunk] = back / np.sqrt(sq * sp)[:, None]
+    return out
+
+
 # ---------------------------------------------------------------------------------------------
 # a17: Savitzky-Golay smoothing used by prep_matchfilt (simple_math.f90:3511-3588)
 # ---------------------------------------------------------------------------------------------

@@ -438,3 +438,19 @@ def test_median_nocopy_positions_and_selec():
             lo = s[max(0, size // 2 - 2)] if size % 2 == 0 else s[max(0, size // 2 - 1)]
             hi = s[min(size - 1, size // 2 + 1)]
             assert lo <= v <= hi
+
+
+def test_gencorrs_refs_fft_equals_per_pair():
+    """The many-references form equals gencorrs_cc_1 pair by pair (direct sums), with and without
+    CTF and with kstop < kto."""
+    pf, _ = make_case(seed=65, n=24, kfrom=2, kto=9, kstop=7, ctf=True, nrefs=5, nptcls=2)
+    for i in range(2):
+        for ctf in (True, False):
+            c = pf.ctfmats[i] if ctf else None
+            many = O.gencorrs_refs_fft(pf.refs_even, pf.ptcls[i], c, pf.ks, chunk=2)
+            saved = pf.ctfmats
+            if not ctf:
+                pf.ctfmats = None
+            for r in range(5):
+                assert np.max(np.abs(many[r] - pf.gencorrs(r, i, None, method="direct"))) < 1e-12
+            pf.ctfmats = saved
