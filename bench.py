@@ -625,6 +625,28 @@ def strategy_form(lib, w, nptcls=10000, reps=2, with_cpu=True):
                          "note": "unshifted search: one pair per reference, so the staged-reference tile does not apply"}}
 
 
+def host_cpu():
+    """CPU model and core counts of this host (the CPU baseline's hardware)."""
+    model, phys = "unknown", set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    if "physical id" in cur and "core id" in cur:
+                        phys.add((cur["physical id"], cur["core id"]))
+                    cur = {}
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                cur[k] = v
+                if k == "model name":
+                    model = v
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": len(phys) or None,
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
 def cpu_baseline(pftcc, w, shifts, nthreads, target_s=12.0):
     """Times the reference CPU algorithm (C++/OpenMP port, oracle/pftcc_cpu.cpp) on a bounded
     sample of the same workload and checks the GPU's assignments on that sample."""
@@ -661,8 +683,11 @@ def cpu_baseline(pftcc, w, shifts, nthreads, target_s=12.0):
     return {
         "value": evals / dt, "unit": "evals/s", "cores": nthreads, "kind": "port",
         "sample": f"{npt} ptcls x {nr} refs x {ns} shifts x {2 * n} rots of the same workload "
-                  f"({dt:.1f} s): the reference's FP32 FFT route (r2c_n + c2c_n + c2r_2n per shell, "
-                  f"own mixed-radix FFT since FFTW is absent), OpenMP over particles",
+                  f"({dt:.1f} s): the reference's FP32 FFT route (r2c_n + c2c_n + c2r_2n per shell), "
+                  f"OpenMP over particles",
+        "fft": "own mixed-radix Stockham FFT (FFTW is absent), eight shells per AVX2 vector, twiddles precomputed",
+        "cpu": host_cpu(),
+        "gflops_per_core": evals / dt / (2 * n) * plain_flops_per_pair(n, nk) / nthreads / 1e9,
         "gpu_assignment_agreement": float(np.mean(same)),
         "max_abs_best_cc_diff": float(np.max(np.abs(gcc - ccc))),
     }

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#include <algorithm>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -138,6 +139,159 @@ struct Transforms {
   }
 };
 
+// ------------------------------------------------------------------------------------------
+// Eight transforms at once: the shells of one (reference, particle) pair in groups of eight, one
+// shell per lane of an 8-wide float vector (AVX2), structure-of-arrays complex values.  The same
+// Stockham passes as above with every twiddle precomputed per pass (no trigonometry per call).
+// ------------------------------------------------------------------------------------------
+typedef float v8 __attribute__((vector_size(32)));
+struct vc {
+  v8 re, im;
+};
+inline vc vadd(const vc& a, const vc& b) { return {a.re + b.re, a.im + b.im}; }
+inline vc vsub(const vc& a, const vc& b) { return {a.re - b.re, a.im - b.im}; }
+inline vc vmul(const vc& a, cf w) {   // by a scalar twiddle
+  const float c = w.real(), s = w.imag();
+  return {a.re * c - a.im * s, a.re * s + a.im * c};
+}
+inline vc vmulc(const vc& a, const vc& b) {   // conj(a) * b, lane by lane
+  return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+
+struct VPlan {
+  int n = 0;
+  std::vector<int> radix;
+  std::vector<std::vector<cf>> ptw;   // per pass: [jm][t] = W_n^(jm t tstep)
+  std::vector<std::vector<cf>> dft;   // per pass: [k][t] = exp(-2 pi i k t / R) for generic radices
+};
+
+VPlan make_vplan(int n) {
+  VPlan p;
+  p.n = n;
+  const Plan sp = make_plan(n);
+  p.radix = sp.radix;
+  int Ns = 1;
+  for (int R : p.radix) {
+    const int tstep = n / (Ns * R);
+    std::vector<cf> tw((size_t)Ns * R);
+    for (int jm = 0; jm < Ns; ++jm)
+      for (int t = 0; t < R; ++t) {
+        const double ang = -2.0 * M_PI * (double)((size_t)jm * t * tstep % n) / n;
+        tw[(size_t)jm * R + t] = cf((float)std::cos(ang), (float)std::sin(ang));
+      }
+    p.ptw.push_back(tw);
+    std::vector<cf> d((size_t)R * R);
+    for (int k = 0; k < R; ++k)
+      for (int t = 0; t < R; ++t) {
+        const double ang = -2.0 * M_PI * (double)((k * t) % R) / R;
+        d[(size_t)k * R + t] = cf((float)std::cos(ang), (float)std::sin(ang));
+      }
+    p.dft.push_back(d);
+    Ns *= R;
+  }
+  return p;
+}
+
+// forward complex FFT (exp(-)) of 8 lanes; x overwritten, w scratch (n each)
+void vfft(const VPlan& p, vc* x, vc* w) {
+  const int n = p.n;
+  int Ns = 1;
+  vc* src = x;
+  vc* dst = w;
+  vc v[32], o[32];
+  for (size_t ip = 0; ip < p.radix.size(); ++ip) {
+    const int R = p.radix[ip];
+    const int nb = n / R;
+    const cf* tw = p.ptw[ip].data();
+    const cf* D = p.dft[ip].data();
+    for (int j = 0; j < nb; ++j) {
+      const int jm = j % Ns;
+      for (int t = 0; t < R; ++t) v[t] = src[j + t * nb];
+      if (Ns > 1)
+        for (int t = 1; t < R; ++t) v[t] = vmul(v[t], tw[(size_t)jm * R + t]);
+      const int base = (j / Ns) * Ns * R + jm;
+      if (R == 4) {
+        const vc t0 = vadd(v[0], v[2]), t1 = vsub(v[0], v[2]), t2 = vadd(v[1], v[3]), t3 = vsub(v[1], v[3]);
+        dst[base] = vadd(t0, t2);
+        dst[base + 2 * Ns] = vsub(t0, t2);
+        dst[base + Ns] = {t1.re + t3.im, t1.im - t3.re};
+        dst[base + 3 * Ns] = {t1.re - t3.im, t1.im + t3.re};
+      } else if (R == 2) {
+        dst[base] = vadd(v[0], v[1]);
+        dst[base + Ns] = vsub(v[0], v[1]);
+      } else {
+        // generic radix: o[k] = sum_t v[t] D[k][t]; k = 0 is the plain sum
+        o[0] = v[0];
+        for (int t = 1; t < R; ++t) o[0] = vadd(o[0], v[t]);
+        for (int k = 1; k < R; ++k) {
+          vc s = v[0];
+          for (int t = 1; t < R; ++t) s = vadd(s, vmul(v[t], D[(size_t)k * R + t]));
+          o[k] = s;
+        }
+        for (int k = 0; k < R; ++k) dst[base + k * Ns] = o[k];
+      }
+    }
+    Ns *= R;
+    std::swap(src, dst);
+  }
+  if (src != x) std::memcpy(static_cast<void*>(x), src, sizeof(vc) * n);
+}
+
+// r2c_n, c2c_n and c2r_2n of the reference (FFTW conventions, unnormalised) on 8 lanes.
+struct Transforms8 {
+  int n;
+  VPlan ph, pn;
+  std::vector<vc> a, b;
+  std::vector<cf> er;   // exp(-2 pi i k / n), k = 0..n/2 (the r2c split)
+  std::vector<cf> ec;   // exp(+i pi m / n), m = 0..n-1 (the c2r pre-twiddle)
+  explicit Transforms8(int n_) : n(n_), ph(make_vplan(n_ / 2)), pn(make_vplan(n_)), a(n_ + 2), b(n_ + 2),
+                                 er(n_ / 2 + 1), ec(n_) {
+    for (int k = 0; k <= n / 2; ++k) {
+      const double ang = -2.0 * M_PI * k / n;
+      er[k] = cf((float)std::cos(ang), (float)std::sin(ang));
+    }
+    for (int m = 0; m < n; ++m) {
+      const double ang = M_PI * m / n;
+      ec[m] = cf((float)std::cos(ang), (float)std::sin(ang));
+    }
+  }
+  // out[0..n/2] = sum_j x[j] exp(-2 pi i j k / n), x real
+  void r2c(const v8* x, vc* out) {
+    const int h = n / 2;
+    for (int j = 0; j < h; ++j) a[j] = {x[2 * j], x[2 * j + 1]};
+    vfft(ph, a.data(), b.data());
+    for (int k = 0; k <= h; ++k) {
+      const vc A = a[k % h], Bc = a[(h - k) % h];            // B = conj(Bc)
+      const vc e = {0.5f * (A.re + Bc.re), 0.5f * (A.im - Bc.im)};
+      const vc d = {A.re - Bc.re, A.im + Bc.im};              // A - B
+      const vc o = {0.5f * d.im, -0.5f * d.re};               // -i/2 (A - B)
+      out[k] = vadd(e, vmul(o, er[k]));
+    }
+  }
+  void c2c(vc* x) { vfft(pn, x, b.data()); }
+  // out[t] = sum_{m=0}^{2n-1} X[m] exp(+2 pi i m t / 2n), X Hermitian from X[0..n]
+  void c2r2n(const vc* X, v8* out) {
+    const v8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int m = 0; m < n; ++m) {
+      vc Xm = X[m], Xn = X[n - m];
+      if (m == 0) {
+        Xm.im = zero;
+        Xn.im = zero;
+      }
+      const vc s = {Xm.re + Xn.re, Xm.im - Xn.im};          // Xm + conj(Xn)
+      const vc df = {Xm.re - Xn.re, Xm.im + Xn.im};         // Xm - conj(Xn)
+      const vc ed = vmul(df, ec[m]);
+      const vc t = {s.re - ed.im, s.im + ed.re};             // s + i e df
+      a[m] = {t.re, -t.im};                                   // conj
+    }
+    vfft(pn, a.data(), b.data());
+    for (int j = 0; j < n; ++j) {
+      out[2 * j] = a[j].re;
+      out[2 * j + 1] = -a[j].im;
+    }
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -178,19 +332,42 @@ void cpu_search_fp32(int nr, int np, int ns, int nk, int kcorr, int n, const flo
   const cf* PA = reinterpret_cast<const cf*>(pa);
   const cf* PB = reinterpret_cast<const cf*>(pb);
   const cf* SH = reinterpret_cast<const cf*>(shmat);
+  const int ng = (kcorr + 7) / 8;   // groups of eight shells
 #pragma omp parallel num_threads(nthreads)
   {
-    Transforms T(n);
-    std::vector<cf> q((size_t)nk * n), A(h + 1), B(n), prod(R + 2);
-    std::vector<float> re(n), back(R), corrs(R);
-    std::vector<cf> im(n);
+    Transforms8 T(n);
+    std::vector<cf> q((size_t)nk * n);
+    std::vector<v8> re(n), back(R), corrs(R);
+    std::vector<vc> A(h + 1), B(n + 2), prod(n + 2);
+    std::vector<vc> pav((size_t)ng * (h + 1)), pbv((size_t)ng * n);   // the particle's memo, 8 shells per vector
     std::vector<cf> fftf(n);
     for (int j = 0; j < n; ++j) {
       const double ang = -M_PI * j / n;
       fftf[j] = cf((float)std::cos(ang), (float)std::sin(ang));
     }
+    const v8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma omp for schedule(static)
     for (int i = 0; i < np; ++i) {
+      for (int g = 0; g < ng; ++g) {
+        for (int t = 0; t <= h; ++t) {
+          vc v = {zero, zero};
+          for (int l = 0; l < 8 && 8 * g + l < kcorr; ++l) {
+            const cf x = PA[((size_t)i * nk + 8 * g + l) * (h + 1) + t];
+            v.re[l] = x.real();
+            v.im[l] = x.imag();
+          }
+          pav[(size_t)g * (h + 1) + t] = v;
+        }
+        for (int t = 0; t < n; ++t) {
+          vc v = {zero, zero};
+          for (int l = 0; l < 8 && 8 * g + l < kcorr; ++l) {
+            const cf x = PB[((size_t)i * nk + 8 * g + l) * n + t];
+            v.re[l] = x.real();
+            v.im[l] = x.imag();
+          }
+          pbv[(size_t)g * n + t] = v;
+        }
+      }
       float bc = -INFINITY;
       int br = 0, bs = 0, bt = 0;
       for (int r = 0; r < nr; ++r)
@@ -205,20 +382,29 @@ void cpu_search_fp32(int nr, int np, int ns, int nk, int kcorr, int n, const flo
               q[(size_t)kk * n + j] = v;
               sqsum += v.real() * v.real() + v.imag() * v.imag();
             }
-          std::fill(corrs.begin(), corrs.end(), 0.f);
-          for (int kk = 0; kk < kcorr; ++kk) {
-            const cf* qk = &q[(size_t)kk * n];
+          // calc_corrs_over_k, eight shells per pass (one per lane); the lanes are summed at the end
+          std::fill(corrs.begin(), corrs.end(), zero);
+          for (int g = 0; g < ng; ++g) {
+            const int nl = std::min(8, kcorr - 8 * g);
             for (int j = 0; j < n; ++j) {
-              re[j] = qk[j].real();
-              im[j] = qk[j].imag() * fftf[j];
+              v8 xr = zero, yr = zero, yi = zero;
+              for (int l = 0; l < nl; ++l) {
+                const cf x = q[(size_t)(8 * g + l) * n + j];
+                xr[l] = x.real();
+                yr[l] = x.imag() * fftf[j].real();
+                yi[l] = x.imag() * fftf[j].imag();
+              }
+              re[j] = xr;
+              B[j] = {yr, yi};
             }
             T.r2c(re.data(), A.data());
-            T.c2c(im.data(), B.data());
-            const cf* pak = PA + ((size_t)i * nk + kk) * (h + 1);
-            const cf* pbk = PB + ((size_t)i * nk + kk) * n;
+            T.c2c(B.data());
+            const vc* pak = &pav[(size_t)g * (h + 1)];
+            const vc* pbk = &pbv[(size_t)g * n];
             for (int t = 0; t <= h; ++t) {
-              prod[2 * t] = 4.f * std::conj(A[t]) * pak[t];
-              prod[2 * t + 1] = 4.f * std::conj(B[t]) * pbk[t];
+              const vc a4 = vmulc(A[t], pak[t]), b4 = vmulc(B[t], pbk[t]);
+              prod[2 * t] = {4.f * a4.re, 4.f * a4.im};
+              if (2 * t + 1 <= n) prod[2 * t + 1] = {4.f * b4.re, 4.f * b4.im};
             }
             T.c2r2n(prod.data(), back.data());
             for (int t = 0; t < R; ++t) corrs[t] += back[t];
@@ -227,7 +413,9 @@ void cpu_search_fp32(int nr, int np, int ns, int nk, int kcorr, int n, const flo
           const float den = std::sqrt(sqsum * sqsums_ptcl[i]);
           float* out = cc_all ? cc_all + (((size_t)i * nr + r) * ns + s) * R : nullptr;
           for (int t = 0; t < R; ++t) {
-            const float c = corrs[t] * scale / den;
+            float csum = 0.f;
+            for (int l = 0; l < 8; ++l) csum += corrs[t][l];
+            const float c = csum * scale / den;
             if (out) out[t] = c;
             if (c > bc) {
               bc = c;
