@@ -100,6 +100,8 @@ int pftcc_set_ref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, float
 int pftcc_set_ptcl_fcomp(pftcc_t* h, int32_t iptcl, int32_t irot, int32_t k, float re,
                          float im);                                                   /* :471 */
 int pftcc_cp_even2odd_ref(pftcc_t* h, int32_t iref);                                 /* :485 */
+/* cp_refs (:491-495): both reference sets of src (same nrefs, pftsz, kfromto, same GPU) into h. */
+int pftcc_cp_refs(pftcc_t* h, const pftcc_t* src);
 int pftcc_cp_even_ref2ptcl(pftcc_t* h, int32_t iref, int32_t iptcl);                 /* :497 */
 int pftcc_swap_ptclsevenodd(pftcc_t* h);                                             /* :504 */
 int pftcc_reallocate_ptcls(pftcc_t* h, int32_t nptcls, const int32_t* pinds);        /* :509 */
@@ -129,6 +131,11 @@ int pftcc_get_nrefs(const pftcc_t* h, int32_t* nrefs);                          
 int pftcc_get_nptcls(const pftcc_t* h, int32_t* nptcls);                             /* :701 */
 int pftcc_ptcl_iseven(const pftcc_t* h, int32_t iptcl, int32_t* iseven);            /* :695 */
 int pftcc_get_ref_pft(pftcc_t* h, int32_t iref, int32_t iseven, float* pft);        /* :670 */
+/* The particle range [pfrom, pto] of the index table (pfromto, :276; reallocate_ptcls :514-515). */
+int pftcc_get_pfromto(const pftcc_t* h, int32_t pfromto[2]);
+/* assign_pinds (:706-710): pinds[i] = the 1-based local index of global particle pfrom + i
+ * (i in [0, pto - pfrom]), 0 when the particle is not held. */
+int pftcc_assign_pinds(const pftcc_t* h, int32_t* pinds);
 int pftcc_get_ptcl_pft(pftcc_t* h, int32_t iptcl, float* pft);                      /* :660 */
 /* Device-resident memo and CTF matrices, for verification: ctfmat float[nk][pftsz];
  * memo float[nk][pftsz+1][2] = real FFT of length 2*pftsz of the particle's ring (the
@@ -151,6 +158,58 @@ int pftcc_gencorr_grad_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const 
                                  int32_t irot, double* f, double grad[2]);
 int pftcc_gencorr_grad_only_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl,
                                       const double shvec[2], int32_t irot, double grad[2]);
+/* The objective-specific forms that gencorr_for_rot(_8) / gencorr_grad(_only)_for_rot_8 dispatch to
+ * (:1632-1643): gencorr_cc_for_rot (:1645), gencorr_cc_for_rot_8 (:1690), gencorr_cc_grad_for_rot_8
+ * (:1863), gencorr_cc_grad_only_for_rot_8 (:1927), gencorr_euclid_for_rot (:2002),
+ * gencorr_euclid_for_rot_8 (:2026), gencorr_euclid_grad_for_rot_8 (:2071),
+ * gencorr_euclid_grad_only_for_rot_8 (:2111).  The objective is fixed instead of read from
+ * cc_objfun; the Euclid forms need assign_sigma2_noise. */
+int pftcc_gencorr_cc_for_rot(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2], int32_t irot,
+                             float* f);
+int pftcc_gencorr_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2], int32_t irot,
+                               double* f);
+int pftcc_gencorr_cc_grad_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                    int32_t irot, double* f, double grad[2]);
+int pftcc_gencorr_cc_grad_only_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                         int32_t irot, double grad[2]);
+int pftcc_gencorr_euclid_for_rot(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2], int32_t irot,
+                                 float* f);
+int pftcc_gencorr_euclid_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                   int32_t irot, double* f);
+int pftcc_gencorr_euclid_grad_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                        int32_t irot, double* f, double grad[2]);
+int pftcc_gencorr_euclid_grad_only_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                             int32_t irot, double grad[2]);
+
+/* Orientation derivatives for continuous refinement (the callers are simple_projector.f90:264 and
+ * simple_pftcc_orisrch_grad.f90:145-204).  set_dref_fcomp (:459-469): dcomp = 3 complex values
+ * (re, im interleaved), the derivatives of reference component (irot in [1, pftsz], k in
+ * [kfrom, kstop]) w.r.t. the three orientation angles.  As in the reference (pfts_drefs,
+ * :321-322) the derivative slots are 1..max(nthr, 1) of the configuration, addressed by iref.
+ * set_dref_pft: all three derivative blocks of slot iref at once, float[3][kstop-kfrom+1][pftsz][2]. */
+int pftcc_set_dref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, const float dcomp[6], int32_t iseven);
+int pftcc_set_dref_pft(pftcc_t* h, int32_t iref, const float* dpft, int32_t iseven);
+/* gencorr_cont_grad_cc_for_rot_8 (:1735-1786): cc and dcc[3], d cc / d(orientation angles), with
+ * q = reference x shell filter x |CTF| x shift phase, dq = dref x |CTF| x shift phase,
+ * cc = y / sqrt(S_q S_p), dcc = (T1 - y T2 / S_q) / sqrt(S_q S_p), S_p the memoized sqsum.  As in
+ * the reference, T1 = Re sum dq conj(P) is taken against particle slot iref (its
+ * calc_T1_T2_for_rot_8 call passes iref as the particle index, :1781), so iref <= nptcls. */
+int pftcc_gencorr_cont_grad_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                         int32_t irot, double* cc, double dcc[3]);
+/* gencorr_cont_shift_grad_cc_for_rot_8 (:1788-1847): f and grad[5] = 3 orientation derivatives
+ * (as above), then d/dshift_x and d/dshift_y (unweighted shells over the same denominator). */
+int pftcc_gencorr_cont_shift_grad_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl,
+                                               const double shvec[2], int32_t irot, double* f,
+                                               double grad[5]);
+/* gencorr_cont_grad_euclid_for_rot_8 / gencorr_cont_shift_grad_euclid_for_rot_8 (:2050-2069):
+ * unimplemented stubs in the reference (cc = 0, derivatives unset); here every output is 0. */
+int pftcc_gencorr_cont_grad_euclid_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                             int32_t irot, double* cc, double dcc[3]);
+int pftcc_gencorr_cont_shift_grad_euclid_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl,
+                                                   const double shvec[2], int32_t irot, double* f,
+                                                   double grad[5]);
+/* calc_ptcl_pspec (:897-904): pspec[k - kfrom] = sum_j |p_k[j]|^2 / pftsz for k in [kfrom, kto]. */
+int pftcc_calc_ptcl_pspec(pftcc_t* h, int32_t iptcl, float* pspec);
 /* gencorr_sigma_contrib (:2150): sigma_contrib[nk] (+ optional ptcl/ref sums of squares). */
 int pftcc_gencorr_sigma_contrib(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2],
                                 int32_t irot, float* sigma_contrib, float* ptcl_sumsq,
@@ -317,7 +376,8 @@ int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const i
  * particle's |CTF| is replaced by its ring mean per shell (the reference applies the particle's
  * angle-dependent |CTF| in the reference frame, :982, which no per-m contraction can express),
  * and ntop > 0 rescores the candidates exactly.
- * prec: PFTCC_PREC_SPLIT_BF16 (hi+lo split, FP32 accumulation), PFTCC_PREC_BF16 or PFTCC_PREC_FP8.
+ * prec: PFTCC_PREC_SPLIT_BF16 (hi+lo split, FP32 accumulation), PFTCC_PREC_BF16, PFTCC_PREC_FP16,
+ * PFTCC_PREC_SPLIT_FP16 or PFTCC_PREC_FP8.
  * Denominators sqrt(sum_k cbar_k^2 S_ref,k * S_ptcl) are formed in FP64.  No counterpart
  * entry point in the reference; it replaces the same greedy loop as pftcc_search
  * (simple_strategy3D_greedy.f90:60-77, gencorrs over all references + maxloc).
@@ -327,6 +387,11 @@ int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const i
  * neighbours, and the best rescored candidate is returned with its FP64 cc. */
 #define PFTCC_PREC_BF16 1
 #define PFTCC_PREC_SPLIT_BF16 3
+/* FP16 operands (one product) or split FP16 (hi+lo, three products), FP32 accumulation.  FP16 has
+ * 5 exponent bits, so per-shell power-of-two scales balance the two operands and a global power of
+ * two per operand maps its largest component to <= 2^15 (as for FP8); Z is scaled back exactly. */
+#define PFTCC_PREC_FP16 2
+#define PFTCC_PREC_SPLIT_FP16 4
 /* FP8 e4m3 coarse pass: per-shell power-of-two scales balance the two operands, a global power of
  * two per operand maps its largest component to <= 448; meant for ntop > 0 (exact rescoring). */
 #define PFTCC_PREC_FP8 8

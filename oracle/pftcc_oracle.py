@@ -770,6 +770,76 @@ class Pftcc:
             g[d] = float(np.sum(w * corrk_for_rot(qt[:ks], p, irot))) / den
         return f, g
 
+    def gencorr_obj_for_rot_8(self, iref: int, i: int, shvec, irot: int, euclid: bool) -> float:
+        """gencorr_cc_for_rot_8 (:1690-1733) or gencorr_euclid_for_rot_8 (:2026-2048), the
+        objective fixed instead of taken from cc_objfun (gencorr_for_rot_8's dispatch, :1632-1643)."""
+        saved = self.euclid
+        self.euclid = euclid
+        try:
+            return self.gencorr_for_rot_8(iref, i, shvec, irot)
+        finally:
+            self.euclid = saved
+
+    def gencorr_obj_grad_for_rot_8(self, iref: int, i: int, shvec, irot: int, euclid: bool):
+        """gencorr_cc_grad_for_rot_8 (:1863-1925) or gencorr_euclid_grad_for_rot_8 (:2071-2109)."""
+        saved = self.euclid
+        self.euclid = euclid
+        try:
+            return self.gencorr_grad_for_rot_8(iref, i, shvec, irot)
+        finally:
+            self.euclid = saved
+
+    # -- orientation derivatives (continuous refinement) ------------------------------------------
+    def cont_terms(self, iref: int, i: int, shvec, irot: int, drefs: np.ndarray, it1: int | None = None):
+        """The sums of gencorr_cont_grad_cc_for_rot_8 (:1735-1786) and
+        gencorr_cont_shift_grad_cc_for_rot_8 (:1788-1847), FP64.
+
+        q = shellnorm(ref) |CTF| shmat_8 (:1760-1775; the shell filter is applied to q only),
+        dq_d = dref_d |CTF| shmat_8 for the three orientation derivatives drefs [3, ks, n]
+        (pfts_drefs of the particle's eo class), over shells kfrom..kstop:
+        y = sum_k corrk(q, P_i), S_q = sum |q|^2, T1_d = sum_k corrk(dq_d, P_it1),
+        T2_d = Re sum dq_d conj(q) (calc_T1_T2_for_rot_8, :1246-1278), and the shift terms
+        g_x, g_y = sum_k corrk(i a_x q, P_i), likewise a_y (:1841-1846).  The reference passes iref as
+        the particle index of the T1 terms (:1781, :1836), so it1 defaults to iref.
+        """
+        it1 = iref if it1 is None else it1
+        ks = self.ks
+        q = self.prep_ref(iref, i, shvec)[:ks]
+        sh = shmat_8(self.grid, shvec)[:ks]
+        c = self.ctf(i)[:ks]
+        p = self.ptcls[i, :ks]
+        pt = self.ptcls[it1, :ks]
+        y = float(np.sum(corrk_for_rot(q, p, irot)))
+        sq = float(np.sum(np.abs(q) ** 2))
+        t1 = np.zeros(3)
+        t2 = np.zeros(3)
+        for d in range(3):
+            dq = drefs[d].astype(np.complex128) * c * sh
+            t1[d] = float(np.sum(corrk_for_rot(dq, pt, irot)))
+            t2[d] = float(np.real(np.sum(dq * np.conj(q))))
+        gs = np.zeros(2)
+        for d, ax in enumerate((self.grid.argtransf_x, self.grid.argtransf_y)):
+            gs[d] = float(np.sum(corrk_for_rot(q * 1j * ax[:ks], p, irot)))
+        return y, sq, t1, t2, gs
+
+    def cont_grad_cc_for_rot_8(self, iref: int, i: int, shvec, irot: int, drefs: np.ndarray, it1=None):
+        """gencorr_cont_grad_cc_for_rot_8 (:1735-1786): cc = y / sqrt(S_q S_p) and
+        dcc_d = (T1_d - y T2_d / S_q) / sqrt(S_q S_p), S_p the memoized particle norm."""
+        y, sq, t1, t2, _ = self.cont_terms(iref, i, shvec, irot, drefs, it1)
+        den = math.sqrt(sq * self.sqsum_ptcl(i))
+        return y / den, (t1 - y * t2 / sq) / den
+
+    def cont_shift_grad_cc_for_rot_8(self, iref: int, i: int, shvec, irot: int, drefs: np.ndarray, it1=None):
+        """gencorr_cont_shift_grad_cc_for_rot_8 (:1788-1847): f and grad[5] (three orientation
+        derivatives, then d/dshift_x, d/dshift_y)."""
+        y, sq, t1, t2, gs = self.cont_terms(iref, i, shvec, irot, drefs, it1)
+        den = math.sqrt(sq * self.sqsum_ptcl(i))
+        return y / den, np.concatenate([(t1 - y * t2 / sq) / den, gs / den])
+
+    def calc_ptcl_pspec(self, i: int) -> np.ndarray:
+        """calc_ptcl_pspec (:897-904): sum_j |p_k[j]|^2 / pftsz over kfrom..kto."""
+        return np.sum(np.abs(self.ptcls[i].astype(np.complex128)) ** 2, axis=-1) / self.pftsz
+
     # -- a15: in-plane angle derivative (continuous rotation psi, radians) ----------------------
     def corr_at_angle(self, iref: int, i: int, shvec, psi: float) -> float:
         """Band-limited rotational interpolant of the gencorr_for_rot_8 objective.

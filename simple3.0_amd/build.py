@@ -57,7 +57,7 @@ def build_fortran():
     run([FLANG, "-O2", "-fPIC", "-c", src_mod, "-module-dir", mod_dir, "-o", obj])
     if os.path.exists(src_drv):
         exe = os.path.join(LIB, "pftcc_fortran_driver")
-        run([FLANG, "-O2", src_drv, obj, "-I" + mod_dir, "-module-dir", mod_dir,
+        run([FLANG, "-O2", "-fopenmp", src_drv, obj, "-I" + mod_dir, "-module-dir", mod_dir,
              "-L" + LIB, "-lpftcc_hip", "-Wl,-rpath,$ORIGIN", "-o", exe])
         return exe
     return obj

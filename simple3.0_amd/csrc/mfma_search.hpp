@@ -25,7 +25,19 @@ __host__ __device__ __forceinline__ size_t zidx(int r, int p, int m, int n, int 
 }
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+// One 16x16x32 MFMA step on packed 16-bit operands (8 per lane): BF16, or FP16 when H.  Both
+// formats keep the sign in bit 15 of each element, so rot_neg below serves both.
+template <bool H>
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+  if constexpr (H)
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                  0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
 // Ring mean of |CTF| per (particle, shell) and its square in FP64.
 __global__ void k_ring_mean_ctf(const float* __restrict__ ctf, int np, int nk, int n, float* __restrict__ cbar,
@@ -63,6 +75,23 @@ __device__ __forceinline__ unsigned short bf16_bits(float x) {
 __device__ __forceinline__ float bf16_float(unsigned short b) {
   return __uint_as_float((unsigned int)b << 16);
 }
+__device__ __forceinline__ unsigned short f16_bits(float x) {
+  return __builtin_bit_cast(unsigned short, (_Float16)x);   // round to nearest even
+}
+__device__ __forceinline__ float f16_float(unsigned short b) {
+  return (float)__builtin_bit_cast(_Float16, b);
+}
+
+// Power of two g with max * g <= lim (1 when the operand is all zero); max is the bit pattern of
+// a non-negative float.
+__device__ __forceinline__ float pow2_gain(const unsigned int* gmax, float lim) {
+  const float m = __uint_as_float(*gmax);
+  if (!(m > 0.f)) return 1.0f;
+  int e;
+  (void)frexpf(lim / m, &e);   // lim / m = f 2^e, f in [0.5, 1)
+  return ldexpf(1.0f, e - 1);
+}
+constexpr float kF16Lim = 32768.0f;   // FP16 operands: largest component <= 2^15 (max normal 65504)
 
 // Split-BF16 operand layout, in 16-byte vectors (8 k' each): [m][row / 16][k' / 32][row % 16][4],
 // so that the 64 lanes of one MFMA operand load (16 rows x 32 k') read one contiguous kilobyte.
@@ -71,10 +100,14 @@ __host__ __device__ __forceinline__ size_t split_vec(int m, int row, int vk, int
 }
 
 // Packs spectra spec[row][kk][m] (float2, m in [0, n]) times an optional per-(row, kk) scale into
-// split-BF16 operands hi/lo (k' = 2 kk + {0: re, 1: im}, zero past 2 kcorr; layout split_vec).
+// split 16-bit operands hi/lo (k' = 2 kk + {0: re, 1: im}, zero past 2 kcorr; layout split_vec):
+// BF16, or FP16 when H, where the values are also multiplied by the shell scale shellscale[kk]
+// and the operand's power-of-two gain (largest component <= 2^15, gmax from k_absmax_scaled).
 // One thread per (m, row, complex k); rows beyond count and k' beyond 2 kcorr are written as zero.
-__global__ void k_pack_split(const float2* __restrict__ spec, const float* __restrict__ scale, int count, int rows_pad,
-                             int nk, int kcorr, int n, int KP, unsigned int* __restrict__ hi,
+template <bool H>
+__global__ void k_pack_split(const float2* __restrict__ spec, const float* __restrict__ scale,
+                             const float* __restrict__ shellscale, const unsigned int* __restrict__ gmax, int count,
+                             int rows_pad, int nk, int kcorr, int n, int KP, unsigned int* __restrict__ hi,
                              unsigned int* __restrict__ lo) {
   const size_t KW = KP / 2;                                  // 32-bit words (one complex) per row
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,12 +119,23 @@ __global__ void k_pack_split(const float2* __restrict__ spec, const float* __res
   float re = 0.f, im = 0.f;
   if (row < count && kk < kcorr) {
     const float2 v = spec[((size_t)row * nk + kk) * (n + 1) + m];
-    const float s = scale ? scale[(size_t)row * nk + kk] : 1.0f;
+    float s = scale ? scale[(size_t)row * nk + kk] : 1.0f;
+    if (H) s *= shellscale[kk] * pow2_gain(gmax, kF16Lim);
     re = v.x * s;
     im = v.y * s;
   }
-  const unsigned short hr = bf16_bits(re), hiim = bf16_bits(im);
-  const unsigned short lr = bf16_bits(re - bf16_float(hr)), li = bf16_bits(im - bf16_float(hiim));
+  unsigned short hr, hiim, lr, li;
+  if (H) {
+    hr = f16_bits(re);
+    hiim = f16_bits(im);
+    lr = f16_bits(re - f16_float(hr));
+    li = f16_bits(im - f16_float(hiim));
+  } else {
+    hr = bf16_bits(re);
+    hiim = bf16_bits(im);
+    lr = bf16_bits(re - bf16_float(hr));
+    li = bf16_bits(im - bf16_float(hiim));
+  }
   const size_t w = split_vec(m, row, kk / 4, rows_pad, KP) * 4 + (kk % 4);
   hi[w] = (unsigned int)hr | ((unsigned int)hiim << 16);
   lo[w] = (unsigned int)lr | ((unsigned int)li << 16);
@@ -110,7 +154,8 @@ __device__ __forceinline__ bf16x8 rot_neg(bf16x8 a) {
 }
 
 // Z[r][p][m] = sum_k' A[m][r][k'] (x) B[m][p][k'] for one 64 x 64 (ref x particle) tile per wave.
-// split = 3: hi.hi + hi.lo + lo.hi; split = 1: hi.hi only (plain BF16, for error studies).
+// split = 3: hi.hi + hi.lo + lo.hi; split = 1: hi.hi only.  Operands are BF16, or FP16 when H (then
+// Z is multiplied back by 1 / (gA gB), the operands' power-of-two gains).
 // Tile coordinates of a 1-D launch of gx * gy * (n+1) work-groups: the XCD-aware order puts the
 // particle tiles that share one reference tile (and m) on one XCD, next to each other in time.
 struct ZTile {
@@ -135,10 +180,12 @@ constexpr int ZGEMM_TR = PFTCC_ZGEMM_TR;   // 16-row reference blocks per wavefr
 constexpr int ZGEMM_TP = PFTCC_ZGEMM_TP;   // 16-column particle blocks per wavefront tile
 
 // One (16 TR) x (16 TP) (reference x particle) tile per wavefront, 32 k' per MFMA step.
-template <int SPLIT, int TR, int TP>
+template <int SPLIT, int TR, int TP, bool H>
 __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
                                               const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
-                                              int rows_pad, int cols_pad, int KP, int n, float2* __restrict__ Z) {
+                                              int rows_pad, int cols_pad, int KP, int n, float2* __restrict__ Z,
+                                              const unsigned int* __restrict__ gmaxA,
+                                              const unsigned int* __restrict__ gmaxB) {
   const int l = threadIdx.x;
   const ZTile zt = ztile(cols_pad / (16 * TP), rows_pad / (16 * TR), 16 * TP, 16 * TR);
   const int p0 = zt.p0, r0 = zt.r0, m = zt.m;
@@ -174,16 +221,17 @@ __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, co
 #pragma unroll
       for (int j = 0; j < TP; ++j) {
         if (SPLIT == 3) {
-          zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], zr[i][j], 0, 0, 0);
-          zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], zr[i][j], 0, 0, 0);
-          zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l, bh[j], zi[i][j], 0, 0, 0);
-          zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bl[j], zi[i][j], 0, 0, 0);
+          zr[i][j] = mfma16<H>(al[i], bh[j], zr[i][j]);
+          zr[i][j] = mfma16<H>(ah[i], bl[j], zr[i][j]);
+          zi[i][j] = mfma16<H>(a2l, bh[j], zi[i][j]);
+          zi[i][j] = mfma16<H>(a2h, bl[j], zi[i][j]);
         }
-        zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], zr[i][j], 0, 0, 0);
-        zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bh[j], zi[i][j], 0, 0, 0);
+        zr[i][j] = mfma16<H>(ah[i], bh[j], zr[i][j]);
+        zi[i][j] = mfma16<H>(a2h, bh[j], zi[i][j]);
       }
     }
   }
+  const float inv = H ? 1.0f / (pow2_gain(gmaxA, kF16Lim) * pow2_gain(gmaxB, kF16Lim)) : 1.0f;
   // D layout: lane l, element e -> row 4 (l/16) + e, column l % 16
 #pragma unroll
   for (int i = 0; i < TR; ++i)
@@ -193,7 +241,7 @@ __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, co
       for (int e = 0; e < 4; ++e) {
         const int r = r0 + i * 16 + 4 * lk + e;
         const int p = p0 + j * 16 + lr;
-        Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e], zi[i][j][e]);
+        Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e] * inv, zi[i][j][e] * inv);
       }
 }
 
@@ -206,10 +254,12 @@ __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, co
 // large operand is read once per launch and only the particle operand (prefetched one step ahead)
 // is loaded in the inner loop.  The XCD-aware order keeps the slabs of one m on one XCD (the
 // particle operand of that m stays in its L2).
-template <int SPLIT, int TR, int NSTEP>
+template <int SPLIT, int TR, int NSTEP, bool H>
 __global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
                                                  const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
-                                                 int rows_pad, int cols_pad, int n, float2* __restrict__ Z) {
+                                                 int rows_pad, int cols_pad, int n, float2* __restrict__ Z,
+                                                 const unsigned int* __restrict__ gmaxA,
+                                                 const unsigned int* __restrict__ gmaxB) {
   constexpr int TP = PFTCC_ZGEMM_AS_TP;        // 16-column particle blocks per tile
   constexpr int KP = NSTEP * 32;
   const int l = threadIdx.x;
@@ -236,6 +286,7 @@ __global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi,
     }
   };
   const int ntile = cols_pad / (16 * TP);
+  const float inv = H ? 1.0f / (pow2_gain(gmaxA, kF16Lim) * pow2_gain(gmaxB, kF16Lim)) : 1.0f;
   load_b(0, 0, bh, bl);
   for (int tile = 0; tile < ntile; ++tile) {
     const int p0 = tile * 16 * TP;
@@ -260,13 +311,13 @@ __global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi,
 #pragma unroll
         for (int j = 0; j < TP; ++j) {
           if (SPLIT == 3) {
-            zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[st][i], bh[j], zr[i][j], 0, 0, 0);
-            zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[st][i], bl[j], zr[i][j], 0, 0, 0);
-            zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l, bh[j], zi[i][j], 0, 0, 0);
-            zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bl[j], zi[i][j], 0, 0, 0);
+            zr[i][j] = mfma16<H>(al[st][i], bh[j], zr[i][j]);
+            zr[i][j] = mfma16<H>(ah[st][i], bl[j], zr[i][j]);
+            zi[i][j] = mfma16<H>(a2l, bh[j], zi[i][j]);
+            zi[i][j] = mfma16<H>(a2h, bl[j], zi[i][j]);
           }
-          zr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[st][i], bh[j], zr[i][j], 0, 0, 0);
-          zi[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h, bh[j], zi[i][j], 0, 0, 0);
+          zr[i][j] = mfma16<H>(ah[st][i], bh[j], zr[i][j]);
+          zi[i][j] = mfma16<H>(a2h, bh[j], zi[i][j]);
         }
       }
       if (more) {
@@ -285,7 +336,7 @@ __global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi,
         for (int e = 0; e < 4; ++e) {
           const int r = r0 + i * 16 + 4 * lk + e;
           const int p = p0 + j * 16 + lr;
-          Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e], zi[i][j][e]);
+          Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e] * inv, zi[i][j][e] * inv);
         }
   }
 }
@@ -634,13 +685,7 @@ __global__ void k_absmax_scaled(const float2* __restrict__ spec, const float* __
 }
 
 // Power of two g with max * g <= 448.
-__device__ __forceinline__ float fp8_gain(const unsigned int* gmax) {
-  const float m = __uint_as_float(*gmax);
-  if (!(m > 0.f)) return 1.0f;
-  int e;
-  (void)frexpf(448.0f / m, &e);   // 448 / m = f 2^e, f in [0.5, 1)
-  return ldexpf(1.0f, e - 1);
-}
+__device__ __forceinline__ float fp8_gain(const unsigned int* gmax) { return pow2_gain(gmax, 448.0f); }
 
 // e4m3 operand tiles out[m][row][k'] (bytes; k' = 2 kk + {re, im}, zero past 2 kcorr), one 32-bit
 // word (two complex values) per thread.

@@ -120,6 +120,10 @@ struct pftcc {
   float *d_refinv_f = nullptr, *d_ofac_f = nullptr, *d_shellw = nullptr;
   double *d_refinv_d = nullptr, *d_ofac_d = nullptr, *d_sigma2w = nullptr, *d_refpw = nullptr;
   int* d_optrow = nullptr;
+  // orientation derivatives of the references (set_dref_fcomp, :459-469): [ndslots][3][kcorr][n]
+  // per eo class, slots 1..ndslots = max(cfg.nthr, 1) as the reference allocates them (:321-322)
+  float2 *d_drefs_e = nullptr, *d_drefs_o = nullptr;
+  int ndslots = 0;
   // polarizer tables (init_polarizer): weights [9][nk n] and logical addresses [6][nk n]
   float* d_pz_w = nullptr;
   int* d_pz_hk = nullptr;
@@ -135,11 +139,11 @@ struct pftcc {
     int rows_pad = 0, KP = 0, kcorr = 0;
     std::vector<float> shifts;        // [ns][2]: rows are (reference, shift), reference-major
     int knorm = 0;                    // shells in sr (kcorr unshifted, nk shifted)
-    bool fp8 = false;                 // e4m3 operand (a8) instead of split BF16 (hi, lo)
+    int fmt = 0;                      // OP_BF16, OP_F16 (split operands hi, lo) or OP_FP8 (a8)
     unsigned int *hi = nullptr, *lo = nullptr;
     double* sr = nullptr;             // [nr][knorm]
-    // FP8: per-shell mean squares of the reference spectra, shell scales a_k (device copies of
-    // a_k and 1/a_k), the operand's largest component and the operand itself
+    // FP16 and FP8: per-shell mean squares of the reference spectra, shell scales a_k (device
+    // copies of a_k and 1/a_k), the operand's largest component and (FP8) the operand itself
     std::vector<double> msV;
     std::vector<float> ak;
     float *d_ak = nullptr, *d_bk = nullptr;
@@ -708,16 +712,24 @@ int ensure_sqp8(pftcc* h) {
   return PFTCC_OK;
 }
 
+// Objective form of the FP64 single-rotation routines: 0 CC k-weighted (match_filt = no), 1 CC
+// match_filt, 2 Euclid.  objfun < 0 follows the handle's cc_objfun (gencorr_for_rot_8's dispatch,
+// :1632-1643); PFTCC_OBJFUN_CC / _EUCLID select gencorr_cc_* / gencorr_euclid_* directly.
+int rot8_mode(const pftcc* h, int objfun) {
+  const bool euclid = objfun < 0 ? h->euclid : objfun == PFTCC_OBJFUN_EUCLID;
+  return euclid ? 2 : (h->match_filt ? 1 : 0);
+}
+
 // Launches k_rot8 on device lists.
 int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, const double* d_shifts,
-                const int* d_rhos, double* d_f, double* d_grad, bool angle) {
+                const int* d_rhos, double* d_f, double* d_grad, bool angle, int objfun = -1) {
   Rot8Args a{};
   a.n = h->n;
   a.nk = h->nk;
   a.kcorr = h->kcorr;
   a.kfrom = h->kfrom;
   a.nrefs = h->nrefs;
-  a.mode = h->euclid ? 2 : (h->match_filt ? 1 : 0);
+  a.mode = rot8_mode(h, objfun);
   a.refs_even = h->d_refs_e;
   a.refs_odd = h->d_refs_o;
   a.ptcls = h->d_ptcls;
@@ -725,7 +737,7 @@ int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, cons
   a.ctf = h->with_ctf ? h->d_ctf : nullptr;
   a.sqsums = h->d_sqsums;
   a.iseven = h->d_iseven;
-  const bool norm = (h->match_filt || h->pssnr) && !h->euclid;
+  const bool norm = (h->match_filt || h->pssnr) && a.mode != 2;
   a.refinv = norm ? h->d_refinv_d : nullptr;
   a.ofac = norm ? h->d_ofac_d : nullptr;
   a.optrow = (norm && h->clsfrcs) ? h->d_optrow : nullptr;
@@ -771,16 +783,24 @@ int round_up(int x, int m) { return (x + m - 1) / m * m; }
 #ifndef PFTCC_ZGEMM_AS_TR
 #define PFTCC_ZGEMM_AS_TR 2
 #endif
+// Operand formats of the precision form.
+enum { OP_BF16 = 0, OP_F16 = 1, OP_FP8 = 2 };
+int op_format(int prec) {
+  return prec == PFTCC_PREC_FP8 ? OP_FP8 : (prec == PFTCC_PREC_FP16 || prec == PFTCC_PREC_SPLIT_FP16) ? OP_F16 : OP_BF16;
+}
+
 // A-stationary contraction for KP = 32 NSTEP; false if NSTEP has no instantiation.
-template <int SPLIT>
+template <int SPLIT, bool H>
 bool zgemm_as_go(int KP, int rows_pad, int cols_pad, int n, hipStream_t s, const unsigned int* ahi,
-                 const unsigned int* alo, const unsigned int* bhi, const unsigned int* blo, float2* Z) {
+                 const unsigned int* alo, const unsigned int* bhi, const unsigned int* blo, float2* Z,
+                 const unsigned int* gA, const unsigned int* gB) {
   constexpr int TR = PFTCC_ZGEMM_AS_TR;
   const dim3 grid((unsigned)((size_t)(rows_pad / (16 * TR)) * (n + 1)));
 #define X(NS)                                                                                         \
   if (KP == 32 * NS) {                                                                                \
-    hipLaunchKernelGGL((k_zgemm_as<SPLIT, TR, NS>), grid, dim3(64), 0, s, (const bf16x8*)ahi,          \
-                       (const bf16x8*)alo, (const bf16x8*)bhi, (const bf16x8*)blo, rows_pad, cols_pad, n, Z); \
+    hipLaunchKernelGGL((k_zgemm_as<SPLIT, TR, NS, H>), grid, dim3(64), 0, s, (const bf16x8*)ahi,       \
+                       (const bf16x8*)alo, (const bf16x8*)bhi, (const bf16x8*)blo, rows_pad, cols_pad, n, Z, \
+                       gA, gB);                                                                       \
     return true;                                                                                      \
   }
   X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
@@ -846,12 +866,13 @@ int shell_ms(pftcc* h, const float2* spec, const int* d_list, int count, std::ve
 }
 
 // Reference operand of eo class eo over the (0-based) reference list rl; cached.  Split BF16, or
-// FP8 e4m3 with shell scales balanced against the particles' mean squares msW.
-int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector<float>& shv, bool fp8,
+// split FP16 / FP8 e4m3 with shell scales balanced against the particles' mean squares msW.
+int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector<float>& shv, int fmt,
                  const std::vector<double>* msW) {
   pftcc::APack& ap = h->apack[eo];
-  if (ap.valid && ap.refs == rl && ap.shifts == shv && ap.kcorr == h->kcorr && ap.fp8 == fp8 &&
-      (!fp8 || fp8_shell_scales(ap.msV, *msW) == ap.ak))
+  const bool fp8 = fmt == OP_FP8, scaled = fmt != OP_BF16;
+  if (ap.valid && ap.refs == rl && ap.shifts == shv && ap.kcorr == h->kcorr && ap.fmt == fmt &&
+      (!scaled || fp8_shell_scales(ap.msV, *msW) == ap.ak))
     return PFTCC_OK;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   free_apack(ap);
@@ -866,10 +887,11 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector
   const size_t words = fp8 ? (size_t)(n + 1) * ap.rows_pad * (ap.KP8 / 4) : (size_t)(n + 1) * ap.rows_pad * (ap.KP / 2);
   int rc;
   if ((rc = dalloc(h, &ap.sr, (size_t)nref * knorm))) return rc;
+  if (scaled && ((rc = dalloc(h, &ap.d_ak, (size_t)h->kcorr)) || (rc = dalloc(h, &ap.d_bk, (size_t)h->kcorr)) ||
+                 (rc = dalloc(h, &ap.gmax, (size_t)1))))
+    return rc;
   if (fp8) {
-    if ((rc = dalloc(h, &ap.a8, words)) || (rc = dalloc(h, &ap.d_ak, (size_t)h->kcorr)) ||
-        (rc = dalloc(h, &ap.d_bk, (size_t)h->kcorr)) || (rc = dalloc(h, &ap.gmax, (size_t)1)))
-      return rc;
+    if ((rc = dalloc(h, &ap.a8, words))) return rc;
   } else if ((rc = dalloc(h, &ap.hi, words)) || (rc = dalloc(h, &ap.lo, words))) {
     return rc;
   }
@@ -901,7 +923,7 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector
   hipLaunchKernelGGL(k_memoize, dim3((nr * nk + F - 1) / F), dim3(256), sizeof(float2) * ((size_t)2 * F * n + 2 * n),
                      h->stream, (const float2*)rowrings, 0, nr, nk, n, h->plan, (const float2*)h->d_tw2, spec, F,
                      (const int*)nullptr);
-  if (fp8) {
+  if (scaled) {
     if ((rc = shell_ms(h, spec, nullptr, nr, ap.msV))) return rc;
     ap.ak = fp8_shell_scales(ap.msV, *msW);
     std::vector<float> bk(ap.ak.size());
@@ -912,13 +934,19 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector
     const size_t tot = (size_t)nr * h->kcorr * (n + 1);
     hipLaunchKernelGGL(k_absmax_scaled, dim3((unsigned)std::min<size_t>(1024, (tot + 255) / 256)), dim3(256), 0, h->stream,
                        (const float2*)spec, (const float*)nullptr, (const float*)ap.d_ak, nr, nk, n, h->kcorr, ap.gmax);
+  }
+  if (fp8) {
     hipLaunchKernelGGL(k_pack_fp8, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream, (const float2*)spec,
                        (const float*)nullptr, (const float*)ap.d_ak, (const unsigned int*)ap.gmax, nr, ap.rows_pad, nk,
                        h->kcorr, n, ap.KP8, ap.a8);
+  } else if (scaled) {
+    hipLaunchKernelGGL(k_pack_split<true>, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
+                       (const float2*)spec, (const float*)nullptr, (const float*)ap.d_ak, (const unsigned int*)ap.gmax,
+                       nr, ap.rows_pad, nk, h->kcorr, n, ap.KP, ap.hi, ap.lo);
   } else {
-    hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
-                       (const float2*)spec, (const float*)nullptr, nr, ap.rows_pad, nk, h->kcorr, n, ap.KP, ap.hi,
-                       ap.lo);
+    hipLaunchKernelGGL(k_pack_split<false>, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
+                       (const float2*)spec, (const float*)nullptr, (const float*)nullptr, (const unsigned int*)nullptr,
+                       nr, ap.rows_pad, nk, h->kcorr, n, ap.KP, ap.hi, ap.lo);
   }
   hipLaunchKernelGGL(k_ring_sq64, dim3((nref * knorm + 3) / 4), dim3(64, 4), 0, h->stream, (const float2*)rings, nref,
                      nk, n, knorm, ap.sr);
@@ -934,7 +962,7 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector
   ap.shifts = shv;
   ap.knorm = knorm;
   ap.kcorr = h->kcorr;
-  ap.fp8 = fp8;
+  ap.fmt = fmt;
   ap.valid = true;
   return PFTCC_OK;
 }
@@ -945,8 +973,9 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
                const std::vector<int>& rl, const std::vector<float>& shv, int prec, unsigned long long* d_best,
                float* d_cc, unsigned long long* d_prbest, const std::vector<double>* msW) {
   int rc;
-  const bool fp8 = prec == PFTCC_PREC_FP8;
-  if ((rc = ensure_apack(h, eo, rl, shv, fp8, msW))) return rc;
+  const int fmt = op_format(prec);
+  const bool fp8 = fmt == OP_FP8, f16 = fmt == OP_F16;
+  if ((rc = ensure_apack(h, eo, rl, shv, fmt, msW))) return rc;
   const pftcc::APack& ap = h->apack[eo];
   const int nse = std::max(1, (int)shv.size() / 2);
   const int n = h->n, nk = h->nk, KP = fp8 ? ap.KP8 : ap.KP, nr = (int)rl.size() * nse;
@@ -992,18 +1021,25 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     if (ctf)
       hipLaunchKernelGGL(k_ring_mean_ctf_list, dim3((cnt * nk + 3) / 4), dim3(64, 4), 0, h->stream,
                          (const float*)h->d_ctf, (const int*)(d_pl + c0), cnt, nk, n, cbar, cbar2);
-    if (fp8) {
+    if (fp8 || f16) {
       hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, h->stream, gmaxB, 1, 0u);
       const size_t tot = (size_t)cnt * h->kcorr * (n + 1);
       hipLaunchKernelGGL(k_absmax_scaled, dim3((unsigned)std::min<size_t>(1024, (tot + 255) / 256)), dim3(256), 0, h->stream,
                          (const float2*)wg, ctf ? (const float*)cbar : nullptr, (const float*)ap.d_bk, cnt, nk, n,
                          h->kcorr, gmaxB);
+    }
+    if (fp8) {
       hipLaunchKernelGGL(k_pack_fp8, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
                          (const float2*)wg, ctf ? (const float*)cbar : nullptr, (const float*)ap.d_bk,
                          (const unsigned int*)gmaxB, cnt, Pc, nk, h->kcorr, n, KP, bhi);
+    } else if (f16) {
+      hipLaunchKernelGGL(k_pack_split<true>, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
+                         (const float2*)wg, ctf ? (const float*)cbar : nullptr, (const float*)ap.d_bk,
+                         (const unsigned int*)gmaxB, cnt, Pc, nk, h->kcorr, n, KP, bhi, blo);
     } else {
-      hipLaunchKernelGGL(k_pack_split, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
-                         (const float2*)wg, ctf ? (const float*)cbar : nullptr, cnt, Pc, nk, h->kcorr, n, KP, bhi, blo);
+      hipLaunchKernelGGL(k_pack_split<false>, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, h->stream,
+                         (const float2*)wg, ctf ? (const float*)cbar : nullptr, (const float*)nullptr,
+                         (const unsigned int*)nullptr, cnt, Pc, nk, h->kcorr, n, KP, bhi, blo);
     }
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 1, &ev))) return rc;
@@ -1013,15 +1049,24 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
       hipLaunchKernelGGL(k_zgemm8, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad, Pc,
                          KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB);
     else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_BF16 &&
-             zgemm_as_go<1>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z)) {
+             zgemm_as_go<1, false>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z, nullptr, nullptr)) {
     } else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_SPLIT_BF16 &&
-               zgemm_as_go<3>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z)) {
-    } else if (prec == PFTCC_PREC_BF16)
-      hipLaunchKernelGGL((k_zgemm<1, ZGEMM_TR, ZGEMM_TP>), gz, dim3(64), 0, h->stream, (const bf16x8*)ap.hi,
-                         (const bf16x8*)ap.lo, (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
-    else
-      hipLaunchKernelGGL((k_zgemm<3, ZGEMM_TR, ZGEMM_TP>), gz, dim3(64), 0, h->stream, (const bf16x8*)ap.hi,
-                         (const bf16x8*)ap.lo, (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z);
+               zgemm_as_go<3, false>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z, nullptr, nullptr)) {
+    } else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_FP16 &&
+               zgemm_as_go<1, true>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z, ap.gmax, gmaxB)) {
+    } else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_SPLIT_FP16 &&
+               zgemm_as_go<3, true>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z, ap.gmax, gmaxB)) {
+    } else {
+#define ZG_LAUNCH(S, H)                                                                                  \
+  hipLaunchKernelGGL((k_zgemm<S, ZGEMM_TR, ZGEMM_TP, H>), gz, dim3(64), 0, h->stream, (const bf16x8*)ap.hi,  \
+                     (const bf16x8*)ap.lo, (const bf16x8*)bhi, (const bf16x8*)blo, ap.rows_pad, Pc, KP, n, Z,   \
+                     (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB)
+      if (prec == PFTCC_PREC_BF16) ZG_LAUNCH(1, false);
+      else if (prec == PFTCC_PREC_SPLIT_BF16) ZG_LAUNCH(3, false);
+      else if (prec == PFTCC_PREC_FP16) ZG_LAUNCH(1, true);
+      else ZG_LAUNCH(3, true);
+#undef ZG_LAUNCH
+    }
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 2, &ev))) return rc;
     ZifftArgs za{};
@@ -1071,7 +1116,8 @@ int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* ire
              int prec, unsigned long long* d_best, float* d_cc, unsigned long long* d_prbest = nullptr,
              std::vector<int>* plocal = nullptr) {
   int rc;
-  if (prec != PFTCC_PREC_BF16 && prec != PFTCC_PREC_SPLIT_BF16 && prec != PFTCC_PREC_FP8)
+  if (prec != PFTCC_PREC_BF16 && prec != PFTCC_PREC_SPLIT_BF16 && prec != PFTCC_PREC_FP16 &&
+      prec != PFTCC_PREC_SPLIT_FP16 && prec != PFTCC_PREC_FP8)
     return fail(h, PFTCC_E_ARG, "unknown precision %d", prec);
   if (h->euclid || h->match_filt || h->pssnr)
     return fail(h, PFTCC_E_UNSUPPORTED, "the MFMA form implements objfun cc without match filter");
@@ -1095,9 +1141,9 @@ int mfma_run(pftcc* h, int np, const int32_t* iptcls, int nr, const int32_t* ire
     pos[eo].push_back(i);
     if (plocal) plocal->push_back(ip);
   }
-  // FP8: per-shell mean squares of the particles' spectra (for the shell scales)
+  // FP16 and FP8: per-shell mean squares of the particles' spectra (for the shell scales)
   std::vector<double> msW;
-  if (prec == PFTCC_PREC_FP8) {
+  if (op_format(prec) != OP_BF16) {
     std::vector<int> all;
     for (int eo = 0; eo < 2; ++eo) all.insert(all.end(), pl[eo].begin(), pl[eo].end());
     int* d_all = nullptr;
@@ -1119,10 +1165,12 @@ std::vector<float> shift_vector(int32_t ns, const float* shifts) {
 
 // One (iref, iptcl, shvec, irot) evaluation through k_rot8; returns f and grad[3].
 int rot8_single(pftcc* h, int iref, int iptcl, const double shvec[2], int irot, double* f, double* grad,
-                bool angle) {
+                bool angle, int objfun = -1) {
   int rc, ip;
   if ((rc = check_ref(h, iref)) || (rc = local_index(h, iptcl, &ip)) || (rc = check_rot(h, irot))) return rc;
   if ((rc = need_direct(h))) return rc;
+  if (rot8_mode(h, objfun) == 2 && !h->have_sigma2)
+    return fail(h, PFTCC_E_STATE, "the Euclid objective needs assign_sigma2_noise");
   if (angle && (rc = ensure_dptcls(h))) return rc;
   struct {
     int iref, ip, rho, pad;
@@ -1134,7 +1182,7 @@ int rot8_single(pftcc* h, int iref, int iptcl, const double shvec[2], int irot, 
   HIPCHK(h, hipMemcpyAsync(d_in, &in, sizeof in, hipMemcpyHostToDevice, h->stream));
   const int* di = reinterpret_cast<const int*>(d_in);
   if ((rc = launch_rot8(h, 1, di, di + 1, reinterpret_cast<const double*>(d_in + 16), di + 2, d_out, d_out + 1,
-                        angle)))
+                        angle, objfun)))
     return rc;
   double out[4];
   HIPCHK(h, hipMemcpyAsync(out, d_out, sizeof out, hipMemcpyDeviceToHost, h->stream));
@@ -1326,6 +1374,8 @@ int pftcc_kill(pftcc_t* h) {
   dfree(h->d_optlp);
   dfree(h->d_pz_w);
   dfree(h->d_pz_hk);
+  dfree(h->d_drefs_e);
+  dfree(h->d_drefs_o);
   for (auto& ap : h->apack) free_apack(ap);
   if (h->d_scratch) (void)hipFree(h->d_scratch);
   delete h;
@@ -2232,6 +2282,295 @@ int pftcc_gencorr_grad_only_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, c
   grad[0] = gg[0];
   grad[1] = gg[1];
   return rc;
+}
+
+// The objective-specific forms: the same kernel with the objective fixed instead of read from
+// cc_objfun.
+int pftcc_gencorr_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2], int32_t irot,
+                               double* f) {
+  if (!h || !f) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  return rot8_single(h, iref, iptcl, shvec, irot, f, nullptr, false, PFTCC_OBJFUN_CC);
+}
+
+int pftcc_gencorr_euclid_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2], int32_t irot,
+                                   double* f) {
+  if (!h || !f) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  return rot8_single(h, iref, iptcl, shvec, irot, f, nullptr, false, PFTCC_OBJFUN_EUCLID);
+}
+
+int pftcc_gencorr_cc_for_rot(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2], int32_t irot, float* f) {
+  if (!h || !f) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  const double sh[2] = {shvec ? (double)shvec[0] : 0.0, shvec ? (double)shvec[1] : 0.0};
+  double v = 0.0;
+  const int rc = rot8_single(h, iref, iptcl, sh, irot, &v, nullptr, false, PFTCC_OBJFUN_CC);
+  *f = (float)v;
+  return rc;
+}
+
+int pftcc_gencorr_euclid_for_rot(pftcc_t* h, int32_t iref, int32_t iptcl, const float shvec[2], int32_t irot,
+                                 float* f) {
+  if (!h || !f) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  const double sh[2] = {shvec ? (double)shvec[0] : 0.0, shvec ? (double)shvec[1] : 0.0};
+  double v = 0.0;
+  const int rc = rot8_single(h, iref, iptcl, sh, irot, &v, nullptr, false, PFTCC_OBJFUN_EUCLID);
+  *f = (float)v;
+  return rc;
+}
+
+static int grad_for_rot_obj(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2], int32_t irot, double* f,
+                            double grad[2], int objfun) {
+  if (!h || !grad) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  double gg[3], fv;
+  const int rc = rot8_single(h, iref, iptcl, shvec, irot, &fv, gg, false, objfun);
+  if (f) *f = fv;
+  grad[0] = gg[0];
+  grad[1] = gg[1];
+  return rc;
+}
+
+int pftcc_gencorr_cc_grad_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2], int32_t irot,
+                                    double* f, double grad[2]) {
+  if (!f) return PFTCC_E_ARG;
+  return grad_for_rot_obj(h, iref, iptcl, shvec, irot, f, grad, PFTCC_OBJFUN_CC);
+}
+
+int pftcc_gencorr_cc_grad_only_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                         int32_t irot, double grad[2]) {
+  return grad_for_rot_obj(h, iref, iptcl, shvec, irot, nullptr, grad, PFTCC_OBJFUN_CC);
+}
+
+int pftcc_gencorr_euclid_grad_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                        int32_t irot, double* f, double grad[2]) {
+  if (!f) return PFTCC_E_ARG;
+  return grad_for_rot_obj(h, iref, iptcl, shvec, irot, f, grad, PFTCC_OBJFUN_EUCLID);
+}
+
+int pftcc_gencorr_euclid_grad_only_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                             int32_t irot, double grad[2]) {
+  return grad_for_rot_obj(h, iref, iptcl, shvec, irot, nullptr, grad, PFTCC_OBJFUN_EUCLID);
+}
+
+// ---------------------------------------------------- orientation derivatives (cont) ---
+static int ensure_drefs(pftcc* h) {
+  if (h->d_drefs_e) return PFTCC_OK;
+  h->ndslots = std::max(1, h->cfg.nthr);
+  const size_t cnt = (size_t)h->ndslots * 3 * h->kcorr * h->n;
+  int rc;
+  if ((rc = dalloc(h, &h->d_drefs_e, cnt)) || (rc = dalloc(h, &h->d_drefs_o, cnt))) return rc;
+  HIPCHK(h, sync_memset(h, h->d_drefs_e, 0, cnt * sizeof(float2)));
+  HIPCHK(h, sync_memset(h, h->d_drefs_o, 0, cnt * sizeof(float2)));
+  return PFTCC_OK;
+}
+
+static int check_dslot(pftcc* h, int iref) {
+  int rc;
+  if ((rc = check_ref(h, iref)) || (rc = ensure_drefs(h))) return rc;
+  if (iref > h->ndslots)
+    return fail(h, PFTCC_E_ARG, "iref %d: the derivative slots are 1..nthr = %d (pfts_drefs, :321-322)", iref,
+                h->ndslots);
+  return PFTCC_OK;
+}
+
+int pftcc_set_dref_fcomp(pftcc_t* h, int32_t iref, int32_t irot, int32_t k, const float dcomp[6], int32_t iseven) {
+  if (!h || !dcomp) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = check_dslot(h, iref))) return rc;
+  if (irot < 1 || irot > h->n || k < h->kfrom || k > h->kstop)
+    return fail(h, PFTCC_E_ARG, "component (%d, %d) outside the derivative block (kfrom..kstop)", irot, k);
+  float2* base = (iseven ? h->d_drefs_e : h->d_drefs_o) + (size_t)(iref - 1) * 3 * h->kcorr * h->n;
+  for (int d = 0; d < 3; ++d) {
+    const float2 v = make_float2(dcomp[2 * d], dcomp[2 * d + 1]);
+    HIPCHK(h, sync_copy(h, base + ((size_t)d * h->kcorr + (k - h->kfrom)) * h->n + (irot - 1), &v, sizeof v,
+                        hipMemcpyHostToDevice));
+  }
+  return PFTCC_OK;
+}
+
+int pftcc_set_dref_pft(pftcc_t* h, int32_t iref, const float* dpft, int32_t iseven) {
+  if (!h || !dpft) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc;
+  if ((rc = check_dslot(h, iref))) return rc;
+  const size_t cnt = (size_t)3 * h->kcorr * h->n;
+  float2* base = (iseven ? h->d_drefs_e : h->d_drefs_o) + (size_t)(iref - 1) * cnt;
+  HIPCHK(h, sync_copy(h, base, dpft, cnt * sizeof(float2), hipMemcpyHostToDevice));
+  return PFTCC_OK;
+}
+
+// y, S_q, T1[3], T2[3], gx, gy of k_cont8 for one (iref, iptcl, shift, irot).
+static int cont8_single(pftcc* h, int iref, int iptcl, const double shvec[2], int irot, double out[9]) {
+  int rc, ip;
+  if ((rc = check_dslot(h, iref)) || (rc = local_index(h, iptcl, &ip)) || (rc = check_rot(h, irot))) return rc;
+  if (iref > h->nptcls)
+    return fail(h, PFTCC_E_ARG, "iref %d: the T1 terms read particle slot iref (:1781, :1836), nptcls = %d", iref,
+                h->nptcls);
+  if ((rc = need_direct(h))) return rc;
+  struct {
+    int iref, ip, it1, rho;
+    double sh[2];
+  } in{iref - 1, ip, iref - 1, irot - 1, {shvec ? shvec[0] : 0.0, shvec ? shvec[1] : 0.0}};
+  if ((rc = scratch_reserve(h, 4096))) return rc;
+  char* d_in = take<char>(h, sizeof in);
+  double* d_out = take<double>(h, 9);
+  HIPCHK(h, hipMemcpyAsync(d_in, &in, sizeof in, hipMemcpyHostToDevice, h->stream));
+  const int* di = reinterpret_cast<const int*>(d_in);
+  Cont8Args a{};
+  a.n = h->n;
+  a.nk = h->nk;
+  a.kcorr = h->kcorr;
+  a.kfrom = h->kfrom;
+  a.nrefs = h->nrefs;
+  a.ndslots = h->ndslots;
+  a.refs_even = h->d_refs_e;
+  a.refs_odd = h->d_refs_o;
+  a.drefs_even = h->d_drefs_e;
+  a.drefs_odd = h->d_drefs_o;
+  a.ptcls = h->d_ptcls;
+  a.ctf = h->with_ctf ? h->d_ctf : nullptr;
+  a.iseven = h->d_iseven;
+  const bool norm = h->match_filt || h->pssnr;
+  a.refinv = norm ? h->d_refinv_d : nullptr;
+  a.ofac = norm ? h->d_ofac_d : nullptr;
+  a.optrow = (norm && h->clsfrcs) ? h->d_optrow : nullptr;
+  a.sn32 = h->d_sn32;
+  a.ncn32 = h->d_ncn32;
+  a.arg_a = kDPi / (double)(h->ldim / 2);
+  a.argx = h->d_argx;
+  a.argy = h->d_argy;
+  a.one_x = h->d_onex;
+  a.one_y = h->d_oney;
+  a.irefs = di;
+  a.ips = di + 1;
+  a.it1s = di + 2;
+  a.rhos = di + 3;
+  a.shifts = reinterpret_cast<const double*>(d_in + 16);
+  a.nitems = 1;
+  a.out = d_out;
+  const int nt = std::min(256, (h->n + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_cont8, dim3(1), dim3(nt), 0, h->stream, a);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(out, d_out, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+int pftcc_gencorr_cont_grad_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                         int32_t irot, double* cc, double dcc[3]) {
+  if (!h || !cc || !dcc) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  double o[9];
+  int rc, ip;
+  if ((rc = cont8_single(h, iref, iptcl, shvec, irot, o))) return rc;
+  (void)local_index(h, iptcl, &ip);
+  float sp;
+  HIPCHK(h, sync_copy(h, &sp, h->d_sqsums + ip, sizeof sp, hipMemcpyDeviceToHost));
+  // cc = sum(fdf_y) / denom, dcc(j) = (sum(T1(:,j)) - sum(fdf_y) sum(T2(:,j)) / sqsum_ref) / denom (:1777-1786)
+  const double denom = sqrt(o[1] * (double)sp);
+  *cc = o[0] / denom;
+  for (int d = 0; d < 3; ++d) dcc[d] = (o[2 + d] - o[0] * o[5 + d] / o[1]) / denom;
+  return PFTCC_OK;
+}
+
+int pftcc_gencorr_cont_shift_grad_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                               int32_t irot, double* f, double grad[5]) {
+  if (!h || !f || !grad) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  double o[9];
+  int rc, ip;
+  if ((rc = cont8_single(h, iref, iptcl, shvec, irot, o))) return rc;
+  (void)local_index(h, iptcl, &ip);
+  float sp;
+  HIPCHK(h, sync_copy(h, &sp, h->d_sqsums + ip, sizeof sp, hipMemcpyDeviceToHost));
+  // (:1832-1846): three orientation derivatives, then the two shift derivatives
+  const double denom = sqrt(o[1] * (double)sp);
+  *f = o[0] / denom;
+  for (int d = 0; d < 3; ++d) grad[d] = (o[2 + d] - o[0] * o[5 + d] / o[1]) / denom;
+  grad[3] = o[7] / denom;
+  grad[4] = o[8] / denom;
+  return PFTCC_OK;
+}
+
+// The reference's Euclid forms of these two are unimplemented stubs (:2050-2069: cc = 0, the
+// derivatives left unset); these return 0 everywhere.
+int pftcc_gencorr_cont_grad_euclid_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                             int32_t irot, double* cc, double dcc[3]) {
+  (void)shvec;
+  if (!h || !cc || !dcc) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = check_ref(h, iref)) || (rc = local_index(h, iptcl, &ip)) || (rc = check_rot(h, irot))) return rc;
+  *cc = 0.0;
+  dcc[0] = dcc[1] = dcc[2] = 0.0;
+  return PFTCC_OK;
+}
+
+int pftcc_gencorr_cont_shift_grad_euclid_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl, const double shvec[2],
+                                                   int32_t irot, double* f, double grad[5]) {
+  (void)shvec;
+  if (!h || !f || !grad) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = check_ref(h, iref)) || (rc = local_index(h, iptcl, &ip)) || (rc = check_rot(h, irot))) return rc;
+  *f = 0.0;
+  for (int d = 0; d < 5; ++d) grad[d] = 0.0;
+  return PFTCC_OK;
+}
+
+// --------------------------------------------------------------- remaining getters ---
+int pftcc_get_pfromto(const pftcc_t* h, int32_t pfromto[2]) {
+  if (!h || !pfromto) return PFTCC_E_ARG;
+  pfromto[0] = h->pfrom;
+  pfromto[1] = h->pto;
+  return PFTCC_OK;
+}
+
+int pftcc_assign_pinds(const pftcc_t* h, int32_t* pinds) {
+  if (!h || !pinds) return PFTCC_E_ARG;
+  for (size_t i = 0; i < h->pinds.size(); ++i) pinds[i] = h->pinds[i];
+  return PFTCC_OK;
+}
+
+int pftcc_cp_refs(pftcc_t* h, const pftcc_t* src) {
+  if (!h || !src) return PFTCC_E_ARG;
+  if (h == src) return PFTCC_OK;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (src->nrefs != h->nrefs || src->nk != h->nk || src->n != h->n || src->kfrom != h->kfrom)
+    return fail(h, PFTCC_E_ARG, "cp_refs: reference blocks differ in shape (%d x %d x %d vs %d x %d x %d)",
+                src->nrefs, src->nk, src->n, h->nrefs, h->nk, h->n);
+  if (src->device != h->device) return fail(h, PFTCC_E_UNSUPPORTED, "cp_refs: handles on different GPUs");
+  const size_t bytes = (size_t)h->nrefs * h->nk * h->n * sizeof(float2);
+  // the source's pending work (its own stream) must be complete before its arrays are read
+  HIPCHK(h, hipStreamSynchronize(src->stream));
+  HIPCHK(h, sync_copy(h, h->d_refs_e, src->d_refs_e, bytes, hipMemcpyDeviceToDevice));
+  HIPCHK(h, sync_copy(h, h->d_refs_o, src->d_refs_o, bytes, hipMemcpyDeviceToDevice));
+  h->refnorm_valid = false;
+  h->apack[0].valid = h->apack[1].valid = false;
+  return PFTCC_OK;
+}
+
+int pftcc_calc_ptcl_pspec(pftcc_t* h, int32_t iptcl, float* pspec) {
+  if (!h || !pspec) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  int rc, ip;
+  if ((rc = local_index(h, iptcl, &ip))) return rc;
+  if ((rc = scratch_reserve(h, aligned((size_t)h->nk * 8)))) return rc;
+  double* d_s = take<double>(h, h->nk);
+  // FP64 shell sums of |p|^2 over the whole ring block, one wavefront per shell
+  hipLaunchKernelGGL(k_shell_sq8, dim3((h->nk + 3) / 4), dim3(256), 0, h->stream,
+                     (const float2*)(h->d_ptcls + (size_t)ip * h->nk * h->n), h->nk, h->n, d_s);
+  HIPCHK(h, hipGetLastError());
+  std::vector<double> s(h->nk);
+  HIPCHK(h, hipMemcpyAsync(s.data(), d_s, h->nk * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  // pspec(:) = sum(csq_fast(pfts_ptcls(:,:,i)), dim=1) / real(pftsz) (:897-904)
+  for (int kk = 0; kk < h->nk; ++kk) pspec[kk] = (float)s[kk] / (float)h->n;
+  return PFTCC_OK;
 }
 
 int pftcc_gencorrs_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, const int32_t* irefs, int32_t ns,

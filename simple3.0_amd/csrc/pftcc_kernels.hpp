@@ -696,4 +696,120 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PFTCC_
   }
 }
 
+// Orientation-derivative objective of continuous refinement: gencorr_cont_grad_cc_for_rot_8 and
+// gencorr_cont_shift_grad_cc_for_rot_8 (:1735-1847).  q = ref x shellnorm x |CTF| x shmat_8 and the
+// three derivative rings dq_d = dref_d x |CTF| x shmat_8 (the shell normalisation is applied to q
+// only, as the reference does), over shells kfrom..kstop, in FP64:
+//   y   = sum_k Re sum_j q conj(P_ip rotated),           S_q = sum_k sum_j |q|^2,
+//   T1d = sum_k Re sum_j dq_d conj(P_it1 rotated),       T2d = sum_k Re sum_j dq_d conj(q),
+//   gx  = Re sum (i a_x q) conj(P_ip rotated),            gy likewise with a_y,
+// where the T1 particle is it1, which the reference takes from its iref argument
+// (calc_T1_T2_for_rot_8(pft_ref, pft_dref, iref, ...), :1781, :1836).  out[9] = {y, S_q, T1[3],
+// T2[3], gx, gy} per item, reduced in a fixed order.  One work-group per item.
+struct Cont8Args {
+  int n, nk, kcorr, kfrom, nrefs, ndslots;
+  const float2* refs_even;
+  const float2* refs_odd;
+  const float2* drefs_even;    // [ndslots][3][kcorr][n]
+  const float2* drefs_odd;
+  const float2* ptcls;
+  const float* ctf;
+  const int8_t* iseven;
+  const double* refinv;        // shell normalisation (match_filt / pssnr), as Rot8Args, or null
+  const double* ofac;
+  const int* optrow;
+  const float* sn32;
+  const float* ncn32;
+  double arg_a;
+  const double* argx;          // [nk][n]; the first shell's row starts the phase recurrence
+  const double* argy;
+  const double* one_x;
+  const double* one_y;
+  const int* irefs;            // 0-based reference (and derivative slot)
+  const int* ips;              // local 0-based particle
+  const int* it1s;             // local 0-based particle of the T1 terms
+  const double* shifts;        // [nitems][2]
+  const int* rhos;             // 0-based rotation
+  int nitems;
+  double* out;                 // [nitems][9]
+};
+
+// Rotated particle value P[(j + rho) mod 2n] on the half ring (conjugated past n).
+__device__ __forceinline__ void rot_ptcl(const float2* ring, int n, int j, int rho, double& pr, double& pi) {
+  int t = j + rho;
+  bool cj = false;
+  if (t >= 2 * n) t -= 2 * n;
+  if (t >= n) { t -= n; cj = true; }
+  const float2 v = ring[t];
+  pr = v.x;
+  pi = cj ? -(double)v.y : (double)v.y;
+}
+
+__global__ void __launch_bounds__(256) k_cont8(Cont8Args a) {
+  const int item = blockIdx.x;
+  if (item >= a.nitems) return;
+  const int n = a.n;
+  const int iref = a.irefs[item], ip = a.ips[item], it1 = a.it1s[item], rho = a.rhos[item];
+  const double sx = a.shifts[2 * item], sy = a.shifts[2 * item + 1];
+  const bool even = a.iseven[ip] != 0;
+  const float2* ref = (even ? a.refs_even : a.refs_odd) + (size_t)iref * a.nk * n;
+  const float2* dref = (even ? a.drefs_even : a.drefs_odd) + (size_t)iref * 3 * a.kcorr * n;
+  const float2* ptc = a.ptcls + (size_t)ip * a.nk * n;
+  const float2* pt1 = a.ptcls + (size_t)it1 * a.nk * n;
+  const float* ctf = a.ctf ? a.ctf + (size_t)ip * a.nk * n : nullptr;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double sa, ca, ss, cs;
+    sincos(a.argx[j] * sx + a.argy[j] * sy, &sa, &ca);
+    sincos(a.one_x[j] * sx + a.one_y[j] * sy, &ss, &cs);
+    double shr = ca, shi = sa;
+    for (int kk = 0; kk < a.kcorr; ++kk) {
+      const double c = ctf ? (double)ctf[(size_t)kk * n + j] : 1.0;
+      double fac = 1.0;
+      if (a.refinv) {
+        const int row = a.optrow ? a.optrow[ip] : iref;
+        fac = a.ofac[(size_t)row * a.kcorr + kk] * a.refinv[((size_t)(even ? 0 : 1) * a.nrefs + iref) * a.kcorr + kk];
+      }
+      const float2 rv = ref[(size_t)kk * n + j];
+      const double ar = (double)rv.x * fac * c, ai = (double)rv.y * fac * c;
+      const double qr = ar * shr - ai * shi, qi = ar * shi + ai * shr;
+      double pr, pim, tr, tim;
+      rot_ptcl(ptc + (size_t)kk * n, n, j, rho, pr, pim);
+      rot_ptcl(pt1 + (size_t)kk * n, n, j, rho, tr, tim);
+      acc[0] += qr * pr + qi * pim;
+      acc[1] += qr * qr + qi * qi;
+      const double im = qi * pr - qr * pim;          // Im(q conj P); Re((i a q) conj P) = -a Im(q conj P)
+      const float kf = (float)(a.kfrom + kk);
+      acc[7] += -((double)(a.sn32[j] * kf) * a.arg_a) * im;
+      acc[8] += -((double)(a.ncn32[j] * kf) * a.arg_a) * im;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float2 dv = dref[((size_t)d * a.kcorr + kk) * n + j];
+        const double br = (double)dv.x * c, bi = (double)dv.y * c;
+        const double dr = br * shr - bi * shi, di = br * shi + bi * shr;
+        acc[2 + d] += dr * tr + di * tim;
+        acc[5 + d] += dr * qr + di * qi;
+      }
+      const double r2 = shr * cs - shi * ss, i2 = shr * ss + shi * cs;
+      shr = r2;
+      shi = i2;
+    }
+  }
+  __shared__ double red[9][4];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    double x = acc[v];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if ((threadIdx.x & 63) == 0) red[v][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    const int nw = (blockDim.x + 63) >> 6;
+    double x = red[threadIdx.x][0];
+    for (int w = 1; w < nw; ++w) x += red[threadIdx.x][w];
+    a.out[(size_t)item * 9 + threadIdx.x] = x;
+  }
+}
+
 }  // namespace pfk

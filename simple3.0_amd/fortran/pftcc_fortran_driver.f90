@@ -6,6 +6,13 @@
 !       search / search_mfma / shsrch_grad_batch / strategy_srch / gencorr_for_rot_8 /
 !       gencorr_grad_for_rot_8 / specscore and writes the results
 !       (raw little-endian; layout in tests/test_gpu_fortran.py).
+!   pftcc_fortran_driver omp <nthreads>
+!       the per-pair calls issued from OpenMP threads as simple_strategy3D_matcher.f90:204 does
+!       (!$omp parallel do over particles): gencorrs (with and without shift),
+!       gencorr_grad_for_rot_8 and gencorr_cont_grad_cc_for_rot_8 for every (ref, ptcl) pair, once
+!       serially and once in parallel; exits non-zero unless both give identical results.  Also
+!       exercises the rest of the module's surface (cp_refs, assign_pinds, calc_ptcl_pspec, the
+!       objective-specific and orientation-derivative forms).
 !   pftcc_fortran_driver shiftsrch
 !       the invariant of production/tests/test_shiftsrch (simple_test_shiftsrch.f90:40-118):
 !       8 (ref, ptcl) pairs, the particle carrying the reference's phase for a +-1 px shift in
@@ -24,8 +31,11 @@ select case(trim(mode))
         call run_parity(trim(fin), trim(fout))
     case('shiftsrch')
         call run_shiftsrch
+    case('omp')
+        call get_command_argument(2, fin)
+        call run_omp(trim(fin))
     case default
-        write(*,*) 'usage: pftcc_fortran_driver parity <in> <out> | shiftsrch'
+        write(*,*) 'usage: pftcc_fortran_driver parity <in> <out> | shiftsrch | omp <nthreads>'
         error stop 2
 end select
 
@@ -237,5 +247,126 @@ contains
         endif
         write(*,'(a)') 'shiftsrch: all shifts recovered'
     end subroutine run_shiftsrch
+
+    subroutine run_omp( nthr_arg )
+        use omp_lib
+        character(len=*), intent(in) :: nthr_arg
+        integer, parameter :: sp = c_float, dp = c_double
+        integer, parameter :: NREFS = 6, NPTCLS = 12, BOX = 64, KFROM = 2, KTO = 20
+        type(polarft_corrcalc) :: pftcc, pftcc2
+        type(pftcc_params)     :: p
+        type(ctfparams)        :: cps(NPTCLS)
+        complex(c_float_complex), allocatable :: ref(:,:), ptcl(:,:)
+        complex(sp) :: dcomp(3)
+        real(c_float),  allocatable :: cc_s(:,:,:,:), cc_p(:,:,:,:), pspec(:)
+        real(c_double), allocatable :: f_s(:,:), f_p(:,:), g_s(:,:,:), g_p(:,:,:)
+        real(c_double), allocatable :: c_s(:,:), c_p(:,:), d_s(:,:,:), d_p(:,:,:)
+        integer, allocatable :: pinds(:)
+        real    :: rnd(2), shv(2)
+        real(dp) :: sh8(2), f8, g8(2), g5(5), ce, de(3)
+        integer :: i, r, j, k, n, nrots, nthr, irot, seed(64), nbad
+        read(nthr_arg, *) nthr
+        p%boxmatch = BOX
+        p%ring2    = 30
+        p%kfromto  = [KFROM, KTO]
+        p%kstop    = KTO
+        p%with_ctf = .true.
+        p%nthr     = NREFS
+        call pftcc%new(NREFS, [1, NPTCLS], p)
+        n     = pftcc%get_pftsz()
+        nrots = pftcc%get_nrots()
+        allocate(ref(n, KFROM:KTO), ptcl(n, KFROM:KTO))
+        seed = 7
+        call random_seed(put=seed)
+        do r = 1, NREFS
+            do k = KFROM, KTO
+                do j = 1, n
+                    call random_number(rnd)
+                    ref(j,k) = cmplx(rnd(1) - 0.5, rnd(2) - 0.5) / real(k)
+                end do
+            end do
+            call pftcc%set_ref_pft(r, ref, .true.)
+            call pftcc%cp_even2odd_ref(r)
+            do k = KFROM, KTO
+                do j = 1, n
+                    call random_number(rnd)
+                    dcomp = cmplx(rnd(1) - 0.5, rnd(2) - 0.5) / real(k)
+                    call pftcc%set_dref_fcomp(r, j, k, dcomp, .true.)
+                    call pftcc%set_dref_fcomp(r, j, k, dcomp, .false.)
+                end do
+            end do
+        end do
+        do i = 1, NPTCLS
+            do k = KFROM, KTO
+                do j = 1, n
+                    call random_number(rnd)
+                    ptcl(j,k) = cmplx(rnd(1) - 0.5, rnd(2) - 0.5) / real(k)
+                end do
+            end do
+            call pftcc%set_ptcl_pft(i, ptcl)
+            call random_number(rnd)
+            cps(i)%smpd = 1.3
+            cps(i)%dfx  = 1.0 + 2.0 * rnd(1)
+            cps(i)%dfy  = cps(i)%dfx - 0.1 * rnd(2)
+            cps(i)%angast = 30.
+        end do
+        call pftcc%set_eos([(mod(i, 2), i = 1, NPTCLS)])
+        call pftcc%create_polar_absctfmats(cps)
+        call pftcc%memoize_ffts
+        allocate(cc_s(nrots, 2, NREFS, NPTCLS), cc_p(nrots, 2, NREFS, NPTCLS))
+        allocate(f_s(NREFS, NPTCLS), f_p(NREFS, NPTCLS), g_s(2, NREFS, NPTCLS), g_p(2, NREFS, NPTCLS))
+        allocate(c_s(NREFS, NPTCLS), c_p(NREFS, NPTCLS), d_s(3, NREFS, NPTCLS), d_p(3, NREFS, NPTCLS))
+        call pairs(1, cc_s, f_s, g_s, c_s, d_s)
+        call pairs(nthr, cc_p, f_p, g_p, c_p, d_p)
+        nbad = count(cc_s /= cc_p) + count(f_s /= f_p) + count(g_s /= g_p) + count(c_s /= c_p) + count(d_s /= d_p)
+        write(*,'(a,i0,a,i0,a,i0)') 'omp: ', NREFS * NPTCLS, ' pairs on ', nthr, ' threads, mismatches: ', nbad
+        ! the rest of the surface: each call must succeed (errors stop the program)
+        call pftcc%assign_pinds(pinds)
+        if( lbound(pinds,1) /= 1 .or. ubound(pinds,1) /= NPTCLS .or. any(pinds /= [(i, i = 1, NPTCLS)]) ) nbad = nbad + 1
+        allocate(pspec(KFROM:KTO))
+        call pftcc%calc_ptcl_pspec(3, pspec)
+        if( any(pspec <= 0.) ) nbad = nbad + 1
+        sh8 = [0.5_dp, -0.25_dp]
+        f8  = pftcc%gencorr_cc_for_rot_8(2, 3, sh8, 5)
+        if( f8 /= pftcc%gencorr_for_rot_8(2, 3, sh8, 5) ) nbad = nbad + 1
+        call pftcc%gencorr_cc_grad_for_rot_8(2, 3, sh8, 5, f8, g8)
+        call pftcc%gencorr_cont_shift_grad_cc_for_rot_8(2, 3, sh8, 5, f8, g5)
+        ce = pftcc%gencorr_cont_grad_euclid_for_rot_8(2, 3, sh8, 5, de)
+        if( ce /= 0._dp .or. any(de /= 0._dp) ) nbad = nbad + 1
+        call pftcc2%new(NREFS, [1, NPTCLS], p)
+        call pftcc2%cp_refs(pftcc)
+        call pftcc2%kill
+        call pftcc%kill
+        if( nbad > 0 )then
+            write(*,'(a,i0)') 'omp: FAILED, mismatches ', nbad
+            error stop 1
+        endif
+        write(*,'(a)') 'omp: parallel per-pair calls equal the serial ones'
+
+    contains
+
+        subroutine pairs( nt, cc, f, g, c, d )
+            integer,        intent(in)  :: nt
+            real(c_float),  intent(out) :: cc(nrots, 2, NREFS, NPTCLS)
+            real(c_double), intent(out) :: f(NREFS, NPTCLS), g(2, NREFS, NPTCLS), c(NREFS, NPTCLS), d(3, NREFS, NPTCLS)
+            integer  :: ip, ir, ir8
+            real     :: s2(2)
+            real(dp) :: s8(2)
+            !$omp parallel do num_threads(nt) default(shared) private(ip,ir,ir8,s2,s8) schedule(static)
+            do ip = 1, NPTCLS
+                do ir = 1, NREFS
+                    s2  = [0.25 * real(ir), -0.5 * real(ip) / real(NPTCLS)]
+                    s8  = real(s2, dp)
+                    ir8 = 1 + mod(7 * ip + 3 * ir, nrots)
+                    call pftcc%gencorrs(ir, ip, cc(:, 1, ir, ip))
+                    call pftcc%gencorrs(ir, ip, s2, cc(:, 2, ir, ip))
+                    call pftcc%gencorr_grad_for_rot_8(ir, ip, s8, ir8, f(ir, ip), g(:, ir, ip))
+                    c(ir, ip) = pftcc%gencorr_cont_grad_cc_for_rot_8(ir, ip, s8, ir8, d(:, ir, ip))
+                end do
+            end do
+            !$omp end parallel do
+        end subroutine pairs
+
+    end subroutine run_omp
 
 end program pftcc_fortran_driver

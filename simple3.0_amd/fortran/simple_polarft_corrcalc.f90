@@ -18,12 +18,12 @@ implicit none
 
 public :: polarft_corrcalc, pftcc_glob, pftcc_params, ctfparams, prep_cfg, OBJFUN_CC, OBJFUN_EUCLID
 public :: strategy_cfg, SRCH_GREEDY2D, SRCH_SNHC2D, SRCH_GREEDY3D, SRCH_SHC3D, SRCH_SNHC3D
-public :: PREC_BF16, PREC_SPLIT_BF16, PREC_FP8
+public :: PREC_BF16, PREC_SPLIT_BF16, PREC_FP16, PREC_SPLIT_FP16, PREC_FP8
 private
 
 integer, parameter :: sp = c_float, dp = c_double
 integer, parameter :: OBJFUN_CC = 0, OBJFUN_EUCLID = 1
-integer, parameter :: PREC_BF16 = 1, PREC_SPLIT_BF16 = 3, PREC_FP8 = 8   ! precision form
+integer, parameter :: PREC_BF16 = 1, PREC_SPLIT_BF16 = 3, PREC_FP16 = 2, PREC_SPLIT_FP16 = 4, PREC_FP8 = 8   ! precision form
 
 ! params_glob fields read by polarft_corrcalc (simple_parameters.f90:1182-1371)
 type :: pftcc_params
@@ -80,6 +80,8 @@ type :: polarft_corrcalc
     procedure          :: set_ref_fcomp
     procedure          :: set_ptcl_fcomp
     procedure          :: cp_even2odd_ref
+    procedure          :: cp_refs
+    procedure          :: set_dref_fcomp
     procedure          :: cp_even_ref2ptcl
     procedure          :: swap_ptclsevenodd
     procedure          :: reallocate_ptcls
@@ -96,6 +98,8 @@ type :: polarft_corrcalc
     procedure          :: get_roind
     procedure          :: get_coord
     procedure          :: get_ref_pft
+    procedure          :: assign_pinds
+    procedure          :: print
     procedure          :: get_ptcl_pft
     procedure          :: get_nrefs
     procedure          :: get_nptcls
@@ -103,6 +107,7 @@ type :: polarft_corrcalc
     procedure          :: exists
     procedure          :: create_polar_absctfmats
     procedure          :: memoize_ffts
+    procedure          :: calc_ptcl_pspec
     procedure, private :: gencorrs_1
     procedure, private :: gencorrs_2
     generic            :: gencorrs => gencorrs_1, gencorrs_2
@@ -111,6 +116,18 @@ type :: polarft_corrcalc
     procedure          :: gencorr_grad_for_rot_8
     procedure          :: gencorr_grad_only_for_rot_8
     procedure          :: gencorr_sigma_contrib
+    procedure          :: gencorr_cc_for_rot
+    procedure          :: gencorr_cc_for_rot_8
+    procedure          :: gencorr_cc_grad_for_rot_8
+    procedure          :: gencorr_cc_grad_only_for_rot_8
+    procedure          :: gencorr_euclid_for_rot
+    procedure          :: gencorr_euclid_for_rot_8
+    procedure          :: gencorr_euclid_grad_for_rot_8
+    procedure          :: gencorr_euclid_grad_only_for_rot_8
+    procedure          :: gencorr_cont_grad_cc_for_rot_8
+    procedure          :: gencorr_cont_shift_grad_cc_for_rot_8
+    procedure          :: gencorr_cont_grad_euclid_for_rot_8
+    procedure          :: gencorr_cont_shift_grad_euclid_for_rot_8
     procedure, private :: specscore_1
     procedure, private :: specscore_2
     generic            :: specscore => specscore_1, specscore_2
@@ -334,6 +351,127 @@ interface
         real(c_float), intent(in) :: shvec(2)
         real(c_float), intent(out) :: sc(*)
         type(c_ptr), value :: ps, rs
+    end function
+    ! objective-specific single-rotation forms (gencorr_cc_* / gencorr_euclid_*)
+    integer(c_int) function c_cc_for_rot(h, iref, iptcl, shvec, irot, f) bind(C, name='pftcc_gencorr_cc_for_rot')
+        import :: c_int, c_int32_t, c_ptr, c_float
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_float), intent(in) :: shvec(2)
+        real(c_float), intent(out) :: f
+    end function
+    integer(c_int) function c_euclid_for_rot(h, iref, iptcl, shvec, irot, f) &
+        &bind(C, name='pftcc_gencorr_euclid_for_rot')
+        import :: c_int, c_int32_t, c_ptr, c_float
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_float), intent(in) :: shvec(2)
+        real(c_float), intent(out) :: f
+    end function
+    integer(c_int) function c_cc_for_rot_8(h, iref, iptcl, shvec, irot, f) bind(C, name='pftcc_gencorr_cc_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: f
+    end function
+    integer(c_int) function c_euclid_for_rot_8(h, iref, iptcl, shvec, irot, f) &
+        &bind(C, name='pftcc_gencorr_euclid_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: f
+    end function
+    integer(c_int) function c_cc_grad_for_rot_8(h, iref, iptcl, shvec, irot, f, grad) &
+        &bind(C, name='pftcc_gencorr_cc_grad_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: f, grad(2)
+    end function
+    integer(c_int) function c_euclid_grad_for_rot_8(h, iref, iptcl, shvec, irot, f, grad) &
+        &bind(C, name='pftcc_gencorr_euclid_grad_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: f, grad(2)
+    end function
+    integer(c_int) function c_cc_grad_only_for_rot_8(h, iref, iptcl, shvec, irot, grad) &
+        &bind(C, name='pftcc_gencorr_cc_grad_only_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: grad(2)
+    end function
+    integer(c_int) function c_euclid_grad_only_for_rot_8(h, iref, iptcl, shvec, irot, grad) &
+        &bind(C, name='pftcc_gencorr_euclid_grad_only_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: grad(2)
+    end function
+    integer(c_int) function c_set_dref_fcomp(h, iref, irot, k, dcomp, iseven) bind(C, name='pftcc_set_dref_fcomp')
+        import :: c_int, c_int32_t, c_ptr, c_float_complex
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, irot, k, iseven
+        complex(c_float_complex), intent(in) :: dcomp(3)
+    end function
+    integer(c_int) function c_cont_grad_cc(h, iref, iptcl, shvec, irot, cc, dcc) &
+        &bind(C, name='pftcc_gencorr_cont_grad_cc_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: cc, dcc(3)
+    end function
+    integer(c_int) function c_cont_grad_euclid(h, iref, iptcl, shvec, irot, cc, dcc) &
+        &bind(C, name='pftcc_gencorr_cont_grad_euclid_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: cc, dcc(3)
+    end function
+    integer(c_int) function c_cont_shift_grad_cc(h, iref, iptcl, shvec, irot, f, grad) &
+        &bind(C, name='pftcc_gencorr_cont_shift_grad_cc_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: f, grad(5)
+    end function
+    integer(c_int) function c_cont_shift_grad_euclid(h, iref, iptcl, shvec, irot, f, grad) &
+        &bind(C, name='pftcc_gencorr_cont_shift_grad_euclid_for_rot_8')
+        import :: c_int, c_int32_t, c_ptr, c_double
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iref, iptcl, irot
+        real(c_double), intent(in) :: shvec(2)
+        real(c_double), intent(out) :: f, grad(5)
+    end function
+    integer(c_int) function c_calc_ptcl_pspec(h, iptcl, pspec) bind(C, name='pftcc_calc_ptcl_pspec')
+        import :: c_int, c_int32_t, c_ptr, c_float
+        type(c_ptr), value :: h
+        integer(c_int32_t), value :: iptcl
+        real(c_float), intent(out) :: pspec(*)
+    end function
+    integer(c_int) function c_get_pfromto(h, pfromto) bind(C, name='pftcc_get_pfromto')
+        import :: c_int, c_int32_t, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int32_t), intent(out) :: pfromto(2)
+    end function
+    integer(c_int) function c_assign_pinds(h, pinds) bind(C, name='pftcc_assign_pinds')
+        import :: c_int, c_int32_t, c_ptr
+        type(c_ptr), value :: h
+        integer(c_int32_t), intent(out) :: pinds(*)
+    end function
+    integer(c_int) function c_cp_refs(h, src) bind(C, name='pftcc_cp_refs')
+        import :: c_int, c_ptr
+        type(c_ptr), value :: h, src
     end function
     integer(c_int) function c_specscore(h, iref, iptcl, irot, shvec, score) bind(C, name='pftcc_specscore')
         import :: c_int, c_int32_t, c_ptr, c_float
@@ -751,14 +889,191 @@ contains
             &int(irot,c_int32_t), grad), 'gencorr_grad_only_for_rot_8')
     end subroutine gencorr_grad_only_for_rot_8
 
-    subroutine gencorr_sigma_contrib( self, iref, iptcl, shvec, irot, sigma_contrib )
+    subroutine gencorr_sigma_contrib( self, iref, iptcl, shvec, irot, sigma_contrib, ptcl_sumsq, ref_sumsq )
         class(polarft_corrcalc), intent(inout) :: self
         integer,                 intent(in)    :: iref, iptcl, irot
         real(sp),                intent(in)    :: shvec(2)
         real(sp),                intent(out)   :: sigma_contrib(self%kfromto(1):self%kfromto(2))
+        real(sp), optional,      intent(out)   :: ptcl_sumsq(self%kfromto(1):self%kfromto(2))
+        real(sp), optional,      intent(out)   :: ref_sumsq(self%kfromto(1):self%kfromto(2))
+        real(c_float), allocatable, target :: ps(:), rs(:)
+        type(c_ptr) :: pps, prs
+        pps = c_null_ptr
+        prs = c_null_ptr
+        if( present(ptcl_sumsq) )then
+            allocate(ps(self%kfromto(2) - self%kfromto(1) + 1))
+            pps = c_loc(ps)
+        endif
+        if( present(ref_sumsq) )then
+            allocate(rs(self%kfromto(2) - self%kfromto(1) + 1))
+            prs = c_loc(rs)
+        endif
         call chk(self, c_gencorr_sigma_contrib(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
-            &int(irot,c_int32_t), sigma_contrib, c_null_ptr, c_null_ptr), 'gencorr_sigma_contrib')
+            &int(irot,c_int32_t), sigma_contrib, pps, prs), 'gencorr_sigma_contrib')
+        if( present(ptcl_sumsq) ) ptcl_sumsq = ps
+        if( present(ref_sumsq)  ) ref_sumsq  = rs
     end subroutine gencorr_sigma_contrib
+
+    ! the objective-specific forms that gencorr_for_rot(_8) dispatches to (:1645-2148)
+    real(sp) function gencorr_cc_for_rot( self, iref, iptcl, shvec, irot )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(sp),                intent(in)    :: shvec(2)
+        real(c_float) :: f
+        call chk(self, c_cc_for_rot(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), f), 'gencorr_cc_for_rot')
+        gencorr_cc_for_rot = f
+    end function gencorr_cc_for_rot
+
+    real(dp) function gencorr_cc_for_rot_8( self, iref, iptcl, shvec, irot )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(c_double) :: f
+        call chk(self, c_cc_for_rot_8(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), f), 'gencorr_cc_for_rot_8')
+        gencorr_cc_for_rot_8 = f
+    end function gencorr_cc_for_rot_8
+
+    subroutine gencorr_cc_grad_for_rot_8( self, iref, iptcl, shvec, irot, f, grad )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(dp),                intent(out)   :: f, grad(2)
+        call chk(self, c_cc_grad_for_rot_8(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), f, grad), 'gencorr_cc_grad_for_rot_8')
+    end subroutine gencorr_cc_grad_for_rot_8
+
+    subroutine gencorr_cc_grad_only_for_rot_8( self, iref, iptcl, shvec, irot, grad )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(dp),                intent(out)   :: grad(2)
+        call chk(self, c_cc_grad_only_for_rot_8(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), grad), 'gencorr_cc_grad_only_for_rot_8')
+    end subroutine gencorr_cc_grad_only_for_rot_8
+
+    real(sp) function gencorr_euclid_for_rot( self, iref, iptcl, shvec, irot )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(sp),                intent(in)    :: shvec(2)
+        real(c_float) :: f
+        call chk(self, c_euclid_for_rot(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), f), 'gencorr_euclid_for_rot')
+        gencorr_euclid_for_rot = f
+    end function gencorr_euclid_for_rot
+
+    real(dp) function gencorr_euclid_for_rot_8( self, iref, iptcl, shvec, irot )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(c_double) :: f
+        call chk(self, c_euclid_for_rot_8(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), f), 'gencorr_euclid_for_rot_8')
+        gencorr_euclid_for_rot_8 = f
+    end function gencorr_euclid_for_rot_8
+
+    subroutine gencorr_euclid_grad_for_rot_8( self, iref, iptcl, shvec, irot, f, grad )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(dp),                intent(out)   :: f, grad(2)
+        call chk(self, c_euclid_grad_for_rot_8(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), f, grad), 'gencorr_euclid_grad_for_rot_8')
+    end subroutine gencorr_euclid_grad_for_rot_8
+
+    subroutine gencorr_euclid_grad_only_for_rot_8( self, iref, iptcl, shvec, irot, grad )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(dp),                intent(out)   :: grad(2)
+        call chk(self, c_euclid_grad_only_for_rot_8(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), grad), 'gencorr_euclid_grad_only_for_rot_8')
+    end subroutine gencorr_euclid_grad_only_for_rot_8
+
+    ! orientation derivatives of continuous refinement (:1735-1847, :2050-2069)
+    function gencorr_cont_grad_cc_for_rot_8( self, iref, iptcl, shvec, irot, dcc ) result( cc )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(dp),                intent(out)   :: dcc(3)
+        real(dp) :: cc
+        call chk(self, c_cont_grad_cc(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), cc, dcc), 'gencorr_cont_grad_cc_for_rot_8')
+    end function gencorr_cont_grad_cc_for_rot_8
+
+    subroutine gencorr_cont_shift_grad_cc_for_rot_8( self, iref, iptcl, shvec, irot, f, grad )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(dp),                intent(out)   :: f, grad(5)
+        call chk(self, c_cont_shift_grad_cc(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), f, grad), 'gencorr_cont_shift_grad_cc_for_rot_8')
+    end subroutine gencorr_cont_shift_grad_cc_for_rot_8
+
+    function gencorr_cont_grad_euclid_for_rot_8( self, iref, iptcl, shvec, irot, dcc ) result( cc )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(dp),                intent(out)   :: dcc(3)
+        real(dp) :: cc
+        call chk(self, c_cont_grad_euclid(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), cc, dcc), 'gencorr_cont_grad_euclid_for_rot_8')
+    end function gencorr_cont_grad_euclid_for_rot_8
+
+    subroutine gencorr_cont_shift_grad_euclid_for_rot_8( self, iref, iptcl, shvec, irot, f, grad )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, iptcl, irot
+        real(dp),                intent(in)    :: shvec(2)
+        real(dp),                intent(out)   :: f, grad(5)
+        call chk(self, c_cont_shift_grad_euclid(self%h, int(iref,c_int32_t), int(iptcl,c_int32_t), shvec, &
+            &int(irot,c_int32_t), f, grad), 'gencorr_cont_shift_grad_euclid_for_rot_8')
+    end subroutine gencorr_cont_shift_grad_euclid_for_rot_8
+
+    subroutine set_dref_fcomp( self, iref, irot, k, dcomp, iseven )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iref, irot, k
+        complex(sp),             intent(in)    :: dcomp(3)
+        logical,                 intent(in)    :: iseven
+        call chk(self, c_set_dref_fcomp(self%h, int(iref,c_int32_t), int(irot,c_int32_t), int(k,c_int32_t), &
+            &dcomp, merge(1_c_int32_t, 0_c_int32_t, iseven)), 'set_dref_fcomp')
+    end subroutine set_dref_fcomp
+
+    subroutine cp_refs( self, self2 )
+        class(polarft_corrcalc), intent(inout) :: self, self2
+        call chk(self, c_cp_refs(self%h, self2%h), 'cp_refs')
+    end subroutine cp_refs
+
+    subroutine calc_ptcl_pspec( self, iptcl, pspec )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer,                 intent(in)    :: iptcl
+        real,                    intent(inout) :: pspec(self%kfromto(1):self%kfromto(2))
+        call chk(self, c_calc_ptcl_pspec(self%h, int(iptcl,c_int32_t), pspec), 'calc_ptcl_pspec')
+    end subroutine calc_ptcl_pspec
+
+    subroutine assign_pinds( self, pinds )
+        class(polarft_corrcalc), intent(inout) :: self
+        integer, allocatable,    intent(out)   :: pinds(:)
+        integer(c_int32_t) :: pft(2)
+        integer(c_int32_t), allocatable :: tmp(:)
+        call chk(self, c_get_pfromto(self%h, pft), 'assign_pinds')
+        allocate(tmp(pft(1):pft(2)))
+        call chk(self, c_assign_pinds(self%h, tmp), 'assign_pinds')
+        allocate(pinds(pft(1):pft(2)))
+        pinds = tmp
+    end subroutine assign_pinds
+
+    subroutine print( self )
+        class(polarft_corrcalc), intent(in) :: self
+        integer(c_int32_t) :: pdim(3)
+        integer(c_int) :: rc
+        rc = c_get_pdim(self%h, pdim)
+        write(*,*) "total n particles in partition         (self%nptcls): ", self%nptcls
+        write(*,*) "number of references                    (self%nrefs): ", self%nrefs
+        write(*,*) "number of rotations                     (self%nrots): ", self%nrots
+        write(*,*) "size of pft                             (self%pftsz): ", self%pftsz
+        write(*,*) "shells held                           (self%kfromto): ", pdim(2:3)
+    end subroutine print
 
     real(sp) function specscore_1( self, iref, iptcl, irot )
         class(polarft_corrcalc), intent(inout) :: self

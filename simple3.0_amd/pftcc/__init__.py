@@ -26,6 +26,8 @@ PFTCC_OBJFUN_CC = 0
 PFTCC_OBJFUN_EUCLID = 1
 PFTCC_PREC_BF16 = 1
 PFTCC_PREC_SPLIT_BF16 = 3
+PFTCC_PREC_FP16 = 2
+PFTCC_PREC_SPLIT_FP16 = 4
 PFTCC_PREC_FP8 = 8
 
 
@@ -127,6 +129,24 @@ SIGNATURES = [
     ("pftcc_gencorr_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD]),
     ("pftcc_gencorr_grad_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD, _PD]),
     ("pftcc_gencorr_grad_only_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD]),
+    ("pftcc_gencorr_cc_for_rot", _I, [_P, _I, _I, _PF, _I, _PF]),
+    ("pftcc_gencorr_cc_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD]),
+    ("pftcc_gencorr_cc_grad_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD, _PD]),
+    ("pftcc_gencorr_cc_grad_only_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD]),
+    ("pftcc_gencorr_euclid_for_rot", _I, [_P, _I, _I, _PF, _I, _PF]),
+    ("pftcc_gencorr_euclid_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD]),
+    ("pftcc_gencorr_euclid_grad_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD, _PD]),
+    ("pftcc_gencorr_euclid_grad_only_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD]),
+    ("pftcc_set_dref_fcomp", _I, [_P, _I, _I, _I, _PF, _I]),
+    ("pftcc_set_dref_pft", _I, [_P, _I, _PF, _I]),
+    ("pftcc_gencorr_cont_grad_cc_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD, _PD]),
+    ("pftcc_gencorr_cont_shift_grad_cc_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD, _PD]),
+    ("pftcc_gencorr_cont_grad_euclid_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD, _PD]),
+    ("pftcc_gencorr_cont_shift_grad_euclid_for_rot_8", _I, [_P, _I, _I, _PD, _I, _PD, _PD]),
+    ("pftcc_calc_ptcl_pspec", _I, [_P, _I, _PF]),
+    ("pftcc_get_pfromto", _I, [_P, _PI]),
+    ("pftcc_assign_pinds", _I, [_P, _PI]),
+    ("pftcc_cp_refs", _I, [_P, _P]),
     ("pftcc_gencorr_sigma_contrib", _I, [_P, _I, _I, _PF, _I, _PF, _PF, _PF]),
     ("pftcc_specscore", _I, [_P, _I, _I, _I, _PF, _PF]),
     ("pftcc_calc_roinv_corrmat", _I, [_P, _PF]),
@@ -468,6 +488,123 @@ class PolarFTCorrCalc:
         self._chk(self._lib.pftcc_gencorr_grad_only_for_rot_8(self._h, iref, iptcl, _dp(s), irot,
                                                               _dp(g)), "gencorr_grad_only_for_rot_8")
         return g
+
+    # objective-specific forms (gencorr_cc_* :1645-1733, :1863-2000; gencorr_euclid_* :2002-2148)
+    def _for_rot_8_obj(self, fn, what, iref, iptcl, shvec, irot):
+        s = np.asarray(shvec, np.float64)
+        v = C.c_double()
+        self._chk(fn(self._h, iref, iptcl, _dp(s), irot, C.byref(v)), what)
+        return v.value
+
+    def _for_rot_obj(self, fn, what, iref, iptcl, shvec, irot):
+        s = np.asarray(shvec, np.float32)
+        v = C.c_float()
+        self._chk(fn(self._h, iref, iptcl, _fp(s), irot, C.byref(v)), what)
+        return v.value
+
+    def _grad_obj(self, fn, what, iref, iptcl, shvec, irot, with_f=True):
+        s = np.asarray(shvec, np.float64)
+        g = np.zeros(2, np.float64)
+        if with_f:
+            f = C.c_double()
+            self._chk(fn(self._h, iref, iptcl, _dp(s), irot, C.byref(f), _dp(g)), what)
+            return f.value, g
+        self._chk(fn(self._h, iref, iptcl, _dp(s), irot, _dp(g)), what)
+        return g
+
+    def gencorr_cc_for_rot(self, iref, iptcl, shvec, irot):
+        return self._for_rot_obj(self._lib.pftcc_gencorr_cc_for_rot, "gencorr_cc_for_rot", iref, iptcl, shvec, irot)
+
+    def gencorr_cc_for_rot_8(self, iref, iptcl, shvec, irot):
+        return self._for_rot_8_obj(self._lib.pftcc_gencorr_cc_for_rot_8, "gencorr_cc_for_rot_8", iref, iptcl,
+                                   shvec, irot)
+
+    def gencorr_euclid_for_rot(self, iref, iptcl, shvec, irot):
+        return self._for_rot_obj(self._lib.pftcc_gencorr_euclid_for_rot, "gencorr_euclid_for_rot", iref, iptcl,
+                                 shvec, irot)
+
+    def gencorr_euclid_for_rot_8(self, iref, iptcl, shvec, irot):
+        return self._for_rot_8_obj(self._lib.pftcc_gencorr_euclid_for_rot_8, "gencorr_euclid_for_rot_8", iref,
+                                   iptcl, shvec, irot)
+
+    def gencorr_cc_grad_for_rot_8(self, iref, iptcl, shvec, irot):
+        return self._grad_obj(self._lib.pftcc_gencorr_cc_grad_for_rot_8, "gencorr_cc_grad_for_rot_8", iref, iptcl,
+                              shvec, irot)
+
+    def gencorr_cc_grad_only_for_rot_8(self, iref, iptcl, shvec, irot):
+        return self._grad_obj(self._lib.pftcc_gencorr_cc_grad_only_for_rot_8, "gencorr_cc_grad_only_for_rot_8",
+                              iref, iptcl, shvec, irot, with_f=False)
+
+    def gencorr_euclid_grad_for_rot_8(self, iref, iptcl, shvec, irot):
+        return self._grad_obj(self._lib.pftcc_gencorr_euclid_grad_for_rot_8, "gencorr_euclid_grad_for_rot_8", iref,
+                              iptcl, shvec, irot)
+
+    def gencorr_euclid_grad_only_for_rot_8(self, iref, iptcl, shvec, irot):
+        return self._grad_obj(self._lib.pftcc_gencorr_euclid_grad_only_for_rot_8,
+                              "gencorr_euclid_grad_only_for_rot_8", iref, iptcl, shvec, irot, with_f=False)
+
+    # orientation derivatives (set_dref_fcomp :459, gencorr_cont_* :1735-1847, :2050-2069)
+    def set_dref_fcomp(self, iref, irot, k, dcomp, iseven=True):
+        d = np.ascontiguousarray(dcomp, np.complex64)
+        if d.shape != (3,):
+            raise PftccError("dcomp must hold 3 complex values")
+        self._chk(self._lib.pftcc_set_dref_fcomp(self._h, iref, irot, k, _fp(d), int(bool(iseven))),
+                  "set_dref_fcomp")
+
+    def set_dref_pft(self, iref, dpft, iseven=True):
+        d = np.ascontiguousarray(dpft, np.complex64)
+        self._chk(self._lib.pftcc_set_dref_pft(self._h, iref, _fp(d), int(bool(iseven))), "set_dref_pft")
+
+    def gencorr_cont_grad_cc_for_rot_8(self, iref, iptcl, shvec, irot):
+        s = np.asarray(shvec, np.float64)
+        cc = C.c_double()
+        d = np.zeros(3, np.float64)
+        self._chk(self._lib.pftcc_gencorr_cont_grad_cc_for_rot_8(self._h, iref, iptcl, _dp(s), irot, C.byref(cc),
+                                                                 _dp(d)), "gencorr_cont_grad_cc_for_rot_8")
+        return cc.value, d
+
+    def gencorr_cont_shift_grad_cc_for_rot_8(self, iref, iptcl, shvec, irot):
+        s = np.asarray(shvec, np.float64)
+        f = C.c_double()
+        g = np.zeros(5, np.float64)
+        self._chk(self._lib.pftcc_gencorr_cont_shift_grad_cc_for_rot_8(self._h, iref, iptcl, _dp(s), irot,
+                                                                       C.byref(f), _dp(g)),
+                  "gencorr_cont_shift_grad_cc_for_rot_8")
+        return f.value, g
+
+    def gencorr_cont_grad_euclid_for_rot_8(self, iref, iptcl, shvec, irot):
+        s = np.asarray(shvec, np.float64)
+        cc = C.c_double()
+        d = np.zeros(3, np.float64)
+        self._chk(self._lib.pftcc_gencorr_cont_grad_euclid_for_rot_8(self._h, iref, iptcl, _dp(s), irot,
+                                                                     C.byref(cc), _dp(d)),
+                  "gencorr_cont_grad_euclid_for_rot_8")
+        return cc.value, d
+
+    def gencorr_cont_shift_grad_euclid_for_rot_8(self, iref, iptcl, shvec, irot):
+        s = np.asarray(shvec, np.float64)
+        f = C.c_double()
+        g = np.zeros(5, np.float64)
+        self._chk(self._lib.pftcc_gencorr_cont_shift_grad_euclid_for_rot_8(self._h, iref, iptcl, _dp(s), irot,
+                                                                           C.byref(f), _dp(g)),
+                  "gencorr_cont_shift_grad_euclid_for_rot_8")
+        return f.value, g
+
+    def calc_ptcl_pspec(self, iptcl):
+        out = np.zeros(self.nk, np.float32)
+        self._chk(self._lib.pftcc_calc_ptcl_pspec(self._h, iptcl, _fp(out)), "calc_ptcl_pspec")
+        return out
+
+    def assign_pinds(self):
+        """pinds over pfromto: (pfrom, array of 1-based local indices, 0 = not held)."""
+        pft = np.zeros(2, np.int32)
+        self._chk(self._lib.pftcc_get_pfromto(self._h, _ip(pft)), "get_pfromto")
+        out = np.zeros(int(pft[1] - pft[0] + 1), np.int32)
+        self._chk(self._lib.pftcc_assign_pinds(self._h, _ip(out)), "assign_pinds")
+        return int(pft[0]), out
+
+    def cp_refs(self, other):
+        self._chk(self._lib.pftcc_cp_refs(self._h, other.handle), "cp_refs")
 
     def gencorr_sigma_contrib(self, iref, iptcl, shvec, irot):
         s = np.asarray(shvec, np.float32)

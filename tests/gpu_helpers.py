@@ -13,7 +13,7 @@ def say(msg):
     sys.stdout.flush()
 
 
-def to_lib(pf: O.Pftcc, cps, pfrom=1, memoize=True):
+def to_lib(pf: O.Pftcc, cps, pfrom=1, memoize=True, nthr=1):
     """Builds a library handle holding exactly the oracle's inputs.
 
     When the case has a CTF, the library computes its own |CTF| matrices from the parameters
@@ -24,7 +24,7 @@ def to_lib(pf: O.Pftcc, cps, pfrom=1, memoize=True):
     nptcls = pf.ptcls.shape[0]
     prm = Params(boxmatch=pf.grid.ldim, kfromto=(pf.kfrom, pf.grid.kto), kstop=pf.kstop,
                  pftsz=pf.pftsz, ctf=cps is not None, match_filt=pf.match_filt, pssnr=pf.pssnr,
-                 objfun=PFTCC_OBJFUN_EUCLID if pf.euclid else PFTCC_OBJFUN_CC)
+                 objfun=PFTCC_OBJFUN_EUCLID if pf.euclid else PFTCC_OBJFUN_CC, nthr=nthr)
     eo = np.where(pf.iseven, 0, 1).astype(np.int32)
     lib = PolarFTCorrCalc().new(nrefs, (pfrom, pfrom + nptcls - 1), prm, eoarr=eo)
     lib.set_ref_pfts(1, pf.refs_even, True)

@@ -123,3 +123,15 @@ def test_fortran_module_parity(tmp_path):
         assert (h_br[i], h_bc[i], h_ne[i], bool(h_fb[i])) == (h[3][0] + 1, h[0][h[3][0]], h[4], h[5])
     assert np.sum(s_rot > 0) >= 6          # random pairs: most improve on the start
     lib.kill()
+
+
+def test_fortran_openmp_per_pair_calls():
+    """gencorrs, gencorr_grad_for_rot_8 and gencorr_cont_grad_cc_for_rot_8 issued from 6 OpenMP
+    threads (as simple_strategy3D_matcher.f90:204 does) return exactly the serial results; the
+    driver also calls the rest of the wrapped surface (cp_refs, assign_pinds, calc_ptcl_pspec,
+    the objective-specific and Euclid cont forms)."""
+    say("fortran driver: omp")
+    r = subprocess.run([DRIVER, "omp", "6"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "parallel per-pair calls equal the serial ones" in r.stdout

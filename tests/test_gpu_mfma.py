@@ -14,11 +14,13 @@ import pytest
 import pftcc_oracle as O
 from cases import make_case
 from gpu_helpers import peak_rel_err, say, to_lib, top2_gap
-from pftcc import PFTCC_PREC_BF16, PFTCC_PREC_FP8, PFTCC_PREC_SPLIT_BF16, PftccError
+from pftcc import (PFTCC_PREC_BF16, PFTCC_PREC_FP8, PFTCC_PREC_FP16, PFTCC_PREC_SPLIT_BF16, PFTCC_PREC_SPLIT_FP16,
+                   PftccError)
 
 pytestmark = pytest.mark.gpu
 
 MFMA_TOL = 1e-4
+FP16_TOL = 1e-3
 
 
 def oracle_cube(pf, ptcl_idx, ref_idx):
@@ -63,6 +65,15 @@ def test_mfma_exact_for_ring_constant_ctf(n, kfrom, kto, ldim, ctf, astig):
     say(f"  split-bf16 max peak-relative error {e:.2e}")
     lo = lib.gencorrs_mfma_batch(ids, prec=PFTCC_PREC_BF16)
     say(f"  plain bf16 max peak-relative error {max(peak_rel_err(lo[p], ref[p]) for p in range(7)):.2e}")
+    # split FP16 (about 22 significant bits per operand): the split-BF16 bar
+    e16s = check_cube(lib.gencorrs_mfma_batch(ids, prec=PFTCC_PREC_SPLIT_FP16), ref)
+    say(f"  split-fp16 max peak-relative error {e16s:.2e}")
+    # FP16, one product (11 significant bits per operand, rounding errors of ~2^-12 that average
+    # over the 2 K terms of each angular frequency and the 2n of the inverse transform)
+    h = lib.gencorrs_mfma_batch(ids, prec=PFTCC_PREC_FP16)
+    e16 = max(peak_rel_err(h[p], ref[p]) for p in range(7))
+    say(f"  fp16 max peak-relative error {e16:.2e}")
+    assert e16 < FP16_TOL, e16
     lib.kill()
 
 
@@ -137,7 +148,7 @@ def test_mfma_rejects_unsupported():
     pf, cps = make_case(seed=44, n=40, kto=12, ctf=False, nrefs=2, nptcls=2)
     lib = to_lib(pf, cps)
     with pytest.raises(PftccError):
-        lib.search_mfma(np.arange(1, 3), prec=2)
+        lib.search_mfma(np.arange(1, 3), prec=5)
     with pytest.raises(PftccError):
         lib.search_mfma(np.array([5], np.int32))
     with pytest.raises(PftccError):
@@ -145,11 +156,12 @@ def test_mfma_rejects_unsupported():
     lib.kill()
 
 
-@pytest.mark.parametrize("astig", [0.0, 0.3])
-def test_mfma_rescored_search_is_exact(astig):
+@pytest.mark.parametrize("astig,prec", [(0.0, PFTCC_PREC_SPLIT_BF16), (0.3, PFTCC_PREC_SPLIT_BF16),
+                                        (0.3, PFTCC_PREC_FP16), (0.3, PFTCC_PREC_SPLIT_FP16)])
+def test_mfma_rescored_search_is_exact(astig, prec):
     """ntop > 0: the candidates are rescored in FP64 with the particle's own |CTF|, so the result is
     the oracle's exact argmax (astigmatic CTF included) and its cc the oracle's FP64 value."""
-    say(f"mfma rescored search astig={astig}")
+    say(f"mfma rescored search astig={astig} prec={prec}")
     pf, cps = make_case(seed=45, n=192, kfrom=2, kto=40, ctf=True, astig=astig, nrefs=12, nptcls=9, ldim=128,
                         eo_mixed=True)
     # plant rotated, CTF-multiplied copies of references so that each particle has a clear winner
@@ -164,14 +176,14 @@ def test_mfma_rescored_search_is_exact(astig):
     lib = to_lib(pf, cps)
     ids = np.arange(1, 10)
     ref = oracle_cube(pf, range(9), range(12))
-    cc, ir, rot = lib.search_mfma(ids, ntop=4)
+    cc, ir, rot = lib.search_mfma(ids, ntop=4, prec=prec)
     pcc, pir, _, prot = lib.search(ids)
     for i in range(9):
         r, t = np.unravel_index(int(np.argmax(ref[i])), ref[i].shape)
         assert (ir[i], rot[i]) == (r + 1, t + 1)
         assert abs(cc[i] - ref[i][r, t]) <= 1e-7
         assert (pir[i], prot[i]) == (ir[i], rot[i])
-    raw_cc, raw_ir, raw_rot = lib.search_mfma(ids, ntop=0)
+    raw_cc, raw_ir, raw_rot = lib.search_mfma(ids, ntop=0, prec=prec)
     say(f"  raw vs exact max |dcc| {np.max(np.abs(raw_cc - cc)):.2e}")
     with pytest.raises(PftccError):
         lib.search_mfma(ids, ntop=65)

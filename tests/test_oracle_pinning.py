@@ -454,3 +454,95 @@ def test_gencorrs_refs_fft_equals_per_pair():
             for r in range(5):
                 assert np.max(np.abs(many[r] - pf.gencorrs(r, i, None, method="direct"))) < 1e-12
             pf.ctfmats = saved
+
+
+# -------------------------------------------- orientation derivatives (gencorr_cont_*) ---
+def _cont_case(seed):
+    """kstop = kto, so the unweighted cont objective equals the shifted gencorrs normalisation."""
+    pf, _ = make_case(seed=seed, n=16, kfrom=2, kto=7, ctf=True, nrefs=3, nptcls=4, ldim=32)
+    pf.refs_even = pf.refs_even.astype(np.complex128)
+    pf.refs_odd = pf.refs_odd.astype(np.complex128)
+    pf.ptcls = pf.ptcls.astype(np.complex128)
+    rng = np.random.default_rng(seed + 1)
+    drefs = random_pfts(rng, 3, pf.ks, pf.pftsz).astype(np.complex128)
+    return pf, drefs
+
+
+def _cc_moved(pf, iref, i, sh, irot, dref, eps):
+    refs = pf.refs_even if pf.iseven[i] else pf.refs_odd
+    saved = refs[iref].copy()
+    refs[iref][:pf.ks] += eps * dref
+    try:
+        return pf.gencorrs(iref, i, sh)[irot - 1]
+    finally:
+        refs[iref][:] = saved
+
+
+@pytest.mark.parametrize("irot", [1, 6, 17, 25])
+def test_cont_grad_equals_fd_of_reference_motion(irot):
+    """gencorr_cont_grad_cc_for_rot_8 (:1735-1786) with the T1 particle equal to the scored one:
+    cc equals the (unweighted, shifted) gencorrs value at irot, and dcc_d is the derivative of
+    that cc when the reference moves along dref_d (central differences, eps 1e-4); the shift
+    terms of the shift form are the derivatives over the shift (h 1e-5 px).  A dropped T2 term,
+    a wrong sign or a CTF/shift factor missing on dq breaks these."""
+    pf, drefs = _cont_case(101)
+    i = iref = 1
+    sh = (0.7, -1.1)
+    cc, dcc = pf.cont_grad_cc_for_rot_8(iref, i, sh, irot, drefs)
+    assert abs(cc - pf.gencorrs(iref, i, sh)[irot - 1]) < 1e-12
+    eps = 1e-4
+    for d in range(3):
+        fd = (_cc_moved(pf, iref, i, sh, irot, drefs[d], eps) - _cc_moved(pf, iref, i, sh, irot, drefs[d], -eps)) / (2 * eps)
+        assert abs(dcc[d] - fd) <= 1e-7 * max(1.0, abs(fd)), (d, dcc[d], fd)
+    f, g = pf.cont_shift_grad_cc_for_rot_8(iref, i, sh, irot, drefs)
+    assert f == cc and np.array_equal(g[:3], dcc)
+    h = 1e-5
+    for d in range(2):
+        sp = list(sh)
+        sm = list(sh)
+        sp[d] += h
+        sm[d] -= h
+        fd = (pf.gencorrs(iref, i, tuple(sp))[irot - 1] - pf.gencorrs(iref, i, tuple(sm))[irot - 1]) / (2 * h)
+        assert abs(g[3 + d] - fd) <= 1e-6 * max(1.0, abs(fd)), (d, g[3 + d], fd)
+
+
+def test_cont_t1_reads_particle_slot_iref():
+    """The reference passes iref as the particle of the T1 terms (:1781, :1836).  With particle
+    slot iref holding a copy of the scored particle the derivative is the true one; with other
+    contents it is not, and it follows that slot."""
+    pf, drefs = _cont_case(103)
+    i, iref, irot, sh = 0, 2, 9, (0.3, 0.4)
+    pf.iseven[:] = True
+    fd = [(_cc_moved(pf, iref, i, sh, irot, drefs[d], 1e-4) - _cc_moved(pf, iref, i, sh, irot, drefs[d], -1e-4)) / 2e-4
+          for d in range(3)]
+    _, d_other = pf.cont_grad_cc_for_rot_8(iref, i, sh, irot, drefs)
+    assert np.max(np.abs(d_other - fd)) > 1e-3
+    pf.ptcls[iref] = pf.ptcls[i]
+    _, d_same = pf.cont_grad_cc_for_rot_8(iref, i, sh, irot, drefs)
+    assert np.max(np.abs(d_same - fd)) < 1e-7 * max(1.0, np.max(np.abs(fd)))
+
+
+def test_calc_ptcl_pspec_sums_to_sqsum():
+    """calc_ptcl_pspec (:897-904) times pftsz, summed over kfrom..kstop, is memoize_sqsum_ptcl's
+    particle norm (:841-845); a full-ring (2 pftsz) or a per-rotation mean breaks it."""
+    pf, _ = make_case(seed=105, n=16, kto=9, kstop=7, nrefs=2, nptcls=3)
+    for i in range(3):
+        ps = pf.calc_ptcl_pspec(i)
+        assert ps.shape == (pf.grid.nk,)
+        assert abs(np.sum(ps[:pf.ks]) * pf.pftsz - pf.sqsum_ptcl(i)) < 1e-9 * pf.sqsum_ptcl(i)
+
+
+def test_objective_specific_forms_ignore_cc_objfun():
+    """gencorr_cc_* / gencorr_euclid_* (:1645-2148) fix the objective that gencorr_for_rot_8
+    otherwise takes from cc_objfun (:1632-1643)."""
+    pf, _ = make_case(seed=107, n=16, kto=7, nrefs=2, nptcls=3, euclid=True)
+    e = pf.gencorr_for_rot_8(1, 2, (0.5, -0.2), 5)
+    assert pf.gencorr_obj_for_rot_8(1, 2, (0.5, -0.2), 5, euclid=True) == e
+    c = pf.gencorr_obj_for_rot_8(1, 2, (0.5, -0.2), 5, euclid=False)
+    assert pf.euclid and c != e and abs(c) <= 1.0
+    pf.euclid = False
+    assert pf.gencorr_for_rot_8(1, 2, (0.5, -0.2), 5) == c
+    f, g = pf.gencorr_grad_for_rot_8(1, 2, (0.5, -0.2), 5)
+    pf.euclid = True
+    f2, g2 = pf.gencorr_obj_grad_for_rot_8(1, 2, (0.5, -0.2), 5, euclid=False)
+    assert f2 == f and np.array_equal(g, g2)
