@@ -2404,7 +2404,7 @@ int pftcc_set_dref_pft(pftcc_t* h, int32_t iref, const float* dpft, int32_t isev
 }
 
 // y, S_q, T1[3], T2[3], gx, gy of k_cont8 for one (iref, iptcl, shift, irot).
-static int cont8_single(pftcc* h, int iref, int iptcl, const double shvec[2], int irot, double out[9]) {
+static int cont8_single(pftcc* h, int iref, int iptcl, const double shvec[2], int irot, double out[kCont8Sums]) {
   int rc, ip;
   if ((rc = check_dslot(h, iref)) || (rc = local_index(h, iptcl, &ip)) || (rc = check_rot(h, irot))) return rc;
   if (iref > h->nptcls)
@@ -2417,7 +2417,7 @@ static int cont8_single(pftcc* h, int iref, int iptcl, const double shvec[2], in
   } in{iref - 1, ip, iref - 1, irot - 1, {shvec ? shvec[0] : 0.0, shvec ? shvec[1] : 0.0}};
   if ((rc = scratch_reserve(h, 4096))) return rc;
   char* d_in = take<char>(h, sizeof in);
-  double* d_out = take<double>(h, 9);
+  double* d_out = take<double>(h, kCont8Sums);
   HIPCHK(h, hipMemcpyAsync(d_in, &in, sizeof in, hipMemcpyHostToDevice, h->stream));
   const int* di = reinterpret_cast<const int*>(d_in);
   Cont8Args a{};
@@ -2455,7 +2455,7 @@ static int cont8_single(pftcc* h, int iref, int iptcl, const double shvec[2], in
   const int nt = std::min(256, (h->n + 63) / 64 * 64);
   hipLaunchKernelGGL(k_cont8, dim3(1), dim3(nt), 0, h->stream, a);
   HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipMemcpyAsync(out, d_out, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(out, d_out, kCont8Sums * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return PFTCC_OK;
 }
@@ -2464,7 +2464,7 @@ int pftcc_gencorr_cont_grad_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t iptcl
                                          int32_t irot, double* cc, double dcc[3]) {
   if (!h || !cc || !dcc) return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
-  double o[9];
+  double o[kCont8Sums];
   int rc, ip;
   if ((rc = cont8_single(h, iref, iptcl, shvec, irot, o))) return rc;
   (void)local_index(h, iptcl, &ip);
@@ -2481,7 +2481,7 @@ int pftcc_gencorr_cont_shift_grad_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t
                                                int32_t irot, double* f, double grad[5]) {
   if (!h || !f || !grad) return PFTCC_E_ARG;
   std::lock_guard<std::mutex> g(h->mtx);
-  double o[9];
+  double o[kCont8Sums];
   int rc, ip;
   if ((rc = cont8_single(h, iref, iptcl, shvec, irot, o))) return rc;
   (void)local_index(h, iptcl, &ip);
@@ -2491,8 +2491,8 @@ int pftcc_gencorr_cont_shift_grad_cc_for_rot_8(pftcc_t* h, int32_t iref, int32_t
   const double denom = sqrt(o[1] * (double)sp);
   *f = o[0] / denom;
   for (int d = 0; d < 3; ++d) grad[d] = (o[2 + d] - o[0] * o[5 + d] / o[1]) / denom;
-  grad[3] = o[7] / denom;
-  grad[4] = o[8] / denom;
+  grad[3] = o[8] / denom;
+  grad[4] = o[9] / denom;
   return PFTCC_OK;
 }
 

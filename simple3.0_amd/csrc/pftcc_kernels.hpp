@@ -704,8 +704,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PFTCC_
 //   T1d = sum_k Re sum_j dq_d conj(P_it1 rotated),       T2d = sum_k Re sum_j dq_d conj(q),
 //   gx  = Re sum (i a_x q) conj(P_ip rotated),            gy likewise with a_y,
 // where the T1 particle is it1, which the reference takes from its iref argument
-// (calc_T1_T2_for_rot_8(pft_ref, pft_dref, iref, ...), :1781, :1836).  out[9] = {y, S_q, T1[3],
-// T2[3], gx, gy} per item, reduced in a fixed order.  One work-group per item.
+// (calc_T1_T2_for_rot_8(pft_ref, pft_dref, iref, ...), :1781, :1836).  out[kCont8Sums] = {y, S_q,
+// T1[3], T2[3], gx, gy} per item, reduced in a fixed order.  One work-group per item.
+constexpr int kCont8Sums = 10;
 struct Cont8Args {
   int n, nk, kcorr, kfrom, nrefs, ndslots;
   const float2* refs_even;
@@ -731,7 +732,7 @@ struct Cont8Args {
   const double* shifts;        // [nitems][2]
   const int* rhos;             // 0-based rotation
   int nitems;
-  double* out;                 // [nitems][9]
+  double* out;                 // [nitems][kCont8Sums]
 };
 
 // Rotated particle value P[(j + rho) mod 2n] on the half ring (conjugated past n).
@@ -757,7 +758,7 @@ __global__ void __launch_bounds__(256) k_cont8(Cont8Args a) {
   const float2* ptc = a.ptcls + (size_t)ip * a.nk * n;
   const float2* pt1 = a.ptcls + (size_t)it1 * a.nk * n;
   const float* ctf = a.ctf ? a.ctf + (size_t)ip * a.nk * n : nullptr;
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double acc[kCont8Sums] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     double sa, ca, ss, cs;
     sincos(a.argx[j] * sx + a.argy[j] * sy, &sa, &ca);
@@ -780,8 +781,8 @@ __global__ void __launch_bounds__(256) k_cont8(Cont8Args a) {
       acc[1] += qr * qr + qi * qi;
       const double im = qi * pr - qr * pim;          // Im(q conj P); Re((i a q) conj P) = -a Im(q conj P)
       const float kf = (float)(a.kfrom + kk);
-      acc[7] += -((double)(a.sn32[j] * kf) * a.arg_a) * im;
-      acc[8] += -((double)(a.ncn32[j] * kf) * a.arg_a) * im;
+      acc[8] += -((double)(a.sn32[j] * kf) * a.arg_a) * im;
+      acc[9] += -((double)(a.ncn32[j] * kf) * a.arg_a) * im;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const float2 dv = dref[((size_t)d * a.kcorr + kk) * n + j];
@@ -795,20 +796,20 @@ __global__ void __launch_bounds__(256) k_cont8(Cont8Args a) {
       shi = i2;
     }
   }
-  __shared__ double red[9][4];
+  __shared__ double red[kCont8Sums][4];
 #pragma unroll
-  for (int v = 0; v < 9; ++v) {
+  for (int v = 0; v < kCont8Sums; ++v) {
     double x = acc[v];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
     if ((threadIdx.x & 63) == 0) red[v][threadIdx.x >> 6] = x;
   }
   __syncthreads();
-  if (threadIdx.x < 9) {
+  if (threadIdx.x < kCont8Sums) {
     const int nw = (blockDim.x + 63) >> 6;
     double x = red[threadIdx.x][0];
     for (int w = 1; w < nw; ++w) x += red[threadIdx.x][w];
-    a.out[(size_t)item * 9 + threadIdx.x] = x;
+    a.out[(size_t)item * kCont8Sums + threadIdx.x] = x;
   }
 }
 
