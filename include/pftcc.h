@@ -241,6 +241,22 @@ int pftcc_search(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, cons
 int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr,
                      const int32_t* d_irefs, int32_t ns, const float* d_shifts, float* d_best_cc,
                      int32_t* d_best_iref, int32_t* d_best_ishift, int32_t* d_best_irot);
+/* pftcc_search_dev that allocates nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as pftcc_search_dev; it
+ * calls no hipMalloc, hipFree or other allocating routine.  Its working memory (index lists, the
+ * shift table, the per-particle best keys) is a static device array of 32 MiB per GPU, held
+ * until the process ends; a call needing more (about np * 16 + nr * 4 + max(ns,1) *
+ * (nk * pftsz * 8 + 16) bytes, so at most ~150 shifts at box 200) returns PFTCC_E_ARG.  Every
+ * call on one GPU shares that array and leaves the previous call's contents in it: calls that
+ * overlap on one GPU (several handles, several host threads, several streams) race and must be
+ * serialised by the caller.  With match_filt or pssnr the shell-normalisation tables must be
+ * current (PFTCC_E_STATE otherwise): one pftcc_search or pftcc_search_dev call after the last
+ * reference or particle change builds them. */
+int pftcc_search_dev_noalloc(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr,
+                             const int32_t* d_irefs, int32_t ns, const float* d_shifts,
+                             float* d_best_cc, int32_t* d_best_iref, int32_t* d_best_ishift,
+                             int32_t* d_best_irot);
 /* The reference-loop search strategies for a batch of particles, each particle against all nrefs
  * references with gencorrs (:1594, unshifted) and maxloc over the rotations, then:
  *   PFTCC_SRCH_GREEDY2D  strategy2D_greedy%srch (simple_strategy2D_greedy.f90:33-58): references

@@ -161,6 +161,11 @@ struct pftcc {
   // scratch arena (stream ordered; calls on a handle are serialised by mtx)
   char* d_scratch = nullptr;
   size_t scratch_cap = 0, scratch_off = 0;
+  // device copy of pinds (global id - pfrom -> local 1-based slot), kept current by new and
+  // reallocate_ptcls, and this GPU's address of the no-allocation search's static scratch
+  int* d_pinds = nullptr;
+  size_t d_pinds_cap = 0;
+  unsigned char* noalloc_scratch = nullptr;
   std::mutex mtx;
   std::string err;
 };
@@ -361,6 +366,50 @@ void free_ptcl_arrays(pftcc* h) {
 int upload_iseven(pftcc* h) {
   HIPCHK(h, hipMemcpyAsync(h->d_iseven, h->iseven.data(), h->nptcls, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PFTCC_OK;
+}
+
+// Device copy of pinds, refreshed whenever the host table changes (new, reallocate_ptcls).
+int upload_pinds(pftcc* h) {
+  const size_t span = h->pinds.size();
+  if (span > h->d_pinds_cap) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(h->d_pinds);
+    h->d_pinds_cap = 0;
+    int rc;
+    if ((rc = dalloc(h, &h->d_pinds, span))) return rc;
+    h->d_pinds_cap = span;
+  }
+  HIPCHK(h, sync_copy(h, h->d_pinds, h->pinds.data(), span * sizeof(int), hipMemcpyHostToDevice));
+  return PFTCC_OK;
+}
+
+// ------------------------------------------------------- scratch of the no-allocation search ---
+// pftcc_search_dev_noalloc keeps its working memory (index lists, the shift table, the per-particle
+// best keys) in this file-scope array instead of allocating: every GPU the library runs on holds
+// one copy, at its own address, until the process ends.  Host code never passes the array's name
+// to a kernel or a copy (on the host it is only a stand-in for the symbol): pftcc_new resolves
+// each device's address once with hipGetSymbolAddress, and the entry point uses that address.
+constexpr size_t kNoallocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+std::mutex g_scratch_mtx;
+std::vector<std::pair<int, unsigned char*>> g_scratch_addr;   // (device, address of its scratch)
+
+// With h->device current: this device's address of scratch, resolved on first use.
+int resolve_noalloc_scratch(pftcc* h) {
+  std::lock_guard<std::mutex> g(g_scratch_mtx);
+  for (const auto& e : g_scratch_addr)
+    if (e.first == h->device) {
+      h->noalloc_scratch = e.second;
+      return PFTCC_OK;
+    }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(h, PFTCC_E_DEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", h->device,
+                hipGetErrorString(e));
+  g_scratch_addr.emplace_back(h->device, (unsigned char*)p);
+  h->noalloc_scratch = (unsigned char*)p;
   return PFTCC_OK;
 }
 
@@ -1258,6 +1307,7 @@ int pftcc_new(const pftcc_cfg* cfg, int32_t nrefs, int32_t pfrom, int32_t pto, c
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bad(fail(h, PFTCC_E_DEVICE, "no HIP device"));
   build_host_grid(h);
   int rc;
+  if ((rc = resolve_noalloc_scratch(h)) || (rc = upload_pinds(h))) return bad(rc);
   const size_t ring = (size_t)h->nk * h->n;
   if ((rc = dalloc(h, &h->d_refs_e, (size_t)nrefs * ring)) || (rc = dalloc(h, &h->d_refs_o, (size_t)nrefs * ring)))
     return bad(rc);
@@ -1377,6 +1427,7 @@ int pftcc_kill(pftcc_t* h) {
   dfree(h->d_drefs_e);
   dfree(h->d_drefs_o);
   for (auto& ap : h->apack) free_apack(ap);
+  dfree(h->d_pinds);
   if (h->d_scratch) (void)hipFree(h->d_scratch);
   delete h;
   return PFTCC_OK;
@@ -1887,7 +1938,7 @@ int pftcc_reallocate_ptcls(pftcc_t* h, int32_t nptcls, const int32_t* pinds) {
   for (int i = 0; i < nptcls; ++i) h->pinds[pinds[i] - lo] = i + 1;
   h->memo_valid = h->dptcls_valid = h->sqp8_valid = false;
   h->refnorm_valid = false;
-  return PFTCC_OK;
+  return upload_pinds(h);
 }
 
 int pftcc_set_eo(pftcc_t* h, int32_t iptcl, int32_t is_even) {
@@ -2613,9 +2664,8 @@ int pftcc_gencorrs_batch(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t 
 }
 
 static int search_impl(pftcc_t* h, int np, const int* d_pl, int nr, const int* d_rl, int ns, const float2* d_tab,
-                       float* d_cc, int* d_ir, int* d_is, int* d_rot) {
+                       unsigned long long* d_best, float* d_cc, int* d_ir, int* d_is, int* d_rot) {
   int rc;
-  unsigned long long* d_best = take<unsigned long long>(h, np);
   hipLaunchKernelGGL(k_fill_u64, dim3((np + 255) / 256), dim3(256), 0, h->stream, d_best, np, 0ull);
   HIPCHK(h, hipGetLastError());
   if ((rc = launch_corr(h, np, d_pl, nr, d_rl, std::max(ns, 1), d_tab, ns > 0, nullptr, d_best))) return rc;
@@ -2661,7 +2711,8 @@ int pftcc_search(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr, cons
     HIPCHK(h, hipMemcpyAsync(d_sh, sh.data(), sh.size() * 8, hipMemcpyHostToDevice, h->stream));
     if ((rc = shift_table(h, ns, d_sh, d_tab))) return rc;
   }
-  if ((rc = search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, d_cc, d_ir, d_is, d_rot))) return rc;
+  if ((rc = search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, take<unsigned long long>(h, np), d_cc, d_ir, d_is, d_rot)))
+    return rc;
   HIPCHK(h, hipMemcpyAsync(best_cc, d_cc, np * 4, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipMemcpyAsync(best_iref, d_ir, np * 4, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipMemcpyAsync(best_ishift, d_is, np * 4, hipMemcpyDeviceToHost, h->stream));
@@ -2736,7 +2787,74 @@ int pftcc_search_dev(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr
   HIPCHK(h, hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (bad) return fail(h, PFTCC_E_ARG, "search_dev: particle or reference index out of range");
-  return search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, d_best_cc, d_best_iref, d_best_ishift, d_best_irot);
+  return search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, take<unsigned long long>(h, np), d_best_cc, d_best_iref,
+                     d_best_ishift, d_best_irot);
+}
+
+int pftcc_search_dev_noalloc(pftcc_t* h, int32_t np, const int32_t* d_iptcls, int32_t nr, const int32_t* d_irefs,
+                             int32_t ns, const float* d_shifts, float* d_best_cc, int32_t* d_best_iref,
+                             int32_t* d_best_ishift, int32_t* d_best_irot) {
+  if (!h || !d_iptcls || np < 1 || nr < 1 || ns < 0 || !d_best_cc || !d_best_iref || !d_best_ishift || !d_best_irot)
+    return PFTCC_E_ARG;
+  if (ns > 0 && !d_shifts) return PFTCC_E_ARG;
+  std::lock_guard<std::mutex> g(h->mtx);
+  if (!d_irefs && nr != h->nrefs) return fail(h, PFTCC_E_ARG, "nr must equal nrefs when irefs is NULL");
+  if (!h->noalloc_scratch || !h->d_pinds) return fail(h, PFTCC_E_STATE, "search_dev_noalloc: handle not set up");
+  // the state checks of need_memo, without building anything: the shell-normalisation tables
+  // (match_filt / pssnr) are built by the allocating entry points
+  if (!h->memo_valid)
+    return fail(h, PFTCC_E_STATE, "particle FFTs are not memoized: call memoize_ffts after setting particles");
+  if (h->with_ctf && !h->ctf_ready)
+    return fail(h, PFTCC_E_STATE, "CTF matrices missing: call create_polar_absctfmats");
+  if (h->euclid && !h->have_sigma2) return fail(h, PFTCC_E_STATE, "objfun euclid needs assign_sigma2_noise");
+  if ((h->match_filt || h->pssnr) && !h->refnorm_valid)
+    return fail(h, PFTCC_E_STATE,
+                "search_dev_noalloc: the shell-normalisation tables are stale; one pftcc_search or "
+                "pftcc_search_dev call after the last reference or particle change builds them");
+  const int nse = std::max(ns, 1);
+  const size_t ring = (size_t)h->nk * h->n;
+  const size_t need = aligned((size_t)np * 4) + aligned((size_t)nr * 4) + aligned((size_t)nse * 16) +
+                      aligned((size_t)nse * ring * 8) + aligned(64) + aligned((size_t)np * 8);
+  if (need > kNoallocScratchBytes)
+    return fail(h, PFTCC_E_ARG,
+                "search_dev_noalloc: %d particles x %d references x %d shifts need %zu bytes of scratch, "
+                "the maximum is %zu (fewer shifts or particles per call)",
+                np, nr, nse, need, kNoallocScratchBytes);
+  // carve the static scratch as scratch_reserve/take carve the arena
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    unsigned char* p = h->noalloc_scratch + off;
+    off += aligned(bytes);
+    return p;
+  };
+  int* d_pl = (int*)carve((size_t)np * 4);
+  int* d_rl = (int*)carve((size_t)nr * 4);
+  double* d_sh = (double*)carve((size_t)nse * 16);
+  float2* d_tab = (float2*)carve((size_t)nse * ring * 8);
+  int* d_bad = (int*)carve(64);
+  unsigned long long* d_best = (unsigned long long*)carve((size_t)np * 8);
+  const int span = h->pto - h->pfrom + 1;
+  // the previous call's contents remain: the flag is cleared by a fill kernel (no memset node)
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, h->stream, (unsigned int*)d_bad, 16, 0u);
+  hipLaunchKernelGGL(k_translate, dim3((np + 255) / 256), dim3(256), 0, h->stream, d_iptcls, np,
+                     (const int*)h->d_pinds, h->pfrom, span, d_pl, d_bad);
+  hipLaunchKernelGGL(k_refs0, dim3((nr + 255) / 256), dim3(256), 0, h->stream, d_irefs, nr, h->nrefs, d_rl, d_bad);
+  HIPCHK(h, hipGetLastError());
+  int rc;
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_f2d, dim3((2 * ns + 255) / 256), dim3(256), 0, h->stream, d_shifts, 2 * ns, d_sh);
+    if ((rc = shift_table(h, ns, d_sh, d_tab))) return rc;
+  }
+  int bad = 0;
+  HIPCHK(h, hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (bad) return fail(h, PFTCC_E_ARG, "search_dev_noalloc: particle or reference index out of range");
+  // kernel timing creates its events lazily, so it stays off for this call
+  const bool timed = h->timing;
+  h->timing = false;
+  rc = search_impl(h, np, d_pl, nr, d_rl, ns, d_tab, d_best, d_best_cc, d_best_iref, d_best_ishift, d_best_irot);
+  h->timing = timed;
+  return rc;
 }
 
 int pftcc_strategy_srch(pftcc_t* h, const pftcc_strategy_cfg* cfg, int32_t np, const int32_t* iptcls,

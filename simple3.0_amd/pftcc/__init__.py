@@ -155,6 +155,7 @@ SIGNATURES = [
     ("pftcc_strategy_srch", _I, [_P, C.POINTER(StrategyCfgC), _I, _PI, _PI, _P, _PF, _PF, _PF, _PI, _PI, _PI, _PI,
                                  _PF, _PI, _PI]),
     ("pftcc_search_dev", _I, [_P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P]),
+    ("pftcc_search_dev_noalloc", _I, [_P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P]),
     ("pftcc_grad_batch", _I, [_P, _I, _PI, _PI, _PD, _PI, _PD, _PD]),
     ("pftcc_grad_batch_dev", _I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     ("pftcc_prep_matchfilt", _I, [_P, _I, _I, _I]),
@@ -694,6 +695,15 @@ class PolarFTCorrCalc:
                                              C.c_void_p(d_shifts) if d_shifts else None,
                                              C.c_void_p(d_cc), C.c_void_p(d_iref),
                                              C.c_void_p(d_ishift), C.c_void_p(d_irot)), "search_dev")
+
+    def search_dev_noalloc(self, np_, d_iptcls, nr, d_irefs, ns, d_shifts, d_cc, d_iref, d_ishift, d_irot):
+        """pftcc_search_dev without device allocation (static per-GPU scratch of 32 MiB)."""
+        self._chk(self._lib.pftcc_search_dev_noalloc(self._h, np_, C.c_void_p(d_iptcls), nr,
+                                                     C.c_void_p(d_irefs) if d_irefs else None, ns,
+                                                     C.c_void_p(d_shifts) if d_shifts else None,
+                                                     C.c_void_p(d_cc), C.c_void_p(d_iref),
+                                                     C.c_void_p(d_ishift), C.c_void_p(d_irot)),
+                  "search_dev_noalloc")
 
     def grad_batch(self, irefs, iptcls, shvecs, irots, with_grad=True):
         r = np.ascontiguousarray(irefs, np.int32)
