@@ -58,7 +58,7 @@ def mfma_flops_per_pair(n, kcorr):
 def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
     """Split-BF16 MFMA form at the 3D-refinement shape, next to the plain FP32 form on the same data."""
     import pftcc
-    from pftcc import Params, PolarFTCorrCalc, PFTCC_PREC_SPLIT_BF16
+    from pftcc import Params, PolarFTCorrCalc, PFTCC_PREC_FP16, PFTCC_PREC_SPLIT_BF16, PFTCC_PREC_SPLIT_FP16
     from pftcc import synth
 
     box, smpd, kfromto = 256, 1.06, (2, 80)
@@ -89,6 +89,22 @@ def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
     wall = (time.perf_counter() - t0) / steps
     ch = {name: lib.get_timing_channel(c) for c, name in ((1, "zgemm"), (2, "zifft"), (3, "pack_rescore"))}
     lib.set_timing(False)
+    # the FP16 operand formats on the same search (one step each, reference operand rebuilt)
+    others = {}
+    for nm, pr in (("fp16", PFTCC_PREC_FP16), ("split_fp16", PFTCC_PREC_SPLIT_FP16)):
+        lib.search_mfma(ids[:256], prec=pr, ntop=ntop)
+        S.synth_device_sync()
+        lib.set_timing(True)
+        t0 = time.perf_counter()
+        lib.set_ref_pfts_dev(1, nrefs, rbuf_refs.ptr, True)
+        lib.set_ref_pfts_dev(1, nrefs, rbuf_refs.ptr, False)
+        occ, oir, orot = lib.search_mfma(ids, prec=pr, ntop=ntop)
+        S.synth_device_sync()
+        owall = time.perf_counter() - t0
+        others[nm] = {"value": nptcls * nrefs * nrots / owall, "unit": "evals/s", "ms_per_step": 1e3 * owall,
+                      "contraction_ms": lib.get_timing_channel(1)[0],
+                      "assignment_agreement_vs_split_bf16": float(np.mean((oir == ir) & (orot == rot)))}
+        lib.set_timing(False)
     rbuf_refs.close()
     value = nptcls * nrefs * nrots / wall
     raw_cc, raw_ir, raw_rot = lib.search_mfma(ids[:plain_sample], prec=PFTCC_PREC_SPLIT_BF16, ntop=0)
@@ -105,7 +121,10 @@ def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
     log("precision form: FP64 checker on a sample")
     cpu = precision_cpu_baseline(lib, nrefs, kcorr, PFTCC_PREC_SPLIT_BF16, ntop,
                                  (cc[:plain_sample], ir[:plain_sample], rot[:plain_sample]), (pir, prot),
-                                 ids[:plain_sample])
+                                 ids[:plain_sample], other_precs=(("fp16", PFTCC_PREC_FP16),
+                                                                  ("split_fp16", PFTCC_PREC_SPLIT_FP16)))
+    for nm in others:
+        others[nm]["contraction_error_vs_fp64"] = cpu["error_vs_fp64"].pop(f"contraction_{nm}")
     lib.kill()
     zg_ms, zg_n = ch["zgemm"]
     zi_ms, zi_n = ch["zifft"]
@@ -136,11 +155,14 @@ def precision_form(steps, nrefs=4684, nptcls=20000, plain_sample=2000, ntop=4):
                                 "frac": zbytes / (zi_ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS},
             "pack_topk_rescore": {"ms_per_step": ch["pack_rescore"][0] / steps},
         },
+        "fp16_operands": {**others, "note": "FP16 (one product) and split FP16 (hi+lo, three products) operands "
+                          "with per-shell and global power-of-two scales, FP32 accumulation, the same top-4 FP64 "
+                          "rescoring; contraction error against the FP64 ring-mean formula on the checker's particles"},
         "cpu_baseline": cpu,
     }
 
 
-def precision_cpu_baseline(lib, nrefs, kcorr, prec, ntop, best, plain, sample_ids, nfp64=6):
+def precision_cpu_baseline(lib, nrefs, kcorr, prec, ntop, best, plain, sample_ids, nfp64=6, other_precs=()):
     """CPU baseline of the precision form, used as its FP64 checker: the reference formula
     (gencorrs_cc_1, FFT route, FP64 numpy restatement of oracle/pftcc_oracle.py) of nfp64 particles
     against all references, timed on one host core, and the errors of the GPU's results against it.
@@ -153,15 +175,17 @@ def precision_cpu_baseline(lib, nrefs, kcorr, prec, ntop, best, plain, sample_id
     * assignments: best (reference, rotation) of the rescored and unrescored searches against the
       exact FP64 best, counted where the FP64 top-2 gap exceeds 1e-5;
     * on the plain-FP32 sample: every particle where the rescored precision form and the plain FP32
-      search disagree is rescored at both answers with the exact FP64 formula.
+      search disagree is rescored at both answers with the exact FP64 formula;
+    * other_precs (name, prec): the contraction error of those operand formats on the same particles.
     Errors are absolute, in units of cc (|cc| <= 1), and per vector relative to its FP64 peak."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pftcc_oracle as O
 
     refs = np.stack([lib.get_ref_pft(r + 1, True) for r in range(nrefs)])
     ids = np.asarray(sample_ids[:nfp64])
-    err = {"contraction": [], "ring_mean_ctf": [], "unrescored_total": []}
-    rel = {"contraction": [], "ring_mean_ctf": [], "unrescored_total": []}
+    keys = ["contraction", "ring_mean_ctf", "unrescored_total"] + [f"contraction_{nm}" for nm, _ in other_precs]
+    err = {k: [] for k in keys}
+    rel = {k: [] for k in keys}
     agree_res = agree_raw = counted = 0
     cpu_s = 0.0
     dev_rs = lib.search_mfma(ids, prec=prec, ntop=ntop)
@@ -175,7 +199,10 @@ def precision_cpu_baseline(lib, nrefs, kcorr, prec, ntop, best, plain, sample_id
         cpu_s += time.perf_counter() - t0
         ringm = O.gencorrs_refs_fft(refs, pt, ctf_rm, kcorr)
         dev = lib.gencorrs_mfma_batch([int(i)], prec=prec)[0].astype(np.float64)
-        for key, a, b in (("contraction", dev, ringm), ("ring_mean_ctf", ringm, exact), ("unrescored_total", dev, exact)):
+        pairs = [("contraction", dev, ringm), ("ring_mean_ctf", ringm, exact), ("unrescored_total", dev, exact)]
+        for nm, pr in other_precs:
+            pairs.append((f"contraction_{nm}", lib.gencorrs_mfma_batch([int(i)], prec=pr)[0].astype(np.float64), ringm))
+        for key, a, b in pairs:
             d = np.abs(a - b)
             err[key].append(d.ravel())
             rel[key].append(np.max(d, axis=1) / np.maximum(np.max(np.abs(b), axis=1), 1e-30))
@@ -221,6 +248,136 @@ def precision_cpu_baseline(lib, nrefs, kcorr, prec, ntop, best, plain, sample_id
         "plain_fp32_sample_disagreements": {"count": nd, "fp64_ties_within_1e-6": ties,
                                             "precision_form_worse_in_fp64": worse,
                                             "identical_or_fp64_tie_frac": 1.0 - worse / max(1, len(pir))},
+    }
+
+
+def coarse_fine_form(nrefs=10000, nptcls=100000, sample=1000, tops=(16, 32, 64, 128, 256), fp64_refs=8):
+    """FP8 e4m3 coarse pass with top-T FP64 rescoring at the large coarse-to-fine shape (1e4
+    references x 1e5 resident particles, box 256, kfromto [2,64]: pftsz 400, K 63), next to split
+    BF16 with top-4 rescoring on the same data.
+
+    Recall is measured against the FP64-best orientation of a sample of particles: the plain FP32
+    form searches every reference at every rotation (per-reference maxima, strategy_srch greedy
+    2-D), the fp64_refs best references are rescored in FP64 (numpy, the reference's gencorrs_cc_1
+    with the exact per-pixel |CTF|) at their best rotation and both neighbours, and the best of
+    those is the FP64 best.  A search's answer counts as recalled when it is that orientation or its
+    FP64 cc is within 1e-6 of it (an FP64 tie).  The smallest T of `tops` with recall >= 99.9 % is
+    then timed over all particles (one call, particles resident in HBM, rescoring included), and so
+    is split BF16 with top-4 rescoring."""
+    import pftcc
+    from pftcc import Params, PolarFTCorrCalc, PFTCC_PREC_FP8, PFTCC_PREC_SPLIT_BF16, SRCH_GREEDY2D
+    from pftcc import synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pftcc_oracle as O
+
+    box, smpd, kfromto = 256, 1.06, (2, 64)
+    pftsz = pftcc.magic_pftsz(126)
+    w = synth.Workload(box=box, smpd=smpd, kfromto=kfromto, pftsz=pftsz, nrefs=nrefs, nptcls=nptcls, snr=0.03,
+                       seed=20251204, dfrange=(1.0, 3.0), astig=0.1, max_shift=0.0, uniform_dirs=True)
+    synth.make_params(w)
+    lib = PolarFTCorrCalc().new(nrefs, (1, nptcls), Params(boxmatch=box, kfromto=kfromto, pftsz=pftsz, ctf=True))
+    t0 = time.perf_counter()
+    synth.load_into(lib, w, chunk=10000)
+    log(f"coarse-to-fine setup: {nptcls} particles x {nrefs} refs in {time.perf_counter() - t0:.1f} s")
+    S = synth.synth_lib()
+    nrots = 2 * pftsz
+    kcorr = w.nk
+    ids = np.arange(1, nptcls + 1, dtype=np.int32)
+    sids = ids[:sample]
+    # FP64 best of the sample
+    t0 = time.perf_counter()
+    g = lib.strategy_srch(SRCH_GREEDY2D, sids, ref_corrs=True)
+    plain_s = time.perf_counter() - t0
+    refs = {}
+
+    def ref_ring(r):
+        if r not in refs:
+            refs[r] = lib.get_ref_pft(r, True).astype(np.complex128)
+        return refs[r]
+
+    ptc = {}
+
+    def cc64(i, iref, irot):
+        if i not in ptc:
+            pt = lib.get_ptcl_pft(int(i)).astype(np.complex128)[:kcorr]
+            ctf = lib.get_ctfmat(int(i)).astype(np.float64)[:kcorr]
+            ptc[i] = (pt, ctf, float(np.sum(np.abs(pt) ** 2)))
+        pt, ctf, sp = ptc[i]
+        q = ref_ring(iref)[:kcorr] * ctf
+        return float(O.corrk_for_rot(q, pt, irot).sum() / math.sqrt(np.sum(np.abs(q) ** 2) * sp))
+
+    t0 = time.perf_counter()
+    best64 = []
+    for q, i in enumerate(sids):
+        rc, rr = g["ref_corrs"][q], g["ref_rots"][q]
+        cands = np.argsort(-rc, kind="stable")[:fp64_refs]
+        bv, bk = -np.inf, None
+        for r0 in cands:
+            for t in ((rr[r0] - 2) % nrots + 1, rr[r0], rr[r0] % nrots + 1):
+                v = cc64(i, int(r0) + 1, int(t))
+                if v > bv:
+                    bv, bk = v, (int(r0) + 1, int(t))
+        best64.append((bk, bv))
+    fp64_s = time.perf_counter() - t0
+    plain_recall = float(np.mean([(int(g["best_ref"][q]), int(g["best_rot"][q])) == best64[q][0]
+                                  or cc64(sids[q], int(g["best_ref"][q]), int(g["best_rot"][q])) >= best64[q][1] - 1e-6
+                                  for q in range(sample)]))
+
+    def recall(ir, rot):
+        hit = 0
+        for q, i in enumerate(sids):
+            (br, bt), bv = best64[q]
+            if (int(ir[q]), int(rot[q])) == (br, bt) or (ir[q] > 0 and cc64(i, int(ir[q]), int(rot[q])) >= bv - 1e-6):
+                hit += 1
+        return hit / len(sids)
+
+    lib.search_mfma(sids[:256], prec=PFTCC_PREC_FP8, ntop=16)          # warm-up (code objects)
+    lib.search_mfma(sids[:256], prec=PFTCC_PREC_SPLIT_BF16, ntop=4)
+    sweep = {}
+    for T in tops:
+        cc, ir, rot = lib.search_mfma(sids, prec=PFTCC_PREC_FP8, ntop=T)
+        sweep[f"fp8_top{T}"] = recall(ir, rot)
+    cc, ir, rot = lib.search_mfma(sids, prec=PFTCC_PREC_FP8, ntop=0)
+    sweep["fp8_unrescored"] = recall(ir, rot)
+    cc, ir, rot = lib.search_mfma(sids, prec=PFTCC_PREC_SPLIT_BF16, ntop=4)
+    sweep["split_bf16_top4"] = recall(ir, rot)
+    ok = [T for T in tops if sweep[f"fp8_top{T}"] >= 0.999]
+    tsel = ok[0] if ok else tops[-1]
+    log(f"coarse-to-fine: recall sweep {sweep}; timing FP8 top-{tsel} over {nptcls} particles")
+
+    def timed(prec, ntop):
+        S.synth_device_sync()
+        lib.set_timing(True)
+        t0 = time.perf_counter()
+        cc, ir, rot = lib.search_mfma(ids, prec=prec, ntop=ntop)
+        S.synth_device_sync()
+        wall = time.perf_counter() - t0
+        ch = {name: lib.get_timing_channel(c)[0] for c, name in ((1, "contraction"), (2, "inverse_fft"),
+                                                                 (3, "select_rescore"))}
+        lib.set_timing(False)
+        return wall, ch, recall(ir[:sample], rot[:sample])
+
+    wall8, ch8, rec8 = timed(PFTCC_PREC_FP8, tsel)
+    wallb, chb, recb = timed(PFTCC_PREC_SPLIT_BF16, 4)
+    lib.kill()
+    evals = float(nptcls) * nrefs * nrots
+    return {
+        "workload": "FP8 e4m3 coarse pass (per-shell power-of-two scales, FP32 accumulation), the top-T references "
+                    "of each particle rescored in FP64 at their coarse best rotation and both neighbours; "
+                    "large coarse-to-fine shape, all particles resident in HBM, 256-particle operand chunks",
+        "value": evals / wall8, "unit": "evals/s (end to end, one call over all particles)",
+        "refs": nrefs, "ptcls": nptcls, "box": box, "smpd": smpd, "kfromto": list(kfromto), "pftsz": pftsz,
+        "nrots": nrots, "ctf": "astigmatic <= 0.1 um, defocus U(1,3) um", "snr": 0.03,
+        "ntop": tsel, "wall_s": wall8, "recall_vs_fp64_best": rec8, "kernel_ms": ch8,
+        "hbm_resident_GB": nptcls * kcorr * (pftsz * 8 + pftsz * 4 + (pftsz + 1) * 8) / 1e9
+                           + 2 * nrefs * kcorr * pftsz * 8 / 1e9,
+        "recall_sweep": {"sample_ptcls": sample, "fp64_best": f"plain FP32 search of every reference, the "
+                         f"{fp64_refs} best rescored in FP64 at their best rotation and both neighbours",
+                         "plain_fp32": plain_recall, **sweep},
+        "split_bf16_top4": {"value": evals / wallb, "wall_s": wallb, "recall_vs_fp64_best": recb, "kernel_ms": chb},
+        "plain_fp32_sample": {"value": sample * nrefs * nrots / plain_s, "unit": "evals/s",
+                              "note": "strategy_srch greedy 2-D with per-reference maxima, the FP64 checker's first stage"},
+        "fp64_checker_s": fp64_s,
     }
 
 
@@ -707,8 +864,9 @@ def main():
     ap.add_argument("--mfma-ptcls", type=int, default=10000,
                     help="particles of the precision-form run on the main workload")
     ap.add_argument("--no-gradient", action="store_true", help="skip the adjoint-kernel line")
-    ap.add_argument("--legs", default="plain,precision,gradient,shsrch,polarizer,prep,strategy",
-                    help="comma list of plain, precision, gradient, shsrch, polarizer, prep, strategy (profiling runs select one)")
+    ap.add_argument("--legs", default="plain,precision,coarse,gradient,shsrch,polarizer,prep,strategy",
+                    help="comma list of plain, precision, coarse, gradient, shsrch, polarizer, prep, strategy "
+                         "(profiling runs select one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device copy (k_copy), the reference for HBM traffic counters")
     args = ap.parse_args()
@@ -734,6 +892,8 @@ def main():
         out = {"metric": METRIC, "n_gpus": 1, "note": "plain leg skipped (--legs)"}
         if "precision" in legs:
             out["precision_form"] = precision_form(args.precision_steps)
+        if "coarse" in legs:
+            out["coarse_to_fine"] = coarse_fine_form()
         if "gradient" in legs:
             out["gradient_form"] = gradient_form(3)
         if "shsrch" in legs:
@@ -837,6 +997,8 @@ def main():
     lib.kill()
     if not args.no_precision and "precision" in legs:
         out["precision_form"] = precision_form(args.precision_steps)
+    if not args.no_precision and "coarse" in legs:
+        out["coarse_to_fine"] = coarse_fine_form()
     if not args.no_gradient and "gradient" in legs:
         out["gradient_form"] = gradient_form(3)
     if "shsrch" in legs:

@@ -397,7 +397,7 @@ int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const i
  * Denominators sqrt(sum_k cbar_k^2 S_ref,k * S_ptcl) are formed in FP64.  No counterpart
  * entry point in the reference; it replaces the same greedy loop as pftcc_search
  * (simple_strategy3D_greedy.f90:60-77, gencorrs over all references + maxloc).
- * ntop = 0: the best (reference, rotation) of the contraction itself.  ntop in [1, 64]: the ntop
+ * ntop = 0: the best (reference, rotation) of the contraction itself.  ntop in [1, 256]: the ntop
  * references with the highest contraction score per particle are rescored exactly in FP64 (the
  * reference's gencorrs formula with the particle's own |CTF|) at their best rotation and its two
  * neighbours, and the best rescored candidate is returned with its FP64 cc. */

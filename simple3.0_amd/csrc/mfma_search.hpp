@@ -834,35 +834,86 @@ __global__ void k_ring_sq64_list(const float2* __restrict__ rings, const int* __
 }
 
 // The T largest (particle, reference) keys of each particle, in descending order (0 when fewer
-// than T references exist).  One work-group per particle; keys are distinct (their low words
-// encode distinct (reference, rotation) indices), so round t takes the maximum below round t-1's.
+// than T references exist).  One work-group per particle.  Keys are distinct (their low words encode
+// distinct (reference, rotation) indices), so the T-th largest is found by a most-significant-digit
+// radix select: per 8-bit digit, a histogram of the keys that share the digits chosen so far picks
+// the bucket holding the T-th largest; it stops when that bucket is taken whole (at the latest at
+// the last digit, where a bucket holds one key).  The selected keys (those at or above the chosen
+// prefix) are compacted into LDS and sorted by a bitonic network.  A few passes over the row
+// instead of one pass per candidate.
+constexpr int TOPT_MAX = 256;
 __global__ void __launch_bounds__(256) k_topt(const unsigned long long* __restrict__ prbest, int np, int nrefs, int T,
                                               unsigned long long* __restrict__ cand) {
-  __shared__ unsigned long long red[256];
+  __shared__ unsigned int hist[256];
+  __shared__ unsigned long long sel[TOPT_MAX];
+  __shared__ int s_bin;
+  __shared__ unsigned int s_need, s_bucket, s_count;
   const int i = blockIdx.x, tid = threadIdx.x;
   if (i >= np) return;
   const unsigned long long* row = prbest + (size_t)i * nrefs;
-  unsigned long long thr = ~0ull;
-  for (int t = 0; t < T; ++t) {
-    unsigned long long m = 0ull;
+  unsigned long long prefix = 0ull, mask = 0ull;
+  unsigned int need = (unsigned int)T;
+  bool take_all = false;
+  for (int d = 7; d >= 0; --d) {
+    const int sh = 8 * d;
+    hist[tid] = 0u;
+    __syncthreads();
     for (int r = tid; r < nrefs; r += 256) {
       const unsigned long long k = row[r];
-      if (k < thr && k > m) m = k;
+      if (k != 0ull && (k & mask) == prefix) atomicAdd(&hist[(k >> sh) & 255ull], 1u);
     }
-    red[tid] = m;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (tid < w && red[tid + w] > red[tid]) red[tid] = red[tid + w];
-      __syncthreads();
+    if (tid == 0) {
+      unsigned int above = 0u;
+      int b = 255;
+      for (; b >= 0; --b) {
+        if (above + hist[b] >= need) break;
+        above += hist[b];
+      }
+      s_bin = b;
+      s_need = b >= 0 ? need - above : 0u;
+      s_bucket = b >= 0 ? hist[b] : 0u;
     }
-    thr = red[0];
-    if (tid == 0) cand[(size_t)i * T + t] = thr;
     __syncthreads();
-    if (thr == 0ull) {
-      for (int u = t + 1 + tid; u < T; u += 256) cand[(size_t)i * T + u] = 0ull;
-      return;
+    if (s_bin < 0) {        // fewer than need keys share the prefix: only at the first digit
+      take_all = true;
+      break;
+    }
+    need = s_need;
+    prefix |= (unsigned long long)s_bin << sh;
+    mask |= 255ull << sh;
+    if (s_bucket == need) break;
+  }
+  if (tid == 0) s_count = 0u;
+  __syncthreads();
+  for (int r = tid; r < nrefs; r += 256) {
+    const unsigned long long k = row[r];
+    if (k != 0ull && (take_all || (k & mask) >= prefix)) {
+      const unsigned int idx = atomicAdd(&s_count, 1u);
+      if (idx < (unsigned int)TOPT_MAX) sel[idx] = k;
     }
   }
+  __syncthreads();
+  const int cnt = (int)min(s_count, (unsigned int)T);
+  int P = 1;
+  while (P < cnt) P <<= 1;
+  if (tid >= cnt && tid < P) sel[tid] = 0ull;
+  __syncthreads();
+  // bitonic sort of sel[0, P) into descending order
+  for (int kk = 2; kk <= P; kk <<= 1)
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      const int x = tid ^ j;
+      if (tid < P && x > tid) {
+        const unsigned long long a = sel[tid], b = sel[x];
+        const bool desc = (tid & kk) == 0;
+        if (desc ? a < b : a > b) {
+          sel[tid] = b;
+          sel[x] = a;
+        }
+      }
+      __syncthreads();
+    }
+  for (int t = tid; t < T; t += 256) cand[(size_t)i * T + t] = t < cnt ? sel[t] : 0ull;
 }
 
 struct RescoreArgs {
@@ -891,110 +942,156 @@ struct RescoreArgs {
 // rotations rho - 1, rho, rho + 1 are evaluated as in gencorrs: q = ref * |CTF| [* e^{i phi}] with
 // the particle's own angle-dependent |CTF| and phi = argx sx + argy sy, cc = sum_k<kcorr sum_j
 // Re q[k,j] conj(P[k,(j+rho) mod 2n]) / sqrt(sum_k<knorm |q|^2 * sum_k<kcorr |p|^2) (knorm = all
-// shells when shifted, gencorrs_cc_2).  One work-group of RW wavefronts per particle, each
-// wavefront taking candidates w, w + RW, ...; the best cc wins, ties to the lowest
-// (reference, shift, rotation).
-constexpr int RESCORE_MAX = 64;
-constexpr int RESCORE_WAVES = 16;
+// shells when shifted, gencorrs_cc_2).  The best cc wins, ties to the lowest (reference, shift,
+// rotation).
+// One work-group of NW wavefronts per particle.  The particle's shell k (ring and |CTF|) is staged in
+// LDS once per shell and shared by every candidate, so the only global reads per candidate are its
+// reference ring: each wavefront scores RESCORE_CB candidates at once (c = w + NW b, b < CB) in a
+// pass over the shells, and T > NW CB candidates take several passes.  Per-lane partial sums are
+// reduced over the wavefront in a fixed order, so a candidate's value does not depend on its slot.
+// SH: the candidates carry shifts (gencorrs_cc_2), whose phases are evaluated per element.
+constexpr int RESCORE_MAX = TOPT_MAX;
+constexpr int RESCORE_WAVES = 8;
+constexpr int RESCORE_CB = 4;
+template <bool SH>
 __global__ void __launch_bounds__(64 * RESCORE_WAVES) k_rescore64(RescoreArgs a) {
-  __shared__ double ccs[RESCORE_MAX * 3];
+  __shared__ double ccs[RESCORE_MAX * 3];          // numerators, then cc
   __shared__ unsigned int lins[RESCORE_MAX * 3];
+  __shared__ double sqs[RESCORE_MAX];              // S_q of each candidate
+  __shared__ double spw[RESCORE_WAVES];
+  extern __shared__ __align__(16) float2 pk[];    // [n] particle ring of the shell, then [n] |CTF|
+  float* ck = reinterpret_cast<float*>(pk + a.n);
   const int i = blockIdx.x;
-  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int nw = blockDim.x >> 6;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = a.n, n2 = 2 * n;
   const int ip = a.plist[i];
   const float2* pt = a.ptcls + (size_t)ip * a.nk * n;
   const float* ct = a.ctf ? a.ctf + (size_t)ip * a.nk * n : nullptr;
-  for (int c = w; c < a.T; c += RESCORE_WAVES) {
-    const unsigned long long key = a.cand[(size_t)i * a.T + c];
-    if (key == 0ull) {
-      if (lane == 0)
+  const float2* refs = a.iseven[ip] ? a.refs_even : a.refs_odd;
+  const int per_pass = nw * RESCORE_CB;
+  double sp = 0.0;                                 // sum |p|^2 over kcorr shells (first pass only)
+  for (int c0 = 0; c0 < a.T; c0 += per_pass) {
+    // this wavefront's candidates of the pass
+    bool live[RESCORE_CB];
+    int rr[RESCORE_CB][3];
+    const float2* ref[RESCORE_CB];
+    double sx[RESCORE_CB], sy[RESCORE_CB];
+    unsigned int row[RESCORE_CB];
+    double num[RESCORE_CB][3], sq[RESCORE_CB];
+#pragma unroll
+    for (int b = 0; b < RESCORE_CB; ++b) {
+      const int c = c0 + w + nw * b;
+      const unsigned long long key = c < a.T ? a.cand[(size_t)i * a.T + c] : 0ull;
+      live[b] = key != 0ull;
+      const unsigned int lin = live[b] ? 0xFFFFFFFFu - (unsigned int)(key & 0xFFFFFFFFull) : 0u;
+      row[b] = lin / (unsigned)n2;
+      const int iref = (int)(row[b] / (unsigned)a.ns), ish = (int)(row[b] % (unsigned)a.ns);
+      const int rho = (int)(lin % (unsigned)n2);
+      rr[b][0] = (rho + n2 - 1) % n2;
+      rr[b][1] = rho;
+      rr[b][2] = (rho + 1) % n2;
+      ref[b] = refs + (size_t)iref * a.nk * n;
+      sx[b] = SH ? a.shifts[2 * ish] : 0.0;
+      sy[b] = SH ? a.shifts[2 * ish + 1] : 0.0;
+      num[b][0] = num[b][1] = num[b][2] = 0.0;
+      sq[b] = 0.0;
+    }
+    for (int k = 0; k < a.knorm; ++k) {
+      __syncthreads();                             // the previous shell's readers are done
+      for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const float2 v = pt[(size_t)k * n + j];
+        pk[j] = v;
+        ck[j] = ct ? ct[(size_t)k * n + j] : 1.f;
+        if (c0 == 0 && k < a.kcorr) sp += (double)v.x * v.x + (double)v.y * v.y;
+      }
+      __syncthreads();
+      const bool corr = k < a.kcorr;
+      for (int j = lane; j < n; j += 64) {
+        const double cv = (double)ck[j];
+        const size_t e = (size_t)k * n + j;
+#pragma unroll
+        for (int b = 0; b < RESCORE_CB; ++b) {
+          if (!live[b]) continue;
+          const float2 rv = ref[b][e];
+          double qx = rv.x * cv, qy = rv.y * cv;
+          if (SH) {
+            double sn, cs;
+            sincos(a.argx[e] * sx[b] + a.argy[e] * sy[b], &sn, &cs);
+            const double tx = qx * cs - qy * sn;
+            qy = qx * sn + qy * cs;
+            qx = tx;
+          }
+          sq[b] += qx * qx + qy * qy;
+          if (!corr) continue;
+#pragma unroll
+          for (int t = 0; t < 3; ++t) {
+            int idx = j + rr[b][t];
+            if (idx >= n2) idx -= n2;
+            float2 P;
+            if (idx < n) {
+              P = pk[idx];
+            } else {
+              P = pk[idx - n];
+              P.y = -P.y;
+            }
+            num[b][t] += qx * P.x + qy * P.y;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < RESCORE_CB; ++b) {
+      double s0 = num[b][0], s1 = num[b][1], s2 = num[b][2], q2 = sq[b];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+        q2 += __shfl_xor(q2, o);
+      }
+      const int c = c0 + w + nw * b;
+      if (lane == 0 && c < a.T) {
+        const double nums[3] = {s0, s1, s2};
         for (int t = 0; t < 3; ++t) {
-          ccs[c * 3 + t] = -INFINITY;
-          lins[c * 3 + t] = 0xFFFFFFFFu;
+          ccs[c * 3 + t] = live[b] ? nums[t] : -INFINITY;   // divided by the norms once S_p is known
+          lins[c * 3 + t] = live[b] ? row[b] * (unsigned int)n2 + (unsigned int)rr[b][t] : 0xFFFFFFFFu;
         }
-      continue;
-    }
-    const unsigned int lin = 0xFFFFFFFFu - (unsigned int)(key & 0xFFFFFFFFull);
-    const unsigned int row = lin / (unsigned)n2;
-    const int iref = (int)(row / (unsigned)a.ns), ish = (int)(row % (unsigned)a.ns);
-    const int rho = (int)(lin % (unsigned)n2);
-    const int rr[3] = {(rho + n2 - 1) % n2, rho, (rho + 1) % n2};
-    const float2* ref = (a.iseven[ip] ? a.refs_even : a.refs_odd) + (size_t)iref * a.nk * n;
-    const double sx = a.shifts ? a.shifts[2 * ish] : 0.0, sy = a.shifts ? a.shifts[2 * ish + 1] : 0.0;
-    double num0 = 0.0, num1 = 0.0, num2 = 0.0, sq = 0.0, sp = 0.0;
-    for (int e = lane; e < a.knorm * n; e += 64) {
-      const int k = e / n, j = e % n;
-      const float2 rv = ref[e];
-      const double cv = ct ? (double)ct[e] : 1.0;
-      double qx = rv.x * cv, qy = rv.y * cv;
-      if (a.shifts) {
-        double sn, cs;
-        sincos(a.argx[e] * sx + a.argy[e] * sy, &sn, &cs);
-        const double tx = qx * cs - qy * sn;
-        qy = qx * sn + qy * cs;
-        qx = tx;
-      }
-      sq += qx * qx + qy * qy;
-      if (e >= a.kcorr * n) continue;
-      const float2 pv = pt[e];
-      sp += (double)pv.x * pv.x + (double)pv.y * pv.y;
-      const float2* prow = pt + (size_t)k * n;
-      double nm[3];
-#pragma unroll
-      for (int t = 0; t < 3; ++t) {
-        int idx = j + rr[t];
-        if (idx >= n2) idx -= n2;
-        float2 P;
-        if (idx < n) {
-          P = prow[idx];
-        } else {
-          P = prow[idx - n];
-          P.y = -P.y;
-        }
-        nm[t] = qx * P.x + qy * P.y;
-      }
-      num0 += nm[0];
-      num1 += nm[1];
-      num2 += nm[2];
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      num0 += __shfl_xor(num0, o);
-      num1 += __shfl_xor(num1, o);
-      num2 += __shfl_xor(num2, o);
-      sq += __shfl_xor(sq, o);
-      sp += __shfl_xor(sp, o);
-    }
-    if (lane == 0) {
-      const double d = sqrt(sq * sp);
-      const double nums[3] = {num0, num1, num2};
-      for (int t = 0; t < 3; ++t) {
-        ccs[c * 3 + t] = nums[t] / d;
-        lins[c * 3 + t] = row * (unsigned int)n2 + (unsigned int)rr[t];
+        sqs[c] = q2;
       }
     }
   }
+  // S_p: the staging threads' partial sums, reduced in a fixed order
+  for (int o = 32; o > 0; o >>= 1) sp += __shfl_xor(sp, o);
+  if (lane == 0) spw[w] = sp;
   __syncthreads();
   if (threadIdx.x == 0) {
+    double spt = 0.0;
+    for (int v = 0; v < nw; ++v) spt += spw[v];
     double bc = -INFINITY;
     unsigned int bl = 0xFFFFFFFFu;
-    for (int c = 0; c < 3 * a.T; ++c)
-      if (lins[c] != 0xFFFFFFFFu && (ccs[c] > bc || (ccs[c] == bc && lins[c] < bl))) {
-        bc = ccs[c];
-        bl = lins[c];
+    for (int c = 0; c < a.T; ++c) {
+      if (lins[c * 3] == 0xFFFFFFFFu) continue;
+      const double d = sqrt(sqs[c] * spt);
+      for (int t = 0; t < 3; ++t) {
+        const double v = ccs[c * 3 + t] / d;
+        if (v > bc || (v == bc && lins[c * 3 + t] < bl)) {
+          bc = v;
+          bl = lins[c * 3 + t];
+        }
       }
+    }
     if (bl == 0xFFFFFFFFu) {
       a.best_cc[i] = __uint_as_float(0x7FC00000u);
       a.best_iref[i] = 0;
       a.best_irot[i] = 0;
       if (a.best_ishift) a.best_ishift[i] = 0;
     } else {
-      const unsigned int row = bl / (unsigned)n2;
+      const unsigned int rw = bl / (unsigned)n2;
       a.best_cc[i] = (float)bc;
-      a.best_iref[i] = (int)(row / (unsigned)a.ns) + 1;
+      a.best_iref[i] = (int)(rw / (unsigned)a.ns) + 1;
       a.best_irot[i] = (int)(bl % (unsigned)n2) + 1;
-      if (a.best_ishift) a.best_ishift[i] = (int)(row % (unsigned)a.ns);
+      if (a.best_ishift) a.best_ishift[i] = (int)(rw % (unsigned)a.ns);
     }
     if (a.best_cc64) a.best_cc64[i] = bc;
   }

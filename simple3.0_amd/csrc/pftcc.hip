@@ -3115,7 +3115,10 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
       ra.best_iref = d_ir;
       ra.best_irot = d_rot;
       ra.best_ishift = d_ish;
-      hipLaunchKernelGGL(k_rescore64, dim3(np), dim3(64 * std::min(ntop, RESCORE_WAVES)), 0, h->stream, ra);
+      const int nw = std::max(1, std::min(RESCORE_WAVES, (ntop + RESCORE_CB - 1) / RESCORE_CB));
+      const size_t lds = (size_t)h->n * (sizeof(float2) + sizeof(float));
+      if (ns > 0) hipLaunchKernelGGL(k_rescore64<true>, dim3(np), dim3(64 * nw), lds, h->stream, ra);
+      else hipLaunchKernelGGL(k_rescore64<false>, dim3(np), dim3(64 * nw), lds, h->stream, ra);
       if (hipGetLastError() != hipSuccess) rc = fail(h, PFTCC_E_DEVICE, "search_mfma: rescoring launch failed");
     }
     const int rc2 = timing_end(h, ev);

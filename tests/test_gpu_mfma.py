@@ -186,7 +186,7 @@ def test_mfma_rescored_search_is_exact(astig, prec):
     raw_cc, raw_ir, raw_rot = lib.search_mfma(ids, ntop=0, prec=prec)
     say(f"  raw vs exact max |dcc| {np.max(np.abs(raw_cc - cc)):.2e}")
     with pytest.raises(PftccError):
-        lib.search_mfma(ids, ntop=65)
+        lib.search_mfma(ids, ntop=257)
     lib.kill()
 
 
@@ -221,6 +221,56 @@ def test_fp8_coarse_pass_error_and_rescoring():
         r, t = np.unravel_index(int(np.argmax(exact[i])), exact[i].shape)
         assert (ir[i], rot[i]) == (r + 1, t + 1)
         assert abs(cc[i] - exact[i][r, t]) <= 1e-7
+    lib.kill()
+
+
+def _fp8_keys(cube, n2):
+    """Per (particle, reference) best key of the coarse pass: its largest value, ties to the lowest
+    rotation; references then ordered by key (value descending, ties to the lowest reference)."""
+    np_, nr = cube.shape[:2]
+    rot = np.argmax(cube, axis=-1)                  # first maximum = lowest rotation
+    val = np.take_along_axis(cube, rot[..., None], axis=-1)[..., 0]
+    order = [sorted(range(nr), key=lambda r: (-val[p, r], r)) for p in range(np_)]
+    return rot, order
+
+
+def test_fp8_coarse_pass_c4_shape_topt_and_rescoring():
+    """FP8 e4m3 coarse pass at the large coarse-to-fine shape (box 256, kfromto [2,64]: pftsz 400,
+    K 63) against the ring-mean oracle, and the rescoring of the top-T references for T from 1 to
+    256 (beyond nrefs): from the device's own coarse correlations and the FP64 oracle the result is
+    predicted exactly for every T (the T references with the largest coarse maxima, each rescored
+    in FP64 at its coarse best rotation and both neighbours, the best winning, ties to the lowest
+    (reference, rotation)); its FP64 cc can only grow with T."""
+    say("fp8 coarse pass, n 400, K 63: top-T selection and FP64 rescoring")
+    nr, npt = 120, 6
+    pf, cps = make_case(seed=57, n=400, kfrom=2, kto=64, ctf=True, astig=0.1, nrefs=nr, nptcls=npt, ldim=256,
+                        eo_mixed=True)
+    ids = np.arange(1, npt + 1)
+    lib = to_lib(pf, cps)
+    coarse = lib.gencorrs_mfma_batch(ids, prec=PFTCC_PREC_FP8).astype(np.float64)
+    ref = oracle_cube(ring_mean_case(copy.deepcopy(pf)), range(npt), range(nr))
+    errs = [peak_rel_err(coarse[p], ref[p]) for p in range(npt)]
+    say(f"  fp8 peak-relative error vs ring-mean oracle: max {max(errs):.2e}")
+    assert max(errs) < 0.15
+    exact = oracle_cube(pf, range(npt), range(nr))
+    n2 = 2 * pf.pftsz
+    rot, order = _fp8_keys(coarse.astype(np.float32), n2)
+    prev = np.full(npt, -np.inf)
+    for T in (1, 4, 16, 64, 100, 120, 256):
+        cc, ir, rt = lib.search_mfma(ids, prec=PFTCC_PREC_FP8, ntop=T)
+        for p in range(npt):
+            best, bkey = -np.inf, None
+            for r in order[p][:T]:
+                for t in ((rot[p, r] - 1) % n2, rot[p, r], (rot[p, r] + 1) % n2):
+                    v = exact[p, r, t]
+                    if v > best or (v == best and (r, t) < bkey):
+                        best, bkey = v, (r, t)
+            assert (ir[p], rt[p]) == (bkey[0] + 1, bkey[1] + 1), (T, p, ir[p], rt[p], bkey)
+            assert abs(cc[p] - best) <= 1e-7, (T, p, cc[p], best)
+            assert best >= prev[p]
+            prev[p] = best
+    with pytest.raises(PftccError):
+        lib.search_mfma(ids, prec=PFTCC_PREC_FP8, ntop=257)
     lib.kill()
 
 
