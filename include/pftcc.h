@@ -400,7 +400,8 @@ int pftcc_shsrch_grad_batch(pftcc_t* h, int32_t n, const int32_t* irefs, const i
  * ntop = 0: the best (reference, rotation) of the contraction itself.  ntop in [1, 256]: the ntop
  * references with the highest contraction score per particle are rescored exactly in FP64 (the
  * reference's gencorrs formula with the particle's own |CTF|) at their best rotation and its two
- * neighbours, and the best rescored candidate is returned with its FP64 cc. */
+ * neighbours, and the best rescored candidate is returned with its FP64 cc (rescoring needs
+ * pftsz <= 1024, PFTCC_E_UNSUPPORTED otherwise). */
 #define PFTCC_PREC_BF16 1
 #define PFTCC_PREC_SPLIT_BF16 3
 /* FP16 operands (one product) or split FP16 (hi+lo, three products), FP32 accumulation.  FP16 has

@@ -944,122 +944,143 @@ struct RescoreArgs {
 // Re q[k,j] conj(P[k,(j+rho) mod 2n]) / sqrt(sum_k<knorm |q|^2 * sum_k<kcorr |p|^2) (knorm = all
 // shells when shifted, gencorrs_cc_2).  The best cc wins, ties to the lowest (reference, shift,
 // rotation).
-// One work-group of NW wavefronts per particle.  The particle's shell k (ring and |CTF|) is staged in
-// LDS once per shell and shared by every candidate, so the only global reads per candidate are its
-// reference ring: each wavefront scores RESCORE_CB candidates at once (c = w + NW b, b < CB) in a
-// pass over the shells, and T > NW CB candidates take several passes.  Per-lane partial sums are
-// reduced over the wavefront in a fixed order, so a candidate's value does not depend on its slot.
+// One work-group per particle, one wavefront per candidate (T > NW candidates take several passes).
+// The particle's shell k (ring and |CTF|) is staged in LDS and shared by every wavefront, so a
+// candidate's only global reads are its reference ring.  While shell k is computed, the reference
+// values of shell k + 1 (JB per lane) and the particle's shell k + 1 are loaded into registers; the
+// latter is written to the other LDS buffer after the arithmetic, so one barrier per shell
+// separates the buffers' writers and readers.  Lane sums are reduced over the wavefront in a fixed
+// order, so a candidate's value does not depend on its slot.  JB = ceil(n / 64) positions per lane.
 // SH: the candidates carry shifts (gencorrs_cc_2), whose phases are evaluated per element.
 constexpr int RESCORE_MAX = TOPT_MAX;
-constexpr int RESCORE_WAVES = 8;
-constexpr int RESCORE_CB = 4;
-template <bool SH>
+constexpr int RESCORE_WAVES = 16;
+template <bool SH, int JB>
 __global__ void __launch_bounds__(64 * RESCORE_WAVES) k_rescore64(RescoreArgs a) {
   __shared__ double ccs[RESCORE_MAX * 3];          // numerators, then cc
   __shared__ unsigned int lins[RESCORE_MAX * 3];
   __shared__ double sqs[RESCORE_MAX];              // S_q of each candidate
   __shared__ double spw[RESCORE_WAVES];
-  extern __shared__ __align__(16) float2 pk[];    // [n] particle ring of the shell, then [n] |CTF|
-  float* ck = reinterpret_cast<float*>(pk + a.n);
+  extern __shared__ __align__(16) float2 pk[];    // 2 x [n] particle ring, then 2 x [n] |CTF|
+  const int n = a.n, n2 = 2 * n;
+  float* ck = reinterpret_cast<float*>(pk + 2 * n);
   const int i = blockIdx.x;
   const int nw = blockDim.x >> 6;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n = a.n, n2 = 2 * n;
   const int ip = a.plist[i];
   const float2* pt = a.ptcls + (size_t)ip * a.nk * n;
   const float* ct = a.ctf ? a.ctf + (size_t)ip * a.nk * n : nullptr;
   const float2* refs = a.iseven[ip] ? a.refs_even : a.refs_odd;
-  const int per_pass = nw * RESCORE_CB;
+  const int kn = a.knorm;
   double sp = 0.0;                                 // sum |p|^2 over kcorr shells (first pass only)
-  for (int c0 = 0; c0 < a.T; c0 += per_pass) {
-    // this wavefront's candidates of the pass
-    bool live[RESCORE_CB];
-    int rr[RESCORE_CB][3];
-    const float2* ref[RESCORE_CB];
-    double sx[RESCORE_CB], sy[RESCORE_CB];
-    unsigned int row[RESCORE_CB];
-    double num[RESCORE_CB][3], sq[RESCORE_CB];
-#pragma unroll
-    for (int b = 0; b < RESCORE_CB; ++b) {
-      const int c = c0 + w + nw * b;
-      const unsigned long long key = c < a.T ? a.cand[(size_t)i * a.T + c] : 0ull;
-      live[b] = key != 0ull;
-      const unsigned int lin = live[b] ? 0xFFFFFFFFu - (unsigned int)(key & 0xFFFFFFFFull) : 0u;
-      row[b] = lin / (unsigned)n2;
-      const int iref = (int)(row[b] / (unsigned)a.ns), ish = (int)(row[b] % (unsigned)a.ns);
-      const int rho = (int)(lin % (unsigned)n2);
-      rr[b][0] = (rho + n2 - 1) % n2;
-      rr[b][1] = rho;
-      rr[b][2] = (rho + 1) % n2;
-      ref[b] = refs + (size_t)iref * a.nk * n;
-      sx[b] = SH ? a.shifts[2 * ish] : 0.0;
-      sy[b] = SH ? a.shifts[2 * ish + 1] : 0.0;
-      num[b][0] = num[b][1] = num[b][2] = 0.0;
-      sq[b] = 0.0;
+  for (int c0 = 0; c0 < a.T; c0 += nw) {
+    const int c = c0 + w;
+    const unsigned long long key = c < a.T ? a.cand[(size_t)i * a.T + c] : 0ull;
+    const bool live = key != 0ull;
+    const unsigned int lin = live ? 0xFFFFFFFFu - (unsigned int)(key & 0xFFFFFFFFull) : 0u;
+    const unsigned int row = lin / (unsigned)n2;
+    const int iref = (int)(row / (unsigned)a.ns), ish = (int)(row % (unsigned)a.ns);
+    const int rho = (int)(lin % (unsigned)n2);
+    const int rr[3] = {(rho + n2 - 1) % n2, rho, (rho + 1) % n2};
+    const float2* ref = refs + (size_t)iref * a.nk * n;
+    const double sx = SH && live ? a.shifts[2 * ish] : 0.0, sy = SH && live ? a.shifts[2 * ish + 1] : 0.0;
+    double num0 = 0.0, num1 = 0.0, num2 = 0.0, sq = 0.0;
+    // prologue: shell 0 staged, its reference values in registers
+    float2 rcur[JB], rnext[JB];
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      const float2 v = pt[e];
+      pk[e] = v;
+      ck[e] = ct ? ct[e] : 1.f;
+      if (c0 == 0 && a.kcorr > 0) sp += (double)v.x * v.x + (double)v.y * v.y;
     }
-    for (int k = 0; k < a.knorm; ++k) {
-      __syncthreads();                             // the previous shell's readers are done
-      for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const float2 v = pt[(size_t)k * n + j];
-        pk[j] = v;
-        ck[j] = ct ? ct[(size_t)k * n + j] : 1.f;
-        if (c0 == 0 && k < a.kcorr) sp += (double)v.x * v.x + (double)v.y * v.y;
-      }
-      __syncthreads();
-      const bool corr = k < a.kcorr;
-      for (int j = lane; j < n; j += 64) {
-        const double cv = (double)ck[j];
-        const size_t e = (size_t)k * n + j;
 #pragma unroll
-        for (int b = 0; b < RESCORE_CB; ++b) {
-          if (!live[b]) continue;
-          const float2 rv = ref[b][e];
+    for (int u = 0; u < JB; ++u) {
+      const int j = lane + 64 * u;
+      rnext[u] = (live && j < n) ? ref[j] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (int k = 0; k < kn; ++k) {
+      const int b = k & 1;
+      const float2* pb = pk + b * n;
+      const float* cb = ck + b * n;
+#pragma unroll
+      for (int u = 0; u < JB; ++u) rcur[u] = rnext[u];
+      // loads of shell k + 1: reference values and the particle's staging values
+      const bool more = k + 1 < kn;
+      float2 pnext = make_float2(0.f, 0.f);
+      float cnext = 1.f;
+      const int e1 = threadIdx.x;
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+          const int j = lane + 64 * u;
+          if (live && j < n) rnext[u] = ref[(size_t)(k + 1) * n + j];
+        }
+        if (e1 < n) {
+          pnext = pt[(size_t)(k + 1) * n + e1];
+          if (ct) cnext = ct[(size_t)(k + 1) * n + e1];
+        }
+      }
+      if (live) {
+        const bool corr = k < a.kcorr;
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+          const int j = lane + 64 * u;
+          if (j >= n) continue;
+          const double cv = (double)cb[j];
+          const float2 rv = rcur[u];
           double qx = rv.x * cv, qy = rv.y * cv;
           if (SH) {
+            const size_t e = (size_t)k * n + j;
             double sn, cs;
-            sincos(a.argx[e] * sx[b] + a.argy[e] * sy[b], &sn, &cs);
+            sincos(a.argx[e] * sx + a.argy[e] * sy, &sn, &cs);
             const double tx = qx * cs - qy * sn;
             qy = qx * sn + qy * cs;
             qx = tx;
           }
-          sq[b] += qx * qx + qy * qy;
+          sq += qx * qx + qy * qy;
           if (!corr) continue;
+          double nm[3];
 #pragma unroll
           for (int t = 0; t < 3; ++t) {
-            int idx = j + rr[b][t];
+            int idx = j + rr[t];
             if (idx >= n2) idx -= n2;
             float2 P;
             if (idx < n) {
-              P = pk[idx];
+              P = pb[idx];
             } else {
-              P = pk[idx - n];
+              P = pb[idx - n];
               P.y = -P.y;
             }
-            num[b][t] += qx * P.x + qy * P.y;
+            nm[t] = qx * P.x + qy * P.y;
           }
+          num0 += nm[0];
+          num1 += nm[1];
+          num2 += nm[2];
         }
       }
+      if (more && e1 < n) {
+        pk[(1 - b) * n + e1] = pnext;
+        ck[(1 - b) * n + e1] = cnext;
+        if (c0 == 0 && k + 1 < a.kcorr) sp += (double)pnext.x * pnext.x + (double)pnext.y * pnext.y;
+      }
+      __syncthreads();
     }
 #pragma unroll
-    for (int b = 0; b < RESCORE_CB; ++b) {
-      double s0 = num[b][0], s1 = num[b][1], s2 = num[b][2], q2 = sq[b];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        s0 += __shfl_xor(s0, o);
-        s1 += __shfl_xor(s1, o);
-        s2 += __shfl_xor(s2, o);
-        q2 += __shfl_xor(q2, o);
-      }
-      const int c = c0 + w + nw * b;
-      if (lane == 0 && c < a.T) {
-        const double nums[3] = {s0, s1, s2};
-        for (int t = 0; t < 3; ++t) {
-          ccs[c * 3 + t] = live[b] ? nums[t] : -INFINITY;   // divided by the norms once S_p is known
-          lins[c * 3 + t] = live[b] ? row[b] * (unsigned int)n2 + (unsigned int)rr[b][t] : 0xFFFFFFFFu;
-        }
-        sqs[c] = q2;
-      }
+    for (int o = 32; o > 0; o >>= 1) {
+      num0 += __shfl_xor(num0, o);
+      num1 += __shfl_xor(num1, o);
+      num2 += __shfl_xor(num2, o);
+      sq += __shfl_xor(sq, o);
     }
+    if (lane == 0 && c < a.T) {
+      const double nums[3] = {num0, num1, num2};
+      for (int t = 0; t < 3; ++t) {
+        ccs[c * 3 + t] = live ? nums[t] : -INFINITY;      // divided by the norms once S_p is known
+        lins[c * 3 + t] = live ? row * (unsigned int)n2 + (unsigned int)rr[t] : 0xFFFFFFFFu;
+      }
+      sqs[c] = sq;
+    }
+    __syncthreads();                               // the next pass restages shell 0
   }
   // S_p: the staging threads' partial sums, reduced in a fixed order
   for (int o = 32; o > 0; o >>= 1) sp += __shfl_xor(sp, o);

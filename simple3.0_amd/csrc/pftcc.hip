@@ -3115,10 +3115,18 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
       ra.best_iref = d_ir;
       ra.best_irot = d_rot;
       ra.best_ishift = d_ish;
-      const int nw = std::max(1, std::min(RESCORE_WAVES, (ntop + RESCORE_CB - 1) / RESCORE_CB));
-      const size_t lds = (size_t)h->n * (sizeof(float2) + sizeof(float));
-      if (ns > 0) hipLaunchKernelGGL(k_rescore64<true>, dim3(np), dim3(64 * nw), lds, h->stream, ra);
-      else hipLaunchKernelGGL(k_rescore64<false>, dim3(np), dim3(64 * nw), lds, h->stream, ra);
+      // one wavefront per candidate; the staging of a shell needs at least n threads
+      const int nw = std::max(std::min(RESCORE_WAVES, ntop), (h->n + 63) / 64);
+      const size_t lds = (size_t)2 * h->n * (sizeof(float2) + sizeof(float));
+      if (h->n > 64 * RESCORE_WAVES) {
+        rc = fail(h, PFTCC_E_UNSUPPORTED, "search_mfma: rescoring supports pftsz <= %d", 64 * RESCORE_WAVES);
+      } else if (h->n <= 512) {
+        if (ns > 0) hipLaunchKernelGGL((k_rescore64<true, 8>), dim3(np), dim3(64 * nw), lds, h->stream, ra);
+        else hipLaunchKernelGGL((k_rescore64<false, 8>), dim3(np), dim3(64 * nw), lds, h->stream, ra);
+      } else {
+        if (ns > 0) hipLaunchKernelGGL((k_rescore64<true, 16>), dim3(np), dim3(64 * nw), lds, h->stream, ra);
+        else hipLaunchKernelGGL((k_rescore64<false, 16>), dim3(np), dim3(64 * nw), lds, h->stream, ra);
+      }
       if (hipGetLastError() != hipSuccess) rc = fail(h, PFTCC_E_DEVICE, "search_mfma: rescoring launch failed");
     }
     const int rc2 = timing_end(h, ev);
