@@ -115,10 +115,11 @@ constexpr int fused_F() {
 
 // Pair tile and register budget measured fastest on MI355X for the ring lengths of the benchmark
 // shapes: n = 308, 9 pairs at >= 3 wavefronts per SIMD (167 VGPRs, no spills, against 2
-// wavefronts at ~190 VGPRs unconstrained); n = 192, 5 pairs.  fused_F elsewhere.
+// wavefronts at ~190 VGPRs unconstrained); n = 192, 5 pairs; n = 400, 4 pairs (4 to 10 pairs and 1
+// to 3 wavefronts measured: within 4 %, larger tiles slower).  fused_F elsewhere.
 template <int N, int NT>
 constexpr int fused_F_tuned() {
-  if (NT == 256 && (N == 192 || N == 308)) return N == 192 ? 5 : 9;
+  if (NT == 256 && (N == 192 || N == 308 || N == 400)) return N == 192 ? 5 : N == 308 ? 9 : 4;
   return fused_F<N, NT>();
 }
 #ifndef PFTCC_FUSED_GROUP
