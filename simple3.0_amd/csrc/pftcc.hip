@@ -816,9 +816,14 @@ int launch_rot8(pftcc* h, int npairs, const int* d_irefs, const int* d_ips, cons
   // one thread per ring position j (rounded up to whole wavefronts; k_rot8's bound is 512, larger
   // rings loop over j)
   const int nt = std::min(512, (h->n + 63) / 64 * 64);
-  if (a.mode == 0) hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_AHEAD, 0>), dim3(npairs), dim3(nt), 0, h->stream, a);
-  else if (a.mode == 1) hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_AHEAD, 1>), dim3(npairs), dim3(nt), 0, h->stream, a);
-  else hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_AHEAD, 2>), dim3(npairs), dim3(nt), 0, h->stream, a);
+  // the shell normalisation (match_filt / pssnr) is a template argument: without it the kernel
+  // carries no per-shell filter factor
+  const bool nrm = a.refinv != nullptr;
+  if (a.mode == 0 && !nrm) hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_AHEAD, 0, false>), dim3(npairs), dim3(nt), 0, h->stream, a);
+  else if (a.mode == 0) hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_AHEAD, 0, true>), dim3(npairs), dim3(nt), 0, h->stream, a);
+  else if (a.mode == 1 && !nrm) hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_AHEAD, 1, false>), dim3(npairs), dim3(nt), 0, h->stream, a);
+  else if (a.mode == 1) hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_AHEAD, 1, true>), dim3(npairs), dim3(nt), 0, h->stream, a);
+  else hipLaunchKernelGGL((k_rot8<PFTCC_ROT8_AHEAD, 2, false>), dim3(npairs), dim3(nt), 0, h->stream, a);
   HIPCHK(h, hipGetLastError());
   return timing_end(h, ev);
 }

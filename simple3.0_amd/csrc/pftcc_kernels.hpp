@@ -518,7 +518,7 @@ struct Rot8Shell {
   double s2w;                  // Euclid shell weight (uniform)
 };
 
-template <int MODE>
+template <int MODE, bool NORM>
 __device__ __forceinline__ void rot8_load(const Rot8Args a, Rot8Shell& s, int kk, int j, int t, int ip, int iref,
                                           bool even, const float2* ref, const float2* ptc, const float2* dpt,
                                           const float* ctf) {
@@ -528,7 +528,7 @@ __device__ __forceinline__ void rot8_load(const Rot8Args a, Rot8Shell& s, int kk
   s.dv = dpt ? dpt[(size_t)kk * n + t] : make_float2(0.f, 0.f);
   s.cf = ctf ? ctf[(size_t)kk * n + j] : 1.f;
   s.fac = 1.0;
-  if (a.refinv) {
+  if (NORM) {
     const int row = a.optrow ? a.optrow[ip] : iref;
     s.fac = a.ofac[(size_t)row * a.kcorr + kk] * a.refinv[((size_t)(even ? 0 : 1) * a.nrefs + iref) * a.kcorr + kk];
   }
@@ -541,13 +541,13 @@ struct Rot8Acc {
 };
 
 // One shell's contributions; advances the shift phase to the next shell (gen_shmat_8's recurrence).
-template <int MODE>
+template <int MODE, bool NORM>
 __device__ __forceinline__ void rot8_step(const Rot8Args a, const Rot8Shell& s, int kk, bool cj, bool angle,
                                           float snj, float ncnj, Rot8Acc& acc, double& shr, double& shi, double cs,
                                           double ss) {
   const double w = (MODE == 0 || MODE == 2) ? (double)(a.kfrom + kk) : 1.0;
   double c = (double)s.cf;
-  if (a.refinv) c *= s.fac;
+  if (NORM) c *= s.fac;
   // argtransf (:311-317) from the FP32 polar coordinate k sin(theta_j), -k cos(theta_j)
   const float kf = (float)(a.kfrom + kk);
   const double ax = (double)(snj * kf) * a.arg_a;
@@ -586,7 +586,7 @@ __device__ __forceinline__ void rot8_step(const Rot8Args a, const Rot8Shell& s, 
 #define PFTCC_ROT8_AHEAD 1     // shells of loads in flight ahead of the arithmetic in k_rot8
 #endif
 #ifndef PFTCC_ROT8_W
-#define PFTCC_ROT8_W 6         // wavefronts per SIMD the register allocation of k_rot8 must allow
+#define PFTCC_ROT8_W 7         // wavefronts per SIMD the register allocation of k_rot8 must allow
 #endif
 
 // FP64 sums of |P|^2 per (particle, shell): one wavefront per ring.
@@ -604,8 +604,9 @@ __global__ void __launch_bounds__(256) k_shell_sq8(const float2* __restrict__ ri
 // Threads own angles j and run the shells in order.  The loads of shell k + D are issued before
 // the arithmetic of shell k (a software pipeline over a ring of D + 1 shells), so a thread always
 // has loads in flight.  MODE: 0 CC k-weighted, 1 CC match_filt, 2 Euclid (equal to a.mode; a
-// template argument so that each form's sums stay in registers).
-template <int D, int MODE>
+// template argument so that each form's sums stay in registers).  NORM: shell normalisation
+// (match_filt / pssnr filter factors per shell), so that the other forms carry no factor.
+template <int D, int MODE, bool NORM>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PFTCC_ROT8_W))) k_rot8(Rot8Args a) {
   const int item = xcd_remap(blockIdx.x, gridDim.x);
   if (item >= a.npairs) return;
@@ -641,20 +642,20 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PFTCC_
     Rot8Shell s[D + 1];
 #pragma unroll
     for (int u = 0; u < D; ++u)
-      if (u < kc) rot8_load<MODE>(a, s[u], u, j, t, ip, iref, even, ref, ptc, dpt, ctf);
+      if (u < kc) rot8_load<MODE, NORM>(a, s[u], u, j, t, ip, iref, even, ref, ptc, dpt, ctf);
     int kk = 0;
     for (; kk + D + 1 <= kc; kk += D + 1) {
 #pragma unroll
       for (int u = 0; u <= D; ++u) {
-        if (kk + u + D < kc) rot8_load<MODE>(a, s[(u + D) % (D + 1)], kk + u + D, j, t, ip, iref, even, ref, ptc, dpt, ctf);
-        rot8_step<MODE>(a, s[u], kk + u, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
+        if (kk + u + D < kc) rot8_load<MODE, NORM>(a, s[(u + D) % (D + 1)], kk + u + D, j, t, ip, iref, even, ref, ptc, dpt, ctf);
+        rot8_step<MODE, NORM>(a, s[u], kk + u, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
       }
     }
 #pragma unroll
     for (int u = 0; u <= D; ++u)
       if (kk + u < kc) {
-        if (kk + u + D < kc) rot8_load<MODE>(a, s[(u + D) % (D + 1)], kk + u + D, j, t, ip, iref, even, ref, ptc, dpt, ctf);
-        rot8_step<MODE>(a, s[u], kk + u, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
+        if (kk + u + D < kc) rot8_load<MODE, NORM>(a, s[(u + D) % (D + 1)], kk + u + D, j, t, ip, iref, even, ref, ptc, dpt, ctf);
+        rot8_step<MODE, NORM>(a, s[u], kk + u, cj, angle, snj, ncnj, acc, shr, shi, cs, ss);
       }
   }
   // wavefront sums, then the waves' partial sums in a fixed order
