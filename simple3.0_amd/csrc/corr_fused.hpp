@@ -145,6 +145,76 @@ constexpr int fused_W_tuned() {
 #ifndef PFTCC_FUSED_TWL
 #define PFTCC_FUSED_TWL 1      // last-pass twiddles from a [t][g] table (conflict-free reads)
 #endif
+#ifndef PFTCC_FUSED_PFA
+#define PFTCC_FUSED_PFA 1      // prime-factor shell transform where N = 4 A B allows it
+#endif
+
+__host__ __device__ constexpr int pfa_modinv(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 0;
+}
+
+// Prime-factor (Good-Thomas) plan of the shell transform, for N = 4 A B with A > B distinct odd
+// primes (fused radix order 4, A, B) and N/4 = 1 (mod 4).  Input index n = CRT(n1, nA, nB)
+// (n = n1 mod 4, nA mod A, nB mod B), output index m = (N/4 k1 + N/A kA + N/B kB) mod N: the three
+// sub-transforms are plain DFTs of 4, A and B points with no twiddle factors between them.
+// LDS layout of one transform: position k1 QS + a + b SB (a = nA or kA, b = nB or kB), with rows
+// padded to SB = A + 2 and QS = B SB; every pass works in place on its own butterfly's positions.
+// The padding spreads the scattered pass-1 stores (column (2j mod A, 2j mod B) of butterfly j)
+// over the LDS banks; passes A and B read rows and columns at unit and SB strides.
+template <int N>
+struct PfaPlan {
+  static constexpr bool ok = Fp<N>::np == 3 && Fp<N>::R(0) == 4 && Fp<N>::R(1) % 2 == 1 &&
+                             Fp<N>::R(2) % 2 == 1 && Fp<N>::R(1) != Fp<N>::R(2) && (N / 4) % 4 == 1;
+  static constexpr int A = ok ? Fp<N>::R(1) : 1;
+  static constexpr int B = ok ? Fp<N>::R(2) : 1;
+  static constexpr int Q = N / 4;
+  static constexpr int SB = A + 2;            // stride of b
+  static constexpr int QS = B * SB;           // stride of k1
+  static constexpr int IA = ok ? pfa_modinv((N / A) % A, A) : 0;   // kA = m IA mod A
+  static constexpr int IB = ok ? pfa_modinv((N / B) % B, B) : 0;   // kB = m IB mod B
+  // LDS position of output bin m (k1 = m mod 4 since N/4 = 1 mod 4); the Nyquist bin N after them
+  static __device__ __forceinline__ int pos(int m) {
+    if (m >= N) return 4 * QS;
+    return (m & 3) * QS + (m * IA) % A + ((m * IB) % B) * SB;
+  }
+  // Butterfly g of the last pass, 0 <= g < 2 A + 1: (k1, kA) and its mirror (-k1, -kA), whose
+  // outputs hold the bins N - m.  g = 0: (0, 0), g = 2: (2, 0) are their own mirrors; g = 1:
+  // (1, 0) with (3, 0); g = 3 + i: k1 = i / H, kA = 1 + i % H with (-k1, A - kA), H = (A - 1) / 2.
+  static __device__ __forceinline__ void rep(int g, int& k1, int& ka) {
+    constexpr int H = (A - 1) / 2;
+    if (g < 3) {
+      k1 = g;
+      ka = 0;
+    } else {
+      k1 = (g - 3) / H;
+      ka = 1 + (g - 3) % H;
+    }
+  }
+};
+
+// Pass A of the prime-factor transform: DFT of A points along a for every (k1, nB), in place.
+template <int N, int F, int NT, int S>
+__device__ __forceinline__ void pfa_pass_a(float2* __restrict__ buf, int tid) {
+  constexpr int A = PfaPlan<N>::A, B = PfaPlan<N>::B, QS = PfaPlan<N>::QS, SB = PfaPlan<N>::SB;
+  constexpr int TASKS = F * 4 * B;
+  constexpr int TPT = (TASKS + NT - 1) / NT;
+#pragma unroll
+  for (int c = 0; c < TPT; ++c) {
+    const int task = tid + c * NT;
+    if (TASKS % NT == 0 || task < TASKS) {
+      const int f = task / (4 * B), r = task % (4 * B);
+      float2* p = buf + f * S + (r / B) * QS + (r % B) * SB;
+      float2 v[A];
+#pragma unroll
+      for (int t = 0; t < A; ++t) v[t] = p[t];
+      Dft<A>::run(v);
+#pragma unroll
+      for (int t = 0; t < A; ++t) p[t] = v[t];
+    }
+  }
+}
 #ifndef PFTCC_FUSED_P1V4
 #define PFTCC_FUSED_P1V4 1     // pass-1 outputs stored as two 16-byte writes
 #endif
@@ -155,10 +225,23 @@ constexpr int fused_W_tuned() {
 #define PFTCC_FUSED_PAD 0      // extra float2 between the transforms of a tile (even: 16-byte rows)
 #endif
 
+// Prime-factor shell transform in the fused kernel (no twiddles between the passes).
+template <int N>
+constexpr bool fused_pfa() {
+  return PFTCC_FUSED_PFA && PFTCC_FUSED_W12 && PfaPlan<N>::ok;
+}
+
 // LDS row stride of one transform of the fused kernel.
 template <int N>
 constexpr int fused_stride() {
-  return N + PFTCC_FUSED_PAD;
+  return fused_pfa<N>() ? 4 * PfaPlan<N>::QS : N + PFTCC_FUSED_PAD;
+}
+
+// Length of each staged particle spectrum (W1, W2) in LDS: N + 1 bins, in the prime-factor layout
+// with its padding.
+template <int N>
+constexpr int fused_wlen() {
+  return fused_pfa<N>() ? 4 * PfaPlan<N>::QS + 1 : N + 1;
 }
 
 // Entries of the last pass's twiddle table: (RL - 1) x NsL.
@@ -171,7 +254,7 @@ constexpr int fused_twl_len() {
 template <int N, int F>
 constexpr size_t fused_lds_bytes() {
   // buf[F N] + Wk[N+1] (W1, W2: 2 (N+1)) + tw2[2N] + twL float2;  ctfk[N] + sq/off slots 2 F N/4 floats
-  return sizeof(float2) * ((size_t)F * fused_stride<N>() + (PFTCC_FUSED_W12 ? 2 : 1) * (N + 1) + 2 * N +
+  return sizeof(float2) * ((size_t)F * fused_stride<N>() + (PFTCC_FUSED_W12 ? 2 : 1) * fused_wlen<N>() + 2 * N +
                            fused_twl_len<N>() + 2 * N) +
          sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
 }
@@ -232,13 +315,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   constexpr int Q = N / 4;                  // pass-1 butterflies per transform
   constexpr int T1 = F * Q;
   constexpr int TPT1 = (T1 + NT - 1) / NT;
+  // prime-factor shell transform (no twiddles between the passes); Stockham passes otherwise
+  constexpr bool PFA = fused_pfa<N>();
+  constexpr int PA = PfaPlan<N>::A, PB = PfaPlan<N>::B, QS = PfaPlan<N>::QS, SB = PfaPlan<N>::SB;
+  constexpr int WL = fused_wlen<N>();        // W1 at Wk[0, WL), W2 at Wk[WL, 2 WL); Nyquist bin at WL - 1
+  static_assert(!PFA || (G == 2 * PA + 1 && RL == PB), "prime-factor plan");
 
   extern __shared__ __align__(16) float2 smem[];
   float2* buf = smem;                        // [F][N]
   constexpr int S = fused_stride<N>();       // transform stride in buf
   static_assert(S % 2 == 0, "16-byte aligned rows");
   float2* Wk = buf + F * S;                  // [N+1]  W_k / 2, or W1 [N+1] then W2 [N+1]
-  float2* tw2 = Wk + (PFTCC_FUSED_W12 ? 2 : 1) * (N + 1);   // [2N]   exp(-i pi a / N)
+  float2* tw2 = Wk + (PFTCC_FUSED_W12 ? 2 : 1) * WL;   // [2N]   exp(-i pi a / N)
   float2* twL = tw2 + 2 * N;                 // [RL-1][NsL]  twL[(t-1) NsL + g] = tw2[2 g t]
   float2* rck = twL + fused_twl_len<N>();    // [2][N] ref x |CTF| of the tile's references (RST)
   float* ctfk = reinterpret_cast<float*>(rck + 2 * N);   // [N]
@@ -273,11 +361,39 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   const int nk = a.nk;
 
   for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
-  if constexpr (PFTCC_FUSED_TWL)
+  if constexpr (PFTCC_FUSED_TWL && !PFA)
     for (int i = tid; i < (RL - 1) * NsL; i += NT) {
       const int t = i / NsL + 1, g = i - (t - 1) * NsL;
       twL[i] = a.tw2g[2 * g * t];
     }
+  // prime-factor index tables, in the last-pass twiddle table's place (read once per shell, so
+  // that no per-lane index is held in registers across the shell loop):
+  //   p1tab[j]: pass-1 butterfly j: LDS column jb (bits 0-7), offset ob of its second q pair
+  //             (bits 8-17), sign of z[b] (bit 30) and of z[b + Q] (bit 31)
+  //   p3tab[g]: last-pass butterfly g: positions o1 (bits 0-8) and o2 (bits 9-17), own mirror (bit 30)
+  //   wptab[m]: LDS position of spectrum bin m, 0 <= m <= N
+  int* const p1tab = reinterpret_cast<int*>(twL);
+  int* const p3tab = p1tab + Q;
+  int* const wptab = p3tab + G;
+  static_assert(!PFA || (size_t)(Q + G + N + 1) * sizeof(int) <= sizeof(float2) * fused_twl_len<N>(),
+                "prime-factor tables fit the twiddle table's space");
+  static_assert(!PFA || (2 * N < 1024 && 4 * QS < 512 && QS < 256), "prime-factor table fields");
+  if constexpr (PFA) {
+    for (int j = tid; j < Q; j += NT) {
+      const bool lo = 2 * j < Q;
+      const int ob = lo ? 4 * j + 2 * Q : 4 * j - 2 * Q;
+      const int jb = (2 * j) % PA + ((2 * j) % PB) * SB;
+      p1tab[j] = jb | (ob << 8) | ((j & 1) << 30) | ((int)((j & 1) != (lo ? 0 : 1)) << 31);
+    }
+    for (int g = tid; g < G; g += NT) {
+      int k1, ka;
+      PfaPlan<N>::rep(g, k1, ka);
+      const int o1 = k1 * QS + ka;
+      const int o2 = ((4 - k1) & 3) * QS + (ka == 0 ? 0 : PA - ka);
+      p3tab[g] = o1 | (o2 << 9) | ((int)(ka == 0 && (k1 & 1) == 0) << 30);
+    }
+    for (int m = tid; m <= N; m += NT) wptab[m] = PfaPlan<N>::pos(m);
+  }
 
   // RST: the tile's references and their rows
   int rr0 = 0;
@@ -384,8 +500,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
             if constexpr (PFTCC_FUSED_W12) {
               if (a.mode == 1) wh = make_float2(wkp * wh.x, wkp * wh.y);
               const float2 t = tw2[m];
-              Wk[m] = cmul(make_float2(1.f + t.y, t.x), wh);
-              Wk[N + 1 + m] = cmul(make_float2(1.f - t.y, -t.x), wh);
+              const int wp = PFA ? wptab[m] : m;
+              Wk[wp] = cmul(make_float2(1.f + t.y, t.x), wh);
+              Wk[WL + wp] = cmul(make_float2(1.f - t.y, -t.x), wh);
             } else {
               Wk[m] = wh;
             }
@@ -418,8 +535,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           // are applied to the staged spectrum (with the Euclid shell weight) once per shell
           if (a.mode == 1) wh = make_float2(wk * wh.x, wk * wh.y);
           const float2 t = tw2[m];
-          Wk[m] = cmul(make_float2(1.f + t.y, t.x), wh);               // conj(A') W / 2
-          Wk[N + 1 + m] = cmul(make_float2(1.f - t.y, -t.x), wh);      // conj(B') W / 2
+          const int wp = PFA ? wptab[m] : m;
+          Wk[wp] = cmul(make_float2(1.f + t.y, t.x), wh);               // conj(A') W / 2
+          Wk[WL + wp] = cmul(make_float2(1.f - t.y, -t.x), wh);      // conj(B') W / 2
         } else {
           Wk[m] = wh;
         }
@@ -432,22 +550,27 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       if (!(T1 % NT == 0 || task < T1)) continue;
       const int f = task / Q, j = task % Q;
       float2 y[4];
+      // q pairs of the butterfly: Stockham z[j], z[j + Q] (+ their mirrors z[j + 2Q], z[j + 3Q]);
+      // prime-factor: z[b], z[b + Q] with b = 2j or 2j + 2Q, i.e. q at 4j and 4j +- 2Q
+      const int p1 = PFA ? p1tab[j] : 0;
+      const int oa = PFA ? 4 * j : 2 * j;
+      const int ob = PFA ? (p1 >> 8) & 1023 : 2 * j + N / 2;
       if (slotok[c]) {
         float2 q0, q1, q2, q3;
         if constexpr (RST) {
           const float2* rc = rck + (((tile * F + f) / a.ns) - rr0) * N;
-          const float4 ra = *reinterpret_cast<const float4*>(rc + 2 * j);
-          const float4 rb = *reinterpret_cast<const float4*>(rc + 2 * j + N / 2);
+          const float4 ra = *reinterpret_cast<const float4*>(rc + oa);
+          const float4 rb = *reinterpret_cast<const float4*>(rc + ob);
           q0 = make_float2(ra.x, ra.y);
           q1 = make_float2(ra.z, ra.w);
           q2 = make_float2(rb.x, rb.y);
           q3 = make_float2(rb.z, rb.w);
         } else {
           const float2* r = rrow[c] + (size_t)kk * N;
-          const float4 ra = *reinterpret_cast<const float4*>(r + 2 * j);
-          const float4 rb = *reinterpret_cast<const float4*>(r + 2 * j + N / 2);
-          const float2 c01 = *reinterpret_cast<const float2*>(ctfk + 2 * j);
-          const float2 c23 = *reinterpret_cast<const float2*>(ctfk + 2 * j + N / 2);
+          const float4 ra = *reinterpret_cast<const float4*>(r + oa);
+          const float4 rb = *reinterpret_cast<const float4*>(r + ob);
+          const float2 c01 = *reinterpret_cast<const float2*>(ctfk + oa);
+          const float2 c23 = *reinterpret_cast<const float2*>(ctfk + ob);
           q0 = make_float2(ra.x * c01.x, ra.y * c01.x);
           q1 = make_float2(ra.z * c01.y, ra.w * c01.y);
           q2 = make_float2(rb.x * c23.x, rb.y * c23.x);
@@ -455,8 +578,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
         }
         if (srow[c]) {
           const float2* s = srow[c] + (size_t)kk * N;
-          const float4 sa = *reinterpret_cast<const float4*>(s + 2 * j);
-          const float4 sb = *reinterpret_cast<const float4*>(s + 2 * j + N / 2);
+          const float4 sa = *reinterpret_cast<const float4*>(s + oa);
+          const float4 sb = *reinterpret_cast<const float4*>(s + ob);
           q0 = cmul(q0, make_float2(sa.x, sa.y));
           q1 = cmul(q1, make_float2(sa.z, sa.w));
           q2 = cmul(q2, make_float2(sb.x, sb.y));
@@ -478,17 +601,32 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           if (donorm) sqp[c] += e2;
           if (a.mode == 1 && docorr) offp[c] = fmaf(wk, e2, offp[c]);
         }
-        // z[j], z[j+Q], z[j+2Q], z[j+3Q]
-        y[0] = make_float2(q0.x + q0.y, q1.x + q1.y);
-        y[1] = make_float2(q2.x + q2.y, q3.x + q3.y);
-        y[2] = make_float2(q0.x - q0.y, q1.x - q1.y);
-        y[3] = make_float2(q2.x - q2.y, q3.x - q3.y);
+        if constexpr (PFA) {
+          // z[b + Q n1], n1 = 0..3: signs +-1 from b >= 2Q (j odd) and b + Q >= 2Q (2j >= Q)
+          const float sa = __int_as_float((int)(((unsigned)p1 << 1) & 0x80000000u) | 0x3f800000);
+          const float sb = __int_as_float((int)((unsigned)p1 & 0x80000000u) | 0x3f800000);
+          y[0] = make_float2(fmaf(sa, q0.y, q0.x), fmaf(sa, q1.y, q1.x));
+          y[1] = make_float2(fmaf(sb, q2.y, q2.x), fmaf(sb, q3.y, q3.x));
+          y[2] = make_float2(fmaf(-sa, q0.y, q0.x), fmaf(-sa, q1.y, q1.x));
+          y[3] = make_float2(fmaf(-sb, q2.y, q2.x), fmaf(-sb, q3.y, q3.x));
+        } else {
+          // z[j], z[j+Q], z[j+2Q], z[j+3Q]
+          y[0] = make_float2(q0.x + q0.y, q1.x + q1.y);
+          y[1] = make_float2(q2.x + q2.y, q3.x + q3.y);
+          y[2] = make_float2(q0.x - q0.y, q1.x - q1.y);
+          y[3] = make_float2(q2.x - q2.y, q3.x - q3.y);
+        }
       } else {
         y[0] = y[1] = y[2] = y[3] = make_float2(0.f, 0.f);
       }
       if (docorr) {
         Dft<4>::run(y);
-        if constexpr (PFTCC_FUSED_P1V4) {
+        if constexpr (PFA) {
+          // outputs k1 at k1 Q + (nA, nB) of b: nA = 2j mod A, nB = 2j mod B
+          const int jb = p1 & 255;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) buf[f * S + t * QS + jb] = y[t];
+        } else if constexpr (PFTCC_FUSED_P1V4) {
           float4* dst = reinterpret_cast<float4*>(buf + f * S + 4 * j);   // 16-byte aligned: S even
           dst[0] = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
           dst[1] = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
@@ -501,8 +639,47 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     if (!docorr) continue;
     __syncthreads();
     // ---- middle passes in LDS ----
-    lds_passes<N, F, NT, 1, NP - 1, S>(buf, tw2, tid);
-    // ---- last pass (radix RL, Ns = NsL) on mirrored butterflies + split + conj(V) W_k ----
+    if constexpr (PFA) {
+      pfa_pass_a<N, F, NT, S>(buf, tid);
+      __syncthreads();
+    } else {
+      lds_passes<N, F, NT, 1, NP - 1, S>(buf, tw2, tid);
+    }
+    // ---- last pass (radix RL) on mirrored butterflies + split + conj(V) W_k ----
+    if constexpr (PFA) {
+#pragma unroll
+      for (int c = 0; c < TPTL; ++c) {
+        const int task = tid + c * NT;
+        if (!(TL % NT == 0 || task < TL)) continue;
+        const int f = task / G, g = task % G;
+        const int p3 = p3tab[g];
+        const bool self = (p3 >> 30) & 1;
+        const int o1 = p3 & 511;              // positions of the bins m(k1, kA, t)
+        const int o2 = (p3 >> 9) & 511;       // and of m(-k1, -kA, t)
+        float2 y1[RL], y2[RL];
+#pragma unroll
+        for (int t = 0; t < RL; ++t) y1[t] = buf[f * S + o1 + t * SB];
+        Dft<RL>::run(y1);
+        if (!self) {
+#pragma unroll
+          for (int t = 0; t < RL; ++t) y2[t] = buf[f * S + o2 + t * SB];
+          Dft<RL>::run(y2);
+        } else {
+#pragma unroll
+          for (int t = 0; t < RL; ++t) y2[t] = make_float2(0.f, 0.f);
+        }
+        // the bin N - m of y1[t] is y1[-t] (self) or y2[-t]; Wk is staged in the same order.  Each
+        // bin is accumulated together with its mirror, so that the staged values are used as read.
+#pragma unroll
+        for (int t = 0; t < RL; ++t) {
+          const int tn = (RL - t) % RL;
+          acc_w12(acc[c][t], y1[t], vsel(self, y1[tn], y2[tn]), Wk[o1 + t * SB], Wk[WL + o1 + t * SB]);
+          if (!self) acc_w12(acc[c][RL + tn], y2[tn], y1[t], Wk[o2 + tn * SB], Wk[WL + o2 + tn * SB]);
+        }
+        if (g == 0) acc_w12(acc[c][2 * RL], y1[0], y1[0], Wk[WL - 1], Wk[2 * WL - 1]);
+      }
+      continue;
+    }
 #pragma unroll
     for (int c = 0; c < TPTL; ++c) {
       const int task = tid + c * NT;
@@ -532,7 +709,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
         const float2 b0 = y1[(RL - t) % RL], b1 = y2[RL - 1 - t], b2 = y1[RL - 1 - t];
         const float2 B = vsel(g == 0, b0, vsel(pair, b1, b2));
         if constexpr (PFTCC_FUSED_W12) {
-          acc_w12(acc[c][t], y1[t], B, Wk[m], Wk[N + 1 + m]);
+          acc_w12(acc[c][t], y1[t], B, Wk[m], Wk[WL + m]);
         } else {
           const float2 V = split_bin2(y1[t], B, tw2[m]);
           const float2 p = cmulcj(V, Wk[m]);
@@ -545,7 +722,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
         for (int t = 0; t < RL; ++t) {
           const int m = j2 + t * NsL;
           if constexpr (PFTCC_FUSED_W12) {
-            acc_w12(acc[c][RL + t], y2[t], y1[RL - 1 - t], Wk[m], Wk[N + 1 + m]);
+            acc_w12(acc[c][RL + t], y2[t], y1[RL - 1 - t], Wk[m], Wk[WL + m]);
           } else {
             const float2 V = split_bin2(y2[t], y1[RL - 1 - t], tw2[m]);
             const float2 p = cmulcj(V, Wk[m]);
@@ -556,7 +733,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       }
       if (g == 0) {
         if constexpr (PFTCC_FUSED_W12) {
-          acc_w12(acc[c][2 * RL], y1[0], y1[0], Wk[N], Wk[N + 1 + N]);
+          acc_w12(acc[c][2 * RL], y1[0], y1[0], Wk[N], Wk[WL + N]);
         } else {
           const float2 V = split_bin2(y1[0], y1[0], tw2[N]);
           const float2 p = cmulcj(V, Wk[N]);
@@ -577,8 +754,43 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     }
   }
   // ---- Y = pre-processed inverse input, conj'd, from the register spectra ----
+  if constexpr (PFA) {
+#pragma unroll
+    for (int c = 0; c < TPTL; ++c) {
+      const int task = tid + c * NT;
+      if (!(TL % NT == 0 || task < TL)) continue;
+      const int f = task / G, g = task % G;
+      int k1, ka;
+      PfaPlan<N>::rep(g, k1, ka);
+      const bool self = ka == 0 && (k1 & 1) == 0;
+      const int mb1 = (Q * k1 + (N / PA) * ka) % N;
+      const int mb2 = (Q * ((4 - k1) & 3) + (N / PA) * (ka == 0 ? 0 : PA - ka)) % N;
+#pragma unroll
+      for (int t = 0; t < RL; ++t) {
+        int m = mb1 + (N / RL) * t;
+        if (m >= N) m -= N;
+        float2 X = acc[c][t];
+        float2 Xn = vsel(self, vsel(g == 0 && t == 0, acc[c][2 * RL], acc[c][(RL - t) % RL]),
+                         acc[c][RL + (RL - t) % RL]);
+        if (m == 0) {
+          X.y = 0.f;
+          Xn.y = 0.f;
+        }
+        buf[f * S + m] = unsplit_bin_conj(X, Xn, cconj(tw2[m]));
+      }
+      if (!self) {
+#pragma unroll
+        for (int t = 0; t < RL; ++t) {
+          int m = mb2 + (N / RL) * t;
+          if (m >= N) m -= N;
+          buf[f * S + m] = unsplit_bin_conj(acc[c][RL + t], acc[c][(RL - t) % RL], cconj(tw2[m]));
+        }
+      }
+    }
+  }
 #pragma unroll
   for (int c = 0; c < TPTL; ++c) {
+    if constexpr (PFA) break;
     const int task = tid + c * NT;
     if (!(TL % NT == 0 || task < TL)) continue;
     const int f = task / G, g = task % G;
