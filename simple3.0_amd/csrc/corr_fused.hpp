@@ -148,6 +148,15 @@ constexpr int fused_W_tuned() {
 #ifndef PFTCC_FUSED_PFA
 #define PFTCC_FUSED_PFA 1      // prime-factor shell transform where N = 4 A B allows it
 #endif
+#ifndef PFTCC_PFA_SA
+#define PFTCC_PFA_SA 1         // prime-factor LDS layout strides (tuning builds; 0: the defaults)
+#endif
+#ifndef PFTCC_PFA_SB
+#define PFTCC_PFA_SB 0
+#endif
+#ifndef PFTCC_PFA_QS
+#define PFTCC_PFA_QS 0
+#endif
 
 __host__ __device__ constexpr int pfa_modinv(int a, int m) {
   for (int x = 1; x < m; ++x)
@@ -170,14 +179,17 @@ struct PfaPlan {
   static constexpr int A = ok ? Fp<N>::R(1) : 1;
   static constexpr int B = ok ? Fp<N>::R(2) : 1;
   static constexpr int Q = N / 4;
-  static constexpr int SB = A + 2;            // stride of b
-  static constexpr int QS = B * SB;           // stride of k1
+  static constexpr int SA = PFTCC_PFA_SA;     // stride of a
+  static constexpr int SB = PFTCC_PFA_SB > 0 ? PFTCC_PFA_SB : A + 2;   // stride of b
+  static constexpr int QS = PFTCC_PFA_QS > 0 ? PFTCC_PFA_QS : B * SB;  // stride of k1
+  static_assert(!ok || (((SA == 1 && SB >= A) || (SB == 1 && SA >= B)) && QS >= (A - 1) * SA + (B - 1) * SB + 1),
+                "prime-factor layout: distinct positions within a row");
   static constexpr int IA = ok ? pfa_modinv((N / A) % A, A) : 0;   // kA = m IA mod A
   static constexpr int IB = ok ? pfa_modinv((N / B) % B, B) : 0;   // kB = m IB mod B
   // LDS position of output bin m (k1 = m mod 4 since N/4 = 1 mod 4); the Nyquist bin N after them
   static __device__ __forceinline__ int pos(int m) {
     if (m >= N) return 4 * QS;
-    return (m & 3) * QS + (m * IA) % A + ((m * IB) % B) * SB;
+    return (m & 3) * QS + ((m * IA) % A) * SA + ((m * IB) % B) * SB;
   }
   // Butterfly g of the last pass, 0 <= g < 2 A + 1: (k1, kA) and its mirror (-k1, -kA), whose
   // outputs hold the bins N - m.  g = 0: (0, 0), g = 2: (2, 0) are their own mirrors; g = 1:
@@ -197,7 +209,7 @@ struct PfaPlan {
 // Pass A of the prime-factor transform: DFT of A points along a for every (k1, nB), in place.
 template <int N, int F, int NT, int S>
 __device__ __forceinline__ void pfa_pass_a(float2* __restrict__ buf, int tid) {
-  constexpr int A = PfaPlan<N>::A, B = PfaPlan<N>::B, QS = PfaPlan<N>::QS, SB = PfaPlan<N>::SB;
+  constexpr int A = PfaPlan<N>::A, B = PfaPlan<N>::B, QS = PfaPlan<N>::QS, SA = PfaPlan<N>::SA, SB = PfaPlan<N>::SB;
   constexpr int TASKS = F * 4 * B;
   constexpr int TPT = (TASKS + NT - 1) / NT;
 #pragma unroll
@@ -208,10 +220,10 @@ __device__ __forceinline__ void pfa_pass_a(float2* __restrict__ buf, int tid) {
       float2* p = buf + f * S + (r / B) * QS + (r % B) * SB;
       float2 v[A];
 #pragma unroll
-      for (int t = 0; t < A; ++t) v[t] = p[t];
+      for (int t = 0; t < A; ++t) v[t] = p[t * SA];
       Dft<A>::run(v);
 #pragma unroll
-      for (int t = 0; t < A; ++t) p[t] = v[t];
+      for (int t = 0; t < A; ++t) p[t * SA] = v[t];
     }
   }
 }
@@ -317,7 +329,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   constexpr int TPT1 = (T1 + NT - 1) / NT;
   // prime-factor shell transform (no twiddles between the passes); Stockham passes otherwise
   constexpr bool PFA = fused_pfa<N>();
-  constexpr int PA = PfaPlan<N>::A, PB = PfaPlan<N>::B, QS = PfaPlan<N>::QS, SB = PfaPlan<N>::SB;
+  constexpr int PA = PfaPlan<N>::A, PB = PfaPlan<N>::B, QS = PfaPlan<N>::QS, SA = PfaPlan<N>::SA, SB = PfaPlan<N>::SB;
   constexpr int WL = fused_wlen<N>();        // W1 at Wk[0, WL), W2 at Wk[WL, 2 WL); Nyquist bin at WL - 1
   static_assert(!PFA || (G == 2 * PA + 1 && RL == PB), "prime-factor plan");
 
@@ -382,14 +394,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     for (int j = tid; j < Q; j += NT) {
       const bool lo = 2 * j < Q;
       const int ob = lo ? 4 * j + 2 * Q : 4 * j - 2 * Q;
-      const int jb = (2 * j) % PA + ((2 * j) % PB) * SB;
+      const int jb = ((2 * j) % PA) * SA + ((2 * j) % PB) * SB;
       p1tab[j] = jb | (ob << 8) | ((j & 1) << 30) | ((int)((j & 1) != (lo ? 0 : 1)) << 31);
     }
     for (int g = tid; g < G; g += NT) {
       int k1, ka;
       PfaPlan<N>::rep(g, k1, ka);
-      const int o1 = k1 * QS + ka;
-      const int o2 = ((4 - k1) & 3) * QS + (ka == 0 ? 0 : PA - ka);
+      const int o1 = k1 * QS + ka * SA;
+      const int o2 = ((4 - k1) & 3) * QS + (ka == 0 ? 0 : PA - ka) * SA;
       p3tab[g] = o1 | (o2 << 9) | ((int)(ka == 0 && (k1 & 1) == 0) << 30);
     }
     for (int m = tid; m <= N; m += NT) wptab[m] = PfaPlan<N>::pos(m);
@@ -408,8 +420,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   double rsum0 = 0.0, rsum1 = 0.0;     // this thread's share of the two references' norms
   // pass-1 slots: pair, reference and shift rows are fixed for all shells
   float sqp[TPT1], offp[TPT1];
-  const float2* rrow[TPT1];
-  const float2* srow[TPT1];
+  // element offsets (32 bits) of the pair's reference and shift rows from the uniform bases refs
+  // and a.shtab; the shell's rows follow at + kk N
+  unsigned int roff[TPT1], soff[TPT1];
   int slotref[TPT1];
   bool slotok[TPT1];
 #pragma unroll
@@ -420,15 +433,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     slotok[c] = task < T1 && P < npairs;
     sqp[c] = 0.f;
     offp[c] = 0.f;
-    rrow[c] = refs;
-    srow[c] = nullptr;
+    roff[c] = 0u;
+    soff[c] = 0u;
     slotref[c] = 0;
     if (slotok[c]) {
       const int ir = P / a.ns, is = P % a.ns;
       const int iref = a.rlist ? a.rlist[(size_t)pi * a.rlist_ld + ir] : ir;
       slotref[c] = iref;
-      rrow[c] = refs + (size_t)iref * nk * N;
-      srow[c] = a.shtab ? a.shtab + (size_t)is * nk * N : nullptr;
+      roff[c] = (unsigned int)iref * (unsigned int)(nk * N);
+      soff[c] = (unsigned int)is * (unsigned int)(nk * N);
     }
   }
 
@@ -566,7 +579,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           q2 = make_float2(rb.x, rb.y);
           q3 = make_float2(rb.z, rb.w);
         } else {
-          const float2* r = rrow[c] + (size_t)kk * N;
+          const float2* r = refs + (roff[c] + (unsigned int)(kk * N));
           const float4 ra = *reinterpret_cast<const float4*>(r + oa);
           const float4 rb = *reinterpret_cast<const float4*>(r + ob);
           const float2 c01 = *reinterpret_cast<const float2*>(ctfk + oa);
@@ -576,8 +589,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           q2 = make_float2(rb.x * c23.x, rb.y * c23.x);
           q3 = make_float2(rb.z * c23.y, rb.w * c23.y);
         }
-        if (srow[c]) {
-          const float2* s = srow[c] + (size_t)kk * N;
+        if (a.shtab) {
+          const float2* s = a.shtab + (soff[c] + (unsigned int)(kk * N));
           const float4 sa = *reinterpret_cast<const float4*>(s + oa);
           const float4 sb = *reinterpret_cast<const float4*>(s + ob);
           q0 = cmul(q0, make_float2(sa.x, sa.y));
@@ -656,24 +669,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
         const bool self = (p3 >> 30) & 1;
         const int o1 = p3 & 511;              // positions of the bins m(k1, kA, t)
         const int o2 = (p3 >> 9) & 511;       // and of m(-k1, -kA, t)
+        // a butterfly that is its own mirror has o2 = o1, so y2 = y1 there: the bin N - m of y1[t]
+        // is y2[-t] in every case, and only the bins of y1 are accumulated.
         float2 y1[RL], y2[RL];
 #pragma unroll
         for (int t = 0; t < RL; ++t) y1[t] = buf[f * S + o1 + t * SB];
         Dft<RL>::run(y1);
-        if (!self) {
 #pragma unroll
-          for (int t = 0; t < RL; ++t) y2[t] = buf[f * S + o2 + t * SB];
-          Dft<RL>::run(y2);
-        } else {
-#pragma unroll
-          for (int t = 0; t < RL; ++t) y2[t] = make_float2(0.f, 0.f);
-        }
-        // the bin N - m of y1[t] is y1[-t] (self) or y2[-t]; Wk is staged in the same order.  Each
-        // bin is accumulated together with its mirror, so that the staged values are used as read.
+        for (int t = 0; t < RL; ++t) y2[t] = buf[f * S + o2 + t * SB];
+        Dft<RL>::run(y2);
+        // Wk is staged in the same order.  Each bin is accumulated together with its mirror, so
+        // that the staged values are used as read.
 #pragma unroll
         for (int t = 0; t < RL; ++t) {
           const int tn = (RL - t) % RL;
-          acc_w12(acc[c][t], y1[t], vsel(self, y1[tn], y2[tn]), Wk[o1 + t * SB], Wk[WL + o1 + t * SB]);
+          acc_w12(acc[c][t], y1[t], y2[tn], Wk[o1 + t * SB], Wk[WL + o1 + t * SB]);
           if (!self) acc_w12(acc[c][RL + tn], y2[tn], y1[t], Wk[o2 + tn * SB], Wk[WL + o2 + tn * SB]);
         }
         if (g == 0) acc_w12(acc[c][2 * RL], y1[0], y1[0], Wk[WL - 1], Wk[2 * WL - 1]);

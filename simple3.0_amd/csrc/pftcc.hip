@@ -557,6 +557,13 @@ int fused_tile(int n, bool rst_ok, int ns, bool* rst) {
   return 0;
 }
 
+// The fused kernel addresses reference and shift rows by 32-bit element offsets from their
+// bases: references x shells x n and shifts x shells x n must stay below 2^32.
+bool fused_offsets_fit(const pftcc* h, int ns) {
+  const size_t row = (size_t)h->nk * h->n;
+  return (size_t)h->nrefs * row < ((size_t)1 << 32) && (size_t)std::max(ns, 1) * row < ((size_t)1 << 32);
+}
+
 void fused_go(int n, unsigned blocks, hipStream_t s, const CorrArgs& a, bool rst) {
 #define X(N) if (n == N) { FusedLaunch<N>::go(blocks, s, a, rst); return; }
   PFTCC_FUSED_SIZES(X)
@@ -601,7 +608,9 @@ int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist
   a.tw2g = h->d_tw2;
   const int npairs = nr * a.ns;
   bool rst = false;
-  const int Ff = h->use_fused ? fused_tile(h->n, a.mode == 0 && !a.refinv && rlist_ld == 0, a.ns, &rst) : 0;
+  const int Ff = h->use_fused && fused_offsets_fit(h, a.ns)
+                     ? fused_tile(h->n, a.mode == 0 && !a.refinv && rlist_ld == 0, a.ns, &rst)
+                     : 0;
   a.F = Ff > 0 ? Ff : pick_F(h->n, npairs);
   a.ntiles = (npairs + a.F - 1) / a.F;
   const size_t lds = corr_lds_bytes(h->n, a.F);
@@ -1396,7 +1405,7 @@ int pftcc_get_corr_tile(const pftcc_t* h, int32_t ns, int32_t* tile, int32_t* st
   const bool shifted = ns > 0;
   const bool norm = (h->match_filt || h->pssnr) && !(h->euclid && shifted);
   bool rst = false;
-  const int Ff = h->use_fused ? fused_tile(h->n, !h->euclid && !norm, std::max(ns, 1), &rst) : 0;
+  const int Ff = h->use_fused && fused_offsets_fit(h, ns) ? fused_tile(h->n, !h->euclid && !norm, std::max(ns, 1), &rst) : 0;
   *tile = Ff;
   *staged = rst ? 1 : 0;
   return PFTCC_OK;
