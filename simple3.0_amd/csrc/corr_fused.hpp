@@ -271,6 +271,12 @@ constexpr size_t fused_lds_bytes() {
          sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
 }
 
+// 16-byte load of elements elem, elem + 1 of a float2 array at a uniform base: a 32-bit byte
+// offset lets the load take the base from scalar registers.
+__device__ __forceinline__ float4 ld16(const float2* __restrict__ base, unsigned int elem) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (elem << 3));
+}
+
 // Value select (a conditional on array lvalues would select an address and force the array
 // out of registers).
 __device__ __forceinline__ float2 vsel(bool c, float2 a, float2 b) {
@@ -424,19 +430,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   // and a.shtab; the shell's rows follow at + kk N
   unsigned int roff[TPT1], soff[TPT1];
   int slotref[TPT1];
-  bool slotok[TPT1];
 #pragma unroll
   for (int c = 0; c < TPT1; ++c) {
     const int task = tid + c * NT;
     const int f = task / Q;
     const int P = tile * F + f;
-    slotok[c] = task < T1 && P < npairs;
+    const bool slotok = task < T1 && P < npairs;
     sqp[c] = 0.f;
     offp[c] = 0.f;
     roff[c] = 0u;
     soff[c] = 0u;
     slotref[c] = 0;
-    if (slotok[c]) {
+    if (slotok) {
       const int ir = P / a.ns, is = P % a.ns;
       const int iref = a.rlist ? a.rlist[(size_t)pi * a.rlist_ld + ir] : ir;
       slotref[c] = iref;
@@ -568,10 +573,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       const int p1 = PFA ? p1tab[j] : 0;
       const int oa = PFA ? 4 * j : 2 * j;
       const int ob = PFA ? (p1 >> 8) & 1023 : 2 * j + N / 2;
-      if (slotok[c]) {
+      // a slot past the tile's last pair reads valid rows (reference 0, shift 0, the second staged
+      // row) and its results are never read
+      {
         float2 q0, q1, q2, q3;
         if constexpr (RST) {
-          const float2* rc = rck + (((tile * F + f) / a.ns) - rr0) * N;
+          const float2* rc = rck + min(((tile * F + f) / a.ns) - rr0, 1) * N;
           const float4 ra = *reinterpret_cast<const float4*>(rc + oa);
           const float4 rb = *reinterpret_cast<const float4*>(rc + ob);
           q0 = make_float2(ra.x, ra.y);
@@ -579,9 +586,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           q2 = make_float2(rb.x, rb.y);
           q3 = make_float2(rb.z, rb.w);
         } else {
-          const float2* r = refs + (roff[c] + (unsigned int)(kk * N));
-          const float4 ra = *reinterpret_cast<const float4*>(r + oa);
-          const float4 rb = *reinterpret_cast<const float4*>(r + ob);
+          const unsigned int r = roff[c] + (unsigned int)(kk * N);
+          const float4 ra = ld16(refs, r + oa);
+          const float4 rb = ld16(refs, r + ob);
           const float2 c01 = *reinterpret_cast<const float2*>(ctfk + oa);
           const float2 c23 = *reinterpret_cast<const float2*>(ctfk + ob);
           q0 = make_float2(ra.x * c01.x, ra.y * c01.x);
@@ -589,10 +596,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           q2 = make_float2(rb.x * c23.x, rb.y * c23.x);
           q3 = make_float2(rb.z * c23.y, rb.w * c23.y);
         }
-        if (a.shtab) {
-          const float2* s = a.shtab + (soff[c] + (unsigned int)(kk * N));
-          const float4 sa = *reinterpret_cast<const float4*>(s + oa);
-          const float4 sb = *reinterpret_cast<const float4*>(s + ob);
+        if (RST || a.shtab) {   // the staged-reference variant runs for shifted searches only
+          const unsigned int r = soff[c] + (unsigned int)(kk * N);
+          const float4 sa = ld16(a.shtab, r + oa);
+          const float4 sb = ld16(a.shtab, r + ob);
           q0 = cmul(q0, make_float2(sa.x, sa.y));
           q1 = cmul(q1, make_float2(sa.z, sa.w));
           q2 = cmul(q2, make_float2(sb.x, sb.y));
@@ -629,8 +636,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           y[2] = make_float2(q0.x - q0.y, q1.x - q1.y);
           y[3] = make_float2(q2.x - q2.y, q3.x - q3.y);
         }
-      } else {
-        y[0] = y[1] = y[2] = y[3] = make_float2(0.f, 0.f);
       }
       if (docorr) {
         Dft<4>::run(y);

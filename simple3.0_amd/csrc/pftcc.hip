@@ -557,10 +557,10 @@ int fused_tile(int n, bool rst_ok, int ns, bool* rst) {
   return 0;
 }
 
-// The fused kernel addresses reference and shift rows by 32-bit element offsets from their
-// bases: references x shells x n and shifts x shells x n must stay below 2^32.
+// The fused kernel addresses reference and shift rows by 32-bit byte offsets from their bases:
+// references x shells x n and shifts x shells x n float2 must stay below 4 GiB.
 bool fused_offsets_fit(const pftcc* h, int ns) {
-  const size_t row = (size_t)h->nk * h->n;
+  const size_t row = (size_t)h->nk * h->n * sizeof(float2);
   return (size_t)h->nrefs * row < ((size_t)1 << 32) && (size_t)std::max(ns, 1) * row < ((size_t)1 << 32);
 }
 
