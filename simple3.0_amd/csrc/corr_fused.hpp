@@ -271,10 +271,16 @@ constexpr size_t fused_lds_bytes() {
          sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
 }
 
-// 16-byte load of elements elem, elem + 1 of a float2 array at a uniform base: a 32-bit byte
-// offset lets the load take the base from scalar registers.
+// Loads from an array at a uniform base by a 32-bit byte offset, which lets the load take the
+// base from scalar registers: ld16 reads elements elem and elem + 1 of a float2 array.
 __device__ __forceinline__ float4 ld16(const float2* __restrict__ base, unsigned int elem) {
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (elem << 3));
+}
+__device__ __forceinline__ float2 ld8(const float2* __restrict__ base, unsigned int elem) {
+  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) + (elem << 3));
+}
+__device__ __forceinline__ float ldf(const float* __restrict__ base, unsigned int elem) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (elem << 2));
 }
 
 // Value select (a conditional on array lvalues would select an address and force the array
@@ -461,16 +467,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   constexpr int JP = (N + NT - 1) / NT, MP = (N + 1 + NT - 1) / NT;
   float pcv[JP];
   float2 pr0[JP], pr1[JP], pwm[MP];
+  // the particle's |CTF| and memoized spectra: uniform bases, 32-bit offsets within the particle
+  const float* ctfp = a.ctf ? a.ctf + (size_t)ip * nk * N : nullptr;
+  const float2* memop = a.memo + (size_t)ip * nk * (N + 1);
   auto fetch_stage = [&](int k) {
-    const size_t pr = (size_t)ip * nk + k;
 #pragma unroll
     for (int u = 0; u < JP; ++u) {
       const int j = tid + u * NT;
       if (j < N) {
-        pcv[u] = a.ctf ? a.ctf[pr * N + j] : 1.0f;
+        pcv[u] = ctfp ? ldf(ctfp, (unsigned int)(k * N + j)) : 1.0f;
         if constexpr (RST) {
-          pr0[u] = rs0[(size_t)k * N + j];
-          pr1[u] = rs1[(size_t)k * N + j];
+          pr0[u] = ld8(rs0, (unsigned int)(k * N + j));
+          pr1[u] = ld8(rs1, (unsigned int)(k * N + j));
         }
       }
     }
@@ -478,14 +486,16 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
 #pragma unroll
       for (int u = 0; u < MP; ++u) {
         const int m = tid + u * NT;
-        if (m <= N) pwm[u] = a.memo[pr * (N + 1) + m];
+        if (m <= N) pwm[u] = ld8(memop, (unsigned int)(k * (N + 1) + m));
       }
   };
   if constexpr (PFTCC_FUSED_PREFETCH)
     if (kloop > 0) fetch_stage(0);
   for (int kk = 0; kk < kloop; ++kk) {
     const bool docorr = kk < a.kcorr;
-    const bool donorm = kk < a.knorm;
+    // the staged-reference variant runs for shifted CC searches, whose norms cover every shell
+    // (knorm = nk = kloop)
+    const bool donorm = RST || kk < a.knorm;
     __syncthreads();   // the previous shell's last pass has finished with buf and Wk
     const size_t prow = (size_t)ip * nk + kk;
     if constexpr (PFTCC_FUSED_PREFETCH) {
