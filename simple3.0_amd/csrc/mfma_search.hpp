@@ -562,34 +562,43 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
     const float sc = live ? (float)(inv4n / den[f]) : 0.f;
     float bv = -INFINITY;      // this lane's best value and its rotation index (ties: lowest)
     int bi = -1;
+    // the lane's butterflies j = lane, lane + 64, ... (JB of them), transformed first; their outputs
+    // jj = j + t NsL are then visited t-major, i.e. in increasing rotation order, so that a strict
+    // comparison keeps the lowest index among equal values
+    constexpr int JB = (NsL + 63) / 64;
+    float2 v[JB][RL];
 #pragma unroll
-    for (int j0 = 0; j0 < NsL; j0 += 64) {
-      const int j = j0 + lane;
+    for (int jb = 0; jb < JB; ++jb) {
+      const int j = jb * 64 + lane;
       if (j >= NsL) continue;
-      float2 v[RL];
 #pragma unroll
-      for (int t = 0; t < RL; ++t) v[t] = buf[f * N + j + t * NsL];
+      for (int t = 0; t < RL; ++t) v[jb][t] = buf[f * N + j + t * NsL];
 #pragma unroll
-      for (int t = 1; t < RL; ++t) v[t] = cmul(v[t], tw2[2 * j * t]);
-      Dft<RL>::run(v);
-      if (!live) continue;
+      for (int t = 1; t < RL; ++t) v[jb][t] = cmul(v[jb][t], tw2[2 * j * t]);
+      Dft<RL>::run(v[jb]);
+    }
+    if (live) {
 #pragma unroll
-      for (int t = 0; t < RL; ++t) {
-        const int jj = j + t * NsL;                   // output index: rotations 2 jj, 2 jj + 1
-        const float x0 = v[t].x * sc, x1 = -v[t].y * sc;
-        if (a.cc_out) {
-          float2* oo = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * N));
-          oo[jj] = make_float2(x0, x1);
+      for (int t = 0; t < RL; ++t)
+#pragma unroll
+        for (int jb = 0; jb < JB; ++jb) {
+          const int j = jb * 64 + lane;
+          if (j >= NsL) continue;
+          const int jj = j + t * NsL;                   // output index: rotations 2 jj, 2 jj + 1
+          const float x0 = v[jb][t].x * sc, x1 = -v[jb][t].y * sc;
+          if (a.cc_out) {
+            float2* oo = reinterpret_cast<float2*>(a.cc_out + ((size_t)a.outpos[p] * a.nrefs + r) * (2 * N));
+            oo[jj] = make_float2(x0, x1);
+          }
+          if (x0 > bv) {
+            bv = x0;
+            bi = 2 * jj;
+          }
+          if (x1 > bv) {
+            bv = x1;
+            bi = 2 * jj + 1;
+          }
         }
-        if (x0 > bv || (x0 == bv && 2 * jj < bi)) {
-          bv = x0;
-          bi = 2 * jj;
-        }
-        if (x1 > bv || (x1 == bv && 2 * jj + 1 < bi)) {
-          bv = x1;
-          bi = 2 * jj + 1;
-        }
-      }
     }
     if (want_key && live) {
       unsigned long long bk = 0ull;
