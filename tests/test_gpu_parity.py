@@ -340,3 +340,29 @@ def test_prep_matchfilt_matches_oracle(pssnr, ctf):
     with pytest.raises(PftccError):
         lib.prep_matchfilt_batch([1, 1], [1, 2], [1, 1])
     lib.kill()
+
+
+@pytest.mark.parametrize("mode", ["euclid", "match_filt"])
+def test_fused_prime_factor_objectives(mode):
+    """Ring length 308 runs the prime-factor shell transform of the fused kernel (308 = 4 x 11 x 7).
+    Its Euclid branch (shell weights staged with the particle spectrum, the offset sums of pass 1)
+    and its shell-normalised branch (per-pair reference scales in pass 1) against the FP64 oracle,
+    unshifted and shifted, over a batch whose last tile is partial and whose particles fall in both
+    eo classes."""
+    say(f"fused prime-factor kernel n=308 {mode}")
+    pf, cps = make_case(seed=57, n=308, kto=24, ctf=True, nrefs=4, nptcls=5, ldim=200, eo_mixed=True,
+                        euclid=(mode == "euclid"), match_filt=(mode == "match_filt"))
+    lib = to_lib(pf, cps)
+    shifts = np.array([[0.0, 0.0], [1.5, -0.5], [-2.0, 1.0]], np.float32)
+    got = lib.gencorrs_batch(np.arange(1, 6), None, shifts)
+    for p in range(5):
+        for r in range(4):
+            for s in range(3):
+                sh = tuple(shifts[s]) if s else None
+                ref = pf.gencorrs_euclid(r, p, sh, method="fft") if mode == "euclid" else \
+                    pf.gencorrs(r, p, sh, method="fft")
+                assert peak_rel_err(got[p, r, s], ref) < FP32_TOL, (p, r, s)
+    for p, r in [(0, 0), (4, 3)]:
+        ref = pf.gencorrs_euclid(r, p, None, method="fft") if mode == "euclid" else pf.gencorrs(r, p, method="fft")
+        assert peak_rel_err(lib.gencorrs(r + 1, p + 1), ref) < FP32_TOL
+    lib.kill()
