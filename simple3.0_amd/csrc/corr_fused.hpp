@@ -289,6 +289,18 @@ __device__ __forceinline__ float2 vsel(bool c, float2 a, float2 b) {
   return make_float2(c ? a.x : b.x, c ? a.y : b.y);
 }
 
+// Staged particle spectra of bin position wp: W1 at wp and W2 at WL + wp, or (I: the prime-factor
+// order) both at float4 position wp.
+template <bool I, int WL>
+__device__ __forceinline__ void wk_store(float2* Wk, int wp, float2 w1, float2 w2) {
+  if constexpr (I) {
+    reinterpret_cast<float4*>(Wk)[wp] = make_float4(w1.x, w1.y, w2.x, w2.y);
+  } else {
+    Wk[wp] = w1;
+    Wk[WL + wp] = w2;
+  }
+}
+
 // acc += conj(Z) w1 + Zn w2: the split and the product conj(V) W of one bin with the split's
 // m-dependent factors folded into w1 = conj(A') W / 2 and w2 = conj(B') W / 2.
 __device__ __forceinline__ void acc_w12(float2& acc, float2 Z, float2 Zn, float2 w1, float2 w2) {
@@ -351,6 +363,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   static_assert(S % 2 == 0, "16-byte aligned rows");
   float2* Wk = buf + F * S;                  // [N+1]  W_k / 2, or W1 [N+1] then W2 [N+1]
   float2* tw2 = Wk + (PFTCC_FUSED_W12 ? 2 : 1) * WL;   // [2N]   exp(-i pi a / N)
+  // prime-factor order: W1 and W2 of a bin interleaved, one 16-byte read per bin in the last pass
+  const float4* Wk4 = reinterpret_cast<const float4*>(Wk);
+  static_assert(!PFA || (F * S) % 2 == 0, "16-byte aligned staged spectra");
   float2* twL = tw2 + 2 * N;                 // [RL-1][NsL]  twL[(t-1) NsL + g] = tw2[2 g t]
   float2* rck = twL + fused_twl_len<N>();    // [2][N] ref x |CTF| of the tile's references (RST)
   float* ctfk = reinterpret_cast<float*>(rck + 2 * N);   // [N]
@@ -529,8 +544,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
               if (a.mode == 1) wh = make_float2(wkp * wh.x, wkp * wh.y);
               const float2 t = tw2[m];
               const int wp = PFA ? wptab[m] : m;
-              Wk[wp] = cmul(make_float2(1.f + t.y, t.x), wh);
-              Wk[WL + wp] = cmul(make_float2(1.f - t.y, -t.x), wh);
+              wk_store<PFA, WL>(Wk, wp, cmul(make_float2(1.f + t.y, t.x), wh), cmul(make_float2(1.f - t.y, -t.x), wh));
             } else {
               Wk[m] = wh;
             }
@@ -564,8 +578,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           if (a.mode == 1) wh = make_float2(wk * wh.x, wk * wh.y);
           const float2 t = tw2[m];
           const int wp = PFA ? wptab[m] : m;
-          Wk[wp] = cmul(make_float2(1.f + t.y, t.x), wh);               // conj(A') W / 2
-          Wk[WL + wp] = cmul(make_float2(1.f - t.y, -t.x), wh);      // conj(B') W / 2
+          // conj(A') W / 2 and conj(B') W / 2
+          wk_store<PFA, WL>(Wk, wp, cmul(make_float2(1.f + t.y, t.x), wh), cmul(make_float2(1.f - t.y, -t.x), wh));
         } else {
           Wk[m] = wh;
         }
@@ -698,10 +712,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
 #pragma unroll
         for (int t = 0; t < RL; ++t) {
           const int tn = (RL - t) % RL;
-          acc_w12(acc[c][t], y1[t], y2[tn], Wk[o1 + t * SB], Wk[WL + o1 + t * SB]);
-          if (!self) acc_w12(acc[c][RL + tn], y2[tn], y1[t], Wk[o2 + tn * SB], Wk[WL + o2 + tn * SB]);
+          const float4 wa = Wk4[o1 + t * SB], wb = Wk4[o2 + tn * SB];
+          acc_w12(acc[c][t], y1[t], y2[tn], make_float2(wa.x, wa.y), make_float2(wa.z, wa.w));
+          if (!self) acc_w12(acc[c][RL + tn], y2[tn], y1[t], make_float2(wb.x, wb.y), make_float2(wb.z, wb.w));
         }
-        if (g == 0) acc_w12(acc[c][2 * RL], y1[0], y1[0], Wk[WL - 1], Wk[2 * WL - 1]);
+        if (g == 0) {
+          const float4 wn = Wk4[WL - 1];
+          acc_w12(acc[c][2 * RL], y1[0], y1[0], make_float2(wn.x, wn.y), make_float2(wn.z, wn.w));
+        }
       }
       continue;
     }
