@@ -574,6 +574,8 @@ void fused_go(int n, unsigned blocks, hipStream_t s, const CorrArgs& a, bool rst
 // rlist_ld > 0: d_rlist holds one list of nr references per particle slot (row stride rlist_ld).
 int launch_corr(pftcc* h, int np, const int* d_plist, int nr, const int* d_rlist, int ns,
                 const float2* d_shtab, bool shifted, float* d_cc, unsigned long long* d_best, int rlist_ld = 0) {
+  // the staged-reference kernel reads the shift table of every shifted search without testing it
+  if (shifted && !d_shtab) return fail(h, PFTCC_E_ARG, "shifted search without a shift table");
   CorrArgs a{};
   a.n = h->n;
   a.nk = h->nk;
