@@ -255,7 +255,11 @@ __global__ void __launch_bounds__(64) k_zgemm(const bf16x8* __restrict__ Ahi, co
 // is loaded in the inner loop.  The XCD-aware order keeps the slabs of one m on one XCD (the
 // particle operand of that m stays in its L2).
 template <int SPLIT, int TR, int NSTEP, bool H>
-__global__ void __launch_bounds__(64) k_zgemm_as(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
+#ifndef PFTCC_ZGEMM_AS_W
+#define PFTCC_ZGEMM_AS_W 1     // minimum wavefronts per SIMD imposed on the register allocation (tuning builds)
+#endif
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFTCC_ZGEMM_AS_W)))
+k_zgemm_as(const bf16x8* __restrict__ Ahi, const bf16x8* __restrict__ Alo,
                                                  const bf16x8* __restrict__ Bhi, const bf16x8* __restrict__ Blo,
                                                  int rows_pad, int cols_pad, int n, float2* __restrict__ Z,
                                                  const unsigned int* __restrict__ gmaxA,
