@@ -848,6 +848,12 @@ int round_up(int x, int m) { return (x + m - 1) / m * m; }
 #ifndef PFTCC_ZGEMM_AS_TR
 #define PFTCC_ZGEMM_AS_TR 2
 #endif
+#ifndef PFTCC_MFMA_CHUNK
+#define PFTCC_MFMA_CHUNK 256   // particles per contraction / inverse-stage launch (multiple of 64)
+#endif
+#ifndef PFTCC_MFMA_ZCAP_GIB
+#define PFTCC_MFMA_ZCAP_GIB 4  // cap on the spectra Z of one chunk
+#endif
 // Operand formats of the precision form.
 enum { OP_BF16 = 0, OP_F16 = 1, OP_FP8 = 2 };
 int op_format(int prec) {
@@ -1048,9 +1054,9 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
   std::vector<int> rowkey(nr);
   for (int r = 0; r < nr; ++r) rowkey[r] = rl[r / nse] * nse + r % nse;
   const int total = (int)pl.size();
-  // chunk of particles: keep Z (n+1) x rows_pad x Pc float2 within ~4 GiB
-  int Pc = 256;
-  while (Pc > 64 && (size_t)(n + 1) * ap.rows_pad * Pc * 8 > ((size_t)4 << 30)) Pc -= 64;
+  // chunk of particles: keep Z (n+1) x rows_pad x Pc float2 within the cap
+  int Pc = PFTCC_MFMA_CHUNK;
+  while (Pc > 64 && (size_t)(n + 1) * ap.rows_pad * Pc * 8 > ((size_t)PFTCC_MFMA_ZCAP_GIB << 30)) Pc -= 64;
   Pc = std::min(Pc, round_up(total, 64));
   const size_t words = (size_t)(n + 1) * Pc * (fp8 ? KP / 4 : KP / 2);
   const size_t zsz = (size_t)(n + 1) * ap.rows_pad * Pc;
