@@ -629,9 +629,12 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
 #define PFTCC_ZIFFT_NT 256
 #endif
 
+// Transforms per work-group: 8 at N = 308, whose 11-point pass has 28 butterflies per transform
+// (4 transforms fill 112 of 256 lanes; 8 ran the inverse stage 12.5 % faster, 16 slower), 4
+// elsewhere (N = 400: 8 and 16 measured slower).
 template <int N>
 struct ZifftV2 {
-  static constexpr int F0 = PFTCC_ZIFFT_F;
+  static constexpr int F0 = (N == 308 && PFTCC_ZIFFT_F == 4) ? 8 : PFTCC_ZIFFT_F;
   static constexpr int F = (sizeof(float2) * (F0 * N + 2 * N) <= 65536) ? F0
                          : (sizeof(float2) * (F0 / 2 * N + 2 * N) <= 65536) ? F0 / 2 : F0 / 4;
   // at most one wavefront per transform
