@@ -1698,6 +1698,22 @@ KCtfConst make_kctf(const pftcc_ctfparams& p, bool gate_phaseplate) {
 }
 
 // tw2 table of a length-L transform: exp(-i pi a / L), a in [0, 2L), FP64 rounded to FP32.
+// Box sizes whose preparation transforms run compile-time passes (others: runtime radices).
+#define PFTCC_PREP_SIZES(X) X(64) X(100) X(128) X(160) X(192) X(200) X(240) X(256) X(320) X(384) X(400)
+
+// Calls fn with std::integral_constant<int, L> for a listed size, else with 0.
+template <typename Fn>
+void prep_dispatch(int L, Fn&& fn) {
+#define X(N)                                   \
+  if (L == N) {                                \
+    fn(std::integral_constant<int, N>{});      \
+    return;                                    \
+  }
+  PFTCC_PREP_SIZES(X)
+#undef X
+  fn(std::integral_constant<int, 0>{});
+}
+
 std::vector<float2> twiddles(int L) {
   std::vector<float2> t(2 * L);
   for (int a = 0; a < 2 * L; ++a) {
@@ -1802,8 +1818,10 @@ int prep_impl(pftcc* h, int count, const void* imgs, bool dev, const pftcc_prep_
       HIPCHK(h, hipMemcpyAsync(d_ctf, kc.data(), (size_t)nb * sizeof(KCtfConst), hipMemcpyHostToDevice, h->stream));
     }
     hipLaunchKernelGGL(k_prep_stats, dim3(nb), dim3(256), 0, h->stream, src, B, cfg->msk, d_stats);
-    hipLaunchKernelGGL(k_prep_rows_fwd, dim3((B / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256), lds_rows, h->stream, src, B,
-                       (const float2*)d_stats, (const float2*)d_twB, pB, d_work);
+    prep_dispatch(B, [&](auto lc) {
+      hipLaunchKernelGGL(k_prep_rows_fwd<decltype(lc)::value>, dim3((B / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256),
+                         lds_rows, h->stream, src, B, (const float2*)d_stats, (const float2*)d_twB, pB, d_work);
+    });
     PrepColArgs ca{};
     ca.data = d_work;
     ca.L = B;
@@ -1816,18 +1834,23 @@ int prep_impl(pftcc* h, int count, const void* imgs, bool dev, const pftcc_prep_
     ca.ctf = flip ? d_ctf : nullptr;
     ca.ratio = 1.0f;
     const dim3 gcB((unsigned)((nhB + PREP_CP - 1) / PREP_CP), nb);
-    hipLaunchKernelGGL(k_prep_cols, gcB, dim3(256), lds_cols, h->stream, ca);
-    ca.inverse = 1;
-    ca.post = 0;
-    hipLaunchKernelGGL(k_prep_cols, gcB, dim3(256), lds_cols, h->stream, ca);
-    hipLaunchKernelGGL(k_prep_rows_inv, dim3((M / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256), lds_rows, h->stream,
-                       (const float2*)d_work, B, M, (const float2*)d_twB, pB, d_img2);
+    prep_dispatch(B, [&](auto lc) {
+      hipLaunchKernelGGL(k_prep_cols<decltype(lc)::value>, gcB, dim3(256), lds_cols, h->stream, ca);
+      ca.inverse = 1;
+      ca.post = 0;
+      hipLaunchKernelGGL(k_prep_cols<decltype(lc)::value>, gcB, dim3(256), lds_cols, h->stream, ca);
+      hipLaunchKernelGGL(k_prep_rows_inv<decltype(lc)::value>, dim3((M / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256),
+                         lds_rows, h->stream, (const float2*)d_work, B, M, (const float2*)d_twB, pB, d_img2);
+    });
     hipLaunchKernelGGL(k_prep_mask, dim3(nb), dim3(256), sizeof(float) * 4 * M, h->stream, d_img2, M, soft,
                        cfg->inner, cfg->width, minlen, cfg->griddev ? (const float*)d_wI : nullptr);
     const size_t lds_rows_m = sizeof(float2) * ((size_t)2 * PREP_RP * M + 2 * M);
     const size_t lds_cols_m = sizeof(float2) * ((size_t)2 * PREP_CP * M + 2 * M);
-    hipLaunchKernelGGL(k_prep_rows_fwd, dim3((M / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256), lds_rows_m, h->stream,
-                       (const float*)d_img2, M, (const float2*)nullptr, (const float2*)d_twM, pM, dst);
+    prep_dispatch(M, [&](auto lc) {
+      hipLaunchKernelGGL(k_prep_rows_fwd<decltype(lc)::value>, dim3((M / 2 + PREP_RP - 1) / PREP_RP, nb), dim3(256),
+                         lds_rows_m, h->stream, (const float*)d_img2, M, (const float2*)nullptr, (const float2*)d_twM,
+                         pM, dst);
+    });
     PrepColArgs cm{};
     cm.data = dst;
     cm.L = M;
@@ -1836,8 +1859,10 @@ int prep_impl(pftcc* h, int count, const void* imgs, bool dev, const pftcc_prep_
     cm.tw2g = d_twM;
     cm.plan = pM;
     cm.ratio = ratio;
-    hipLaunchKernelGGL(k_prep_cols, dim3((unsigned)((nhM + PREP_CP - 1) / PREP_CP), nb), dim3(256), lds_cols_m, h->stream,
-                       cm);
+    prep_dispatch(M, [&](auto lc) {
+      hipLaunchKernelGGL(k_prep_cols<decltype(lc)::value>, dim3((unsigned)((nhM + PREP_CP - 1) / PREP_CP), nb),
+                         dim3(256), lds_cols_m, h->stream, cm);
+    });
     HIPCHK(h, hipGetLastError());
     if (!dev)
       HIPCHK(h, hipMemcpyAsync(static_cast<float2*>(cmats) + (size_t)b0 * out_img, d_out, (size_t)nb * out_img * 8,

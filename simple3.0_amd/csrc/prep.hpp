@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "corr_fused.hpp"
 #include "fft_lds.hpp"
 #include "pftcc_kernels.hpp"
 
@@ -22,6 +23,21 @@ namespace pfk {
 
 constexpr int PREP_RP = 4;     // complex row transforms (pairs of real rows) per work-group
 constexpr int PREP_CP = 8;     // columns per work-group
+
+// Forward transforms of F length-L rows in LDS (bufA, stride L).  LC > 0: the compile-time
+// in-place Stockham passes of the correlation kernel (radix order 4, then the other factors in
+// descending order); LC = 0: the runtime-radix passes, ping-ponging with bufB.  Returns the buffer
+// that holds the result.
+template <int LC, int F>
+__device__ __forceinline__ float2* prep_fft(float2* bufA, float2* bufB, int L, const FftPlan& plan,
+                                            const float2* tw2, int tid, int nt) {
+  if constexpr (LC > 0) {
+    lds_passes<LC, F, 256, 0, Fp<LC>::np, LC>(bufA, tw2, tid);
+    return bufA;
+  } else {
+    return fft_forward_oop(bufA, bufB, F, L, L, plan, tw2, tid, nt);
+  }
+}
 constexpr float PREP_TINY = 1e-10f;          // simple_defs.f90:28
 constexpr float PREP_SHTHRESH = 0.001f;      // simple_strategy2D3D_common.f90:31
 
@@ -108,10 +124,13 @@ __global__ void __launch_bounds__(256) k_prep_stats(const float* __restrict__ im
 // Forward real transforms of the rows of the quadrant-swapped image: two real rows per complex
 // transform, split into their half spectra.  src [count][L][L]; stats (noise_norm) or null;
 // dst [count][L][L/2+1].  Grid (ceil(L/2 / RP), count).
-__global__ void __launch_bounds__(256) k_prep_rows_fwd(const float* __restrict__ src, int L,
+// LC > 0: compile-time length (L = LC).
+template <int LC>
+__global__ void __launch_bounds__(256) k_prep_rows_fwd(const float* __restrict__ src, int Lrt,
                                                        const float2* __restrict__ stats,
                                                        const float2* __restrict__ tw2g, FftPlan plan,
                                                        float2* __restrict__ dst) {
+  const int L = LC > 0 ? LC : Lrt;
   extern __shared__ float2 smem[];
   float2* bufA = smem;
   float2* bufB = bufA + PREP_RP * L;
@@ -142,7 +161,7 @@ __global__ void __launch_bounds__(256) k_prep_rows_fwd(const float* __restrict__
     bufA[e] = z;
   }
   __syncthreads();
-  const float2* res = fft_forward_oop(bufA, bufB, PREP_RP, L, L, plan, tw2, tid, nt);
+  const float2* res = prep_fft<LC, PREP_RP>(bufA, bufB, L, plan, tw2, tid, nt);
   float2* out = dst + (size_t)img * L * nh;
   for (int e = tid; e < PREP_RP * nh; e += nt) {
     const int r = e / nh, m = e - r * nh;
@@ -185,9 +204,10 @@ __device__ __forceinline__ float prep_ctf_value(const KCtfConst& c, int h, int k
   return sinf(arg + c.ampc);
 }
 
+template <int LC>
 __global__ void __launch_bounds__(256) k_prep_cols(PrepColArgs a) {
   extern __shared__ float2 smem[];
-  const int L = a.L, nh = L / 2 + 1;
+  const int L = LC > 0 ? LC : a.L, nh = L / 2 + 1;
   float2* bufA = smem;
   float2* bufB = bufA + PREP_CP * L;
   float2* tw2 = bufB + PREP_CP * L;
@@ -202,7 +222,7 @@ __global__ void __launch_bounds__(256) k_prep_cols(PrepColArgs a) {
     bufA[c * L + y] = a.inverse ? cconj(v) : v;
   }
   __syncthreads();
-  const float2* res = fft_forward_oop(bufA, bufB, PREP_CP, L, L, a.plan, tw2, tid, nt);
+  const float2* res = prep_fft<LC, PREP_CP>(bufA, bufB, L, a.plan, tw2, tid, nt);
   // per-image shift (shift2Dserial([-x, -y]) only above SHTHRESH, :338)
   bool doshift = false;
   double sh1 = 0.0, sh2 = 0.0;
@@ -248,9 +268,11 @@ __global__ void __launch_bounds__(256) k_prep_cols(PrepColArgs a) {
 // quadrant swap and the real-space clip to M: out[y'][x'] = row (y' + s + B/2) % B at
 // (x' + s + B/2) % B, s = (B - M)/2.  Two output rows per complex transform.  Grid
 // (ceil(M/2 / RP), count).
-__global__ void __launch_bounds__(256) k_prep_rows_inv(const float2* __restrict__ data, int B, int M,
+template <int LC>
+__global__ void __launch_bounds__(256) k_prep_rows_inv(const float2* __restrict__ data, int Brt, int M,
                                                        const float2* __restrict__ tw2g, FftPlan plan,
                                                        float* __restrict__ out) {
+  const int B = LC > 0 ? LC : Brt;
   extern __shared__ float2 smem[];
   float2* bufA = smem;
   float2* bufB = bufA + PREP_RP * B;
@@ -284,7 +306,7 @@ __global__ void __launch_bounds__(256) k_prep_rows_inv(const float2* __restrict_
     bufA[e] = z;
   }
   __syncthreads();
-  const float2* res = fft_forward_oop(bufA, bufB, PREP_RP, B, B, plan, tw2, tid, nt);
+  const float2* res = prep_fft<LC, PREP_RP>(bufA, bufB, B, plan, tw2, tid, nt);
   float* o = out + (size_t)img * M * M;
   for (int e = tid; e < PREP_RP * 2 * M; e += nt) {
     const int r = e / (2 * M), rem = e - r * (2 * M);
