@@ -21,8 +21,14 @@
 
 namespace pfk {
 
-constexpr int PREP_RP = 4;     // complex row transforms (pairs of real rows) per work-group
-constexpr int PREP_CP = 8;     // columns per work-group
+#ifndef PFTCC_PREP_RP
+#define PFTCC_PREP_RP 4
+#endif
+#ifndef PFTCC_PREP_CP
+#define PFTCC_PREP_CP 8
+#endif
+constexpr int PREP_RP = PFTCC_PREP_RP;     // complex row transforms (pairs of real rows) per work-group
+constexpr int PREP_CP = PFTCC_PREP_CP;     // columns per work-group
 
 // Forward transforms of F length-L rows in LDS (bufA, stride L).  LC > 0: the compile-time
 // in-place Stockham passes of the correlation kernel (radix order 4, then the other factors in
