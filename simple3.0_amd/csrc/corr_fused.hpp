@@ -398,6 +398,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   const float2* refs = even ? a.refs_even : a.refs_odd;
   const int npairs = a.nr * a.ns;
   const int nk = a.nk;
+  // the Euclid objective (never in the staged-reference variant, which runs the CC objective)
+  const bool EUC = !RST && a.mode == 1;
 
   for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
   if constexpr (PFTCC_FUSED_TWL && !PFA)
@@ -441,8 +443,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   if constexpr (RST) {
     rr0 = (tile * F) / a.ns;
     const int rr1 = min(rr0 + 1, a.nr - 1);
-    rs0 = refs + (size_t)(a.rlist ? a.rlist[rr0] : rr0) * nk * N;
-    rs1 = refs + (size_t)(a.rlist ? a.rlist[rr1] : rr1) * nk * N;
+    // uniform across the work-group: row numbers in scalar registers
+    const int row0 = __builtin_amdgcn_readfirstlane(a.rlist ? a.rlist[rr0] : rr0);
+    const int row1 = __builtin_amdgcn_readfirstlane(a.rlist ? a.rlist[rr1] : rr1);
+    rs0 = refs + (size_t)row0 * nk * N;
+    rs1 = refs + (size_t)row1 * nk * N;
   }
   double rsum0 = 0.0, rsum1 = 0.0;     // this thread's share of the two references' norms
   // pass-1 slots: pair, reference and shift rows are fixed for all shells
@@ -515,7 +520,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     const size_t prow = (size_t)ip * nk + kk;
     if constexpr (PFTCC_FUSED_PREFETCH) {
       // the prefetched operands go to LDS, then the next shell's loads are issued
-      const float wkp = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
+      const float wkp = (EUC && docorr) ? a.shellw[prow] : 1.0f;
 #pragma unroll
       for (int u = 0; u < JP; ++u) {
         const int j = tid + u * NT;
@@ -541,7 +546,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           if (m <= N) {
             float2 wh = make_float2(0.5f * pwm[u].x, 0.5f * pwm[u].y);
             if constexpr (PFTCC_FUSED_W12) {
-              if (a.mode == 1) wh = make_float2(wkp * wh.x, wkp * wh.y);
+              if (EUC) wh = make_float2(wkp * wh.x, wkp * wh.y);
               const float2 t = tw2[m];
               const int wp = PFA ? wptab[m] : m;
               wk_store<PFA, WL>(Wk, wp, cmul(make_float2(1.f + t.y, t.x), wh), cmul(make_float2(1.f - t.y, -t.x), wh));
@@ -566,7 +571,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     } else {
       for (int j = tid; j < N; j += NT) ctfk[j] = a.ctf ? a.ctf[prow * N + j] : 1.0f;
     }
-    const float wk = (a.mode == 1 && docorr) ? a.shellw[prow] : 1.0f;
+    const float wk = (EUC && docorr) ? a.shellw[prow] : 1.0f;
     if (!PFTCC_FUSED_PREFETCH && docorr)
       for (int m = tid; m <= N; m += NT) {
         const float2 w = a.memo[prow * (N + 1) + m];
@@ -575,7 +580,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           // conj(V) W = conj(Z[m]) conj(A) W + Z[N-m] conj(B) W with 2V = A' Z[m] + B' conj(Z[N-m]),
           // A' = 1 - i w, B' = 1 + i w, w = exp(-i pi m / N): both factors depend on m only, so they
           // are applied to the staged spectrum (with the Euclid shell weight) once per shell
-          if (a.mode == 1) wh = make_float2(wk * wh.x, wk * wh.y);
+          if (EUC) wh = make_float2(wk * wh.x, wk * wh.y);
           const float2 t = tw2[m];
           const int wp = PFA ? wptab[m] : m;
           // conj(A') W / 2 and conj(B') W / 2
@@ -643,7 +648,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           const float e2 = q0.x * q0.x + q0.y * q0.y + q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y +
                            q3.x * q3.x + q3.y * q3.y;
           if (donorm) sqp[c] += e2;
-          if (a.mode == 1 && docorr) offp[c] = fmaf(wk, e2, offp[c]);
+          if (EUC && docorr) offp[c] = fmaf(wk, e2, offp[c]);
         }
         if constexpr (PFA) {
           // z[b + Q n1], n1 = 0..3: signs +-1 from b >= 2Q (j odd) and b + Q >= 2Q (2j >= Q)
