@@ -33,9 +33,9 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch of k_corr_fused<308,9,256,3,true> at 1000 particles x 200 refs x 169 shifts,
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script (--legs plain --copy-calibration):
-# FETCH 414809400 KiB x 2.000 + WRITE 117375 KiB x 1.000, the factors read from k_copy moving
+# FETCH 411073084 KiB x 2.000 + WRITE 117375 KiB x 1.000, the factors read from k_copy moving
 # 1 GiB in the same runs (profiles/k_corr_fused_308_plain_search.md).
-PLAIN_TRAFFIC_BYTES_B1000 = (414809400.0 * 2.0 + 117375.0) * 1024.0
+PLAIN_TRAFFIC_BYTES_B1000 = (411073084.0 * 2.0 + 117375.0) * 1024.0
 FP64_PEAK_TFLOPS = 78.6
 
 
