@@ -115,11 +115,12 @@ constexpr int fused_F() {
 
 // Pair tile and register budget measured fastest on MI355X for the ring lengths of the benchmark
 // shapes: n = 308, 9 pairs at >= 3 wavefronts per SIMD (167 VGPRs, no spills, against 2
-// wavefronts at ~190 VGPRs unconstrained); n = 192, 5 pairs; n = 400, 4 pairs (4 to 10 pairs and 1
-// to 3 wavefronts measured: within 4 %, larger tiles slower).  fused_F elsewhere.
+// wavefronts at ~190 VGPRs unconstrained); n = 192, 5 pairs; n = 400, 3 pairs with the reference
+// prefetch (128 VGPRs, 4 wavefronts; 240 against 254 ms without it and 265 ms with 4 pairs, which
+// drop to 3 wavefronts with it; 2 pairs 318 ms, 5 to 10 pairs slower).  fused_F elsewhere.
 template <int N, int NT>
 constexpr int fused_F_tuned() {
-  if (NT == 256 && (N == 192 || N == 308 || N == 400)) return N == 192 ? 5 : N == 308 ? 9 : 4;
+  if (NT == 256 && (N == 192 || N == 308 || N == 400)) return N == 192 ? 5 : N == 308 ? 9 : 3;
   return fused_F<N, NT>();
 }
 #ifndef PFTCC_FUSED_GROUP
@@ -128,6 +129,9 @@ constexpr int fused_F_tuned() {
 #ifndef PFTCC_FUSED_PREFETCH
 #define PFTCC_FUSED_PREFETCH 1 // the next shell's staged rows (|CTF|, reference rows, spectrum) loaded ahead
 #endif
+#ifndef PFTCC_FUSED_PREFETCH_REF
+#define PFTCC_FUSED_PREFETCH_REF 1   // per-pair variant: pass-1 reference values loaded a shell ahead
+#endif
 
 // Tile of the staged-reference variant.  Its pass 1 holds no per-pair reference pointers or norm
 // sums: at N = 308 that leaves room for 11 pairs without the staging prefetch, and for 9 pairs
@@ -135,11 +139,18 @@ constexpr int fused_F_tuned() {
 template <int N, int NT>
 constexpr int fused_F_rst_tuned() {
   if (NT == 256 && N == 308) return PFTCC_FUSED_PREFETCH ? 9 : 11;
+  if (NT == 256 && N == 400) return 4;
   return fused_F_tuned<N, NT>();
 }
 template <int N, int NT>
 constexpr int fused_W_tuned() {
   return (NT == 256 && N == 308) ? 3 : 1;
+}
+// The per-pair variant's reference prefetch holds 8 more registers per pass-1 task, so it is used
+// where no register budget is imposed on the tile (at N = 308 it spills).
+template <int N, int NT>
+constexpr bool fused_prefetch_ref() {
+  return PFTCC_FUSED_PREFETCH_REF && fused_W_tuned<N, NT>() == 1;
 }
 
 #ifndef PFTCC_FUSED_TWL
@@ -483,6 +494,30 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     for (int b = 0; b < NB; ++b) acc[c][b] = make_float2(0.f, 0.f);
 
   const int kloop = a.kcorr > a.knorm ? a.kcorr : a.knorm;
+  // per-pair variant: the reference values of this thread's pass-1 butterflies for the next shell,
+  // loaded while the current shell runs (element offsets oa, ob of the butterfly's q pairs)
+  constexpr bool PRF = !RST && fused_prefetch_ref<N, NT>();
+  float4 pra[PRF ? TPT1 : 1], prb[PRF ? TPT1 : 1];
+  auto p1_offsets = [&](int j, int& oa, int& ob) {
+    if constexpr (PFA) {
+      oa = 4 * j;
+      ob = 2 * j < Q ? 4 * j + 2 * Q : 4 * j - 2 * Q;
+    } else {
+      oa = 2 * j;
+      ob = 2 * j + N / 2;
+    }
+  };
+  if constexpr (PRF)
+    if (kloop > 0)
+#pragma unroll
+      for (int c = 0; c < TPT1; ++c) {
+        const int task = tid + c * NT;
+        if (!(T1 % NT == 0 || task < T1)) continue;
+        int oa, ob;
+        p1_offsets(task % Q, oa, ob);
+        pra[c] = ld16(refs, roff[c] + oa);
+        prb[c] = ld16(refs, roff[c] + ob);
+      }
   // the staging operands of shell kk + 1, loaded while shell kk runs
   constexpr int JP = (N + NT - 1) / NT, MP = (N + 1 + NT - 1) / NT;
   float pcv[JP];
@@ -615,9 +650,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
           q2 = make_float2(rb.x, rb.y);
           q3 = make_float2(rb.z, rb.w);
         } else {
-          const unsigned int r = roff[c] + (unsigned int)(kk * N);
-          const float4 ra = ld16(refs, r + oa);
-          const float4 rb = ld16(refs, r + ob);
+          float4 ra, rb;
+          if constexpr (PRF) {
+            ra = pra[c];
+            rb = prb[c];
+            if (kk + 1 < kloop) {
+              const unsigned int r = roff[c] + (unsigned int)((kk + 1) * N);
+              pra[c] = ld16(refs, r + oa);
+              prb[c] = ld16(refs, r + ob);
+            }
+          } else {
+            const unsigned int r = roff[c] + (unsigned int)(kk * N);
+            ra = ld16(refs, r + oa);
+            rb = ld16(refs, r + ob);
+          }
           const float2 c01 = *reinterpret_cast<const float2*>(ctfk + oa);
           const float2 c23 = *reinterpret_cast<const float2*>(ctfk + ob);
           q0 = make_float2(ra.x * c01.x, ra.y * c01.x);

@@ -366,3 +366,34 @@ def test_fused_prime_factor_objectives(mode):
         ref = pf.gencorrs_euclid(r, p, None, method="fft") if mode == "euclid" else pf.gencorrs(r, p, method="fft")
         assert peak_rel_err(lib.gencorrs(r + 1, p + 1), ref) < FP32_TOL
     lib.kill()
+
+
+@pytest.mark.parametrize("mode", ["cc", "euclid"])
+@pytest.mark.parametrize("n,kto,ldim", [(192, 30, 128), (400, 30, 256)])
+def test_per_pair_kernel_reference_prefetch(n, kto, ldim, mode):
+    """At ring lengths 192 and 400 the per-pair variant of the fused kernel loads its pass-1
+    reference values one shell ahead.  Batches of 20 particles x 7 references (the last tile
+    partial, the particle groups of 16 wrapping), unshifted and with two shifts, against the FP64
+    oracle; the search returns the batch's first maximum."""
+    say(f"per-pair kernel with reference prefetch n={n} {mode}")
+    np_, nr = 20, 7
+    pf, cps = make_case(seed=300 + n, n=n, kto=kto, ctf=True, nrefs=nr, nptcls=np_, ldim=ldim, eo_mixed=True,
+                        euclid=(mode == "euclid"))
+    lib = to_lib(pf, cps)
+    ids = np.arange(1, np_ + 1)
+    shifts = np.array([[0.0, 0.0], [1.5, -2.0]], np.float32)
+    for shv in (None, shifts):
+        got = lib.gencorrs_batch(ids, None, shv)
+        for p in range(np_):
+            for r in range(nr):
+                for s in range(got.shape[2]):
+                    sh = tuple(shv[s]) if (shv is not None and s) else None
+                    ref = pf.gencorrs_euclid(r, p, sh, method="fft") if mode == "euclid" else \
+                        pf.gencorrs(r, p, sh, method="fft")
+                    assert peak_rel_err(got[p, r, s], ref) < FP32_TOL, (p, r, s)
+        cc, ir, ish, rot = lib.search(ids, None, shv)
+        for p in range(np_):
+            r, s, t = np.unravel_index(int(np.argmax(got[p])), got[p].shape)
+            assert (ir[p], ish[p], rot[p]) == (r + 1, s, t + 1)
+            assert cc[p] == got[p, r, s, t]
+    lib.kill()
