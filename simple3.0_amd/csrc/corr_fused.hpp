@@ -131,7 +131,7 @@ constexpr int fused_F_tuned() {
 #endif
 #ifndef PFTCC_FUSED_PREFETCH_REF
 #define PFTCC_FUSED_PREFETCH_REF 1   // per-pair variant: pass-1 reference values loaded a shell ahead
-#endif
+#endif                               // (1: where no register budget is imposed, 2: always)
 
 // Tile of the staged-reference variant.  Its pass 1 holds no per-pair reference pointers or norm
 // sums: at N = 308 that leaves room for 11 pairs without the staging prefetch, and for 9 pairs
@@ -150,7 +150,7 @@ constexpr int fused_W_tuned() {
 // where no register budget is imposed on the tile (at N = 308 it spills).
 template <int N, int NT>
 constexpr bool fused_prefetch_ref() {
-  return PFTCC_FUSED_PREFETCH_REF && fused_W_tuned<N, NT>() == 1;
+  return PFTCC_FUSED_PREFETCH_REF == 2 || (PFTCC_FUSED_PREFETCH_REF == 1 && fused_W_tuned<N, NT>() == 1);
 }
 
 #ifndef PFTCC_FUSED_TWL
