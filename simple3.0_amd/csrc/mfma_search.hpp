@@ -23,6 +23,15 @@ constexpr int ZG = 16;   // particles per Z group (the MFMA tile's column count)
 __host__ __device__ __forceinline__ size_t zidx(int r, int p, int m, int n, int cols_pad) {
   return ((size_t)(r * (cols_pad / ZG) + p / ZG) * (n + 1) + m) * ZG + (p % ZG);
 }
+// FP16 spectra (4 bytes per value): groups of ZG16 particles, so that one (row, m) run fills a
+// 128-byte line
+#ifndef PFTCC_ZG16
+#define PFTCC_ZG16 32
+#endif
+constexpr int ZG16 = PFTCC_ZG16;
+__host__ __device__ __forceinline__ size_t zidx16(int r, int p, int m, int n, int cols_pad) {
+  return ((size_t)(r * (cols_pad / ZG16) + p / ZG16) * (n + 1) + m) * ZG16 + (p % ZG16);
+}
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
@@ -361,6 +370,12 @@ struct ZifftArgs {
   unsigned long long* prbest;  // [npos][nrefs] best key of each (particle, reference), or null
   const float2* tw2g;
   FftPlan plan;
+  // FP16 spectra (FP8 coarse pass): Z16 holds Z gA gB zs as FP16 pairs in the zidx16 layout, with
+  // gA, gB the operands' power-of-two gains (from gmaxA, gmaxB) and zs a power of two
+  const unsigned int* Z16;
+  const unsigned int* gmaxA;
+  const unsigned int* gmaxB;
+  double zs;
 };
 
 // Inverse real FFT, FP64-normalised correlations and argmax for F pairs (one reference row,
@@ -466,7 +481,13 @@ __device__ __forceinline__ void store4(float2* o, const float2 (&y)[4]) {
   d[1] = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
 }
 
-template <int N, int F, int NT>
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 h2_to_f2(unsigned int w) {
+  const half2v h = __builtin_bit_cast(half2v, w);
+  return make_float2((float)h.x, (float)h.y);
+}
+
+template <int N, int F, int NT, bool Z16>
 __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   static_assert(ZG % F == 0, "F divides the Z group");
   static_assert(Fp<N>::fusable, "first radix must be 4");
@@ -492,7 +513,9 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   const int p0 = (t % gx) * F;
   for (int i = tid; i < 2 * N; i += NT) tw2[i] = a.tw2g[i];
   // ---- pass 1: loads first (they do not depend on LDS), then the butterflies ----
+  static_assert(ZG16 % F == 0, "F divides the FP16 Z group");
   const float2* Zb = a.Z + zidx(r, p0, 0, N, a.cols_pad);
+  const unsigned int* Zh = a.Z16 + (Z16 ? zidx16(r, p0, 0, N, a.cols_pad) : 0);
   float2 za[TPT1][4], zb[TPT1][4], zn[TPT1];
 #pragma unroll
   for (int c = 0; c < TPT1; ++c) {
@@ -501,8 +524,13 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
     const int g = task / F, f = task % F;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      za[c][t] = Zb[(size_t)(g + t * Q) * ZG + f];
-      zb[c][t] = Zb[(size_t)((t + 1) * Q - g) * ZG + f];
+      if constexpr (Z16) {
+        za[c][t] = h2_to_f2(Zh[(size_t)(g + t * Q) * ZG16 + f]);
+        zb[c][t] = h2_to_f2(Zh[(size_t)((t + 1) * Q - g) * ZG16 + f]);
+      } else {
+        za[c][t] = Zb[(size_t)(g + t * Q) * ZG + f];
+        zb[c][t] = Zb[(size_t)((t + 1) * Q - g) * ZG + f];
+      }
     }
     zn[c] = zb[c][3];   // Z[N - g]; for g = 0 this is Z[N]
   }
@@ -562,8 +590,14 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
   for (int f = wave; f < F; f += NW) {
     const int p = p0 + f;
     const bool live = p < a.np;
-    // FP64 normalisation factor, applied as one FP32 multiply per value
-    const float sc = live ? (float)(inv4n / den[f]) : 0.f;
+    // FP64 normalisation factor, applied as one FP32 multiply per value (FP16 spectra: with the
+    // powers of two that scaled them)
+    float sc;
+    if constexpr (Z16)
+      sc = live ? (float)(inv4n / den[f] / ((double)pow2_gain(a.gmaxA, 448.0f) * pow2_gain(a.gmaxB, 448.0f) * a.zs))
+                : 0.f;
+    else
+      sc = live ? (float)(inv4n / den[f]) : 0.f;
     float bv = -INFINITY;      // this lane's best value and its rotation index (ties: lowest)
     int bi = -1;
     // the lane's butterflies j = lane, lane + 64, ... (JB of them), transformed first; their outputs
@@ -700,7 +734,7 @@ __global__ void k_absmax_scaled(const float2* __restrict__ spec, const float* __
   }
 }
 
-// Power of two g with max * g <= 448.
+// Power of two g with max * g <= 448 (the inverse stage's FP16-spectra scale uses the same gain).
 __device__ __forceinline__ float fp8_gain(const unsigned int* gmax) { return pow2_gain(gmax, 448.0f); }
 
 // e4m3 operand tiles out[m][row][k'] (bytes; k' = 2 kk + {re, im}, zero past 2 kcorr), one 32-bit
@@ -750,52 +784,68 @@ __device__ __forceinline__ i32x8 rot_neg8(i32x8 a) {
 
 // FP8 contraction: Z[r][p][m] = (A1 . B, A2 . B) / (gA gB) for one 64 x 64 tile per wave, 128 k'
 // per MFMA (16x16x128, e4m3 x e4m3, FP32 accumulation).
-__global__ void __launch_bounds__(64) k_zgemm8(const i32x8* __restrict__ A, const i32x8* __restrict__ B, int rows_pad,
-                                               int cols_pad, int KP8, int n, float2* __restrict__ Z,
-                                               const unsigned int* __restrict__ gmaxA,
-                                               const unsigned int* __restrict__ gmaxB) {
+// Z16: Z gA gB zs as FP16 pairs (round to nearest) in the zidx16 layout, |e4m3 products| <= 448^2
+// bounding |Z gA gB| by KP8 448^2 so that zs can be chosen to exclude overflow.
+// Tile: 64 rows x 16 ZGEMM8_TJ columns per wavefront.
+#ifndef PFTCC_ZGEMM8_TJ
+#define PFTCC_ZGEMM8_TJ 4
+#endif
+#ifndef PFTCC_ZGEMM8_W
+#define PFTCC_ZGEMM8_W 1       // minimum wavefronts per SIMD imposed on the register allocation
+#endif
+constexpr int ZGEMM8_TJ = PFTCC_ZGEMM8_TJ;
+template <bool Z16>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFTCC_ZGEMM8_W)))
+k_zgemm8(const i32x8* __restrict__ A, const i32x8* __restrict__ B, int rows_pad, int cols_pad, int KP8, int n,
+         float2* __restrict__ Z, const unsigned int* __restrict__ gmaxA, const unsigned int* __restrict__ gmaxB,
+         float zs) {
+  constexpr int TJ = ZGEMM8_TJ;
   const int l = threadIdx.x;
-  const ZTile zt = ztile(cols_pad / 64, rows_pad / 64, 64, 64);
+  const ZTile zt = ztile(cols_pad / (16 * TJ), rows_pad / 64, 16 * TJ, 64);
   const int p0 = zt.p0, r0 = zt.r0, m = zt.m;
   const int KV = KP8 / 32;                                 // i32x8 vectors per row
   const size_t abase = ((size_t)m * rows_pad + r0) * KV;
   const size_t bbase = ((size_t)m * cols_pad + p0) * KV;
-  f32x4 zr[4][4], zi[4][4];
+  f32x4 zr[4][TJ], zi[4][TJ];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < TJ; ++j) {
       zr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       zi[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
   const int lr = l % 16, lk = l / 16;
   for (int ks = 0; ks < KV; ks += 4) {
-    i32x8 a[4], b[4];
+    i32x8 a[4], b[TJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a[i] = A[abase + (size_t)(i * 16 + lr) * KV + ks + lk];
-      b[i] = B[bbase + (size_t)(i * 16 + lr) * KV + ks + lk];
-    }
+    for (int i = 0; i < 4; ++i) a[i] = A[abase + (size_t)(i * 16 + lr) * KV + ks + lk];
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) b[j] = B[bbase + (size_t)(j * 16 + lr) * KV + ks + lk];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const i32x8 a2 = rot_neg8(a[i]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < TJ; ++j) {
         zr[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], zr[i][j], 0, 0, 0, 0, 0, 0);
         zi[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a2, b[j], zi[i][j], 0, 0, 0, 0, 0, 0);
       }
     }
   }
-  const float inv = 1.0f / (fp8_gain(gmaxA) * fp8_gain(gmaxB));
+  const float inv = Z16 ? zs : 1.0f / (fp8_gain(gmaxA) * fp8_gain(gmaxB));
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < TJ; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = r0 + i * 16 + 4 * lk + e;
         const int p = p0 + j * 16 + lr;
-        Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e] * inv, zi[i][j][e] * inv);
+        if constexpr (Z16) {
+          const half2v h = {(_Float16)(zr[i][j][e] * inv), (_Float16)(zi[i][j][e] * inv)};
+          reinterpret_cast<unsigned int*>(Z)[zidx16(r, p, m, n, cols_pad)] = __builtin_bit_cast(unsigned int, h);
+        } else {
+          Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e] * inv, zi[i][j][e] * inv);
+        }
       }
 }
 

@@ -879,15 +879,27 @@ bool zgemm_as_go(int KP, int rows_pad, int cols_pad, int n, hipStream_t s, const
   return false;
 }
 
-// Compile-time inverse stage for the fused ring lengths; false if n has none.
+// Compile-time inverse stage for the fused ring lengths; false if n has none.  a.Z16 selects the
+// FP16-spectra variant.
 bool zifft_ct_go(int n, int cnt, int nr, hipStream_t s, const ZifftArgs& a) {
 #define X(N)                                                                                          \
   if (n == N) {                                                                                       \
-    hipLaunchKernelGGL((k_zifft_v2<N, ZifftV2<N>::F, ZifftV2<N>::NT>),                               \
-                       dim3((unsigned)((cnt + ZifftV2<N>::F - 1) / ZifftV2<N>::F * nr)),               \
-                       dim3(ZifftV2<N>::NT), ZifftV2<N>::lds, s, a);                                  \
+    const dim3 grid((unsigned)((cnt + ZifftV2<N>::F - 1) / ZifftV2<N>::F * nr));                      \
+    if (a.Z16)                                                                                        \
+      hipLaunchKernelGGL((k_zifft_v2<N, ZifftV2<N>::F, ZifftV2<N>::NT, true>), grid,                  \
+                         dim3(ZifftV2<N>::NT), ZifftV2<N>::lds, s, a);                                \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_zifft_v2<N, ZifftV2<N>::F, ZifftV2<N>::NT, false>), grid,                 \
+                         dim3(ZifftV2<N>::NT), ZifftV2<N>::lds, s, a);                                \
     return true;                                                                                      \
   }
+  PFTCC_FUSED_SIZES(X)
+#undef X
+  return false;
+}
+
+bool zifft_ct_has(int n) {
+#define X(N) if (n == N) return true;
   PFTCC_FUSED_SIZES(X)
 #undef X
   return false;
@@ -1038,6 +1050,10 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector
   return PFTCC_OK;
 }
 
+#ifndef PFTCC_FP8_Z16
+#define PFTCC_FP8_Z16 1   // FP8 coarse pass: spectra Z stored as FP16 pairs (half the round trip's bytes)
+#endif
+
 // Runs the precision form for particles pl (local ids) that share eo class eo, against the cached
 // operand of refs rl.  outpos[i] = position of pl[i] in best/cc_out.
 int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<int>& outpos,
@@ -1054,6 +1070,11 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
   std::vector<int> rowkey(nr);
   for (int r = 0; r < nr; ++r) rowkey[r] = rl[r / nse] * nse + r % nse;
   const int total = (int)pl.size();
+  // FP8 coarse pass: Z as FP16 pairs, scaled by the power of two zs16 <= 65504 / (KP 448^2), which
+  // bounds |Z gA gB| over e4m3 operands (|x| <= 448) so that no value overflows FP16
+  const bool z16 = fp8 && PFTCC_FP8_Z16 && zifft_ct_has(n);
+  double zs16 = 1.0;
+  while (zs16 * KP * 448.0 * 448.0 > 65504.0) zs16 *= 0.5;
   // chunk of particles: keep Z (n+1) x rows_pad x Pc float2 within the cap
   int Pc = PFTCC_MFMA_CHUNK;
   while (Pc > 64 && (size_t)(n + 1) * ap.rows_pad * Pc * 8 > ((size_t)PFTCC_MFMA_ZCAP_GIB << 30)) Pc -= 64;
@@ -1114,11 +1135,14 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     }
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 1, &ev))) return rc;
-    const dim3 gg((unsigned)((size_t)(Pc / 64) * (ap.rows_pad / 64) * (n + 1)));
+    const dim3 gg((unsigned)((size_t)(Pc / (16 * ZGEMM8_TJ)) * (ap.rows_pad / 64) * (n + 1)));
     const dim3 gz((unsigned)((size_t)(Pc / (16 * ZGEMM_TP)) * (ap.rows_pad / (16 * ZGEMM_TR)) * (n + 1)));
-    if (fp8)
-      hipLaunchKernelGGL(k_zgemm8, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad, Pc,
-                         KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB);
+    if (fp8 && z16)
+      hipLaunchKernelGGL(k_zgemm8<true>, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad,
+                         Pc, KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB, (float)zs16);
+    else if (fp8)
+      hipLaunchKernelGGL(k_zgemm8<false>, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad,
+                         Pc, KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB, 1.0f);
     else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_BF16 &&
              zgemm_as_go<1, false>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z, nullptr, nullptr)) {
     } else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_SPLIT_BF16 &&
@@ -1161,6 +1185,12 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     za.prbest = d_prbest;
     za.tw2g = h->d_tw2;
     za.plan = h->plan;
+    if (z16) {
+      za.Z16 = reinterpret_cast<const unsigned int*>(Z);
+      za.gmaxA = (const unsigned int*)ap.gmax;
+      za.gmaxB = (const unsigned int*)gmaxB;
+      za.zs = zs16;
+    }
     if (zifft_ct_go(n, cnt, nr, h->stream, za)) {
       HIPCHK(h, hipGetLastError());
       if ((rc = timing_end(h, ev))) return rc;
