@@ -793,8 +793,12 @@ __device__ __forceinline__ i32x8 rot_neg8(i32x8 a) {
 #ifndef PFTCC_ZGEMM8_W
 #define PFTCC_ZGEMM8_W 1       // minimum wavefronts per SIMD imposed on the register allocation
 #endif
+#ifndef PFTCC_ZGEMM8_MT
+#define PFTCC_ZGEMM8_MT 8      // consecutive m per wavefront when KP8 = 128 (next operands loaded ahead)
+#endif
 constexpr int ZGEMM8_TJ = PFTCC_ZGEMM8_TJ;
-template <bool Z16>
+constexpr int ZGEMM8_MT = PFTCC_ZGEMM8_MT;
+template <bool Z16, int MT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PFTCC_ZGEMM8_W)))
 k_zgemm8(const i32x8* __restrict__ A, const i32x8* __restrict__ B, int rows_pad, int cols_pad, int KP8, int n,
          float2* __restrict__ Z, const unsigned int* __restrict__ gmaxA, const unsigned int* __restrict__ gmaxB,
@@ -802,25 +806,19 @@ k_zgemm8(const i32x8* __restrict__ A, const i32x8* __restrict__ B, int rows_pad,
   constexpr int TJ = ZGEMM8_TJ;
   const int l = threadIdx.x;
   const ZTile zt = ztile(cols_pad / (16 * TJ), rows_pad / 64, 16 * TJ, 64);
-  const int p0 = zt.p0, r0 = zt.r0, m = zt.m;
+  const int p0 = zt.p0, r0 = zt.r0;
   const int KV = KP8 / 32;                                 // i32x8 vectors per row
-  const size_t abase = ((size_t)m * rows_pad + r0) * KV;
-  const size_t bbase = ((size_t)m * cols_pad + p0) * KV;
-  f32x4 zr[4][TJ], zi[4][TJ];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < TJ; ++j) {
-      zr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-      zi[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
   const int lr = l % 16, lk = l / 16;
-  for (int ks = 0; ks < KV; ks += 4) {
-    i32x8 a[4], b[TJ];
+  const float inv = Z16 ? zs : 1.0f / (fp8_gain(gmaxA) * fp8_gain(gmaxB));
+  auto load_ops = [&](int m, int ks, i32x8 (&a)[4], i32x8 (&b)[TJ]) {
+    const size_t abase = ((size_t)m * rows_pad + r0) * KV;
+    const size_t bbase = ((size_t)m * cols_pad + p0) * KV;
 #pragma unroll
     for (int i = 0; i < 4; ++i) a[i] = A[abase + (size_t)(i * 16 + lr) * KV + ks + lk];
 #pragma unroll
     for (int j = 0; j < TJ; ++j) b[j] = B[bbase + (size_t)(j * 16 + lr) * KV + ks + lk];
+  };
+  auto mma = [&](const i32x8 (&a)[4], const i32x8 (&b)[TJ], f32x4 (&zr)[4][TJ], f32x4 (&zi)[4][TJ]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const i32x8 a2 = rot_neg8(a[i]);
@@ -830,23 +828,82 @@ k_zgemm8(const i32x8* __restrict__ A, const i32x8* __restrict__ B, int rows_pad,
         zi[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a2, b[j], zi[i][j], 0, 0, 0, 0, 0, 0);
       }
     }
-  }
-  const float inv = Z16 ? zs : 1.0f / (fp8_gain(gmaxA) * fp8_gain(gmaxB));
+  };
+  auto zero = [&](f32x4 (&zr)[4][TJ], f32x4 (&zi)[4][TJ]) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < TJ; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = r0 + i * 16 + 4 * lk + e;
-        const int p = p0 + j * 16 + lr;
-        if constexpr (Z16) {
-          const half2v h = {(_Float16)(zr[i][j][e] * inv), (_Float16)(zi[i][j][e] * inv)};
-          reinterpret_cast<unsigned int*>(Z)[zidx16(r, p, m, n, cols_pad)] = __builtin_bit_cast(unsigned int, h);
-        } else {
-          Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e] * inv, zi[i][j][e] * inv);
-        }
+      for (int j = 0; j < TJ; ++j) {
+        zr[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        zi[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
+  };
+  auto store = [&](int m, const f32x4 (&zr)[4][TJ], const f32x4 (&zi)[4][TJ]) {
+    if constexpr (Z16) {
+      // element (r0 + dr, p0 + dp, m) of zidx16 lies at zidx16(r0, p0, m) + dr RS + (dp / 32) (n + 1) 32
+      // + dp % 32 (p0 is a multiple of 32): a wave-uniform base and 32-bit offsets below 64 RS
+      static_assert(16 * TJ % ZG16 == 0 || ZG16 % (16 * TJ) == 0, "column tile and FP16 group");
+      char* const zb = reinterpret_cast<char*>(reinterpret_cast<unsigned int*>(Z) + zidx16(r0, p0, m, n, cols_pad));
+      const unsigned int RS = (unsigned int)cols_pad * (unsigned int)(n + 1);
+      const unsigned int lo = 4u * lk * RS + (unsigned int)lr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const unsigned int dp = 16u * j;
+            const unsigned int cst = (16u * i + e) * RS + (dp / ZG16) * (unsigned int)(n + 1) * ZG16 + dp % ZG16;
+            const half2v h = {(_Float16)(zr[i][j][e] * inv), (_Float16)(zi[i][j][e] * inv)};
+            *reinterpret_cast<unsigned int*>(zb + ((lo + cst) << 2)) = __builtin_bit_cast(unsigned int, h);
+          }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = r0 + i * 16 + 4 * lk + e;
+            const int p = p0 + j * 16 + lr;
+            Z[zidx(r, p, m, n, cols_pad)] = make_float2(zr[i][j][e] * inv, zi[i][j][e] * inv);
+          }
+    }
+  };
+  if constexpr (MT == 1) {
+    const int m = zt.m;
+    f32x4 zr[4][TJ], zi[4][TJ];
+    zero(zr, zi);
+    for (int ks = 0; ks < KV; ks += 4) {
+      i32x8 a[4], b[TJ];
+      load_ops(m, ks, a, b);
+      mma(a, b, zr, zi);
+    }
+    store(m, zr, zi);
+  } else {
+    // one k' step per m (KP8 = 128, checked by the host); MT consecutive m per wavefront, the next
+    // m's operands loaded before the current m's products
+    const int m0 = zt.m * MT;
+    i32x8 a[4], b[TJ];
+    load_ops(m0, 0, a, b);
+    for (int mm = 0; mm < MT; ++mm) {
+      const int m = m0 + mm;
+      if (m > n) break;
+      const bool more = mm + 1 < MT && m + 1 <= n;
+      i32x8 an[4], bn[TJ];
+      if (more) load_ops(m + 1, 0, an, bn);
+      f32x4 zr[4][TJ], zi[4][TJ];
+      zero(zr, zi);
+      mma(a, b, zr, zi);
+      store(m, zr, zi);
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = an[i];
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) b[j] = bn[j];
+      }
+    }
+  }
 }
 
 // Shifted reference rings out[r * ns + s][kk][j] = rings[r][kk][j] * shtab[s][kk][j].

@@ -1135,15 +1135,21 @@ int mfma_group(pftcc* h, int eo, const std::vector<int>& pl, const std::vector<i
     }
     HIPCHK(h, hipGetLastError());
     if ((rc = timing_end(h, ev)) || (rc = timing_begin(h, 1, &ev))) return rc;
-    const dim3 gg((unsigned)((size_t)(Pc / (16 * ZGEMM8_TJ)) * (ap.rows_pad / 64) * (n + 1)));
+    // FP8: MT consecutive m per wavefront where one k' step covers the shells (KP = 128)
+    const int mt8 = KP == 128 ? ZGEMM8_MT : 1;
+    const dim3 gg((unsigned)((size_t)(Pc / (16 * ZGEMM8_TJ)) * (ap.rows_pad / 64) * ((n + 1 + mt8 - 1) / mt8)));
     const dim3 gz((unsigned)((size_t)(Pc / (16 * ZGEMM_TP)) * (ap.rows_pad / (16 * ZGEMM_TR)) * (n + 1)));
-    if (fp8 && z16)
-      hipLaunchKernelGGL(k_zgemm8<true>, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad,
-                         Pc, KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB, (float)zs16);
-    else if (fp8)
-      hipLaunchKernelGGL(k_zgemm8<false>, gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, ap.rows_pad,
-                         Pc, KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB, 1.0f);
-    else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_BF16 &&
+    if (fp8) {
+#define Z8_LAUNCH(Z16, MT)                                                                             \
+  hipLaunchKernelGGL((k_zgemm8<Z16, MT>), gg, dim3(64), 0, h->stream, (const i32x8*)ap.a8, (const i32x8*)bhi, \
+                     ap.rows_pad, Pc, KP, n, Z, (const unsigned int*)ap.gmax, (const unsigned int*)gmaxB,  \
+                     (float)zs16)
+      if (z16 && mt8 > 1) Z8_LAUNCH(true, ZGEMM8_MT);
+      else if (z16) Z8_LAUNCH(true, 1);
+      else if (mt8 > 1) Z8_LAUNCH(false, ZGEMM8_MT);
+      else Z8_LAUNCH(false, 1);
+#undef Z8_LAUNCH
+    } else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_BF16 &&
              zgemm_as_go<1, false>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z, nullptr, nullptr)) {
     } else if (PFTCC_ZGEMM_AS && prec == PFTCC_PREC_SPLIT_BF16 &&
                zgemm_as_go<3, false>(KP, ap.rows_pad, Pc, n, h->stream, ap.hi, ap.lo, bhi, blo, Z, nullptr, nullptr)) {
