@@ -362,7 +362,7 @@ def coarse_fine_form(nrefs=10000, nptcls=100000, sample=1000, tops=(16, 32, 64, 
     lib.kill()
     evals = float(nptcls) * nrefs * nrots
     return {
-        "workload": "FP8 e4m3 coarse pass (per-shell power-of-two scales, FP32 accumulation), the top-T references "
+        "workload": "FP8 e4m3 coarse pass (per-shell power-of-two scales, FP32 accumulation, spectra stored as FP16), the top-T references "
                     "of each particle rescored in FP64 at their coarse best rotation and both neighbours; "
                     "large coarse-to-fine shape, all particles resident in HBM, 256-particle operand chunks",
         "value": evals / wall8, "unit": "evals/s (end to end, one call over all particles)",
