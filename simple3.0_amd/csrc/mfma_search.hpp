@@ -1059,6 +1059,7 @@ struct RescoreArgs {
   int* best_irot;
   int* best_ishift;            // or null
   double* best_cc64;           // or null
+  const int* order;            // [np] output position taken by the t-th work-group (XCD order), or null
 };
 
 // Exact rescoring of the candidates in FP64: for each candidate (reference, shift, rho) the
@@ -1075,6 +1076,53 @@ struct RescoreArgs {
 // separates the buffers' writers and readers.  Lane sums are reduced over the wavefront in a fixed
 // order, so a candidate's value does not depend on its slot.  JB = ceil(n / 64) positions per lane.
 // SH: the candidates carry shifts (gencorrs_cc_2), whose phases are evaluated per element.
+// Processing order of the rescoring: output positions counting-sorted by the key row (reference x
+// shift) of their best candidate, the first of k_topt's descending list (row nb - 1 when there is
+// none).  The work-groups that run together on one XCD then take particles whose candidates share
+// reference rings, which stay in its L2.  The order within a row follows atomics and is not fixed;
+// each position's result does not depend on it.
+__global__ void k_order_hist(const unsigned long long* __restrict__ cand, int np, int T, int n2, int nb,
+                             int* __restrict__ bin, int* __restrict__ hist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const unsigned long long key = cand[(size_t)i * T];
+  int b = nb - 1;
+  if (key) b = min(nb - 1, (int)((0xFFFFFFFFu - (unsigned int)(key & 0xFFFFFFFFull)) / (unsigned int)n2));
+  bin[i] = b;
+  atomicAdd(&hist[b], 1);
+}
+
+// Exclusive prefix sum of hist[0, nb) in place: one work-group of 1024 threads, each a contiguous
+// chunk.
+__global__ void __launch_bounds__(1024) k_order_scan(int* __restrict__ hist, int nb) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+  int s = 0;
+  for (int b = b0; b < b1; ++b) s += hist[b];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t ? part[t - 1] : 0;
+  for (int b = b0; b < b1; ++b) {
+    const int c = hist[b];
+    hist[b] = run;
+    run += c;
+  }
+}
+
+__global__ void k_order_scatter(const int* __restrict__ bin, int np, int* __restrict__ offs, int* __restrict__ order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  order[atomicAdd(&offs[bin[i]], 1)] = i;
+}
+
 constexpr int RESCORE_MAX = TOPT_MAX;
 constexpr int RESCORE_WAVES = 16;
 template <bool SH, int JB>
@@ -1086,7 +1134,7 @@ __global__ void __launch_bounds__(64 * RESCORE_WAVES) k_rescore64(RescoreArgs a)
   extern __shared__ __align__(16) float2 pk[];    // 2 x [n] particle ring, then 2 x [n] |CTF|
   const int n = a.n, n2 = 2 * n;
   float* ck = reinterpret_cast<float*>(pk + 2 * n);
-  const int i = blockIdx.x;
+  const int i = a.order ? a.order[xcd_remap(blockIdx.x, gridDim.x)] : blockIdx.x;
   const int nw = blockDim.x >> 6;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ip = a.plist[i];

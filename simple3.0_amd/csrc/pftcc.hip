@@ -1050,6 +1050,9 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector
   return PFTCC_OK;
 }
 
+#ifndef PFTCC_RESCORE_ORDER
+#define PFTCC_RESCORE_ORDER 1   // rescoring work-groups take particles grouped by their best candidate's row
+#endif
 #ifndef PFTCC_FP8_Z16
 #define PFTCC_FP8_Z16 1   // FP8 coarse pass: spectra Z stored as FP16 pairs (half the round trip's bytes)
 #endif
@@ -3155,8 +3158,15 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
   unsigned long long *d_best = nullptr, *d_prbest = nullptr, *d_cand = nullptr;
   float* d_cc = nullptr;
   int *d_ir = nullptr, *d_rot = nullptr, *d_ish = nullptr, *d_pl = nullptr;
+  int *d_bin = nullptr, *d_hist = nullptr, *d_order = nullptr;
   double* d_sh = nullptr;
   int rc;
+  // rescoring order: one bin per key row (global reference x shift) and one for particles without
+  // candidates
+  const size_t nbins = (size_t)h->nrefs * nse + 1;
+  if (ntop > 0 && PFTCC_RESCORE_ORDER &&
+      ((rc = dalloc(h, &d_bin, (size_t)np)) || (rc = dalloc(h, &d_hist, nbins)) || (rc = dalloc(h, &d_order, (size_t)np))))
+    return rc;
   if ((rc = dalloc(h, &d_best, (size_t)np)) || (rc = dalloc(h, &d_cc, (size_t)np)) || (rc = dalloc(h, &d_ir, (size_t)np)) ||
       (rc = dalloc(h, &d_rot, (size_t)np)) || (rc = dalloc(h, &d_ish, (size_t)np)))
     return rc;
@@ -3176,6 +3186,15 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
     if (rc == PFTCC_OK) {
       hipLaunchKernelGGL(k_topt, dim3(np), dim3(256), 0, h->stream, (const unsigned long long*)d_prbest, np,
                          (int)nrows, ntop, d_cand);
+      if (d_order) {
+        const unsigned g = (unsigned)((np + 255) / 256);
+        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, h->stream,
+                           (unsigned int*)d_hist, (int)nbins, 0u);
+        hipLaunchKernelGGL(k_order_hist, dim3(g), dim3(256), 0, h->stream, (const unsigned long long*)d_cand, np, ntop,
+                           2 * h->n, (int)nbins, d_bin, d_hist);
+        hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, h->stream, d_hist, (int)nbins);
+        hipLaunchKernelGGL(k_order_scatter, dim3(g), dim3(256), 0, h->stream, (const int*)d_bin, np, d_hist, d_order);
+      }
       RescoreArgs ra{};
       ra.n = h->n;
       ra.nk = h->nk;
@@ -3198,6 +3217,7 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
       ra.best_iref = d_ir;
       ra.best_irot = d_rot;
       ra.best_ishift = d_ish;
+      ra.order = d_order;
       // one wavefront per candidate; the staging of a shell needs at least n threads
       const int nw = std::max(std::min(RESCORE_WAVES, ntop), (h->n + 63) / 64);
       const size_t lds = (size_t)2 * h->n * (sizeof(float2) + sizeof(float));
@@ -3237,6 +3257,9 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
   dfree(d_cand);
   dfree(d_pl);
   dfree(d_sh);
+  dfree(d_bin);
+  dfree(d_hist);
+  dfree(d_order);
   return rc;
 }
 
