@@ -224,6 +224,41 @@ def test_fp8_coarse_pass_error_and_rescoring():
     lib.kill()
 
 
+@pytest.mark.parametrize("n,kto,ldim", [(66, 20, 48), (192, 70, 160)])
+def test_fp8_spectra_paths(n, kto, ldim):
+    """The FP8 contraction's two other paths: at ring length 66 (no compile-time inverse stage) the
+    spectra stay float2 and each wavefront takes 8 consecutive m (K' = 128); with 69 shells
+    (K' = 256, two k' steps) one m per wavefront, with FP16 spectra at ring length 192.  Coarse
+    correlations against the ring-mean oracle; with rescoring, the FP64 oracle's exact argmax and
+    cc for planted winners."""
+    say(f"fp8 spectra paths n={n} K=[2,{kto}]")
+    nr, npt = 12, 5
+    pf, cps = make_case(seed=400 + n, n=n, kfrom=2, kto=kto, ctf=True, astig=0.1, nrefs=nr, nptcls=npt, ldim=ldim,
+                        eo_mixed=True)
+    rng = np.random.default_rng(401 + n)
+    for i in range(npt):
+        r = int(rng.integers(0, nr))
+        rho = int(rng.integers(0, 2 * pf.pftsz))
+        src = pf.refs_even[r] if pf.iseven[i] else pf.refs_odd[r]
+        rolled = np.roll(O.full_ring(src.astype(np.complex128)), rho, axis=-1)[:, :pf.pftsz]
+        noise = 1.0 * (rng.standard_normal(rolled.shape) + 1j * rng.standard_normal(rolled.shape))
+        pf.ptcls[i] = (rolled * pf.ctfmats[i] + noise * np.abs(rolled).mean()).astype(np.complex64)
+    ids = np.arange(1, npt + 1)
+    lib = to_lib(pf, cps)
+    coarse = lib.gencorrs_mfma_batch(ids, prec=PFTCC_PREC_FP8).astype(np.float64)
+    ref = oracle_cube(ring_mean_case(copy.deepcopy(pf)), range(npt), range(nr))
+    errs = [peak_rel_err(coarse[p], ref[p]) for p in range(npt)]
+    say(f"  fp8 peak-relative error vs ring-mean oracle: max {max(errs):.2e}")
+    assert max(errs) < 0.15
+    exact = oracle_cube(pf, range(npt), range(nr))
+    cc, ir, rot = lib.search_mfma(ids, prec=PFTCC_PREC_FP8, ntop=nr)
+    for i in range(npt):
+        r, t = np.unravel_index(int(np.argmax(exact[i])), exact[i].shape)
+        assert (ir[i], rot[i]) == (r + 1, t + 1)
+        assert abs(cc[i] - exact[i][r, t]) <= 1e-7
+    lib.kill()
+
+
 def _fp8_keys(cube, n2):
     """Per (particle, reference) best key of the coarse pass: its largest value, ties to the lowest
     rotation; references then ordered by key (value descending, ties to the lowest reference)."""
