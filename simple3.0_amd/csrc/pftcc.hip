@@ -1678,7 +1678,8 @@ int polarize_impl(pftcc* h, int count, const int32_t* img_inds, const void* cmat
                                hipMemcpyHostToDevice, h->stream));
       src = d_img;
     }
-    hipLaunchKernelGGL(k_polarize, dim3((np + 255) / 256, (nb + PZ_IMGS - 1) / PZ_IMGS), dim3(256), 0, h->stream, src,
+    hipLaunchKernelGGL(k_polarize, dim3((unsigned)((np + 255) / 256 * ((nb + PZ_IMGS - 1) / PZ_IMGS))), dim3(256), 0,
+                       h->stream, src,
                        ldim, nb, (const float*)h->d_pz_w, (const int*)h->d_pz_hk, nk, n, (const int*)d_slot + b0,
                        mask ? (const int8_t*)d_mask : (const int8_t*)nullptr, out);
     HIPCHK(h, hipGetLastError());

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pftcc_kernels.hpp"
+
 namespace pfk {
 
 constexpr int PZ_WDIM = 3;                   // sqwin of half-width 1.5: nint(x) - 1 .. nint(x) + 1
@@ -110,6 +112,8 @@ __global__ void k_polar_tables(const float* __restrict__ polar_x, const float* _
 // point's table once and grids PZ_IMGS consecutive images with it.  cmats: [nimg][ldim][ldim/2 + 1]
 // complex; out: the handle's ring block, image b written at slot[b] * nk * n; mask: [nk] shell
 // flags or null.  FP32 accumulation in the reference's order (l fastest, then m).
+// 1-D grid of (point blocks) x (image groups) in XCD-aware order: the point blocks of one image
+// group run on one XCD and read its images from that XCD's L2.
 constexpr int PZ_IMGS = 8;
 
 __global__ void __launch_bounds__(256) k_polarize(const float2* __restrict__ cmats, int ldim, int nimg,
@@ -117,9 +121,12 @@ __global__ void __launch_bounds__(256) k_polarize(const float2* __restrict__ cma
                                                   int nk, int n, const int* __restrict__ slot,
                                                   const int8_t* __restrict__ mask, float2* __restrict__ out) {
   const int np = nk * n;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gx = (np + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int by = t / gx, bx = t - by * gx;
+  const int p = bx * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const int b0 = blockIdx.y * PZ_IMGS;
+  const int b0 = by * PZ_IMGS;
   const int kk = p / n;
   const bool zero = mask && !mask[kk];
   const int hc = ldim / 2 + 1;
