@@ -522,14 +522,17 @@ __global__ void __launch_bounds__(NT) k_zifft_v2(ZifftArgs a) {
     const int task = tid + c * NT;
     if (!(T1 % NT == 0 || task < T1)) continue;
     const int g = task / F, f = task % F;
+    // element offsets below (N + 1) ZG16 from the wave-uniform block base: 32-bit
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       if constexpr (Z16) {
-        za[c][t] = h2_to_f2(Zh[(size_t)(g + t * Q) * ZG16 + f]);
-        zb[c][t] = h2_to_f2(Zh[(size_t)((t + 1) * Q - g) * ZG16 + f]);
+        za[c][t] = h2_to_f2(*reinterpret_cast<const unsigned int*>(
+            reinterpret_cast<const char*>(Zh) + ((unsigned int)((g + t * Q) * ZG16 + f) << 2)));
+        zb[c][t] = h2_to_f2(*reinterpret_cast<const unsigned int*>(
+            reinterpret_cast<const char*>(Zh) + ((unsigned int)(((t + 1) * Q - g) * ZG16 + f) << 2)));
       } else {
-        za[c][t] = Zb[(size_t)(g + t * Q) * ZG + f];
-        zb[c][t] = Zb[(size_t)((t + 1) * Q - g) * ZG + f];
+        za[c][t] = ld8(Zb, (unsigned int)((g + t * Q) * ZG + f));
+        zb[c][t] = ld8(Zb, (unsigned int)(((t + 1) * Q - g) * ZG + f));
       }
     }
     zn[c] = zb[c][3];   // Z[N - g]; for g = 0 this is Z[N]
