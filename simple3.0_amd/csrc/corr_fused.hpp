@@ -55,10 +55,22 @@ struct Fp {
   static constexpr bool fusable = np >= 2 && plan.r[0] == 4 && N % 4 == 0;
 };
 
+// Offset of middle pass P's twiddle table [t - 1][jm] (t = 1..R-1, jm < Ns) in a block holding the
+// tables of passes 1, 2, ...: entry (t, jm) = tw2[2 jm t N / (Ns R)], read with stride 1 across
+// lanes where tw2's index strides land on few banks.
+template <int N>
+constexpr int twm_off(int P) {
+  int off = 0;
+  for (int q = 1; q < P; ++q) off += (Fp<N>::R(q) - 1) * Fp<N>::Ns(q);
+  return off;
+}
+
 // In-place Stockham pass P of F transforms held in LDS (transform f at buf + f S); each thread
-// stages its butterflies' inputs in registers between two barriers.
-template <int N, int F, int NT, int P, int S = N>
-__device__ __forceinline__ void lds_pass(float2* __restrict__ buf, const float2* __restrict__ tw2, int tid) {
+// stages its butterflies' inputs in registers between two barriers.  TM: twiddles from the
+// middle-pass tables twm (twm_off layout) instead of tw2.
+template <int N, int F, int NT, int P, int S = N, bool TM = false>
+__device__ __forceinline__ void lds_pass(float2* __restrict__ buf, const float2* __restrict__ tw2, int tid,
+                                         const float2* __restrict__ twm = nullptr) {
   constexpr int R = Fp<N>::R(P);
   constexpr int Ns = Fp<N>::Ns(P);
   constexpr int nb = N / R;
@@ -84,7 +96,8 @@ __device__ __forceinline__ void lds_pass(float2* __restrict__ buf, const float2*
       const int jm = j % Ns;
       if constexpr (Ns > 1) {
 #pragma unroll
-        for (int t = 1; t < R; ++t) v[c][t] = cmul(v[c][t], tw2[2 * jm * t * tstep]);
+        for (int t = 1; t < R; ++t)
+          v[c][t] = cmul(v[c][t], TM ? twm[twm_off<N>(P) + (t - 1) * Ns + jm] : tw2[2 * jm * t * tstep]);
       }
       Dft<R>::run(v[c]);
       const int base = (j / Ns) * Ns * R + jm;
@@ -95,11 +108,11 @@ __device__ __forceinline__ void lds_pass(float2* __restrict__ buf, const float2*
   __syncthreads();
 }
 
-template <int N, int F, int NT, int P, int PEND, int S = N>
-__device__ __forceinline__ void lds_passes(float2* buf, const float2* tw2, int tid) {
+template <int N, int F, int NT, int P, int PEND, int S = N, bool TM = false>
+__device__ __forceinline__ void lds_passes(float2* buf, const float2* tw2, int tid, const float2* twm = nullptr) {
   if constexpr (P < PEND) {
-    lds_pass<N, F, NT, P, S>(buf, tw2, tid);
-    lds_passes<N, F, NT, P + 1, PEND, S>(buf, tw2, tid);
+    lds_pass<N, F, NT, P, S, TM>(buf, tw2, tid, twm);
+    lds_passes<N, F, NT, P + 1, PEND, S, TM>(buf, tw2, tid, twm);
   }
 }
 
@@ -274,11 +287,26 @@ constexpr int fused_twl_len() {
   return PFTCC_FUSED_TWL ? (RL - 1) * (N / RL) : 0;
 }
 
+// Entries of the middle passes' twiddle tables (Stockham passes 1 .. np - 2), 0 when unused.
+// Measured at n 400 (radix-5 pass with Ns = 20: tw2's index stride 8 jm t put 16 lanes on two
+// banks): 239.9 -> 223.0 ms per 600-particle search; n 192 not faster (192.3 against 193.7 ms).
+#ifndef PFTCC_FUSED_TWM
+#define PFTCC_FUSED_TWM 1      // middle-pass twiddles from [t][jm] tables (1: at n 400, 2: wherever Stockham)
+#endif
+template <int N>
+constexpr bool twm_on() {
+  return PFTCC_FUSED_TWM == 2 || (PFTCC_FUSED_TWM == 1 && N == 400);
+}
+template <int N>
+constexpr int fused_twm_len() {
+  return (twm_on<N>() && !fused_pfa<N>()) ? twm_off<N>(Fp<N>::np - 1) : 0;
+}
+
 template <int N, int F>
 constexpr size_t fused_lds_bytes() {
   // buf[F N] + Wk[N+1] (W1, W2: 2 (N+1)) + tw2[2N] + twL float2;  ctfk[N] + sq/off slots 2 F N/4 floats
   return sizeof(float2) * ((size_t)F * fused_stride<N>() + (PFTCC_FUSED_W12 ? 2 : 1) * fused_wlen<N>() + 2 * N +
-                           fused_twl_len<N>() + 2 * N) +
+                           fused_twl_len<N>() + fused_twm_len<N>() + 2 * N) +
          sizeof(float) * ((size_t)N + (size_t)F * (N / 4) * 2);
 }
 
@@ -378,7 +406,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
   const float4* Wk4 = reinterpret_cast<const float4*>(Wk);
   static_assert(!PFA || (F * S) % 2 == 0, "16-byte aligned staged spectra");
   float2* twL = tw2 + 2 * N;                 // [RL-1][NsL]  twL[(t-1) NsL + g] = tw2[2 g t]
-  float2* rck = twL + fused_twl_len<N>();    // [2][N] ref x |CTF| of the tile's references (RST)
+  float2* twM = twL + fused_twl_len<N>();    // middle-pass twiddle tables (Stockham sizes)
+  float2* rck = twM + fused_twm_len<N>();    // [2][N] ref x |CTF| of the tile's references (RST)
   float* ctfk = reinterpret_cast<float*>(rck + 2 * N);   // [N]
   float* sqs = ctfk + N;                     // [F][Q]
   float* offs = sqs + F * Q;                 // [F][Q]
@@ -417,6 +446,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
     for (int i = tid; i < (RL - 1) * NsL; i += NT) {
       const int t = i / NsL + 1, g = i - (t - 1) * NsL;
       twL[i] = a.tw2g[2 * g * t];
+    }
+  constexpr bool TWM = fused_twm_len<N>() > 0;
+  if constexpr (TWM)
+    for (int q = 1; q < NP - 1; ++q) {
+      const int R = Fp<N>::R(q), ns = Fp<N>::Ns(q), tstep = N / (ns * R);
+      for (int i = tid; i < (R - 1) * ns; i += NT) {
+        const int t = i / ns + 1, jm = i - (t - 1) * ns;
+        twM[twm_off<N>(q) + i] = a.tw2g[2 * jm * t * tstep];
+      }
     }
   // prime-factor index tables, in the last-pass twiddle table's place (read once per shell, so
   // that no per-lane index is held in registers across the shell loop):
@@ -736,7 +774,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W))) k_
       pfa_pass_a<N, F, NT, S>(buf, tid);
       __syncthreads();
     } else {
-      lds_passes<N, F, NT, 1, NP - 1, S>(buf, tw2, tid);
+      lds_passes<N, F, NT, 1, NP - 1, S, TWM>(buf, tw2, tid, twM);
     }
     // ---- last pass (radix RL) on mirrored butterflies + split + conj(V) W_k ----
     if constexpr (PFA) {
