@@ -3162,18 +3162,34 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
   int *d_bin = nullptr, *d_hist = nullptr, *d_order = nullptr;
   double* d_sh = nullptr;
   int rc;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(h->stream);
+    dfree(d_best);
+    dfree(d_cc);
+    dfree(d_ir);
+    dfree(d_rot);
+    dfree(d_ish);
+    dfree(d_prbest);
+    dfree(d_cand);
+    dfree(d_pl);
+    dfree(d_sh);
+    dfree(d_bin);
+    dfree(d_hist);
+    dfree(d_order);
+  };
   // rescoring order: one bin per key row (global reference x shift) and one for particles without
   // candidates
   const size_t nbins = (size_t)h->nrefs * nse + 1;
-  if (ntop > 0 && PFTCC_RESCORE_ORDER &&
-      ((rc = dalloc(h, &d_bin, (size_t)np)) || (rc = dalloc(h, &d_hist, nbins)) || (rc = dalloc(h, &d_order, (size_t)np))))
+  if ((ntop > 0 && PFTCC_RESCORE_ORDER &&
+       ((rc = dalloc(h, &d_bin, (size_t)np)) || (rc = dalloc(h, &d_hist, nbins)) ||
+        (rc = dalloc(h, &d_order, (size_t)np)))) ||
+      (rc = dalloc(h, &d_best, (size_t)np)) || (rc = dalloc(h, &d_cc, (size_t)np)) || (rc = dalloc(h, &d_ir, (size_t)np)) ||
+      (rc = dalloc(h, &d_rot, (size_t)np)) || (rc = dalloc(h, &d_ish, (size_t)np)) ||
+      (ntop > 0 && ((rc = dalloc(h, &d_prbest, (size_t)np * nrows)) || (rc = dalloc(h, &d_cand, (size_t)np * ntop)) ||
+                    (rc = dalloc(h, &d_pl, (size_t)np)) || (ns > 0 && (rc = dalloc(h, &d_sh, (size_t)2 * ns)))))) {
+    release();
     return rc;
-  if ((rc = dalloc(h, &d_best, (size_t)np)) || (rc = dalloc(h, &d_cc, (size_t)np)) || (rc = dalloc(h, &d_ir, (size_t)np)) ||
-      (rc = dalloc(h, &d_rot, (size_t)np)) || (rc = dalloc(h, &d_ish, (size_t)np)))
-    return rc;
-  if (ntop > 0 && ((rc = dalloc(h, &d_prbest, (size_t)np * nrows)) || (rc = dalloc(h, &d_cand, (size_t)np * ntop)) ||
-                   (rc = dalloc(h, &d_pl, (size_t)np)) || (ns > 0 && (rc = dalloc(h, &d_sh, (size_t)2 * ns)))))
-    return rc;
+  }
   hipLaunchKernelGGL(k_fill_u64, dim3((np + 255) / 256), dim3(256), 0, h->stream, d_best, np, 0ull);
   std::vector<int> plocal;
   rc = mfma_run(h, np, iptcls, nr, irefs, shv, prec, ntop > 0 ? nullptr : d_best, nullptr, d_prbest, &plocal);
@@ -3248,19 +3264,7 @@ int pftcc_search_mfma(pftcc_t* h, int32_t np, const int32_t* iptcls, int32_t nr,
         hipStreamSynchronize(h->stream) != hipSuccess)
       rc = fail(h, PFTCC_E_DEVICE, "search_mfma: copy of the results failed");
   }
-  (void)hipStreamSynchronize(h->stream);
-  dfree(d_best);
-  dfree(d_cc);
-  dfree(d_ir);
-  dfree(d_rot);
-  dfree(d_ish);
-  dfree(d_prbest);
-  dfree(d_cand);
-  dfree(d_pl);
-  dfree(d_sh);
-  dfree(d_bin);
-  dfree(d_hist);
-  dfree(d_order);
+  release();
   return rc;
 }
 
