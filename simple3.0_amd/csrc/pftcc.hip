@@ -204,6 +204,17 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// Runs f when the scope ends, on every return path (temporary device buffers).
+template <typename F>
+struct ScopeExit {
+  F f;
+  ~ScopeExit() { f(); }
+};
+template <typename F>
+ScopeExit<F> scope_exit(F f) {
+  return ScopeExit<F>{f};
+}
+
 // Host <-> device copies and fills issued on the handle's stream and completed before returning.
 // The stream may be non-blocking (a pool stream of the caller), which the null stream's implicit
 // ordering would not cover: a synchronous null-stream copy could overtake pending kernels that
@@ -981,6 +992,15 @@ int ensure_apack(pftcc* h, int eo, const std::vector<int>& rl, const std::vector
   int* d_list = nullptr;
   float2 *rings = nullptr, *spec = nullptr, *srings = nullptr, *shtab = nullptr;
   double* d_sh = nullptr;
+  // the temporaries are released on every return path (hipFree waits for the device)
+  const auto temps = scope_exit([&] {
+    dfree(d_list);
+    dfree(rings);
+    dfree(spec);
+    dfree(srings);
+    dfree(shtab);
+    dfree(d_sh);
+  });
   if ((rc = dalloc(h, &d_list, (size_t)nref)) || (rc = dalloc(h, &rings, (size_t)nref * nk * n)) ||
       (rc = dalloc(h, &spec, (size_t)nr * nk * (n + 1))))
     return rc;
