@@ -130,10 +130,13 @@ constexpr int fused_F() {
 // shapes: n = 308, 9 pairs at >= 3 wavefronts per SIMD (167 VGPRs, no spills, against 2
 // wavefronts at ~190 VGPRs unconstrained); n = 192, 5 pairs; n = 400, 3 pairs with the reference
 // prefetch (128 VGPRs, 4 wavefronts; 240 against 254 ms without it and 265 ms with 4 pairs, which
-// drop to 3 wavefronts with it; 2 pairs 318 ms, 5 to 10 pairs slower).  fused_F elsewhere.
+// drop to 3 wavefronts with it; 2 pairs 318 ms, 5 to 10 pairs slower); n = 500 (box 320), 4 pairs
+// (150 VGPRs, 3 wavefronts: 202.8 ms per 300-particle search against 217.6 / 242.8 / 265.3 ms
+// with 3 / 5 / 6).  fused_F elsewhere.
 template <int N, int NT>
 constexpr int fused_F_tuned() {
   if (NT == 256 && (N == 192 || N == 308 || N == 400)) return N == 192 ? 5 : N == 308 ? 9 : 3;
+  if (NT == 256 && N == 500) return 4;
   return fused_F<N, NT>();
 }
 #ifndef PFTCC_FUSED_GROUP
@@ -153,6 +156,7 @@ template <int N, int NT>
 constexpr int fused_F_rst_tuned() {
   if (NT == 256 && N == 308) return PFTCC_FUSED_PREFETCH ? 9 : 11;
   if (NT == 256 && N == 400) return 4;
+  if (NT == 256 && N == 500) return fused_F<N, NT>();   // not measured: the generic tile
   return fused_F_tuned<N, NT>();
 }
 template <int N, int NT>
