@@ -38,7 +38,16 @@ __host__ __device__ constexpr FftPlan make_fused_plan(int n) {
         rest[i] = rest[k];
         rest[k] = t;
       }
-  for (int i = 0; i < nr; ++i) q.r[q.np++] = rest[i];
+  // a radix-2 last pass would leave N / 4 + 1 mirrored groups per transform (one transform per
+  // work-group at n 600): where a radix 2 remains, it goes first among the middle passes and the
+  // others follow in increasing order, so that the last pass has the largest radix
+  bool has2 = false;
+  for (int i = 0; i < nr; ++i) has2 = has2 || rest[i] == 2;
+  if (!has2) {
+    for (int i = 0; i < nr; ++i) q.r[q.np++] = rest[i];
+  } else {
+    for (int i = nr - 1; i >= 0; --i) q.r[q.np++] = rest[i];
+  }
   return q;
 }
 

@@ -551,7 +551,7 @@ struct FusedLaunch {
   }
 };
 
-#define PFTCC_FUSED_SIZES(X) X(16) X(24) X(32) X(40) X(48) X(64) X(100) X(192) X(240) X(308) X(400) X(500)
+#define PFTCC_FUSED_SIZES(X) X(16) X(24) X(32) X(40) X(48) X(64) X(100) X(192) X(240) X(308) X(400) X(500) X(600)
 
 // Tile (pairs per work-group) of the fused kernel for ring length n, 0 if n has no fused
 // instantiation.  *rst: the staged-reference variant runs, which needs a tile to span at most two

@@ -369,9 +369,10 @@ def test_fused_prime_factor_objectives(mode):
 
 
 @pytest.mark.parametrize("mode", ["cc", "euclid"])
-@pytest.mark.parametrize("n,kto,ldim", [(192, 30, 128), (240, 30, 160), (400, 30, 256), (500, 30, 320)])
+@pytest.mark.parametrize("n,kto,ldim", [(192, 30, 128), (240, 30, 160), (400, 30, 256), (500, 30, 320),
+                                        (600, 30, 384)])
 def test_per_pair_kernel_reference_prefetch(n, kto, ldim, mode):
-    """At ring lengths 192, 240, 400 and 500 the per-pair variant of the fused kernel loads its pass-1
+    """At ring lengths 192, 240, 400, 500 and 600 (radices 4, 2, 3, 5, 5) the per-pair variant of the fused kernel loads its pass-1
     reference values one shell ahead (at 400 with the middle passes' twiddle tables).  Batches of 20 particles x 7 references (the last tile
     partial, the particle groups of 16 wrapping), unshifted and with two shifts, against the FP64
     oracle; the search returns the batch's first maximum."""
