@@ -142,11 +142,13 @@ constexpr int fused_F() {
 // drop to 3 wavefronts with it; 2 pairs 318 ms, 5 to 10 pairs slower); n = 500 (box 320), 4 pairs
 // (150 VGPRs, 3 wavefronts: 202.8 ms per 300-particle search against 217.6 / 242.8 / 265.3 ms
 // with 3 / 5 / 6); n = 240 (box 160), 4 pairs (96 VGPRs: 143.5 ms per 1000-particle search against
-// 148.1 / 156.4 / 165.2 ms with 6 / 5 / 8).  fused_F elsewhere.
+// 148.1 / 156.4 / 165.2 ms with 6 / 5 / 8); n = 600 (box 384), 2 pairs (281.2 ms per 200-particle
+// search against 288.9 / 305.4 ms with 4 / 3).  fused_F elsewhere.
 template <int N, int NT>
 constexpr int fused_F_tuned() {
   if (NT == 256 && (N == 192 || N == 308 || N == 400)) return N == 192 ? 5 : N == 308 ? 9 : 3;
   if (NT == 256 && (N == 240 || N == 500)) return 4;
+  if (NT == 256 && N == 600) return 2;
   return fused_F<N, NT>();
 }
 #ifndef PFTCC_FUSED_GROUP
@@ -166,7 +168,7 @@ template <int N, int NT>
 constexpr int fused_F_rst_tuned() {
   if (NT == 256 && N == 308) return PFTCC_FUSED_PREFETCH ? 9 : 11;
   if (NT == 256 && N == 400) return 4;
-  if (NT == 256 && (N == 240 || N == 500)) return fused_F<N, NT>();   // not measured: the generic tile
+  if (NT == 256 && (N == 240 || N == 500 || N == 600)) return fused_F<N, NT>();   // not measured: the generic tile
   return fused_F_tuned<N, NT>();
 }
 template <int N, int NT>
