@@ -52,6 +52,7 @@ def check_cube(got, ref):
     (192, 2, 40, 128, True, 0.0),
     (308, 2, 60, 200, False, 0.0),
     (400, 2, 80, 256, True, 0.0),
+    (600, 2, 40, 384, True, 0.0),     # radices 4, 2, 3, 5, 5 in the inverse stage
 ])
 def test_mfma_exact_for_ring_constant_ctf(n, kfrom, kto, ldim, ctf, astig):
     say(f"mfma vs oracle n={n} K=[{kfrom},{kto}] ctf={ctf} astig={astig}")
@@ -224,11 +225,12 @@ def test_fp8_coarse_pass_error_and_rescoring():
     lib.kill()
 
 
-@pytest.mark.parametrize("n,kto,ldim", [(66, 20, 48), (192, 70, 160)])
+@pytest.mark.parametrize("n,kto,ldim", [(66, 20, 48), (192, 70, 160), (600, 20, 384)])
 def test_fp8_spectra_paths(n, kto, ldim):
-    """The FP8 contraction's two other paths: at ring length 66 (no compile-time inverse stage) the
+    """The FP8 contraction's other paths: at ring length 66 (no compile-time inverse stage) the
     spectra stay float2 and each wavefront takes 8 consecutive m (K' = 128); with 69 shells
-    (K' = 256, two k' steps) one m per wavefront, with FP16 spectra at ring length 192.  Coarse
+    (K' = 256, two k' steps) one m per wavefront, with FP16 spectra at ring length 192; FP16
+    spectra and 8 m per wavefront at ring length 600 (inverse-stage radices 4, 2, 3, 5, 5).  Coarse
     correlations against the ring-mean oracle; with rescoring, the FP64 oracle's exact argmax and
     cc for planted winners."""
     say(f"fp8 spectra paths n={n} K=[2,{kto}]")
