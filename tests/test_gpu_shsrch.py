@@ -22,8 +22,8 @@ pytestmark = pytest.mark.gpu
 TRS = 3.0
 
 
-def _planted_case(seed, ctf, objective="cc"):
-    pf, cps = make_case(seed=seed, n=48, kfrom=2, kto=16, nrefs=3, nptcls=5, ctf=ctf, ldim=64,
+def _planted_case(seed, ctf, objective="cc", n=48, kto=16, ldim=64, nptcls=5):
+    pf, cps = make_case(seed=seed, n=n, kfrom=2, kto=kto, nrefs=3, nptcls=nptcls, ctf=ctf, ldim=ldim,
                         euclid=objective == "euclid", match_filt=objective == "match_filt")
     rng = np.random.default_rng(seed)
     n = pf.pftsz
@@ -81,6 +81,34 @@ def test_shift_search_matches_oracle(opt_angle, ctf, objective, coarse):
         for p in range(3):
             i = pairs.index((p, p))
             assert rot[i] > 0 and cxy[i, 0] > (0.8 if ctf else 0.95)
+
+
+def test_shift_search_ring_length_600():
+    """The in-plane callback's compile-time kernel at ring length 600 (box 384; transform radices
+    4, 2, 3, 5, 5): planted pairs against the oracle's L-BFGS-B restatement."""
+    say("shift search parity at n 600")
+    pf, cps = _planted_case(61, True, n=600, kto=20, ldim=384, nptcls=3)
+    lib = to_lib(pf, cps)
+    pairs = [(r, p) for p in range(3) for r in range(3)]
+    irots = [int(np.argmax(pf.gencorrs(r, p, (0.0, 0.0), method="fft"))) + 1 for r, p in pairs]
+    cxy, rot = lib.shsrch_grad_batch([r + 1 for r, _ in pairs], [p + 1 for _, p in pairs], trs=TRS, maxits=60,
+                                     opt_angle=True, irots=irots)
+    lib.kill()
+    found = 0
+    for i, (r, p) in enumerate(pairs):
+        ocxy, orot = _oracle_pair(pf, r, p, True, irots[i])
+        assert rot[i] == orot, (r, p, rot[i], orot)
+        if orot == 0:
+            continue
+        found += 1
+        assert abs(cxy[i, 0] - ocxy[0]) <= 1e-5 * max(1.0, abs(ocxy[0])), (r, p, cxy[i], ocxy)
+        np.testing.assert_allclose(cxy[i, 1:], ocxy[1:], rtol=0, atol=1e-4, err_msg=f"pair {(r, p)}")
+    assert found >= 3
+    # the planted pairs are found (planted without the |CTF| the correlation applies to the
+    # reference: 0.76 at this shape)
+    for p in range(3):
+        i = pairs.index((p, p))
+        assert rot[i] > 0 and cxy[i, 0] > 0.6
 
 
 def test_shift_search_batch_is_order_independent():
