@@ -168,7 +168,9 @@ template <int N, int NT>
 constexpr int fused_F_rst_tuned() {
   if (NT == 256 && N == 308) return PFTCC_FUSED_PREFETCH ? 9 : 11;
   if (NT == 256 && N == 400) return 4;
-  if (NT == 256 && (N == 240 || N == 500 || N == 600)) return fused_F<N, NT>();   // not measured: the generic tile
+  if (NT == 256 && (N == 240 || N == 600)) return fused_F<N, NT>();   // not measured: the generic tile
+  // n 500: 4 pairs as in the per-pair variant (742 against 781 ms for a 100-particle x 9-shift
+  // search with 5)
   return fused_F_tuned<N, NT>();
 }
 template <int N, int NT>

@@ -40,6 +40,7 @@ def shift_grid(lim, step):
     (308, 60, 200, shift_grid(3.0, 0.5), 17, 3),     # 169 shifts, tile 9: 507 pairs, 3 left over
     (192, 40, 128, shift_grid(1.5, 1.5)[:7], 21, 3),  # 7 shifts, tile 5
     (400, 80, 256, shift_grid(1.0, 1.0), 17, 2),      # 9 shifts, tile 4
+    (500, 40, 320, shift_grid(1.0, 1.0), 17, 2),      # 9 shifts, tile 4
     (16, 7, 32, shift_grid(2.0, 0.5)[:33], 18, 2),    # 33 shifts, tile 32
 ])
 def test_staged_reference_kernel_matches_oracle(n, kto, ldim, shifts, nptcls, nrefs):
